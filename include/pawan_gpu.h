@@ -1,0 +1,163 @@
+/*
+ * pawan_gpu.h -- C-ABI of the MI355X (gfx950) implementation of PAWAN's
+ * particle-interaction and time-integration hot path.
+ *
+ * The reference keeps this path in-process C++ (no FFI): the integrators
+ * call the abstract solver __system (src/system/system.h:20-90) through
+ * setStates / solve / getRates, and __interaction implements it over a list
+ * of __wake objects (src/interaction/interaction.h:22-261).  The C++ class
+ * API shipped with this library (pawan_amd/csrc/host/pawan/ headers) keeps those
+ * class names and virtual shapes and calls the functions below; every entry
+ * point names the reference member it replaces.
+ *
+ * Conventions
+ *   - Every function returns an int status: PW_OK (0) or a PW_ERR_* code.
+ *     pw_last_error() returns a thread-local message for the last failure.
+ *     No C++ exception crosses this boundary.
+ *   - Host pointers are borrowed for the duration of the call only.
+ *   - Per-particle 3-vectors are N x 3 row-major doubles (the layout of the
+ *     reference's gsl_matrix rows, src/wake/wake.h:49-52).
+ *   - A context owns its device memory and one HIP stream; work is
+ *     asynchronous on that stream and every function that returns host
+ *     data synchronises it.  A context is driven by one host thread.
+ */
+#ifndef PAWAN_GPU_H_
+#define PAWAN_GPU_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pw_ctx pw_ctx;
+
+/* status codes */
+enum {
+  PW_OK = 0,
+  PW_ERR_ARG = 1,         /* bad argument (null pointer, size, index)      */
+  PW_ERR_HIP = 2,         /* a HIP runtime call failed                      */
+  PW_ERR_STATE = 3,       /* call not valid in the context's current state  */
+  PW_ERR_NONFINITE = 4,   /* a rate or state came out NaN / Inf             */
+  PW_ERR_UNSUPPORTED = 5, /* option not available for this configuration    */
+  PW_ERR_NOMEM = 6        /* device allocation failed                       */
+};
+
+/* pair arithmetic */
+enum {
+  PW_FP64 = 0,  /* FP64 pair math and accumulation (the plain form)         */
+  PW_FP32C = 1, /* FP32 pair math, compensated per-target accumulation,
+                   FP64 state, stage combination and diagnostics           */
+  PW_LEARNED = 2, /* learned far field (reserved)                           */
+  PW_FP64_REF = 3 /* FP64 with the reference's kernel formula evaluated for
+                     every pair, operation by operation (PW_FP64 replaces
+                     it by its closed far-field limit where the Gaussian
+                     terms are below FP64 rounding, rho > 9.49 sigma)      */
+};
+
+/* time integration schemes (the value is the reference's order) */
+enum { PW_EULER = 1, PW_RK3 = 3, PW_RK4 = 4 };
+
+/* Library version string. */
+const char *pw_version(void);
+
+/* Message for the last failing call on this thread ("" if none). */
+const char *pw_last_error(void);
+
+/* Number of visible HIP devices. */
+int pw_device_count(int *count);
+
+/* Create a solver context on HIP device `device`.
+ * Replaces __interaction::__interaction() (src/interaction/interaction.cpp:
+ * 11-22): the reference hard-codes nu = 1.56e-5 there; the context starts
+ * with the same value and pw_set_nu changes it. */
+int pw_create(pw_ctx **out, int device);
+
+/* Release the context and all of its device memory (~__interaction,
+ * interaction.cpp:37-42). */
+int pw_destroy(pw_ctx *ctx);
+
+/* Append a wake of n particles (__interaction::addWake,
+ * interaction.cpp:44-52).  maxn >= n is the wake's _maxnumParticles, which
+ * fixes its block of the state vector: 6*maxn doubles laid out
+ * [pos(maxn x 3) ; alpha(maxn x 3)] (src/wake/wake.cpp:856-893).
+ * pos, alpha: n x 3; radius, volume: n.  Returns the wake index in *wake
+ * (may be NULL).  Wakes can only be added before the first solve/step. */
+int pw_add_wake(pw_ctx *ctx, int64_t n, int64_t maxn, const double *pos,
+                const double *alpha, const double *radius,
+                const double *volume, int *wake);
+
+/* Kinematic viscosity of the PSE term (__interaction::_nu,
+ * interaction.h:25; hard-coded in the reference, settable here). */
+int pw_set_nu(pw_ctx *ctx, double nu);
+int pw_get_nu(pw_ctx *ctx, double *nu);
+
+/* coupled = 1 (default): all wakes interact, as __interaction::interact()
+ * does for its wake list (interaction.cpp:196-210).  coupled = 0: every
+ * wake is an independent system (the reference equivalent is one
+ * __interaction object per wake); they are advanced together, each wake's
+ * targets seeing only its own wake's sources. */
+int pw_set_coupled(pw_ctx *ctx, int coupled);
+
+/* Total particles and wakes in the context. */
+int pw_get_size(pw_ctx *ctx, int64_t *num_particles, int *num_wakes);
+
+/* Per-wake state and rates, n x 3 each (the wake's _position/_vorticity and
+ * _velocity/_retvorcity rows, src/wake/wake.h:49-52). */
+int pw_set_wake_state(pw_ctx *ctx, int wake, const double *pos,
+                      const double *alpha);
+int pw_get_wake_state(pw_ctx *ctx, int wake, double *pos, double *alpha);
+int pw_get_wake_rates(pw_ctx *ctx, int wake, double *vel, double *retvor);
+
+/* Whole-system state / rate vectors in the reference layout: wakes
+ * concatenated, wake w occupying 6*maxn_w doubles [pos ; alpha] or
+ * [vel ; retvor]; padding rows are read as zero and written as zero.
+ * len must equal sum_w 6*maxn_w (__system::_totalmaxsize).
+ * Replace __interaction::setStates / getStates / getRates
+ * (interaction.cpp:337-365 with wake.cpp:856-893). */
+int pw_set_states(pw_ctx *ctx, const double *state, int64_t len);
+int pw_get_states(pw_ctx *ctx, double *state, int64_t len);
+int pw_get_rates(pw_ctx *ctx, double *rate, int64_t len);
+
+/* One right-hand-side evaluation: zero the rates, then all self and (when
+ * coupled) cross-wake pair interactions, leaving velocity and vorticity
+ * rate of every particle on the device (__interaction::solve ->
+ * interact(), interaction.cpp:54-57, 196-210; __parallel::interact,
+ * parallel.cpp:14-86).  precision is PW_FP64 or PW_FP32C. */
+int pw_solve(pw_ctx *ctx, int precision);
+
+/* Advance the device-resident state by nsteps steps of dt with scheme
+ * PW_EULER (__integration::step, src/integration/integration.cpp:126-136),
+ * PW_RK3 (__rk3::step, src/integration/rk3.cpp:12-65) or PW_RK4
+ * (__rk4::step, src/integration/rk4.cpp:12-77).  The stage combination is
+ * done in FP64 in the reference's operation order.  After the call the
+ * rates on the device are those of the last stage evaluated, as in the
+ * reference (the wake rates hold k of the final solve()). */
+int pw_step(pw_ctx *ctx, int scheme, double dt, int precision,
+            int64_t nsteps);
+
+/* FP64 diagnostics of the current state, 15 doubles in the order
+ * __interaction::writediagnosis writes them (interaction.cpp:325-335):
+ * total vorticity (3), linear impulse (3), angular impulse (3), helicity,
+ * enstrophy, div-free enstrophy, kinetic energy, div-free kinetic energy,
+ * centroid Zc (__interaction::diagnose, interaction.cpp:64-194).
+ * For an uncoupled context out holds 15 doubles per wake, each wake
+ * diagnosed as its own system. */
+int pw_diagnose(pw_ctx *ctx, double *out, int64_t out_len);
+
+/* Wait for all queued work of the context. */
+int pw_synchronize(pw_ctx *ctx);
+
+/* Kernel timing with HIP events on the context's stream.  While enabled,
+ * every launch of the pair-interaction kernel is bracketed by events; the
+ * totals accumulate until reset.  pw_get_pair_timing synchronises and
+ * returns the summed kernel time in milliseconds and the number of
+ * launches. */
+int pw_set_profiling(pw_ctx *ctx, int enable);
+int pw_get_pair_timing(pw_ctx *ctx, double *total_ms, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

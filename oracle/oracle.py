@@ -1,0 +1,200 @@
+"""ctypes binding of the oracle library (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py.  The product package never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+RHS_FN = C.CFUNCTYPE(None, C.c_int64, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p)
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def _load() -> C.CDLL:
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = C.CDLL(LIB_PATH)
+    d, i64, dp, ip = C.c_double, C.c_int64, _dp, _ip
+    pd = C.POINTER(C.c_double)
+    lib.or_dnrm2.argtypes = [dp]
+    lib.or_dnrm2.restype = d
+    lib.or_kernel.argtypes = [d, d, pd, pd, pd, pd]
+    lib.or_kernel_q128.argtypes = [d, d, pd, pd, pd]
+    lib.or_qsig.argtypes = [d, d]
+    lib.or_qsig.restype = d
+    lib.or_interact.argtypes = [d, d, d, dp, dp, dp, dp, d, d, dp, dp, dp, dp]
+    sysargs = [C.c_int, ip, dp, dp, dp, dp, d]
+    lib.or_solve.argtypes = sysargs + [dp, dp]
+    lib.or_solve_omp.argtypes = sysargs + [dp, dp]
+    lib.or_solve_q128.argtypes = sysargs + [dp, dp]
+    lib.or_solve_omp_rows.argtypes = [i64, dp, dp, dp, dp, d, ip, i64, dp, dp]
+    lib.or_rates_targets.argtypes = sysargs + [ip, i64, dp, dp, C.c_int]
+    lib.or_rates_targets_q128.argtypes = sysargs + [ip, i64, dp, dp]
+    lib.or_step.argtypes = [C.c_int, d, i64, C.c_int, ip, dp, dp, dp, dp, d, C.c_int]
+    lib.or_step.restype = C.c_int
+    lib.or_step_generic.argtypes = [C.c_int, d, i64, i64, dp, RHS_FN, C.c_void_p]
+    lib.or_step_generic.restype = C.c_int
+    lib.or_diagnose.argtypes = [C.c_int, ip, dp, dp, dp, d, dp, C.c_int]
+    return lib
+
+
+_LIB: C.CDLL | None = None
+
+
+def lib() -> C.CDLL:
+    global _LIB
+    if _LIB is None:
+        _LIB = _load()
+    return _LIB
+
+
+def _arrs(s):
+    return (s.nw, np.ascontiguousarray(s.off, dtype=np.int64), np.ascontiguousarray(s.pos),
+            np.ascontiguousarray(s.alpha), np.ascontiguousarray(s.radius),
+            np.ascontiguousarray(s.volume))
+
+
+def kernel(rho: float, sigma: float):
+    q, F, Z, n = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+    lib().or_kernel(rho, sigma, q, F, Z, n)
+    return q.value, F.value, Z.value, n.value
+
+
+def kernel_q128(rho: float, sigma: float):
+    q, F, Z = C.c_double(), C.c_double(), C.c_double()
+    lib().or_kernel_q128(rho, sigma, q, F, Z)
+    return q.value, F.value, Z.value
+
+
+def qsig(rho: float, sigma: float) -> float:
+    return lib().or_qsig(rho, sigma)
+
+
+def dnrm2(x) -> float:
+    return lib().or_dnrm2(np.ascontiguousarray(x, dtype=np.float64))
+
+
+def interact(nu, s_src, s_trg, r_src, r_trg, a_src, a_trg, v_src, v_trg):
+    """One symmetric pair; returns (dr_src, dr_trg, da_src, da_trg)."""
+    out = [np.zeros(3) for _ in range(4)]
+    f = lambda x: np.ascontiguousarray(x, dtype=np.float64)
+    lib().or_interact(nu, s_src, s_trg, f(r_src), f(r_trg), f(a_src), f(a_trg), v_src, v_trg, *out)
+    return tuple(out)
+
+
+def _check_coupled(s):
+    if not s.coupled and s.nw > 1:
+        raise ValueError("the oracle solves one coupled system; split independent wakes first")
+
+
+def solve(s, nu=None):
+    """Serial reference loops (interaction.cpp:196-302)."""
+    _check_coupled(s)
+    nu = s.nu if nu is None else nu
+    vel, ret = np.zeros((s.n, 3)), np.zeros((s.n, 3))
+    lib().or_solve(*_arrs(s), nu, vel, ret)
+    return vel, ret
+
+
+def solve_omp(s, nu=None):
+    _check_coupled(s)
+    nu = s.nu if nu is None else nu
+    vel, ret = np.zeros((s.n, 3)), np.zeros((s.n, 3))
+    lib().or_solve_omp(*_arrs(s), nu, vel, ret)
+    return vel, ret
+
+
+def solve_q128(s, nu=None):
+    _check_coupled(s)
+    nu = s.nu if nu is None else nu
+    vel, ret = np.zeros((s.n, 3)), np.zeros((s.n, 3))
+    lib().or_solve_q128(*_arrs(s), nu, vel, ret)
+    return vel, ret
+
+
+def rates_targets(s, targets=None, nu=None, use_omp=True):
+    """Rates of the given targets, bitwise equal to :func:`solve` rows."""
+    _check_coupled(s)
+    nu = s.nu if nu is None else nu
+    t = np.arange(s.n, dtype=np.int64) if targets is None else np.ascontiguousarray(targets, dtype=np.int64)
+    vel, ret = np.zeros((len(t), 3)), np.zeros((len(t), 3))
+    lib().or_rates_targets(*_arrs(s), nu, t, len(t), vel, ret, int(use_omp))
+    return vel, ret
+
+
+def rates_targets_q128(s, targets, nu=None):
+    """__float128 rates of the given targets (all sources, no ordering)."""
+    _check_coupled(s)
+    nu = s.nu if nu is None else nu
+    t = np.ascontiguousarray(targets, dtype=np.int64)
+    vel, ret = np.zeros((len(t), 3)), np.zeros((len(t), 3))
+    lib().or_rates_targets_q128(*_arrs(s), nu, t, len(t), vel, ret)
+    return vel, ret
+
+
+def solve_omp_rows(s, rows, nu=None):
+    """Run the parallel.cpp source-row loop body for the listed rows only."""
+    if s.nw != 1:
+        raise ValueError("single wake only")
+    nu = s.nu if nu is None else nu
+    vel, ret = np.zeros((s.n, 3)), np.zeros((s.n, 3))
+    r = np.ascontiguousarray(rows, dtype=np.int64)
+    _, _, pos, alpha, rad, vol = _arrs(s)
+    lib().or_solve_omp_rows(s.n, pos, alpha, rad, vol, nu, r, len(r), vel, ret)
+    return vel, ret
+
+
+SCHEMES = {"euler": 1, "rk3": 3, "rk4": 4}
+
+
+def step(s, scheme: str, dt: float, nsteps: int = 1, nu=None, use_omp=True):
+    """Returns a copy of ``s`` advanced by ``nsteps`` steps."""
+    _check_coupled(s)
+    nu = s.nu if nu is None else nu
+    out = s.copy()
+    rc = lib().or_step(SCHEMES[scheme], dt, nsteps, *_arrs(out)[:2], out.pos, out.alpha,
+                       out.radius, out.volume, nu, int(use_omp))
+    if rc != 0:
+        raise ValueError(f"bad scheme {scheme}")
+    return out
+
+
+def step_generic(scheme: str, dt: float, nsteps: int, y: np.ndarray, f) -> np.ndarray:
+    """The steppers over a flat state with a Python right-hand side f(y)->k."""
+    y = np.ascontiguousarray(y, dtype=np.float64).copy()
+    L = len(y)
+
+    def cb(n, yp, kp, _ctx):
+        yy = np.ctypeslib.as_array(yp, shape=(n,))
+        kk = np.ctypeslib.as_array(kp, shape=(n,))
+        kk[:] = f(yy.copy())
+
+    fn = RHS_FN(cb)
+    rc = lib().or_step_generic(SCHEMES[scheme], dt, nsteps, L, y, fn, None)
+    if rc != 0:
+        raise ValueError(f"bad scheme {scheme}")
+    return y
+
+
+DIAG_NAMES = ["Ox", "Oy", "Oz", "Ix", "Iy", "Iz", "Ax", "Ay", "Az", "H", "E", "EF", "KE", "KEF", "Zc"]
+
+
+def diagnose(s, zc_prev: float = 0.0, use_omp=True) -> np.ndarray:
+    _check_coupled(s)
+    out = np.zeros(15)
+    nw, off, pos, alpha, rad, _ = _arrs(s)
+    lib().or_diagnose(nw, off, pos, alpha, rad, zc_prev, out, int(use_omp))
+    return out

@@ -1,0 +1,648 @@
+/*
+ * pawan_oracle.c -- FP64 CPU restatement of PAWAN's interaction and
+ * integration hot path.  TEST INFRASTRUCTURE ONLY (see pawan_oracle.h).
+ *
+ * Reference paths are relative to the reference repository root.  The
+ * arithmetic follows the reference operation by operation (same operand
+ * order, no fused multiply-add: built with -ffp-contract=off) so that the
+ * per-pair results match what the reference computes; only the GSL
+ * containers are replaced by plain arrays.
+ *
+ * Third-party arithmetic restated here: GSL's cblas dnrm2/ddot/dscal and
+ * gsl_vector_add/sub (GSL version unpinned by the reference,
+ * CMakeLists.txt:21).  dnrm2 is the scaled sum-of-squares algorithm of the
+ * netlib reference BLAS DNRM2, which GSL's cblas implements; ddot is the
+ * plain left-to-right sum.  glibc libm supplies erf/exp/pow/sqrt, as in the
+ * reference build.
+ */
+#include "pawan_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+#ifndef M_1_PI
+#define M_1_PI 0.31830988618379067154
+#endif
+#ifndef M_SQRT1_2
+#define M_SQRT1_2 0.70710678118654752440
+#endif
+
+/* ---------------------------------------------------------------------- */
+/* GSL BLAS-1 pieces used on the hot path                                  */
+/* ---------------------------------------------------------------------- */
+
+/* gsl_blas_dnrm2 on a 3-vector: netlib DNRM2 scaled sum of squares.
+ * Used at src/interaction/interaction_utils.h:412,474,545 and
+ * src/interaction/interaction.cpp:144.
+ * Pinned: tests compare against math.hypot / sqrt(sum x^2) to 1 ulp and
+ * against exact Pythagorean triples. */
+double or_dnrm2(const double *x) {
+  double scale = 0.0, ssq = 1.0;
+  for (int i = 0; i < 3; ++i) {
+    if (x[i] != 0.0) {
+      const double ax = fabs(x[i]);
+      if (scale < ax) {
+        ssq = 1.0 + ssq * (scale / ax) * (scale / ax);
+        scale = ax;
+      } else {
+        ssq += (ax / scale) * (ax / scale);
+      }
+    }
+  }
+  return scale * sqrt(ssq);
+}
+
+/* gsl_blas_ddot: r = 0; r += x[i]*y[i], i = 0,1,2. */
+static inline double ddot3(const double *x, const double *y) {
+  double r = 0.0;
+  r += x[0] * y[0];
+  r += x[1] * y[1];
+  r += x[2] * y[2];
+  return r;
+}
+
+/* gsl_cross, src/utils/gsl_utils.h:142-153. */
+static inline void cross3(const double *a, const double *b, double *c) {
+  const double ax = a[0], ay = a[1], az = a[2];
+  const double bx = b[0], by = b[1], bz = b[2];
+  c[0] = ay * bz - az * by;
+  c[1] = az * bx - ax * bz;
+  c[2] = ax * by - ay * bx;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Gaussian smoothing kernel, src/interaction/gaussian.h                   */
+/* ---------------------------------------------------------------------- */
+
+/* ZETASIG, gaussian.h:20-23. */
+static inline double zetasig(double rho, double sigma) {
+  double rho_bar = rho / sigma;
+  return exp(-0.5 * rho_bar * rho_bar) / pow(sigma, 3.0) /
+         pow(2.0 * M_PI, 1.5);
+}
+
+/* KERNEL, gaussian.h:47-60.
+ * Pinned: q against the quadrature (1/rho^3) * int_0^rho Z(r) r^2 dr,
+ * F against the finite difference q'(rho)/rho, Z against its normalisation
+ * int Z 4 pi r^2 dr = 1, all three against the __float128 twin where no
+ * cancellation occurs, and the small/large rho_bar series limits. */
+void or_kernel(double rho, double sigma, double *q, double *F, double *Z,
+               double *n) {
+  double rho_bar = rho / sigma;
+  double sig3 = sigma * sigma * sigma;
+  *Z = zetasig(rho, sigma);
+  double phi = 0.25 * M_1_PI * erf(M_SQRT1_2 * rho_bar) / sig3;
+  *q = (phi / rho_bar - *Z) / (rho_bar * rho_bar);
+  *n = zetasig(rho, sigma);
+  *F = (*Z - 3.0 * *q) / (rho * rho);
+}
+
+/* QSIG, gaussian.h:30-37 (used by HELICITY). */
+double or_qsig(double rho, double sigma) {
+  double rho_bar = rho / sigma;
+  double sig3 = sigma * sigma * sigma;
+  double Z = zetasig(rho, sigma);
+  double phi = 0.25 * M_1_PI * erf(M_SQRT1_2 * rho_bar) / sig3;
+  return (phi / rho_bar - Z) / (rho_bar * rho_bar);
+}
+
+/* ---------------------------------------------------------------------- */
+/* Pair interaction                                                        */
+/* ---------------------------------------------------------------------- */
+
+/* INTERACT, src/interaction/interaction_utils.h:456-501, with VELOCITY
+ * (:55-62), VORSTRETCH (:73-105) and DIFFUSION (:344-378) inlined in the
+ * same operation order.
+ * Pinned: antisymmetry of the vorticity rate, velocity = regularised
+ * Biot-Savart (far-field limit), stretching = transposed contraction of the
+ * finite-difference velocity gradient, PSE against nu * Laplacian on a
+ * lattice, rotation/translation equivariance (tests/test_oracle_pair.py). */
+void or_interact(double nu, double s_src, double s_trg, const double *r_src,
+                 const double *r_trg, const double *a_src,
+                 const double *a_trg, double v_src, double v_trg,
+                 double *dr_src, double *dr_trg, double *da_src,
+                 double *da_trg) {
+  double d[3] = {r_trg[0] - r_src[0], r_trg[1] - r_src[1],
+                 r_trg[2] - r_src[2]};
+  double rho = or_dnrm2(d);
+  double q, F, Z, n;
+  double sigma = sqrt(0.5 * (s_src * s_src + s_trg * s_trg));
+  or_kernel(rho, sigma, &q, &F, &Z, &n);
+
+  double c[3];
+  /* VELOCITY(q, a_src, d): target */
+  cross3(a_src, d, c);
+  for (int i = 0; i < 3; ++i) dr_trg[i] += q * c[i];
+  /* VELOCITY(-q, a_trg, d): source */
+  cross3(a_trg, d, c);
+  for (int i = 0; i < 3; ++i) dr_src[i] += (-q) * c[i];
+
+  /* VORSTRETCH */
+  double txs[3];
+  cross3(a_trg, a_src, txs);
+  double roaxa = ddot3(d, txs);
+  double da[3];
+  for (int i = 0; i < 3; ++i) da[i] = q * txs[i] + (F * roaxa) * d[i];
+  /* DIFFUSION */
+  double sig12 = 0.5 * sigma * sigma;
+  double coef = n * nu / sig12;
+  for (int i = 0; i < 3; ++i) {
+    double dva = v_trg * a_src[i] - v_src * a_trg[i];
+    da[i] += coef * dva;
+  }
+  for (int i = 0; i < 3; ++i) {
+    da_trg[i] += da[i];
+    da_src[i] -= da[i];
+  }
+}
+
+/* Target-side half of INTERACT: the contribution of source j to target t.
+ * Written with t in the reference's "target" role.  Because every
+ * operation in INTERACT is sign-symmetric under swapping the roles
+ * (d -> -d, a_trg x a_src -> -(a_src x a_trg)), this equals, bit for bit,
+ * the contribution the reference gives particle t when t is in the
+ * "source" role of the pair. */
+static inline void pair_target(double nu, const double *rt, const double *at,
+                               double st, double vt, const double *rs,
+                               const double *as, double ss, double vs,
+                               double *u, double *w) {
+  double d[3] = {rt[0] - rs[0], rt[1] - rs[1], rt[2] - rs[2]};
+  double rho = or_dnrm2(d);
+  double q, F, Z, n;
+  double sigma = sqrt(0.5 * (ss * ss + st * st));
+  or_kernel(rho, sigma, &q, &F, &Z, &n);
+  double c[3];
+  cross3(as, d, c);
+  for (int i = 0; i < 3; ++i) u[i] += q * c[i];
+  double txs[3];
+  cross3(at, as, txs);
+  double roaxa = ddot3(d, txs);
+  double sig12 = 0.5 * sigma * sigma;
+  double coef = n * nu / sig12;
+  for (int i = 0; i < 3; ++i) {
+    double da = q * txs[i] + (F * roaxa) * d[i];
+    da += coef * (vt * as[i] - vs * at[i]);
+    w[i] += da;
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* Right-hand side                                                         */
+/* ---------------------------------------------------------------------- */
+
+/* __interaction::interact(), interaction.cpp:196-210, with the serial
+ * interact(W) (:262-281) and interact(W1,W2) (:283-302). */
+void or_solve(int nw, const int64_t *off, const double *pos,
+              const double *alpha, const double *radius,
+              const double *volume, double nu, double *vel, double *retvor) {
+  const int64_t N = off[nw];
+  memset(vel, 0, sizeof(double) * 3 * N);
+  memset(retvor, 0, sizeof(double) * 3 * N);
+  for (int w = 0; w < nw; ++w) {
+    for (int64_t s = off[w]; s < off[w + 1]; ++s)
+      for (int64_t t = s + 1; t < off[w + 1]; ++t)
+        or_interact(nu, radius[s], radius[t], pos + 3 * s, pos + 3 * t,
+                    alpha + 3 * s, alpha + 3 * t, volume[s], volume[t],
+                    vel + 3 * s, vel + 3 * t, retvor + 3 * s, retvor + 3 * t);
+  }
+  for (int wi = 0; wi < nw; ++wi)
+    for (int wj = wi + 1; wj < nw; ++wj)
+      for (int64_t s = off[wi]; s < off[wi + 1]; ++s)
+        for (int64_t t = off[wj]; t < off[wj + 1]; ++t)
+          or_interact(nu, radius[s], radius[t], pos + 3 * s, pos + 3 * t,
+                      alpha + 3 * s, alpha + 3 * t, volume[s], volume[t],
+                      vel + 3 * s, vel + 3 * t, retvor + 3 * s,
+                      retvor + 3 * t);
+}
+
+/* The variant of INTERACT that returns the source terms in scalars,
+ * interaction_utils.h:523-576 (identical arithmetic to or_interact). */
+static inline void interact3(double nu, double s_src, double s_trg,
+                             const double *r_src, const double *r_trg,
+                             const double *a_src, const double *a_trg,
+                             double v_src, double v_trg, double *dr_trg,
+                             double *da_trg, double *vs, double *qs) {
+  double z_u[3] = {0, 0, 0}, z_w[3] = {0, 0, 0};
+  or_interact(nu, s_src, s_trg, r_src, r_trg, a_src, a_trg, v_src, v_trg, z_u,
+              dr_trg, z_w, da_trg);
+  for (int i = 0; i < 3; ++i) {
+    vs[i] = z_u[i];
+    qs[i] = z_w[i];
+  }
+}
+
+/* One source row of __parallel::interact(W), parallel.cpp:15-48. */
+static void omp_row(int64_t s, int64_t t0, int64_t t1, const double *pos,
+                    const double *alpha, const double *radius,
+                    const double *volume, double nu, double *vel,
+                    double *retvor) {
+  double vx = 0.0, vy = 0.0, vz = 0.0, qx = 0.0, qy = 0.0, qz = 0.0;
+#pragma omp parallel for reduction(+ : vx, vy, vz, qx, qy, qz)
+  for (int64_t t = t0; t < t1; ++t) {
+    double vsrc[3], qsrc[3];
+    interact3(nu, radius[s], radius[t], pos + 3 * s, pos + 3 * t,
+              alpha + 3 * s, alpha + 3 * t, volume[s], volume[t],
+              vel + 3 * t, retvor + 3 * t, vsrc, qsrc);
+    vx += vsrc[0];
+    vy += vsrc[1];
+    vz += vsrc[2];
+    qx += qsrc[0];
+    qy += qsrc[1];
+    qz += qsrc[2];
+  }
+  vel[3 * s + 0] = vx + vel[3 * s + 0];
+  vel[3 * s + 1] = vy + vel[3 * s + 1];
+  vel[3 * s + 2] = vz + vel[3 * s + 2];
+  retvor[3 * s + 0] = qx + retvor[3 * s + 0];
+  retvor[3 * s + 1] = qy + retvor[3 * s + 1];
+  retvor[3 * s + 2] = qz + retvor[3 * s + 2];
+}
+
+/* __parallel::interact(W) / interact(W1,W2), parallel.cpp:14-86, driven by
+ * __interaction::interact(), interaction.cpp:196-210. */
+void or_solve_omp(int nw, const int64_t *off, const double *pos,
+                  const double *alpha, const double *radius,
+                  const double *volume, double nu, double *vel,
+                  double *retvor) {
+  const int64_t N = off[nw];
+  memset(vel, 0, sizeof(double) * 3 * N);
+  memset(retvor, 0, sizeof(double) * 3 * N);
+  for (int w = 0; w < nw; ++w)
+    for (int64_t s = off[w]; s < off[w + 1]; ++s)
+      omp_row(s, s + 1, off[w + 1], pos, alpha, radius, volume, nu, vel,
+              retvor);
+  for (int wi = 0; wi < nw; ++wi)
+    for (int wj = wi + 1; wj < nw; ++wj)
+      for (int64_t s = off[wi]; s < off[wi + 1]; ++s)
+        omp_row(s, off[wj], off[wj + 1], pos, alpha, radius, volume, nu, vel,
+                retvor);
+}
+
+void or_solve_omp_rows(int64_t n, const double *pos, const double *alpha,
+                       const double *radius, const double *volume, double nu,
+                       const int64_t *rows, int64_t nrows, double *vel,
+                       double *retvor) {
+  for (int64_t r = 0; r < nrows; ++r)
+    omp_row(rows[r], rows[r] + 1, n, pos, alpha, radius, volume, nu, vel,
+            retvor);
+}
+
+static void target_rates(int nw, const int64_t *off, const double *pos,
+                         const double *alpha, const double *radius,
+                         const double *volume, double nu, int64_t t,
+                         double *u, double *w) {
+  int wt = 0;
+  while (!(t >= off[wt] && t < off[wt + 1])) ++wt;
+  u[0] = u[1] = u[2] = 0.0;
+  w[0] = w[1] = w[2] = 0.0;
+  const double *rt = pos + 3 * t, *at = alpha + 3 * t;
+  const double st = radius[t], vt = volume[t];
+  /* own wake first (interact(W)), then the cross-wake pairs in the order
+   * the i<j loop of interaction.cpp:205-209 reaches them */
+  for (int64_t s = off[wt]; s < off[wt + 1]; ++s) {
+    if (s == t) continue;
+    pair_target(nu, rt, at, st, vt, pos + 3 * s, alpha + 3 * s, radius[s],
+                volume[s], u, w);
+  }
+  for (int ws = 0; ws < nw; ++ws) {
+    if (ws == wt) continue;
+    for (int64_t s = off[ws]; s < off[ws + 1]; ++s)
+      pair_target(nu, rt, at, st, vt, pos + 3 * s, alpha + 3 * s, radius[s],
+                  volume[s], u, w);
+  }
+}
+
+void or_rates_targets(int nw, const int64_t *off, const double *pos,
+                      const double *alpha, const double *radius,
+                      const double *volume, double nu, const int64_t *targets,
+                      int64_t ntargets, double *vel, double *retvor,
+                      int use_omp) {
+  if (use_omp) {
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t k = 0; k < ntargets; ++k)
+      target_rates(nw, off, pos, alpha, radius, volume, nu, targets[k],
+                   vel + 3 * k, retvor + 3 * k);
+  } else {
+    for (int64_t k = 0; k < ntargets; ++k)
+      target_rates(nw, off, pos, alpha, radius, volume, nu, targets[k],
+                   vel + 3 * k, retvor + 3 * k);
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* Time integration                                                        */
+/* ---------------------------------------------------------------------- */
+
+/* Rates of the whole particle system: k = [vel ; retvor] for the state
+ * y = [pos ; alpha] (the per-wake state layout of wake.cpp:856-893 with the
+ * wakes concatenated and no padding rows). */
+struct particle_ctx {
+  int nw;
+  const int64_t *off;
+  const double *radius, *volume;
+  double nu;
+  int use_omp;
+  int64_t *idx;
+  double *u, *w;
+};
+
+static void particle_rhs(int64_t len, const double *y, double *k, void *vctx) {
+  struct particle_ctx *c = (struct particle_ctx *)vctx;
+  const int64_t N = c->off[c->nw];
+  or_rates_targets(c->nw, c->off, y, y + 3 * N, c->radius, c->volume, c->nu,
+                   c->idx, N, c->u, c->w, c->use_omp);
+  memcpy(k, c->u, sizeof(double) * 3 * N);
+  memcpy(k + 3 * N, c->w, sizeof(double) * 3 * N);
+}
+
+/* out = k*c + y  (gsl_vector_memcpy(out,k); gsl_vector_scale(out,c);
+ * gsl_vector_add(out,y)) */
+void or_axpy_stage(int64_t len, double c, const double *k, const double *y,
+                   double *out) {
+  for (int64_t i = 0; i < len; ++i) out[i] = k[i] * c + y[i];
+}
+
+int or_step_generic(int scheme, double dt, int64_t nsteps, int64_t L,
+                    double *y, or_rhs_fn f, void *ctx) {
+  if (scheme != 1 && scheme != 3 && scheme != 4) return -1;
+  size_t bytes = sizeof(double) * (L > 0 ? L : 1);
+  double *x1 = (double *)malloc(bytes);
+  double *k1 = (double *)malloc(bytes);
+  double *k2 = (double *)malloc(bytes);
+  double *k3 = (double *)malloc(bytes);
+  double *k4 = (double *)malloc(bytes);
+  for (int64_t it = 0; it < nsteps; ++it) {
+    if (scheme == 1) {
+      /* __integration::step, integration.cpp:126-136 */
+      f(L, y, k1, ctx);
+      for (int64_t i = 0; i < L; ++i) k1[i] = k1[i] * dt;
+      for (int64_t i = 0; i < L; ++i) y[i] = y[i] + k1[i];
+    } else if (scheme == 4) {
+      /* __rk4::step, rk4.cpp:12-77 */
+      f(L, y, k1, ctx);
+      or_axpy_stage(L, 0.5 * dt, k1, y, x1);
+      f(L, x1, k2, ctx);
+      or_axpy_stage(L, 0.5 * dt, k2, y, x1);
+      f(L, x1, k3, ctx);
+      or_axpy_stage(L, dt, k3, y, x1);
+      f(L, x1, k4, ctx);
+      const double c6 = dt / 6., c3 = dt / 3.;
+      for (int64_t i = 0; i < L; ++i) {
+        double a = (k1[i] + k4[i]) * c6;
+        double b = (k2[i] + k3[i]) * c3;
+        y[i] = y[i] + (a + b);
+      }
+    } else {
+      /* __rk3::step, rk3.cpp:12-65 */
+      f(L, y, k1, ctx);
+      or_axpy_stage(L, 0.5 * dt, k1, y, x1);
+      f(L, x1, k2, ctx);
+      for (int64_t i = 0; i < L; ++i) {
+        double x2 = k2[i] * (2. * dt) + y[i];
+        k1[i] = k1[i] * dt;
+        x1[i] = x2 - k1[i];
+      }
+      f(L, x1, k3, ctx);
+      const double c6 = dt / 6.0, s6 = 1.0 / 6.0, c23 = dt * 2.0 / 3.0;
+      for (int64_t i = 0; i < L; ++i) {
+        double a = k1[i] * s6 + k3[i] * c6;
+        double b = k2[i] * c23;
+        y[i] = y[i] + (a + b);
+      }
+    }
+  }
+  free(x1);
+  free(k1);
+  free(k2);
+  free(k3);
+  free(k4);
+  return 0;
+}
+
+int or_step(int scheme, double dt, int64_t nsteps, int nw, const int64_t *off,
+            double *pos, double *alpha, const double *radius,
+            const double *volume, double nu, int use_omp) {
+  const int64_t N = off[nw];
+  const int64_t L = 6 * N;
+  struct particle_ctx c;
+  c.nw = nw;
+  c.off = off;
+  c.radius = radius;
+  c.volume = volume;
+  c.nu = nu;
+  c.use_omp = use_omp;
+  c.idx = (int64_t *)malloc(sizeof(int64_t) * (N > 0 ? N : 1));
+  c.u = (double *)malloc(sizeof(double) * 3 * (N > 0 ? N : 1));
+  c.w = (double *)malloc(sizeof(double) * 3 * (N > 0 ? N : 1));
+  for (int64_t i = 0; i < N; ++i) c.idx[i] = i;
+  double *y = (double *)malloc(sizeof(double) * (L > 0 ? L : 1));
+  memcpy(y, pos, sizeof(double) * 3 * N);
+  memcpy(y + 3 * N, alpha, sizeof(double) * 3 * N);
+  int rc = or_step_generic(scheme, dt, nsteps, L, y, particle_rhs, &c);
+  memcpy(pos, y, sizeof(double) * 3 * N);
+  memcpy(alpha, y + 3 * N, sizeof(double) * 3 * N);
+  free(y);
+  free(c.idx);
+  free(c.u);
+  free(c.w);
+  return rc;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Diagnostics, interaction.cpp:64-194 and interaction_utils.h:107-331     */
+/* ---------------------------------------------------------------------- */
+
+/* HELICITY, interaction_utils.h:214-242 (x1,a1 = particle I; x2,a2 = J). */
+static inline double helicity_pair(double s1, double s2, const double *x1,
+                                   const double *x2, const double *a1,
+                                   const double *a2) {
+  double x12[3] = {x2[0] - x1[0], x2[1] - x1[1], x2[2] - x1[2]};
+  double rho = or_dnrm2(x12);
+  double sigma = sqrt(0.5 * (s1 * s1 + s2 * s2));
+  double q = or_qsig(rho, sigma);
+  double c[3];
+  cross3(a1, a2, c);
+  return q * ddot3(x12, c);
+}
+
+/* KINETICENERGY pair, interaction_utils.h:278-306. */
+static inline double ke_pair(double s1, double s2, const double *x1,
+                             const double *x2, const double *a1,
+                             const double *a2) {
+  double x12[3] = {x2[0] - x1[0], x2[1] - x1[1], x2[2] - x1[2]};
+  double rho2 = ddot3(x12, x12);
+  double sigma2 = 0.5 * (s1 * s1 + s2 * s2);
+  double a1a2 = ddot3(a1, a2);
+  double x12a1 = ddot3(x12, a1);
+  double x12a2 = ddot3(x12, a2);
+  return ((rho2 + 2.0 * sigma2) * a1a2 + (x12a1 * x12a2)) /
+         pow(rho2 + sigma2, 1.5) / 16.0 / M_PI;
+}
+
+/* KINETICENERGYF pair, interaction_utils.h:308-331. */
+static inline double kef_pair(double s1, double s2, const double *x1,
+                              const double *x2, const double *a1,
+                              const double *a2) {
+  double x12[3] = {x2[0] - x1[0], x2[1] - x1[1], x2[2] - x1[2]};
+  double rho2 = ddot3(x12, x12);
+  double sigma2 = 0.5 * (s1 * s1 + s2 * s2);
+  double a1a2 = ddot3(a1, a2);
+  return (rho2 + 1.5 * sigma2) * a1a2 / pow(rho2 + sigma2, 1.5) / 8.0 / M_PI;
+}
+
+/* KINETICENERGY / KINETICENERGYF self terms, interaction_utils.h:249-267. */
+static inline double ke_self(double s, const double *a) {
+  return M_1_PI * ddot3(a, a) / s / 8.0;
+}
+static inline double kef_self(double s, const double *a) {
+  return 3 * M_1_PI * ddot3(a, a) / (s * 16.0);
+}
+
+enum { Q_H = 0, Q_KE = 1, Q_KEF = 2 };
+
+/* Sum over J of one row I of a quadratic diagnostic (the J loop of
+ * interaction.cpp:608-718 / parallel.cpp:196-317). */
+static double quad_row(int which, int64_t I, int64_t j0, int64_t j1,
+                       const double *pos, const double *alpha,
+                       const double *radius) {
+  double acc = 0.0;
+  const double *rI = pos + 3 * I, *aI = alpha + 3 * I;
+  for (int64_t J = j0; J < j1; ++J) {
+    const double *rJ = pos + 3 * J, *aJ = alpha + 3 * J;
+    double v;
+    if (which == Q_H)
+      v = helicity_pair(radius[I], radius[J], rI, rJ, aI, aJ);
+    else if (which == Q_KE)
+      v = ke_pair(radius[I], radius[J], rI, rJ, aI, aJ);
+    else
+      v = kef_pair(radius[I], radius[J], rI, rJ, aI, aJ);
+    acc += 2.0 * v;
+  }
+  return acc;
+}
+
+/* calculateHelicity / calculateKineticEnergy(F) for one wake (j1 == -1) or
+ * a wake pair.  Serial form (use_omp == 0) accumulates every term straight
+ * into the total, as interaction.cpp:608-718; the OpenMP form sums each row
+ * first and then adds the row, as parallel.cpp:196-317. */
+static double quad_block(int which, int64_t i0, int64_t i1, int64_t j0,
+                         int64_t j1, int self, const double *pos,
+                         const double *alpha, const double *radius,
+                         int use_omp) {
+  double tot = 0.0;
+  if (!use_omp) {
+    for (int64_t I = i0; I < i1; ++I) {
+      const double *rI = pos + 3 * I, *aI = alpha + 3 * I;
+      if (self) {
+        if (which == Q_KE) tot += ke_self(radius[I], aI);
+        if (which == Q_KEF) tot += kef_self(radius[I], aI);
+      }
+      int64_t js = self ? I + 1 : j0, je = self ? i1 : j1;
+      for (int64_t J = js; J < je; ++J) {
+        const double *rJ = pos + 3 * J, *aJ = alpha + 3 * J;
+        if (which == Q_H)
+          tot += 2.0 * helicity_pair(radius[I], radius[J], rI, rJ, aI, aJ);
+        else if (which == Q_KE)
+          tot += 2.0 * ke_pair(radius[I], radius[J], rI, rJ, aI, aJ);
+        else
+          tot += 2.0 * kef_pair(radius[I], radius[J], rI, rJ, aI, aJ);
+      }
+    }
+    return tot;
+  }
+  int64_t n = i1 - i0;
+  double *row = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
+#pragma omp parallel for schedule(dynamic, 8)
+  for (int64_t k = 0; k < n; ++k) {
+    int64_t I = i0 + k;
+    int64_t js = self ? I + 1 : j0, je = self ? i1 : j1;
+    row[k] = quad_row(which, I, js, je, pos, alpha, radius);
+  }
+  for (int64_t k = 0; k < n; ++k) {
+    int64_t I = i0 + k;
+    if (self) {
+      if (which == Q_KE) tot += ke_self(radius[I], alpha + 3 * I);
+      if (which == Q_KEF) tot += kef_self(radius[I], alpha + 3 * I);
+    }
+    tot += row[k];
+  }
+  free(row);
+  return tot;
+}
+
+static double quad_total(int which, int nw, const int64_t *off,
+                         const double *pos, const double *alpha,
+                         const double *radius, int use_omp) {
+  /* interaction.cpp:93-131: per wake, then each pair i<j */
+  double tot = 0.0;
+  for (int i = 0; i < nw; ++i) {
+    tot += quad_block(which, off[i], off[i + 1], 0, 0, 1, pos, alpha, radius,
+                      use_omp);
+    for (int j = i + 1; j < nw; ++j)
+      tot += quad_block(which, off[i], off[i + 1], off[j], off[j + 1], 0, pos,
+                        alpha, radius, use_omp);
+  }
+  return tot;
+}
+
+void or_diagnose(int nw, const int64_t *off, const double *pos,
+                 const double *alpha, const double *radius, double zc_prev,
+                 double *out, int use_omp) {
+  double O[3] = {0, 0, 0}, I[3] = {0, 0, 0}, A[3] = {0, 0, 0};
+  for (int w = 0; w < nw; ++w) {
+    double o[3] = {0, 0, 0}, li[3] = {0, 0, 0}, ai[3] = {0, 0, 0};
+    for (int64_t i = off[w]; i < off[w + 1]; ++i) {
+      const double *r = pos + 3 * i, *a = alpha + 3 * i;
+      double rxa[3], rxrxa[3];
+      /* calculateTotalVorticity, interaction.cpp:489-496 */
+      for (int k = 0; k < 3; ++k) o[k] += a[k];
+      /* calculateLinearImpulse, :498-510 */
+      cross3(r, a, rxa);
+      for (int k = 0; k < 3; ++k) li[k] += rxa[k];
+      /* calculateAngularImpulse, :512-527 */
+      cross3(r, rxa, rxrxa);
+      for (int k = 0; k < 3; ++k) ai[k] += rxrxa[k];
+    }
+    for (int k = 0; k < 3; ++k) {
+      li[k] = li[k] * 0.5;
+      ai[k] = ai[k] * (1.0 / 3.0);
+      O[k] += o[k];
+      I[k] += li[k];
+      A[k] += ai[k];
+    }
+  }
+  double H = quad_total(Q_H, nw, off, pos, alpha, radius, use_omp);
+  /* Gaussian ENST/ENSTF are identically zero (gaussian.h:69-92), so the
+   * enstrophy sums (interaction.cpp:101-116) are zero. */
+  double E = 0.0, EF = 0.0;
+  double KE = quad_total(Q_KE, nw, off, pos, alpha, radius, use_omp);
+  double KEF = quad_total(Q_KEF, nw, off, pos, alpha, radius, use_omp);
+  /* Centroid, interaction.cpp:135-157: the last wake with a non-zero
+   * denominator wins. */
+  double Zc = zc_prev;
+  for (int w = 0; w < nw; ++w) {
+    double num = 0.0, den = 0.0;
+    for (int64_t i = off[w]; i < off[w + 1]; ++i) {
+      double oy = or_dnrm2(alpha + 3 * i);
+      double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+      num += oy * z * (x * x + y * y);
+      den += oy * (x * x + y * y);
+    }
+    if (den != 0.0) Zc = num / den;
+  }
+  for (int k = 0; k < 3; ++k) {
+    out[k] = O[k];
+    out[3 + k] = I[k];
+    out[6 + k] = A[k];
+  }
+  out[9] = H;
+  out[10] = E;
+  out[11] = EF;
+  out[12] = KE;
+  out[13] = KEF;
+  out[14] = Zc;
+}

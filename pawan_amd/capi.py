@@ -1,0 +1,244 @@
+"""ctypes binding of the C-ABI in include/pawan_gpu.h (libpawan_gpu.so).
+
+Thin Python access to the GPU solver for tests and benchmarks.  There is no
+CPU fallback: if the library is missing or no HIP device is visible, calls
+raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import workloads as _wl
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libpawan_gpu.so")
+
+PW_OK, PW_ERR_ARG, PW_ERR_HIP, PW_ERR_STATE, PW_ERR_NONFINITE, PW_ERR_UNSUPPORTED, PW_ERR_NOMEM = range(7)
+PW_FP64, PW_FP32C, PW_LEARNED, PW_FP64_REF = 0, 1, 2, 3
+PW_EULER, PW_RK3, PW_RK4 = 1, 3, 4
+
+PRECISIONS = {"fp64": PW_FP64, "fp64_ref": PW_FP64_REF, "fp32c": PW_FP32C, "learned": PW_LEARNED}
+SCHEMES = {"euler": PW_EULER, "rk3": PW_RK3, "rk4": PW_RK4}
+
+# Every extern "C" symbol declared in include/pawan_gpu.h.
+EXPORTS = [
+    "pw_version", "pw_last_error", "pw_device_count", "pw_create", "pw_destroy",
+    "pw_add_wake", "pw_set_nu", "pw_get_nu", "pw_set_coupled", "pw_get_size",
+    "pw_set_wake_state", "pw_get_wake_state", "pw_get_wake_rates", "pw_set_states",
+    "pw_get_states", "pw_get_rates", "pw_solve", "pw_step", "pw_diagnose",
+    "pw_synchronize", "pw_set_profiling", "pw_get_pair_timing",
+]
+
+
+class PawanError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_LIB: C.CDLL | None = None
+
+
+def load(path: str | None = None) -> C.CDLL:
+    """Load libpawan_gpu.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise FileNotFoundError(
+            f"{path} is missing: build it with `python -m pawan_amd.build` "
+            "(or __graft_entry__.build()); there is no CPU fallback")
+    lib = C.CDLL(path)
+    vp, i64, d, i = C.c_void_p, C.c_int64, C.c_double, C.c_int
+    pvp = C.POINTER(C.c_void_p)
+    sig = {
+        "pw_version": ([], C.c_char_p),
+        "pw_last_error": ([], C.c_char_p),
+        "pw_device_count": ([C.POINTER(C.c_int)], i),
+        "pw_create": ([pvp, i], i),
+        "pw_destroy": ([vp], i),
+        "pw_add_wake": ([vp, i64, i64, _dp, _dp, _dp, _dp, C.POINTER(C.c_int)], i),
+        "pw_set_nu": ([vp, d], i),
+        "pw_get_nu": ([vp, C.POINTER(C.c_double)], i),
+        "pw_set_coupled": ([vp, i], i),
+        "pw_get_size": ([vp, C.POINTER(C.c_int64), C.POINTER(C.c_int)], i),
+        "pw_set_wake_state": ([vp, i, _dp, _dp], i),
+        "pw_get_wake_state": ([vp, i, _dp, _dp], i),
+        "pw_get_wake_rates": ([vp, i, _dp, _dp], i),
+        "pw_set_states": ([vp, _dp, i64], i),
+        "pw_get_states": ([vp, _dp, i64], i),
+        "pw_get_rates": ([vp, _dp, i64], i),
+        "pw_solve": ([vp, i], i),
+        "pw_step": ([vp, i, d, i, i64], i),
+        "pw_diagnose": ([vp, _dp, i64], i),
+        "pw_synchronize": ([vp], i),
+        "pw_set_profiling": ([vp, i], i),
+        "pw_get_pair_timing": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)], i),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _LIB = lib
+    return lib
+
+
+def _check(rc: int) -> None:
+    if rc != PW_OK:
+        raise PawanError(rc, load().pw_last_error().decode())
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    rc = load().pw_device_count(C.byref(n))
+    return n.value if rc == PW_OK else 0
+
+
+class Solver:
+    """One pw_ctx: a set of wakes on one GPU (the reference's __interaction)."""
+
+    def __init__(self, system: "_wl.ParticleSystem | None" = None, device: int = 0,
+                 nu: float | None = None, coupled: bool | None = None):
+        self._lib = load()
+        h = C.c_void_p()
+        _check(self._lib.pw_create(C.byref(h), device))
+        self._h = h
+        self.wake_sizes: list[int] = []
+        self.wake_max: list[int] = []
+        if system is not None:
+            if coupled is None:
+                coupled = system.coupled
+            self.set_coupled(coupled)
+            for w in range(system.nw):
+                a, b = int(system.off[w]), int(system.off[w + 1])
+                self.add_wake(system.pos[a:b], system.alpha[a:b], system.radius[a:b],
+                              system.volume[a:b])
+            self.set_nu(system.nu if nu is None else nu)
+        elif nu is not None:
+            self.set_nu(nu)
+
+    # -- lifetime ----------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.pw_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- configuration -------------------------------------------------------
+    def add_wake(self, pos, alpha, radius, volume, maxn: int | None = None) -> int:
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        n = len(pos)
+        alpha = np.ascontiguousarray(alpha, dtype=np.float64).reshape(n, 3)
+        radius = np.ascontiguousarray(radius, dtype=np.float64).reshape(n)
+        volume = np.ascontiguousarray(volume, dtype=np.float64).reshape(n)
+        maxn = n if maxn is None else int(maxn)
+        w = C.c_int(-1)
+        _check(self._lib.pw_add_wake(self._h, n, maxn, pos, alpha, radius, volume, C.byref(w)))
+        self.wake_sizes.append(n)
+        self.wake_max.append(maxn)
+        return w.value
+
+    def set_nu(self, nu: float) -> None:
+        _check(self._lib.pw_set_nu(self._h, float(nu)))
+
+    @property
+    def nu(self) -> float:
+        v = C.c_double()
+        _check(self._lib.pw_get_nu(self._h, C.byref(v)))
+        return v.value
+
+    def set_coupled(self, coupled: bool) -> None:
+        _check(self._lib.pw_set_coupled(self._h, int(bool(coupled))))
+
+    @property
+    def n(self) -> int:
+        return int(sum(self.wake_sizes))
+
+    @property
+    def state_len(self) -> int:
+        return 6 * int(sum(self.wake_max))
+
+    # -- state ---------------------------------------------------------------
+    def set_wake_state(self, w: int, pos, alpha) -> None:
+        n = self.wake_sizes[w]
+        _check(self._lib.pw_set_wake_state(
+            self._h, w, np.ascontiguousarray(pos, dtype=np.float64).reshape(n, 3),
+            np.ascontiguousarray(alpha, dtype=np.float64).reshape(n, 3)))
+
+    def wake_state(self, w: int):
+        n = self.wake_sizes[w]
+        pos, alpha = np.zeros((n, 3)), np.zeros((n, 3))
+        _check(self._lib.pw_get_wake_state(self._h, w, pos, alpha))
+        return pos, alpha
+
+    def wake_rates(self, w: int):
+        n = self.wake_sizes[w]
+        vel, ret = np.zeros((n, 3)), np.zeros((n, 3))
+        _check(self._lib.pw_get_wake_rates(self._h, w, vel, ret))
+        return vel, ret
+
+    def state(self):
+        """(pos, alpha) of all particles, wakes concatenated."""
+        parts = [self.wake_state(w) for w in range(len(self.wake_sizes))]
+        return np.concatenate([p for p, _ in parts]), np.concatenate([a for _, a in parts])
+
+    def rates(self):
+        """(vel, retvor) of all particles from the last right-hand side."""
+        parts = [self.wake_rates(w) for w in range(len(self.wake_sizes))]
+        return np.concatenate([p for p, _ in parts]), np.concatenate([a for _, a in parts])
+
+    def set_states(self, state: np.ndarray) -> None:
+        state = np.ascontiguousarray(state, dtype=np.float64)
+        _check(self._lib.pw_set_states(self._h, state, len(state)))
+
+    def get_states(self) -> np.ndarray:
+        out = np.zeros(self.state_len)
+        _check(self._lib.pw_get_states(self._h, out, len(out)))
+        return out
+
+    def get_rates(self) -> np.ndarray:
+        out = np.zeros(self.state_len)
+        _check(self._lib.pw_get_rates(self._h, out, len(out)))
+        return out
+
+    # -- compute -------------------------------------------------------------
+    def solve(self, precision: str = "fp64"):
+        _check(self._lib.pw_solve(self._h, PRECISIONS[precision]))
+        return self.rates()
+
+    def step(self, scheme: str, dt: float, precision: str = "fp64", nsteps: int = 1) -> None:
+        _check(self._lib.pw_step(self._h, SCHEMES[scheme], float(dt), PRECISIONS[precision],
+                                 int(nsteps)))
+
+    def diagnose(self) -> np.ndarray:
+        nsys = 1
+        out = np.zeros(15 * max(1, len(self.wake_sizes)))
+        _check(self._lib.pw_diagnose(self._h, out, len(out)))
+        return out
+
+    def synchronize(self) -> None:
+        _check(self._lib.pw_synchronize(self._h))
+
+    def set_profiling(self, on: bool) -> None:
+        _check(self._lib.pw_set_profiling(self._h, int(bool(on))))
+
+    def pair_timing(self):
+        ms, n = C.c_double(), C.c_int64()
+        _check(self._lib.pw_get_pair_timing(self._h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value

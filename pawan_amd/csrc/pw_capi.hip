@@ -1,0 +1,648 @@
+// pw_capi.hip -- solver context and the extern "C" entry points declared in
+// include/pawan_gpu.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/pawan_gpu.h"
+#include "pw_internal.h"
+
+using pw::Rate64;
+using pw::Rec64;
+using pw::WorkItem;
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg) {
+  g_error = msg;
+  return code;
+}
+
+#define PW_HIP(call)                                                        \
+  do {                                                                      \
+    hipError_t e_ = (call);                                                 \
+    if (e_ != hipSuccess)                                                   \
+      return fail(PW_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr int kTargetsPerItem = 256;
+
+template <typename T>
+hipError_t dalloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  return hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+}
+
+}  // namespace
+
+struct pw_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  double nu = 1.56e-5;  // interaction.cpp:15
+  bool coupled = true;
+
+  // host description of the wakes
+  std::vector<int64_t> wn, wmax, woff{0};
+  std::vector<double> radius, volume;   // per particle
+  std::vector<Rec64> pending;           // state until the device is built
+  bool built = false;
+  int64_t n = 0;
+
+  // device buffers
+  Rec64* y = nullptr;    // current state
+  Rec64* ys = nullptr;   // stage state
+  Rate64* k[4] = {nullptr, nullptr, nullptr, nullptr};
+  Rate64* rates = nullptr;  // rates of the last right-hand side
+  double* d_radius = nullptr;
+  WorkItem* pair_items = nullptr;
+  int npair = 0;
+  WorkItem* lin_items = nullptr;
+  int nlin = 0;
+  std::vector<WorkItem> h_pair_items, h_lin_items;
+  std::vector<int> pair_sys, lin_wake;  // system / wake of each item
+  double* d_lin = nullptr;
+  double* d_quad = nullptr;
+  int* d_flag = nullptr;
+
+  std::vector<double> zc;  // persistent centroid per system
+
+  // profiling
+  bool profiling = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_used, ev_free;
+  double prof_ms = 0.0;
+  int64_t prof_launches = 0;
+
+  int nsys() const { return coupled ? 1 : static_cast<int>(wn.size()); }
+};
+
+namespace {
+
+void release(pw_ctx* c) {
+  (void)hipFree(c->y);
+  (void)hipFree(c->ys);
+  for (auto& p : c->k) (void)hipFree(p);
+  (void)hipFree(c->d_radius);
+  (void)hipFree(c->pair_items);
+  (void)hipFree(c->lin_items);
+  (void)hipFree(c->d_lin);
+  (void)hipFree(c->d_quad);
+  (void)hipFree(c->d_flag);
+  for (auto& e : c->ev_used) {
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+  for (auto& e : c->ev_free) {
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+// Work items: blocks of up to 256 consecutive targets of one system, each
+// against all sources of that system.  For independent wakes the largest
+// systems come first (longest-processing-time order), so the tail of the
+// grid is made of short blocks.
+void build_items(pw_ctx* c) {
+  c->h_pair_items.clear();
+  c->pair_sys.clear();
+  c->h_lin_items.clear();
+  c->lin_wake.clear();
+  const int nw = static_cast<int>(c->wn.size());
+  struct Sys {
+    int64_t a, b;
+    int id;
+  };
+  std::vector<Sys> sys;
+  if (c->coupled)
+    sys.push_back({0, c->n, 0});
+  else
+    for (int w = 0; w < nw; ++w) sys.push_back({c->woff[w], c->woff[w + 1], w});
+  std::stable_sort(sys.begin(), sys.end(),
+                   [](const Sys& p, const Sys& q) { return (p.b - p.a) > (q.b - q.a); });
+  for (const Sys& s : sys)
+    for (int64_t t0 = s.a; t0 < s.b; t0 += kTargetsPerItem) {
+      c->h_pair_items.push_back({static_cast<int32_t>(t0),
+                                 static_cast<int32_t>(std::min<int64_t>(t0 + kTargetsPerItem, s.b)),
+                                 static_cast<int32_t>(s.a), static_cast<int32_t>(s.b)});
+      c->pair_sys.push_back(s.id);
+    }
+  for (int w = 0; w < nw; ++w)
+    for (int64_t t0 = c->woff[w]; t0 < c->woff[w + 1]; t0 += kTargetsPerItem) {
+      c->h_lin_items.push_back(
+          {static_cast<int32_t>(t0),
+           static_cast<int32_t>(std::min<int64_t>(t0 + kTargetsPerItem, c->woff[w + 1])), w, 0});
+      c->lin_wake.push_back(w);
+    }
+  c->npair = static_cast<int>(c->h_pair_items.size());
+  c->nlin = static_cast<int>(c->h_lin_items.size());
+}
+
+int ensure_built(pw_ctx* c) {
+  if (c->built) return PW_OK;
+  if (c->n <= 0) return fail(PW_ERR_STATE, "context has no particles (add a wake first)");
+  if (c->n > INT32_MAX / 2) return fail(PW_ERR_UNSUPPORTED, "too many particles");
+  PW_HIP(hipSetDevice(c->device));
+  build_items(c);
+  const size_t n = static_cast<size_t>(c->n);
+  if (dalloc(&c->y, n) != hipSuccess || dalloc(&c->ys, n) != hipSuccess ||
+      dalloc(&c->k[0], n) != hipSuccess || dalloc(&c->k[1], n) != hipSuccess ||
+      dalloc(&c->k[2], n) != hipSuccess || dalloc(&c->k[3], n) != hipSuccess ||
+      dalloc(&c->d_radius, n) != hipSuccess ||
+      dalloc(&c->pair_items, c->h_pair_items.size()) != hipSuccess ||
+      dalloc(&c->lin_items, c->h_lin_items.size()) != hipSuccess ||
+      dalloc(&c->d_lin, 11 * c->h_lin_items.size()) != hipSuccess ||
+      dalloc(&c->d_quad, 3 * c->h_pair_items.size()) != hipSuccess ||
+      dalloc(&c->d_flag, 1) != hipSuccess) {
+    release(c);
+    c->y = c->ys = nullptr;
+    for (auto& p : c->k) p = nullptr;
+    c->d_radius = nullptr;
+    c->pair_items = c->lin_items = nullptr;
+    c->d_lin = c->d_quad = nullptr;
+    c->d_flag = nullptr;
+    c->stream = nullptr;
+    (void)hipGetLastError();
+    return fail(PW_ERR_NOMEM, "device allocation failed");
+  }
+  PW_HIP(hipMemcpyAsync(c->y, c->pending.data(), n * sizeof(Rec64), hipMemcpyHostToDevice,
+                        c->stream));
+  PW_HIP(hipMemcpyAsync(c->d_radius, c->radius.data(), n * sizeof(double),
+                        hipMemcpyHostToDevice, c->stream));
+  PW_HIP(hipMemcpyAsync(c->pair_items, c->h_pair_items.data(),
+                        c->h_pair_items.size() * sizeof(WorkItem), hipMemcpyHostToDevice,
+                        c->stream));
+  PW_HIP(hipMemcpyAsync(c->lin_items, c->h_lin_items.data(),
+                        c->h_lin_items.size() * sizeof(WorkItem), hipMemcpyHostToDevice,
+                        c->stream));
+  // rates start at zero (a calloc'd rate vector in the reference)
+  std::vector<Rate64> zero(n);
+  std::memset(zero.data(), 0, n * sizeof(Rate64));
+  PW_HIP(hipMemcpyAsync(c->k[0], zero.data(), n * sizeof(Rate64), hipMemcpyHostToDevice,
+                        c->stream));
+  PW_HIP(hipStreamSynchronize(c->stream));
+  c->rates = c->k[0];
+  c->zc.assign(c->nsys(), 0.0);
+  c->pending.clear();
+  c->pending.shrink_to_fit();
+  c->built = true;
+  return PW_OK;
+}
+
+int kind_of(int precision, int* kind) {
+  switch (precision) {
+    case PW_FP64: *kind = pw::kPairFP64; return PW_OK;
+    case PW_FP64_REF: *kind = pw::kPairFP64Exact; return PW_OK;
+    case PW_FP32C: *kind = pw::kPairFP32C; return PW_OK;
+    case PW_LEARNED:
+      return fail(PW_ERR_UNSUPPORTED, "PW_LEARNED is not available through pw_solve/pw_step");
+    default:
+      return fail(PW_ERR_ARG, "unknown precision " + std::to_string(precision));
+  }
+}
+
+// One right-hand side f(state) -> out, with optional event timing.
+int rhs(pw_ctx* c, int kind, const Rec64* state, Rate64* out) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (c->profiling) {
+    if (c->ev_free.empty()) {
+      PW_HIP(hipEventCreate(&e0));
+      PW_HIP(hipEventCreate(&e1));
+    } else {
+      e0 = c->ev_free.back().first;
+      e1 = c->ev_free.back().second;
+      c->ev_free.pop_back();
+    }
+    PW_HIP(hipEventRecord(e0, c->stream));
+  }
+  PW_HIP(pw::launch_pair(kind, state, c->pair_items, c->npair, out, c->nu, c->stream));
+  if (c->profiling) {
+    PW_HIP(hipEventRecord(e1, c->stream));
+    c->ev_used.emplace_back(e0, e1);
+  }
+  PW_HIP(pw::launch_check_finite(out, static_cast<int>(c->n), c->d_flag, c->stream));
+  return PW_OK;
+}
+
+int check_flag(pw_ctx* c, const char* what) {
+  int flag = 0;
+  PW_HIP(hipMemcpyAsync(&flag, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  PW_HIP(hipStreamSynchronize(c->stream));
+  if (flag) return fail(PW_ERR_NONFINITE, std::string(what) + ": non-finite rate (coincident particles?)");
+  return PW_OK;
+}
+
+int collect_profile(pw_ctx* c) {
+  if (c->ev_used.empty()) return PW_OK;
+  PW_HIP(hipStreamSynchronize(c->stream));
+  for (auto& e : c->ev_used) {
+    float ms = 0.0f;
+    PW_HIP(hipEventElapsedTime(&ms, e.first, e.second));
+    c->prof_ms += ms;
+    c->prof_launches += 1;
+    c->ev_free.push_back(e);
+  }
+  c->ev_used.clear();
+  return PW_OK;
+}
+
+#define PW_CTX(c)                                                   \
+  do {                                                              \
+    if (!(c)) return fail(PW_ERR_ARG, "null context");              \
+    if (hipSetDevice((c)->device) != hipSuccess)                    \
+      return fail(PW_ERR_HIP, "hipSetDevice failed");               \
+  } while (0)
+
+#define PW_TRY(expr)                     \
+  do {                                   \
+    int rc_ = (expr);                    \
+    if (rc_ != PW_OK) return rc_;        \
+  } while (0)
+
+int64_t total_maxsize(const pw_ctx* c) {
+  int64_t s = 0;
+  for (int64_t m : c->wmax) s += 6 * m;
+  return s;
+}
+
+int download_state(pw_ctx* c, std::vector<Rec64>& h) {
+  h.resize(static_cast<size_t>(c->n));
+  if (!c->built) {
+    h = c->pending;
+    return PW_OK;
+  }
+  PW_HIP(hipMemcpyAsync(h.data(), c->y, h.size() * sizeof(Rec64), hipMemcpyDeviceToHost,
+                        c->stream));
+  PW_HIP(hipStreamSynchronize(c->stream));
+  return PW_OK;
+}
+
+int upload_state(pw_ctx* c, const std::vector<Rec64>& h) {
+  if (!c->built) {
+    c->pending = h;
+    return PW_OK;
+  }
+  PW_HIP(hipMemcpyAsync(c->y, h.data(), h.size() * sizeof(Rec64), hipMemcpyHostToDevice,
+                        c->stream));
+  PW_HIP(hipStreamSynchronize(c->stream));
+  return PW_OK;
+}
+
+int download_rates(pw_ctx* c, std::vector<Rate64>& h) {
+  h.resize(static_cast<size_t>(c->n));
+  if (!c->built) {
+    std::memset(h.data(), 0, h.size() * sizeof(Rate64));
+    return PW_OK;
+  }
+  PW_HIP(hipMemcpyAsync(h.data(), c->rates, h.size() * sizeof(Rate64), hipMemcpyDeviceToHost,
+                        c->stream));
+  PW_HIP(hipStreamSynchronize(c->stream));
+  return PW_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pw_version(void) { return "pawan-mi355x 0.1 (gfx950)"; }
+
+const char* pw_last_error(void) { return g_error.c_str(); }
+
+int pw_device_count(int* count) {
+  if (!count) return fail(PW_ERR_ARG, "null count");
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) {
+    *count = 0;
+    return fail(PW_ERR_HIP, std::string("hipGetDeviceCount: ") + hipGetErrorString(e));
+  }
+  *count = n;
+  return PW_OK;
+}
+
+int pw_create(pw_ctx** out, int device) {
+  if (!out) return fail(PW_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(PW_ERR_HIP, "no HIP device available");
+  if (device < 0 || device >= ndev)
+    return fail(PW_ERR_ARG, "device index out of range");
+  pw_ctx* c = new (std::nothrow) pw_ctx();
+  if (!c) return fail(PW_ERR_NOMEM, "host allocation failed");
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return fail(PW_ERR_HIP, "stream creation failed");
+  }
+  *out = c;
+  return PW_OK;
+}
+
+int pw_destroy(pw_ctx* c) {
+  if (!c) return PW_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  release(c);
+  delete c;
+  return PW_OK;
+}
+
+int pw_add_wake(pw_ctx* c, int64_t n, int64_t maxn, const double* pos, const double* alpha,
+                const double* radius, const double* volume, int* wake) {
+  PW_CTX(c);
+  if (c->built) return fail(PW_ERR_STATE, "wakes must be added before the first solve/step");
+  if (n < 0 || maxn < n) return fail(PW_ERR_ARG, "need 0 <= n <= maxn");
+  if (n > 0 && (!pos || !alpha || !radius || !volume))
+    return fail(PW_ERR_ARG, "null particle array");
+  if (c->n + n > INT32_MAX / 2) return fail(PW_ERR_UNSUPPORTED, "too many particles");
+  for (int64_t i = 0; i < n; ++i) {
+    Rec64 r;
+    r.x = pos[3 * i];
+    r.y = pos[3 * i + 1];
+    r.z = pos[3 * i + 2];
+    r.s2 = radius[i] * radius[i];  // gsl_pow_2(s), interaction_utils.h:476
+    r.ax = alpha[3 * i];
+    r.ay = alpha[3 * i + 1];
+    r.az = alpha[3 * i + 2];
+    r.vol = volume[i];
+    c->pending.push_back(r);
+    c->radius.push_back(radius[i]);
+    c->volume.push_back(volume[i]);
+  }
+  c->wn.push_back(n);
+  c->wmax.push_back(maxn);
+  c->n += n;
+  c->woff.push_back(c->n);
+  if (wake) *wake = static_cast<int>(c->wn.size()) - 1;
+  return PW_OK;
+}
+
+int pw_set_nu(pw_ctx* c, double nu) {
+  PW_CTX(c);
+  if (!std::isfinite(nu) || nu < 0) return fail(PW_ERR_ARG, "nu must be finite and >= 0");
+  c->nu = nu;
+  return PW_OK;
+}
+
+int pw_get_nu(pw_ctx* c, double* nu) {
+  PW_CTX(c);
+  if (!nu) return fail(PW_ERR_ARG, "null output");
+  *nu = c->nu;
+  return PW_OK;
+}
+
+int pw_set_coupled(pw_ctx* c, int coupled) {
+  PW_CTX(c);
+  if (c->built) return fail(PW_ERR_STATE, "coupling must be set before the first solve/step");
+  c->coupled = coupled != 0;
+  return PW_OK;
+}
+
+int pw_get_size(pw_ctx* c, int64_t* num_particles, int* num_wakes) {
+  PW_CTX(c);
+  if (num_particles) *num_particles = c->n;
+  if (num_wakes) *num_wakes = static_cast<int>(c->wn.size());
+  return PW_OK;
+}
+
+int pw_set_wake_state(pw_ctx* c, int w, const double* pos, const double* alpha) {
+  PW_CTX(c);
+  if (w < 0 || w >= static_cast<int>(c->wn.size())) return fail(PW_ERR_ARG, "wake index out of range");
+  if (c->wn[w] > 0 && (!pos || !alpha)) return fail(PW_ERR_ARG, "null array");
+  std::vector<Rec64> h;
+  PW_TRY(download_state(c, h));
+  for (int64_t i = 0; i < c->wn[w]; ++i) {
+    Rec64& r = h[c->woff[w] + i];
+    r.x = pos[3 * i]; r.y = pos[3 * i + 1]; r.z = pos[3 * i + 2];
+    r.ax = alpha[3 * i]; r.ay = alpha[3 * i + 1]; r.az = alpha[3 * i + 2];
+  }
+  return upload_state(c, h);
+}
+
+int pw_get_wake_state(pw_ctx* c, int w, double* pos, double* alpha) {
+  PW_CTX(c);
+  if (w < 0 || w >= static_cast<int>(c->wn.size())) return fail(PW_ERR_ARG, "wake index out of range");
+  std::vector<Rec64> h;
+  PW_TRY(download_state(c, h));
+  for (int64_t i = 0; i < c->wn[w]; ++i) {
+    const Rec64& r = h[c->woff[w] + i];
+    if (pos) { pos[3 * i] = r.x; pos[3 * i + 1] = r.y; pos[3 * i + 2] = r.z; }
+    if (alpha) { alpha[3 * i] = r.ax; alpha[3 * i + 1] = r.ay; alpha[3 * i + 2] = r.az; }
+  }
+  return PW_OK;
+}
+
+int pw_get_wake_rates(pw_ctx* c, int w, double* vel, double* retvor) {
+  PW_CTX(c);
+  if (w < 0 || w >= static_cast<int>(c->wn.size())) return fail(PW_ERR_ARG, "wake index out of range");
+  std::vector<Rate64> h;
+  PW_TRY(download_rates(c, h));
+  for (int64_t i = 0; i < c->wn[w]; ++i) {
+    const Rate64& r = h[c->woff[w] + i];
+    if (vel) { vel[3 * i] = r.ux; vel[3 * i + 1] = r.uy; vel[3 * i + 2] = r.uz; }
+    if (retvor) { retvor[3 * i] = r.wx; retvor[3 * i + 1] = r.wy; retvor[3 * i + 2] = r.wz; }
+  }
+  return PW_OK;
+}
+
+int pw_set_states(pw_ctx* c, const double* state, int64_t len) {
+  PW_CTX(c);
+  if (!state) return fail(PW_ERR_ARG, "null state");
+  if (len != total_maxsize(c)) return fail(PW_ERR_ARG, "state length != sum of 6*maxn");
+  std::vector<Rec64> h;
+  PW_TRY(download_state(c, h));
+  int64_t base = 0;
+  for (size_t w = 0; w < c->wn.size(); ++w) {
+    const double* p = state + base;                 // pos block, maxn x 3
+    const double* a = state + base + 3 * c->wmax[w];  // alpha block (wake.cpp:860-866)
+    for (int64_t i = 0; i < c->wn[w]; ++i) {
+      Rec64& r = h[c->woff[w] + i];
+      r.x = p[3 * i]; r.y = p[3 * i + 1]; r.z = p[3 * i + 2];
+      r.ax = a[3 * i]; r.ay = a[3 * i + 1]; r.az = a[3 * i + 2];
+    }
+    base += 6 * c->wmax[w];
+  }
+  return upload_state(c, h);
+}
+
+int pw_get_states(pw_ctx* c, double* state, int64_t len) {
+  PW_CTX(c);
+  if (!state) return fail(PW_ERR_ARG, "null state");
+  if (len != total_maxsize(c)) return fail(PW_ERR_ARG, "state length != sum of 6*maxn");
+  std::vector<Rec64> h;
+  PW_TRY(download_state(c, h));
+  std::memset(state, 0, sizeof(double) * static_cast<size_t>(len));
+  int64_t base = 0;
+  for (size_t w = 0; w < c->wn.size(); ++w) {
+    double* p = state + base;
+    double* a = state + base + 3 * c->wmax[w];
+    for (int64_t i = 0; i < c->wn[w]; ++i) {
+      const Rec64& r = h[c->woff[w] + i];
+      p[3 * i] = r.x; p[3 * i + 1] = r.y; p[3 * i + 2] = r.z;
+      a[3 * i] = r.ax; a[3 * i + 1] = r.ay; a[3 * i + 2] = r.az;
+    }
+    base += 6 * c->wmax[w];
+  }
+  return PW_OK;
+}
+
+int pw_get_rates(pw_ctx* c, double* rate, int64_t len) {
+  PW_CTX(c);
+  if (!rate) return fail(PW_ERR_ARG, "null rate");
+  if (len != total_maxsize(c)) return fail(PW_ERR_ARG, "rate length != sum of 6*maxn");
+  std::vector<Rate64> h;
+  PW_TRY(download_rates(c, h));
+  std::memset(rate, 0, sizeof(double) * static_cast<size_t>(len));
+  int64_t base = 0;
+  for (size_t w = 0; w < c->wn.size(); ++w) {
+    double* u = rate + base;
+    double* q = rate + base + 3 * c->wmax[w];  // wake.cpp:876
+    for (int64_t i = 0; i < c->wn[w]; ++i) {
+      const Rate64& r = h[c->woff[w] + i];
+      u[3 * i] = r.ux; u[3 * i + 1] = r.uy; u[3 * i + 2] = r.uz;
+      q[3 * i] = r.wx; q[3 * i + 1] = r.wy; q[3 * i + 2] = r.wz;
+    }
+    base += 6 * c->wmax[w];
+  }
+  return PW_OK;
+}
+
+int pw_solve(pw_ctx* c, int precision) {
+  PW_CTX(c);
+  int kind = 0;
+  PW_TRY(kind_of(precision, &kind));
+  PW_TRY(ensure_built(c));
+  PW_HIP(pw::launch_fill_int(c->d_flag, 1, 0, c->stream));
+  PW_TRY(rhs(c, kind, c->y, c->k[0]));
+  c->rates = c->k[0];
+  return check_flag(c, "pw_solve");
+}
+
+int pw_step(pw_ctx* c, int scheme, double dt, int precision, int64_t nsteps) {
+  PW_CTX(c);
+  int kind = 0;
+  PW_TRY(kind_of(precision, &kind));
+  if (scheme != PW_EULER && scheme != PW_RK3 && scheme != PW_RK4)
+    return fail(PW_ERR_ARG, "scheme must be PW_EULER, PW_RK3 or PW_RK4");
+  if (!std::isfinite(dt)) return fail(PW_ERR_ARG, "dt must be finite");
+  if (nsteps < 0) return fail(PW_ERR_ARG, "nsteps must be >= 0");
+  PW_TRY(ensure_built(c));
+  const int n = static_cast<int>(c->n);
+  hipStream_t st = c->stream;
+  Rate64 **k = c->k;
+  PW_HIP(pw::launch_fill_int(c->d_flag, 1, 0, st));
+  for (int64_t it = 0; it < nsteps; ++it) {
+    if (scheme == PW_RK4) {
+      // rk4.cpp:20-66
+      PW_TRY(rhs(c, kind, c->y, k[0]));
+      PW_HIP(pw::launch_stage(c->y, k[0], 0.5 * dt, c->ys, n, st));
+      PW_TRY(rhs(c, kind, c->ys, k[1]));
+      PW_HIP(pw::launch_stage(c->y, k[1], 0.5 * dt, c->ys, n, st));
+      PW_TRY(rhs(c, kind, c->ys, k[2]));
+      PW_HIP(pw::launch_stage(c->y, k[2], dt, c->ys, n, st));
+      PW_TRY(rhs(c, kind, c->ys, k[3]));
+      PW_HIP(pw::launch_rk4_final(c->y, k[0], k[1], k[2], k[3], dt / 6., dt / 3., n, st));
+      c->rates = k[3];
+    } else if (scheme == PW_RK3) {
+      // rk3.cpp:20-55
+      PW_TRY(rhs(c, kind, c->y, k[0]));
+      PW_HIP(pw::launch_stage(c->y, k[0], 0.5 * dt, c->ys, n, st));
+      PW_TRY(rhs(c, kind, c->ys, k[1]));
+      PW_HIP(pw::launch_rk3_mid(c->y, k[0], k[1], dt, 2. * dt, c->ys, n, st));
+      PW_TRY(rhs(c, kind, c->ys, k[2]));
+      PW_HIP(pw::launch_rk3_final(c->y, k[0], k[1], k[2], dt / 6.0, dt * 2.0 / 3.0, n, st));
+      c->rates = k[2];
+    } else {
+      // integration.cpp:126-136
+      PW_TRY(rhs(c, kind, c->y, k[0]));
+      PW_HIP(pw::launch_euler(c->y, k[0], dt, n, st));
+      c->rates = k[0];
+    }
+  }
+  return check_flag(c, "pw_step");
+}
+
+int pw_diagnose(pw_ctx* c, double* out, int64_t out_len) {
+  PW_CTX(c);
+  if (!out) return fail(PW_ERR_ARG, "null output");
+  PW_TRY(ensure_built(c));
+  const int ns = c->nsys();
+  if (out_len < 15 * static_cast<int64_t>(ns))
+    return fail(PW_ERR_ARG, "output needs 15 doubles per system");
+  PW_HIP(pw::launch_diag_linear(c->y, c->lin_items, c->nlin, c->d_lin, c->stream));
+  PW_HIP(pw::launch_diag_pairs(c->y, c->d_radius, c->pair_items, c->npair, c->d_quad, c->stream));
+  std::vector<double> lin(11 * static_cast<size_t>(c->nlin)), quad(3 * static_cast<size_t>(c->npair));
+  PW_HIP(hipMemcpyAsync(lin.data(), c->d_lin, lin.size() * sizeof(double), hipMemcpyDeviceToHost,
+                        c->stream));
+  PW_HIP(hipMemcpyAsync(quad.data(), c->d_quad, quad.size() * sizeof(double),
+                        hipMemcpyDeviceToHost, c->stream));
+  PW_HIP(hipStreamSynchronize(c->stream));
+  const int nw = static_cast<int>(c->wn.size());
+  // per-wake linear sums in block order
+  std::vector<double> wl(11 * static_cast<size_t>(nw), 0.0);
+  for (int b = 0; b < c->nlin; ++b)
+    for (int m = 0; m < 11; ++m) wl[11 * c->lin_wake[b] + m] += lin[11 * b + m];
+  // per-system quadratic sums in item order (items of one system are
+  // contiguous and in target order)
+  std::vector<double> sq(3 * static_cast<size_t>(ns), 0.0);
+  for (int b = 0; b < c->npair; ++b)
+    for (int m = 0; m < 3; ++m) sq[3 * c->pair_sys[b] + m] += quad[3 * b + m];
+  for (int s = 0; s < ns; ++s) {
+    double* o = out + 15 * s;
+    for (int m = 0; m < 9; ++m) o[m] = 0.0;
+    const int w0 = c->coupled ? 0 : s, w1 = c->coupled ? nw : s + 1;
+    for (int w = w0; w < w1; ++w) {
+      const double* v = &wl[11 * w];
+      // calculateTotalVorticity / LinearImpulse (* 0.5) / AngularImpulse
+      // (* 1/3), interaction.cpp:489-527, summed over wakes (:74-84)
+      for (int m = 0; m < 3; ++m) {
+        o[m] += v[m];
+        o[3 + m] += v[3 + m] * 0.5;
+        o[6 + m] += v[6 + m] * (1.0 / 3.0);
+      }
+      if (v[10] != 0.0) c->zc[s] = v[9] / v[10];  // interaction.cpp:153-155
+    }
+    o[9] = sq[3 * s + 0];   // helicity
+    o[10] = 0.0;            // enstrophy: ENST == 0 for the Gaussian kernel
+    o[11] = 0.0;            // div-free enstrophy: ENSTF == 0
+    o[12] = sq[3 * s + 1];  // kinetic energy
+    o[13] = sq[3 * s + 2];  // div-free kinetic energy
+    o[14] = c->zc[s];
+  }
+  return PW_OK;
+}
+
+int pw_synchronize(pw_ctx* c) {
+  PW_CTX(c);
+  PW_HIP(hipStreamSynchronize(c->stream));
+  return PW_OK;
+}
+
+int pw_set_profiling(pw_ctx* c, int enable) {
+  PW_CTX(c);
+  PW_TRY(collect_profile(c));
+  c->profiling = enable != 0;
+  c->prof_ms = 0.0;
+  c->prof_launches = 0;
+  return PW_OK;
+}
+
+int pw_get_pair_timing(pw_ctx* c, double* total_ms, int64_t* launches) {
+  PW_CTX(c);
+  PW_TRY(collect_profile(c));
+  if (total_ms) *total_ms = c->prof_ms;
+  if (launches) *launches = c->prof_launches;
+  return PW_OK;
+}
+
+}  // extern "C"
