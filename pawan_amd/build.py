@@ -31,7 +31,7 @@ HIPFLAGS = [
     "-I", os.path.join(ROOT, "include"), "-I", CSRC, "-I", HOST,
 ]
 
-HIP_SOURCES = ["pw_pair.hip", "pw_diag.hip", "pw_capi.hip"]
+HIP_SOURCES = ["pw_pair.hip", "pw_stage.hip", "pw_diag.hip", "pw_capi.hip"]
 HOST_SOURCES = ["pawan_host.cpp"]
 
 
