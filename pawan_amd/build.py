@@ -67,8 +67,15 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp])
     os.replace(tmp, LIB)
+    # the class-API driver, linked against the library with an rpath to it
+    os.makedirs(BINDIR, exist_ok=True)
+    exe = os.path.join(BINDIR, "pawan_run")
+    _run([HIPCC, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-I", HOST,
+          os.path.join(HOST, "pawan_run.cpp"), "-L", LIBDIR, "-lpawan_gpu",
+          "-Wl,-rpath,$ORIGIN/../lib", "-o", exe + ".tmp"])
+    os.replace(exe + ".tmp", exe)
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {exe}")
     return LIB
 
 

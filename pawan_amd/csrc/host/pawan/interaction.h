@@ -1,0 +1,94 @@
+// interaction.h -- pawan::__interaction and pawan::__parallel, the
+// multi-wake vortex-particle system (reference:
+// src/interaction/interaction.h:22-261, src/interaction/parallel.h:20-155).
+//
+// Same class names, constructors and virtual shapes as the reference; the
+// pair loops and diagnostics run on the GPU through the C-ABI of
+// include/pawan_gpu.h.  The wakes stay owned by the caller (non-owning
+// pointers, interaction.cpp:44-52); their host arrays are kept in step with
+// the device so that drivers written against the reference read the same
+// state after every call.
+#pragma once
+
+#include <vector>
+
+#include "system.h"
+#include "wake.h"
+
+struct pw_ctx;
+
+namespace pawan {
+
+class __interaction : public __system {
+ protected:
+  double _nu;                       /*!< kinematic viscosity             */
+  size_t _nWake;                    /*!< number of wakes                 */
+  vec *_totalVorticity;             /*!< diagnostics (interaction.h:27-35) */
+  vec *_linearImpulse;
+  vec *_angularImpulse;
+  double _enstrophy;
+  double _helicity;
+  double _kineticEnergy;
+  double _enstrophyF;
+  double _kineticEnergyF;
+  double _Zc = 0.0;
+  int _precision;                   /*!< PW_FP64 (default), PW_FP32C, ... */
+
+  //! zero the rates, then all self and cross pairs (interaction.cpp:196-210)
+  virtual void interact();
+
+ private:
+  std::vector<__wake *> _W;
+  pw_ctx *_ctx = nullptr;
+  bool _deviceStale = true;   // host wake state newer than the device's
+  void ensureDevice();
+  void pushState();           // host wakes -> device
+  void pullState();           // device -> host wakes (positions, strengths)
+  void pullRates();           // device rates -> host wakes
+  [[noreturn]] void fail(const char *what) const;
+
+ public:
+  __interaction();
+  explicit __interaction(__wake *W);
+  __interaction(__wake *W1, __wake *W2);
+  virtual ~__interaction();
+
+  virtual void diagnose() override;
+  virtual void addWake(__wake *W);
+  virtual void solve() override;
+  virtual void write(FILE *f) override;
+  virtual void writediagnosis(FILE *fdiag) override;
+  virtual void writenu(FILE *fdiag) override;
+  virtual void setStates(const vec *state) override;
+  virtual void getRates(vec *rate) override;
+  virtual void getStates(vec *state) override;
+  virtual void getIdealRates(vec *rate);
+  virtual bool stepOnDevice(int scheme, const double &dt, vec *state) override;
+
+  //! The reference hard-codes nu = 1.56e-5 (interaction.cpp:15); settable.
+  void setNu(double nu) { _nu = nu; }
+  double getNu() const { return _nu; }
+  //! Pair arithmetic of solve() and the device steps (PW_FP64 default).
+  void setPrecision(int precision) { _precision = precision; }
+  int getPrecision() const { return _precision; }
+
+  // diagnostics of the last diagnose() call
+  const vec *totalVorticity() const { return _totalVorticity; }
+  const vec *linearImpulse() const { return _linearImpulse; }
+  const vec *angularImpulse() const { return _angularImpulse; }
+  double helicity() const { return _helicity; }
+  double enstrophy() const { return _enstrophy; }
+  double enstrophyF() const { return _enstrophyF; }
+  double kineticEnergy() const { return _kineticEnergy; }
+  double kineticEnergyF() const { return _kineticEnergyF; }
+  double centroid() const { return _Zc; }
+};
+
+//! The reference's OpenMP system (parallel.cpp); here the same GPU path.
+class __parallel : public __interaction {
+ public:
+  explicit __parallel(__wake *W) : __interaction(W) {}
+  __parallel(__wake *W1, __wake *W2) : __interaction(W1, W2) {}
+};
+
+}  // namespace pawan

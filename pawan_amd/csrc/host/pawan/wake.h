@@ -1,0 +1,79 @@
+// wake.h -- pawan::__wake, the particle container
+// (reference: src/wake/wake.h:26-172, src/wake/wake.cpp).
+//
+// Same public members, state-vector layout and constructors as the
+// reference, with the gsl containers replaced by pawan::vec / pawan::mat.
+// Dymore coupling members (addParticles(PawanRecvData), split/merge/relax,
+// bound-vortex effects) are not part of this library.
+#pragma once
+
+#include <cstdio>
+
+#include "pw_vector.h"
+
+#define MAXNUMPARTICLES 100000  // max particles of a copied wake (wake.h:22)
+
+namespace pawan {
+
+class __wake {
+ protected:
+  // wake.cpp:10-25: arrays of exactly n particles
+  virtual void create_particles(const int &n);
+  // wake.cpp:27-44: arrays of MAXNUMPARTICLES rows, no particles yet
+  virtual void initialise_memory();
+
+ public:
+  size_t _size;             /*!< state entries of the particles, 6 N     */
+  size_t _maxsize;          /*!< state-vector block of the wake, 6 maxN  */
+  size_t _numParticles;     /*!< particles N                            */
+  size_t _maxnumParticles;  /*!< rows allocated, maxN                   */
+  mat *_position;           /*!< N x 3                                  */
+  mat *_velocity;
+  mat *_vorticity;
+  mat *_retvorcity;
+  vec *_radius;
+  vec *_volume;
+  vec *_birthstrength;
+  mat *_vorticityfield;
+  vec *_active;
+
+  //! One particle at the origin, radius and volume 1 (wake.cpp:46-59)
+  __wake();
+  //! n particles, zero positions and strengths (create_particles)
+  explicit __wake(const int &n);
+  //! n particles from row-major arrays (pos, alpha: n x 3; radius,
+  //! volume: n).  Not in the reference, whose wakes are built by its
+  //! initial-condition subclasses; this is the generic way in.
+  __wake(size_t n, const double *pos, const double *alpha, const double *radius,
+         const double *volume);
+  //! Copy into MAXNUMPARTICLES-row arrays (wake.cpp:558-562)
+  explicit __wake(__wake *W);
+  //! Copy the particles of W into this wake (wake.cpp:542-556)
+  virtual void addParticles(__wake *W);
+  virtual ~__wake();
+
+  virtual void print();
+  //! [N][pos][alpha][radius][volume][birthstrength] (wake.cpp:821-830)
+  virtual void save(FILE *f);
+  //! [N][pos][alpha][radius][active][volume][birthstrength]
+  //! (wake.cpp:832-841), the layout scripts/readWake.py reads
+  virtual void write(FILE *f);
+
+  //! state = [pos (maxN x 3) ; alpha (maxN x 3)] (wake.cpp:856-867)
+  virtual void setStates(const vec *state);
+  //! rate = [vel ; retvor] for the first N rows (wake.cpp:869-880)
+  virtual void getRates(vec *rate);
+  //! state = [pos ; alpha] for the first N rows (wake.cpp:882-893)
+  virtual void getStates(vec *state);
+  //! zero rates (wake.cpp:895-899)
+  virtual void getIdealRates(vec *rate);
+
+  //! add x to coordinate n of every particle (wake.cpp:901-906)
+  virtual void translate(const size_t &n, const double &x);
+  //! add the 3-vector x to every particle (wake.cpp:908-914)
+  virtual void translate(const double *x);
+  //! elemental rotation by angle a about axis n (wake.cpp:916-937)
+  virtual void rotate(const size_t &n, const double &a);
+};
+
+}  // namespace pawan

@@ -1,0 +1,520 @@
+// pawan_host.cpp -- the C++ class API (pawan::__wake, __interaction,
+// __integration, __rk4, __rk3, __io) over the C-ABI of include/pawan_gpu.h.
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <stdexcept>
+#include <string>
+
+#include "pawan_gpu.h"
+#include "pawan/integration.h"
+#include "pawan/interaction.h"
+#include "pawan/wake.h"
+
+namespace pawan {
+
+// ---------------------------------------------------------------------------
+// __wake (src/wake/wake.cpp)
+// ---------------------------------------------------------------------------
+
+void __wake::create_particles(const int &n) {
+  _numParticles = n;
+  _size = 2 * _numParticles * 3;
+  _maxnumParticles = _numParticles;
+  _maxsize = 2 * _maxnumParticles * 3;
+  _position = new mat(_numParticles, 3);
+  _velocity = new mat(_numParticles, 3);
+  _vorticity = new mat(_numParticles, 3);
+  _retvorcity = new mat(_numParticles, 3);
+  _radius = new vec(_numParticles);
+  _volume = new vec(_numParticles);
+  _birthstrength = new vec(_numParticles);
+  _vorticityfield = new mat(_numParticles, 3);
+  _active = new vec(_numParticles);
+}
+
+void __wake::initialise_memory() {
+  _numParticles = 0;
+  _size = 0;
+  _maxnumParticles = MAXNUMPARTICLES;
+  _maxsize = 2 * _maxnumParticles * 3;
+  _position = new mat(_maxnumParticles, 3);
+  _velocity = new mat(_maxnumParticles, 3);
+  _vorticity = new mat(_maxnumParticles, 3);
+  _retvorcity = new mat(_maxnumParticles, 3);
+  _radius = new vec(_maxnumParticles);
+  _volume = new vec(_maxnumParticles);
+  _birthstrength = new vec(_maxnumParticles);
+  _vorticityfield = new mat(_maxnumParticles, 3);
+  _active = new vec(_maxnumParticles);
+}
+
+__wake::__wake() {
+  create_particles(1);
+  _radius->set(0, 1.0);
+  _volume->set(0, 1.0);
+  _birthstrength->set(0, 1.0);
+}
+
+__wake::__wake(const int &n) { create_particles(n); }
+
+__wake::__wake(size_t n, const double *pos, const double *alpha, const double *radius,
+               const double *volume) {
+  create_particles(static_cast<int>(n));
+  for (size_t i = 0; i < n; ++i) {
+    for (size_t j = 0; j < 3; ++j) {
+      _position->set(i, j, pos[3 * i + j]);
+      _vorticity->set(i, j, alpha[3 * i + j]);
+    }
+    _radius->set(i, radius[i]);
+    _volume->set(i, volume[i]);
+    _birthstrength->set(i, std::sqrt(alpha[3 * i] * alpha[3 * i] + alpha[3 * i + 1] * alpha[3 * i + 1] +
+                                     alpha[3 * i + 2] * alpha[3 * i + 2]));
+    _active->set(i, 1.0);
+  }
+}
+
+void __wake::addParticles(__wake *W) {
+  for (size_t i = 0; i < W->_numParticles; ++i) {
+    for (size_t j = 0; j < 3; ++j) {
+      _position->set(i, j, W->_position->get(i, j));
+      _vorticity->set(i, j, W->_vorticity->get(i, j));
+      _vorticityfield->set(i, j, W->_vorticityfield->get(i, j));
+    }
+    _radius->set(i, W->_radius->get(i));
+    _volume->set(i, W->_volume->get(i));
+    _birthstrength->set(i, W->_birthstrength->get(i));
+  }
+  _numParticles = W->_numParticles;
+  _size = 2 * _numParticles * 3;
+}
+
+__wake::__wake(__wake *W) {
+  if (W->_numParticles > MAXNUMPARTICLES)
+    throw std::length_error("__wake copy: more than MAXNUMPARTICLES particles");
+  initialise_memory();
+  addParticles(W);
+}
+
+__wake::~__wake() {
+  delete _position;
+  delete _velocity;
+  delete _vorticity;
+  delete _retvorcity;
+  delete _radius;
+  delete _volume;
+  delete _birthstrength;
+  delete _vorticityfield;
+  delete _active;
+}
+
+void __wake::print() {
+  std::printf("_numParticles %zu\n", _numParticles);
+  for (size_t i = 0; i < _numParticles; ++i)
+    std::printf("%zu: x=(%g %g %g) a=(%g %g %g) s=%g V=%g\n", i, _position->get(i, 0),
+                _position->get(i, 1), _position->get(i, 2), _vorticity->get(i, 0),
+                _vorticity->get(i, 1), _vorticity->get(i, 2), _radius->get(i), _volume->get(i));
+}
+
+static void fwrite_all(const double *p, size_t n, FILE *f) {
+  if (n && std::fwrite(p, sizeof(double), n, f) != n) throw std::runtime_error("fwrite failed");
+}
+
+void __wake::save(FILE *f) {
+  std::fwrite(&_numParticles, sizeof(size_t), 1, f);
+  fwrite_all(_position->ptr(), _position->data.size(), f);
+  fwrite_all(_vorticity->ptr(), _vorticity->data.size(), f);
+  fwrite_all(_radius->ptr(), _radius->size, f);
+  fwrite_all(_volume->ptr(), _volume->size, f);
+  fwrite_all(_birthstrength->ptr(), _birthstrength->size, f);
+}
+
+void __wake::write(FILE *f) {
+  std::fwrite(&_numParticles, sizeof(size_t), 1, f);
+  fwrite_all(_position->ptr(), _position->data.size(), f);
+  fwrite_all(_vorticity->ptr(), _vorticity->data.size(), f);
+  fwrite_all(_radius->ptr(), _radius->size, f);
+  fwrite_all(_active->ptr(), _active->size, f);
+  fwrite_all(_volume->ptr(), _volume->size, f);
+  fwrite_all(_birthstrength->ptr(), _birthstrength->size, f);
+}
+
+void __wake::setStates(const vec *state) {
+  const size_t off = _maxnumParticles * 3;
+  for (size_t i = 0; i < _maxnumParticles; ++i)
+    for (size_t j = 0; j < 3; ++j) {
+      _position->set(i, j, state->get(3 * i + j));
+      _vorticity->set(i, j, state->get(off + 3 * i + j));
+    }
+}
+
+void __wake::getRates(vec *rate) {
+  for (size_t i = 0; i < _numParticles; ++i)
+    for (size_t j = 0; j < 3; ++j) {
+      size_t ind = i * 3 + j;
+      rate->set(ind, _velocity->get(i, j));
+      ind += _maxsize / 2;
+      rate->set(ind, _retvorcity->get(i, j));
+    }
+}
+
+void __wake::getStates(vec *state) {
+  for (size_t i = 0; i < _numParticles; ++i)
+    for (size_t j = 0; j < 3; ++j) {
+      size_t ind = i * 3 + j;
+      state->set(ind, _position->get(i, j));
+      ind += _maxsize / 2;
+      state->set(ind, _vorticity->get(i, j));
+    }
+}
+
+void __wake::getIdealRates(vec *rate) { rate->set_zero(); }
+
+void __wake::translate(const size_t &n, const double &x) {
+  for (size_t i = 0; i < _numParticles; ++i) _position->set(i, n, x + _position->get(i, n));
+}
+
+void __wake::translate(const double *x) {
+  for (size_t i = 0; i < _numParticles; ++i)
+    for (size_t j = 0; j < 3; ++j) _position->set(i, j, x[j] + _position->get(i, j));
+}
+
+// The second updated coordinate uses the first one already rotated, exactly
+// as wake.cpp:916-937 does, so that drivers get the same wakes.
+void __wake::rotate(const size_t &n, const double &a) {
+  const double c = std::cos(a), s = std::sin(a);
+  for (size_t i = 0; i < _numParticles; ++i) {
+    if (n == 0) {
+      _position->set(i, 1, c * _position->get(i, 1) - s * _position->get(i, 2));
+      _position->set(i, 2, s * _position->get(i, 1) + c * _position->get(i, 2));
+    } else if (n == 1) {
+      _position->set(i, 0, c * _position->get(i, 0) + s * _position->get(i, 2));
+      _position->set(i, 2, -s * _position->get(i, 0) + c * _position->get(i, 2));
+    } else if (n == 2) {
+      _position->set(i, 1, c * _position->get(i, 0) - s * _position->get(i, 1));
+      _position->set(i, 2, s * _position->get(i, 0) + c * _position->get(i, 1));
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// __interaction (src/interaction/interaction.cpp)
+// ---------------------------------------------------------------------------
+
+__interaction::__interaction() {
+  _nu = 1.56e-5;  // interaction.cpp:15
+  _nWake = 0;
+  _totalVorticity = new vec(3);
+  _linearImpulse = new vec(3);
+  _angularImpulse = new vec(3);
+  _enstrophy = _helicity = _kineticEnergy = _enstrophyF = _kineticEnergyF = 0.0;
+  _precision = PW_FP64;
+}
+
+__interaction::__interaction(__wake *W) : __interaction() { addWake(W); }
+
+__interaction::__interaction(__wake *W1, __wake *W2) : __interaction() {
+  addWake(W1);
+  addWake(W2);
+}
+
+__interaction::~__interaction() {
+  if (_ctx) pw_destroy(_ctx);
+  delete _totalVorticity;
+  delete _linearImpulse;
+  delete _angularImpulse;
+}
+
+void __interaction::fail(const char *what) const {
+  throw std::runtime_error(std::string(what) + ": " + pw_last_error());
+}
+
+void __interaction::addWake(__wake *W) {
+  _W.push_back(W);
+  _size = W->_size;  // interaction.cpp:47
+  _totalmaxsize += W->_maxsize;
+  _nWake++;
+  if (_ctx) {  // the device copy is rebuilt with the new wake on next use
+    pw_destroy(_ctx);
+    _ctx = nullptr;
+  }
+}
+
+void __interaction::ensureDevice() {
+  if (_ctx) return;
+  if (pw_create(&_ctx, 0) != PW_OK) fail("pw_create");
+  for (__wake *w : _W) {
+    if (pw_add_wake(_ctx, static_cast<int64_t>(w->_numParticles),
+                    static_cast<int64_t>(w->_maxnumParticles), w->_position->ptr(),
+                    w->_vorticity->ptr(), w->_radius->ptr(), w->_volume->ptr(), nullptr) != PW_OK)
+      fail("pw_add_wake");
+  }
+  _deviceStale = false;
+}
+
+void __interaction::pushState() {
+  ensureDevice();
+  if (pw_set_nu(_ctx, _nu) != PW_OK) fail("pw_set_nu");
+  if (!_deviceStale) return;
+  for (size_t k = 0; k < _W.size(); ++k)
+    if (pw_set_wake_state(_ctx, static_cast<int>(k), _W[k]->_position->ptr(),
+                          _W[k]->_vorticity->ptr()) != PW_OK)
+      fail("pw_set_wake_state");
+  _deviceStale = false;
+}
+
+void __interaction::pullState() {
+  for (size_t k = 0; k < _W.size(); ++k)
+    if (pw_get_wake_state(_ctx, static_cast<int>(k), _W[k]->_position->ptr(),
+                          _W[k]->_vorticity->ptr()) != PW_OK)
+      fail("pw_get_wake_state");
+}
+
+void __interaction::pullRates() {
+  for (size_t k = 0; k < _W.size(); ++k)
+    if (pw_get_wake_rates(_ctx, static_cast<int>(k), _W[k]->_velocity->ptr(),
+                          _W[k]->_retvorcity->ptr()) != PW_OK)
+      fail("pw_get_wake_rates");
+}
+
+void __interaction::interact() {
+  pushState();
+  if (pw_solve(_ctx, _precision) != PW_OK) fail("pw_solve");
+  pullRates();
+}
+
+void __interaction::solve() { interact(); }
+
+bool __interaction::stepOnDevice(int scheme, const double &dt, vec *state) {
+  setStates(state);
+  pushState();
+  if (pw_step(_ctx, scheme, dt, _precision, 1) != PW_OK) fail("pw_step");
+  pullState();
+  pullRates();
+  getStates(state);
+  return true;
+}
+
+void __interaction::diagnose() {
+  pushState();
+  double out[15];
+  if (pw_diagnose(_ctx, out, 15) != PW_OK) fail("pw_diagnose");
+  for (int k = 0; k < 3; ++k) {
+    _totalVorticity->set(k, out[k]);
+    _linearImpulse->set(k, out[3 + k]);
+    _angularImpulse->set(k, out[6 + k]);
+  }
+  _helicity = out[9];
+  _enstrophy = out[10];
+  _enstrophyF = out[11];
+  _kineticEnergy = out[12];
+  _kineticEnergyF = out[13];
+  _Zc = out[14];
+}
+
+void __interaction::write(FILE *f) {
+  std::fwrite(&_nWake, sizeof(size_t), 1, f);
+  // interaction.cpp:308-318 writes the int maxnumparticles with
+  // sizeof(size_t); readers take the first 4 bytes and skip 4 (readWake.py
+  // :94-96).  Written here as a zero-extended 8-byte value.
+  if (!_W.empty()) {
+    const int64_t maxn = static_cast<int64_t>(_W[0]->_maxnumParticles);
+    std::fwrite(&maxn, sizeof(int64_t), 1, f);
+  }
+  for (__wake *w : _W) w->write(f);
+}
+
+void __interaction::writenu(FILE *fdiag) { std::fwrite(&_nu, sizeof(double), 1, fdiag); }
+
+void __interaction::writediagnosis(FILE *fdiag) {
+  std::fwrite(_totalVorticity->ptr(), sizeof(double), 3, fdiag);
+  std::fwrite(_linearImpulse->ptr(), sizeof(double), 3, fdiag);
+  std::fwrite(_angularImpulse->ptr(), sizeof(double), 3, fdiag);
+  std::fwrite(&_helicity, sizeof(double), 1, fdiag);
+  std::fwrite(&_enstrophy, sizeof(double), 1, fdiag);
+  std::fwrite(&_enstrophyF, sizeof(double), 1, fdiag);
+  std::fwrite(&_kineticEnergy, sizeof(double), 1, fdiag);
+  std::fwrite(&_kineticEnergyF, sizeof(double), 1, fdiag);
+  std::fwrite(&_Zc, sizeof(double), 1, fdiag);
+}
+
+// state-vector views per wake (interaction.cpp:337-365)
+static vec subvector(const vec *v, size_t off, size_t len) {
+  vec s(len);
+  for (size_t i = 0; i < len; ++i) s.data[i] = v->data[off + i];
+  return s;
+}
+
+void __interaction::setStates(const vec *state) {
+  size_t offset = 0;
+  for (__wake *w : _W) {
+    vec st = subvector(state, offset, w->_maxsize);
+    w->setStates(&st);
+    offset += w->_maxsize;
+  }
+  _deviceStale = true;
+}
+
+void __interaction::getRates(vec *rate) {
+  size_t offset = 0;
+  for (__wake *w : _W) {
+    vec rt = subvector(rate, offset, w->_maxsize);
+    w->getRates(&rt);
+    for (size_t i = 0; i < w->_maxsize; ++i) rate->data[offset + i] = rt.data[i];
+    offset += w->_maxsize;
+  }
+}
+
+void __interaction::getStates(vec *state) {
+  size_t offset = 0;
+  for (__wake *w : _W) {
+    vec st = subvector(state, offset, w->_maxsize);
+    w->getStates(&st);
+    for (size_t i = 0; i < w->_maxsize; ++i) state->data[offset + i] = st.data[i];
+    offset += w->_maxsize;
+  }
+}
+
+void __interaction::getIdealRates(vec *rate) { rate->set_zero(); }
+
+// ---------------------------------------------------------------------------
+// __io (src/io/io.cpp)
+// ---------------------------------------------------------------------------
+
+static std::string data_root() {
+  const char *r = std::getenv("PAWAN_DATA_DIR");
+  std::string s = r ? r : ".";
+  if (!s.empty() && s.back() != '/') s += '/';
+  return s;
+}
+
+__io::__io() : _root(data_root()), _name("temp") {}
+__io::__io(std::string name) : _root(data_root()), _name(name + "Wake") {}
+void __io::print() { std::printf("%s\n%s\n", _root.c_str(), _name.c_str()); }
+std::string __io::getFile() { return _root + _name; }
+
+FILE *__io::create_binary_file(std::string suffix) {
+  std::string fn = _root + _name + suffix;
+  FILE *f = std::fopen(fn.c_str(), "wb");
+  if (!f) throw std::runtime_error("cannot create " + fn);
+  return f;
+}
+
+FILE *__io::open_binary_file(std::string suffix) {
+  std::string fn = _root + _name + suffix;
+  FILE *f = std::fopen(fn.c_str(), "rb");
+  if (!f) throw std::runtime_error("cannot open " + fn);
+  return f;
+}
+
+// ---------------------------------------------------------------------------
+// __integration, __rk4, __rk3 (src/integration/)
+// ---------------------------------------------------------------------------
+
+__integration::__integration(const double &t, const size_t &n) : _dt(t / n), _t(t), _n(n) {}
+__integration::__integration() : _dt(0.0), _t(0.0), _n(0) {}
+
+void __integration::integrate(__system *S, __io *IO, bool diagnose) {
+  vec states(S->_totalmaxsize);
+  FILE *f = IO->create_binary_file(".wake");
+  FILE *fdiag = IO->create_binary_file(".diagnosis");
+  double t = 0.0;
+  std::fwrite(&t, sizeof(double), 1, f);
+  S->write(f);
+  if (diagnose) {
+    S->writenu(fdiag);
+    std::fwrite(&t, sizeof(double), 1, fdiag);
+    S->diagnose();
+    S->writediagnosis(fdiag);
+  }
+  const auto t0 = std::chrono::steady_clock::now();
+  for (size_t i = 1; i <= _n; ++i) {
+    S->getStates(&states);
+    t = i * _dt;
+    step(_dt, S, &states);
+    std::fwrite(&t, sizeof(double), 1, f);
+    S->write(f);
+    if (diagnose) {
+      S->diagnose();
+      std::fwrite(&t, sizeof(double), 1, fdiag);
+      S->writediagnosis(fdiag);
+    }
+  }
+  std::fclose(f);
+  std::fclose(fdiag);
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  std::printf("Total Time (s) %g\n", secs);
+}
+
+// integration.cpp:126-136
+void __integration::step(const double &dt, __system *S, vec *states) {
+  if (S->stepOnDevice(PW_EULER, dt, states)) return;
+  vec rates(S->_totalmaxsize);
+  S->solve();
+  S->getRates(&rates);
+  for (size_t i = 0; i < rates.size; ++i) rates.data[i] *= dt;
+  for (size_t i = 0; i < states->size; ++i) states->data[i] += rates.data[i];
+  S->setStates(states);
+}
+
+__rk4::__rk4(const double &t, const size_t &n) : __integration(t, n) {}
+
+// rk4.cpp:12-77
+void __rk4::step(const double &dt, __system *S, vec *states) {
+  if (S->stepOnDevice(PW_RK4, dt, states)) return;
+  const size_t L = states->size;
+  vec x1(L), k1(L), k2(L), k3(L), k4(L);
+  double *y = states->ptr();
+  S->setStates(states);
+  S->solve();
+  S->getRates(&k1);
+  for (size_t i = 0; i < L; ++i) x1.data[i] = k1.data[i] * (0.5 * dt) + y[i];
+  S->setStates(&x1);
+  S->solve();
+  S->getRates(&k2);
+  for (size_t i = 0; i < L; ++i) x1.data[i] = k2.data[i] * (0.5 * dt) + y[i];
+  S->setStates(&x1);
+  S->solve();
+  S->getRates(&k3);
+  for (size_t i = 0; i < L; ++i) x1.data[i] = k3.data[i] * dt + y[i];
+  S->setStates(&x1);
+  S->solve();
+  S->getRates(&k4);
+  const double c6 = dt / 6., c3 = dt / 3.;
+  for (size_t i = 0; i < L; ++i)
+    y[i] = y[i] + ((k1.data[i] + k4.data[i]) * c6 + (k2.data[i] + k3.data[i]) * c3);
+  S->setStates(states);
+}
+
+__rk3::__rk3(const double &t, const size_t &n) : __integration(t, n) {}
+
+// rk3.cpp:12-65
+void __rk3::step(const double &dt, __system *S, vec *states) {
+  if (S->stepOnDevice(PW_RK3, dt, states)) return;
+  const size_t L = states->size;
+  vec x1(L), k1(L), k2(L), k3(L);
+  double *y = states->ptr();
+  S->setStates(states);
+  S->solve();
+  S->getRates(&k1);
+  for (size_t i = 0; i < L; ++i) x1.data[i] = k1.data[i] * (0.5 * dt) + y[i];
+  S->setStates(&x1);
+  S->solve();
+  S->getRates(&k2);
+  for (size_t i = 0; i < L; ++i) {
+    const double x2 = k2.data[i] * (2. * dt) + y[i];
+    k1.data[i] = k1.data[i] * dt;
+    x1.data[i] = x2 - k1.data[i];
+  }
+  S->setStates(&x1);
+  S->solve();
+  S->getRates(&k3);
+  const double c6 = dt / 6.0, s6 = 1.0 / 6.0, c23 = dt * 2.0 / 3.0;
+  for (size_t i = 0; i < L; ++i)
+    y[i] = y[i] + ((k1.data[i] * s6 + k3.data[i] * c6) + k2.data[i] * c23);
+  S->setStates(states);
+}
+
+}  // namespace pawan
