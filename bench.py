@@ -41,10 +41,17 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def time_path(system, precision: str, dt: float, steps: int, warmup: int):
+def time_path(system, precision: str, dt: float, steps: int, warmup: int, tune: bool = True):
     from pawan_amd import capi
 
     with capi.Solver(system) as g:
+        tuned = None
+        if tune:
+            t = time.perf_counter()
+            ms = g.autotune(precision)
+            tuned = dict(g.pair_config(precision), rhs_ms=ms)
+            log(f"{precision} autotune {time.perf_counter() - t:.1f} s -> {tuned}")
+        g.set_graph(True)
         for i in range(warmup):
             t = time.perf_counter()
             g.step("rk4", dt, precision, 1)
@@ -62,8 +69,7 @@ def time_path(system, precision: str, dt: float, steps: int, warmup: int):
         wall = time.perf_counter() - t0
         ms, launches = g.pair_timing()
         g.set_profiling(False)
-        diag = None
-    return wall, ms, launches
+    return wall, ms, launches, tuned
 
 
 def _spread(n: int, count: int) -> np.ndarray:
@@ -135,7 +141,10 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=1 << 20, help="particles (default 1,048,576)")
+    ap.add_argument("--n", type=int, default=1 << 20, help="helix particles (default 1,048,576)")
+    ap.add_argument("--workload", default="helix",
+                    choices=["helix", "batched_256", "leapfrog_64k", "ring_1k"])
+    ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -152,22 +161,32 @@ def main() -> int:
     if capi.device_count() < 1:
         log("no HIP device visible")
         return 1
-    dt = 1e-3
     t = time.perf_counter()
-    system = workloads.helix_wake(args.n)
+    if args.workload == "helix":
+        system, dt = workloads.helix_wake(args.n), 1e-3
+        wdesc = (f"helical rotor wake N={system.n} (4 tip helices + inboard sheet), "
+                 f"RK4 dt=1e-3, FP32-compensated pairs, FP64 state")
+    else:
+        spec = workloads.WORKLOADS[args.workload]
+        system, dt = spec["make"](), spec["dt"]
+        wdesc = (f"{args.workload}: {system.nw} {'coupled' if system.coupled else 'independent'} "
+                 f"wake(s), N={system.n}, RK4 dt={dt}, FP32-compensated pairs, FP64 state")
     log(f"generated {system.name}: N={system.n} in {time.perf_counter() - t:.2f} s")
     pairs = system.ordered_pairs()
 
-    wall32, ms32, l32 = time_path(system, "fp32c", dt, args.steps, args.warmup)
+    wall32, ms32, l32, tune32 = time_path(system, "fp32c", dt, args.steps, args.warmup,
+                                          not args.no_tune)
     sps32 = args.steps / wall32
     avg32 = ms32 / max(1, l32)
     log(f"fp32c: {sps32:.4f} steps/s, pair kernel avg {avg32:.1f} ms over {l32} launches")
 
     out64 = {}
     if not args.no_fp64 and args.fp64_steps > 0:
-        wall64, ms64, l64 = time_path(system, "fp64", dt, args.fp64_steps, 0)
+        wall64, ms64, l64, tune64 = time_path(system, "fp64", dt, args.fp64_steps, 0,
+                                              not args.no_tune)
         avg64 = ms64 / max(1, l64)
         out64 = {
+            "fp64_launch_config": tune64,
             "fp64_steps_per_s": args.fp64_steps / wall64,
             "fp64_pair_kernel_ms": avg64,
             "fp64_tflops": FLOP_PER_PAIR * pairs / (avg64 * 1e-3) / 1e12,
@@ -190,9 +209,9 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"helical rotor wake N={system.n} (4 tip helices + inboard sheet), "
-                        f"RK4 dt=1e-3, FP32-compensated pairs, FP64 state",
+            "workload": wdesc,
             "particles": system.n,
+            "wakes": system.nw,
             "ordered_pairs_per_rhs": pairs,
             "parallelism": "single GPU",
         },
@@ -204,7 +223,8 @@ def main() -> int:
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
             "traffic": None,
-            "kernel": "k_pair_fp32c",
+            "kernel": "k_pair_fp32c" + ("_w8" if tune32 and tune32.get("waves8") else ""),
+            "launch_config": tune32,
             "avg_launch_ms": avg32,
             "flop_per_launch": FLOP_PER_PAIR * pairs,
         },
@@ -213,7 +233,7 @@ def main() -> int:
     result.update(out64)
     if out64:
         result["fp32c_over_fp64"] = sps32 / out64["fp64_steps_per_s"]
-    if not args.no_cpu:
+    if not args.no_cpu and system.coupled:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(system, args.cpu_seconds)
         log(f"cpu baseline: {result['cpu_baseline']['value']:.3e} steps/s on "

@@ -157,6 +157,31 @@ int pw_synchronize(pw_ctx *ctx);
 int pw_set_profiling(pw_ctx *ctx, int enable);
 int pw_get_pair_timing(pw_ctx *ctx, double *total_ms, int64_t *launches);
 
+/* How the pair kernel of `precision` is launched.  parts: source ranges
+ * each 256-target block is split into (partial sums reduced per target in
+ * FP64, in a fixed order); persistent: resident blocks pull work items from
+ * an atomic queue (largest first) instead of one block per item; waves8:
+ * FP32C kernel built for 8 waves per SIMD instead of 7.  No counterpart in
+ * the reference (its OpenMP loops have no launch shape); defaults are
+ * parts 1, grid launch, 7 waves. */
+int pw_set_pair_config(pw_ctx *ctx, int precision, int parts, int persistent,
+                       int waves8);
+int pw_get_pair_config(pw_ctx *ctx, int precision, int *parts,
+                       int *persistent, int *waves8);
+
+/* Time the candidate launch configurations of `precision` on the
+ * context's current state (one right-hand side each) and keep the fastest;
+ * *best_ms (may be NULL) gets its time.  Results are cached per problem
+ * shape for the process, and in the text file named by $PAWAN_TUNE_CACHE
+ * when set.  Leaves the rates of the current state on the device, as
+ * pw_solve does. */
+int pw_autotune(pw_ctx *ctx, int precision, double *best_ms);
+
+/* enable = 1: pw_step captures one step as a HIP graph (per scheme,
+ * precision and dt) and replays it for every step, instead of launching
+ * each kernel.  Ignored while kernel profiling is on. */
+int pw_set_graph(pw_ctx *ctx, int enable);
+
 #ifdef __cplusplus
 }
 #endif

@@ -30,6 +30,7 @@ EXPORTS = [
     "pw_set_wake_state", "pw_get_wake_state", "pw_get_wake_rates", "pw_set_states",
     "pw_get_states", "pw_get_rates", "pw_solve", "pw_step", "pw_diagnose",
     "pw_synchronize", "pw_set_profiling", "pw_get_pair_timing",
+    "pw_set_pair_config", "pw_get_pair_config", "pw_autotune", "pw_set_graph",
 ]
 
 
@@ -79,6 +80,10 @@ def load(path: str | None = None) -> C.CDLL:
         "pw_synchronize": ([vp], i),
         "pw_set_profiling": ([vp, i], i),
         "pw_get_pair_timing": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)], i),
+        "pw_set_pair_config": ([vp, i, i, i, i], i),
+        "pw_get_pair_config": ([vp, i, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)], i),
+        "pw_autotune": ([vp, i, C.POINTER(C.c_double)], i),
+        "pw_set_graph": ([vp, i], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -227,7 +232,6 @@ class Solver:
                                  int(nsteps)))
 
     def diagnose(self) -> np.ndarray:
-        nsys = 1
         out = np.zeros(15 * max(1, len(self.wake_sizes)))
         _check(self._lib.pw_diagnose(self._h, out, len(out)))
         return out
@@ -237,6 +241,25 @@ class Solver:
 
     def set_profiling(self, on: bool) -> None:
         _check(self._lib.pw_set_profiling(self._h, int(bool(on))))
+
+    def set_pair_config(self, precision: str, parts: int = 1, persistent: bool = False,
+                        waves8: bool = False) -> None:
+        _check(self._lib.pw_set_pair_config(self._h, PRECISIONS[precision], int(parts),
+                                            int(bool(persistent)), int(bool(waves8))))
+
+    def pair_config(self, precision: str) -> dict:
+        p, q, w = C.c_int(), C.c_int(), C.c_int()
+        _check(self._lib.pw_get_pair_config(self._h, PRECISIONS[precision], C.byref(p), C.byref(q),
+                                            C.byref(w)))
+        return {"parts": p.value, "persistent": bool(q.value), "waves8": bool(w.value)}
+
+    def autotune(self, precision: str) -> float:
+        ms = C.c_double()
+        _check(self._lib.pw_autotune(self._h, PRECISIONS[precision], C.byref(ms)))
+        return ms.value
+
+    def set_graph(self, on: bool) -> None:
+        _check(self._lib.pw_set_graph(self._h, int(bool(on))))
 
     def pair_timing(self):
         ms, n = C.c_double(), C.c_int64()

@@ -5,8 +5,11 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
 #include <new>
+#include <tuple>
 #include <string>
 #include <utility>
 #include <vector>
@@ -32,6 +35,19 @@ int fail(int code, const std::string& msg) {
     hipError_t e_ = (call);                                                 \
     if (e_ != hipSuccess)                                                   \
       return fail(PW_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define PW_CTX(c)                                                   \
+  do {                                                              \
+    if (!(c)) return fail(PW_ERR_ARG, "null context");              \
+    if (hipSetDevice((c)->device) != hipSuccess)                    \
+      return fail(PW_ERR_HIP, "hipSetDevice failed");               \
+  } while (0)
+
+#define PW_TRY(expr)                     \
+  do {                                   \
+    int rc_ = (expr);                    \
+    if (rc_ != PW_OK) return rc_;        \
   } while (0)
 
 constexpr int kTargetsPerItem = 256;
@@ -74,6 +90,28 @@ struct pw_ctx {
   double* d_quad = nullptr;
   int* d_flag = nullptr;
 
+  // per pair-kernel kind: launch configuration and its work items
+  struct KindState {
+    pw::PairConfig cfg;
+    WorkItem* d_items = nullptr;
+    int nitems = 0;
+    int built_parts = 0;  // parts the items were built for (0: none yet)
+    bool tuned = false;
+  } ks[3];
+  Rate64* partial = nullptr;  // source-split partial rates, parts x n
+  int partial_parts = 0;
+  int* d_queue = nullptr;     // work counter of the persistent kernels
+  int num_cus = 256;
+  struct SysRange {
+    int64_t a, b;
+    int id;
+  };
+  std::vector<SysRange> systems;  // largest first
+
+  // HIP graphs of one step, per (scheme, kind, dt)
+  bool use_graph = false;
+  std::map<std::tuple<int, int, uint64_t>, hipGraphExec_t> graphs;
+
   std::vector<double> zc;  // persistent centroid per system
 
   // profiling
@@ -97,6 +135,11 @@ void release(pw_ctx* c) {
   (void)hipFree(c->d_lin);
   (void)hipFree(c->d_quad);
   (void)hipFree(c->d_flag);
+  for (auto& k : c->ks) (void)hipFree(k.d_items);
+  (void)hipFree(c->partial);
+  (void)hipFree(c->d_queue);
+  for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.second);
+  c->graphs.clear();
   for (auto& e : c->ev_used) {
     (void)hipEventDestroy(e.first);
     (void)hipEventDestroy(e.second);
@@ -129,18 +172,21 @@ void build_items(pw_ctx* c) {
     for (int w = 0; w < nw; ++w) sys.push_back({c->woff[w], c->woff[w + 1], w});
   std::stable_sort(sys.begin(), sys.end(),
                    [](const Sys& p, const Sys& q) { return (p.b - p.a) > (q.b - q.a); });
+  c->systems.clear();
+  for (const Sys& s : sys) c->systems.push_back({s.a, s.b, s.id});
   for (const Sys& s : sys)
     for (int64_t t0 = s.a; t0 < s.b; t0 += kTargetsPerItem) {
       c->h_pair_items.push_back({static_cast<int32_t>(t0),
                                  static_cast<int32_t>(std::min<int64_t>(t0 + kTargetsPerItem, s.b)),
-                                 static_cast<int32_t>(s.a), static_cast<int32_t>(s.b)});
+                                 static_cast<int32_t>(s.a), static_cast<int32_t>(s.b), -1, 0, 0, 0});
       c->pair_sys.push_back(s.id);
     }
   for (int w = 0; w < nw; ++w)
     for (int64_t t0 = c->woff[w]; t0 < c->woff[w + 1]; t0 += kTargetsPerItem) {
       c->h_lin_items.push_back(
           {static_cast<int32_t>(t0),
-           static_cast<int32_t>(std::min<int64_t>(t0 + kTargetsPerItem, c->woff[w + 1])), w, 0});
+           static_cast<int32_t>(std::min<int64_t>(t0 + kTargetsPerItem, c->woff[w + 1])), w, 0,
+           0, 0, 0, 0});
       c->lin_wake.push_back(w);
     }
   c->npair = static_cast<int>(c->h_pair_items.size());
@@ -162,7 +208,7 @@ int ensure_built(pw_ctx* c) {
       dalloc(&c->lin_items, c->h_lin_items.size()) != hipSuccess ||
       dalloc(&c->d_lin, 11 * c->h_lin_items.size()) != hipSuccess ||
       dalloc(&c->d_quad, 3 * c->h_pair_items.size()) != hipSuccess ||
-      dalloc(&c->d_flag, 1) != hipSuccess) {
+      dalloc(&c->d_flag, 1) != hipSuccess || dalloc(&c->d_queue, 1) != hipSuccess) {
     release(c);
     c->y = c->ys = nullptr;
     for (auto& p : c->k) p = nullptr;
@@ -192,6 +238,11 @@ int ensure_built(pw_ctx* c) {
   PW_HIP(hipStreamSynchronize(c->stream));
   c->rates = c->k[0];
   c->zc.assign(c->nsys(), 0.0);
+  {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess && prop.multiProcessorCount > 0)
+      c->num_cus = prop.multiProcessorCount;
+  }
   c->pending.clear();
   c->pending.shrink_to_fit();
   c->built = true;
@@ -210,8 +261,73 @@ int kind_of(int precision, int* kind) {
   }
 }
 
+// Work items for a launch split into `parts` source ranges per target
+// block (parts == 1: one item per block writing its rates directly).
+// Split points fall on tile boundaries of the block's system, so the
+// diagonal tile (and its self pairs) stays inside one part.  Items are
+// ordered by cost, largest first.
+std::vector<WorkItem> make_items(const pw_ctx* c, int parts) {
+  std::vector<WorkItem> items;
+  for (const auto& s : c->systems) {
+    const int64_t ntile = (s.b - s.a + kTargetsPerItem - 1) / kTargetsPerItem;
+    for (int64_t t0 = s.a; t0 < s.b; t0 += kTargetsPerItem) {
+      const int32_t t1 = static_cast<int32_t>(std::min<int64_t>(t0 + kTargetsPerItem, s.b));
+      for (int p = 0; p < parts; ++p) {
+        const int64_t a = s.a + kTargetsPerItem * (ntile * p / parts);
+        const int64_t b = std::min<int64_t>(s.b, s.a + kTargetsPerItem * (ntile * (p + 1) / parts));
+        items.push_back({static_cast<int32_t>(t0), t1, static_cast<int32_t>(a),
+                         static_cast<int32_t>(std::max(a, b)), parts == 1 ? -1 : p, 0, 0, 0});
+      }
+    }
+  }
+  std::stable_sort(items.begin(), items.end(), [](const WorkItem& x, const WorkItem& y) {
+    return int64_t(x.s1 - x.s0) * (x.t1 - x.t0) > int64_t(y.s1 - y.s0) * (y.t1 - y.t0);
+  });
+  return items;
+}
+
+void drop_graphs(pw_ctx* c) {
+  for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.second);
+  c->graphs.clear();
+}
+
+int ensure_kind(pw_ctx* c, int kind) {
+  auto& k = c->ks[kind];
+  if (k.built_parts == k.cfg.parts && k.d_items) return PW_OK;
+  std::vector<WorkItem> items = make_items(c, k.cfg.parts);
+  PW_HIP(hipStreamSynchronize(c->stream));
+  (void)hipFree(k.d_items);
+  k.d_items = nullptr;
+  if (dalloc(&k.d_items, items.size()) != hipSuccess) return fail(PW_ERR_NOMEM, "work items");
+  PW_HIP(hipMemcpy(k.d_items, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
+  k.nitems = static_cast<int>(items.size());
+  k.built_parts = k.cfg.parts;
+  if (k.cfg.parts > 1 && c->partial_parts < k.cfg.parts) {
+    (void)hipFree(c->partial);
+    c->partial = nullptr;
+    c->partial_parts = 0;
+    if (dalloc(&c->partial, static_cast<size_t>(k.cfg.parts) * c->n) != hipSuccess)
+      return fail(PW_ERR_NOMEM, "partial-sum buffer");
+    c->partial_parts = k.cfg.parts;
+  }
+  drop_graphs(c);
+  return PW_OK;
+}
+
 // One right-hand side f(state) -> out, with optional event timing.
 int rhs(pw_ctx* c, int kind, const Rec64* state, Rate64* out) {
+  PW_TRY(ensure_kind(c, kind));
+  const auto& k = c->ks[kind];
+  pw::PairArgs args;
+  args.state = state;
+  args.items = k.d_items;
+  args.nitems = k.nitems;
+  args.n = static_cast<int>(c->n);
+  args.rates = out;
+  args.partial = c->partial;
+  args.queue = c->d_queue;
+  args.nu = c->nu;
+  const int grid = c->num_cus * pw::pair_kernel_waves(kind, k.cfg);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c->profiling) {
     if (c->ev_free.empty()) {
@@ -224,7 +340,7 @@ int rhs(pw_ctx* c, int kind, const Rec64* state, Rate64* out) {
     }
     PW_HIP(hipEventRecord(e0, c->stream));
   }
-  PW_HIP(pw::launch_pair(kind, state, c->pair_items, c->npair, out, c->nu, c->stream));
+  PW_HIP(pw::launch_pair(kind, k.cfg, args, grid, c->stream));
   if (c->profiling) {
     PW_HIP(hipEventRecord(e1, c->stream));
     c->ev_used.emplace_back(e0, e1);
@@ -254,19 +370,6 @@ int collect_profile(pw_ctx* c) {
   c->ev_used.clear();
   return PW_OK;
 }
-
-#define PW_CTX(c)                                                   \
-  do {                                                              \
-    if (!(c)) return fail(PW_ERR_ARG, "null context");              \
-    if (hipSetDevice((c)->device) != hipSuccess)                    \
-      return fail(PW_ERR_HIP, "hipSetDevice failed");               \
-  } while (0)
-
-#define PW_TRY(expr)                     \
-  do {                                   \
-    int rc_ = (expr);                    \
-    if (rc_ != PW_OK) return rc_;        \
-  } while (0)
 
 int64_t total_maxsize(const pw_ctx* c) {
   int64_t s = 0;
@@ -528,6 +631,81 @@ int pw_solve(pw_ctx* c, int precision) {
   return check_flag(c, "pw_solve");
 }
 
+}  // extern "C"
+
+namespace {
+
+// The kernels of one step of `scheme` on the context stream.
+int enqueue_step(pw_ctx* c, int scheme, int kind, double dt) {
+  const int n = static_cast<int>(c->n);
+  hipStream_t st = c->stream;
+  Rate64** k = c->k;
+  if (scheme == PW_RK4) {
+    // rk4.cpp:20-66
+    PW_TRY(rhs(c, kind, c->y, k[0]));
+    PW_HIP(pw::launch_stage(c->y, k[0], 0.5 * dt, c->ys, n, st));
+    PW_TRY(rhs(c, kind, c->ys, k[1]));
+    PW_HIP(pw::launch_stage(c->y, k[1], 0.5 * dt, c->ys, n, st));
+    PW_TRY(rhs(c, kind, c->ys, k[2]));
+    PW_HIP(pw::launch_stage(c->y, k[2], dt, c->ys, n, st));
+    PW_TRY(rhs(c, kind, c->ys, k[3]));
+    PW_HIP(pw::launch_rk4_final(c->y, k[0], k[1], k[2], k[3], dt / 6., dt / 3., n, st));
+  } else if (scheme == PW_RK3) {
+    // rk3.cpp:20-55
+    PW_TRY(rhs(c, kind, c->y, k[0]));
+    PW_HIP(pw::launch_stage(c->y, k[0], 0.5 * dt, c->ys, n, st));
+    PW_TRY(rhs(c, kind, c->ys, k[1]));
+    PW_HIP(pw::launch_rk3_mid(c->y, k[0], k[1], dt, 2. * dt, c->ys, n, st));
+    PW_TRY(rhs(c, kind, c->ys, k[2]));
+    PW_HIP(pw::launch_rk3_final(c->y, k[0], k[1], k[2], dt / 6.0, dt * 2.0 / 3.0, n, st));
+  } else {
+    // integration.cpp:126-136
+    PW_TRY(rhs(c, kind, c->y, k[0]));
+    PW_HIP(pw::launch_euler(c->y, k[0], dt, n, st));
+  }
+  return PW_OK;
+}
+
+Rate64* last_rates(pw_ctx* c, int scheme) {
+  return scheme == PW_RK4 ? c->k[3] : scheme == PW_RK3 ? c->k[2] : c->k[0];
+}
+
+// One step captured as a HIP graph (per scheme, kind and dt), replayed for
+// every step: one launch instead of up to 13.
+int step_graph(pw_ctx* c, int scheme, int kind, double dt, hipGraphExec_t* out) {
+  uint64_t bits;
+  std::memcpy(&bits, &dt, sizeof bits);
+  const auto key = std::make_tuple(scheme, kind, bits);
+  auto it = c->graphs.find(key);
+  if (it != c->graphs.end()) {
+    *out = it->second;
+    return PW_OK;
+  }
+  PW_TRY(ensure_kind(c, kind));  // allocations happen outside the capture
+  hipGraph_t g = nullptr;
+  PW_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = enqueue_step(c, scheme, kind, dt);
+  hipGraph_t captured = nullptr;
+  const hipError_t ec = hipStreamEndCapture(c->stream, &captured);
+  if (rc != PW_OK) {
+    if (captured) (void)hipGraphDestroy(captured);
+    return rc;
+  }
+  if (ec != hipSuccess) return fail(PW_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+  g = captured;
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess) return fail(PW_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+  c->graphs[key] = exec;
+  *out = exec;
+  return PW_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int pw_step(pw_ctx* c, int scheme, double dt, int precision, int64_t nsteps) {
   PW_CTX(c);
   int kind = 0;
@@ -537,38 +715,16 @@ int pw_step(pw_ctx* c, int scheme, double dt, int precision, int64_t nsteps) {
   if (!std::isfinite(dt)) return fail(PW_ERR_ARG, "dt must be finite");
   if (nsteps < 0) return fail(PW_ERR_ARG, "nsteps must be >= 0");
   PW_TRY(ensure_built(c));
-  const int n = static_cast<int>(c->n);
-  hipStream_t st = c->stream;
-  Rate64 **k = c->k;
-  PW_HIP(pw::launch_fill_int(c->d_flag, 1, 0, st));
-  for (int64_t it = 0; it < nsteps; ++it) {
-    if (scheme == PW_RK4) {
-      // rk4.cpp:20-66
-      PW_TRY(rhs(c, kind, c->y, k[0]));
-      PW_HIP(pw::launch_stage(c->y, k[0], 0.5 * dt, c->ys, n, st));
-      PW_TRY(rhs(c, kind, c->ys, k[1]));
-      PW_HIP(pw::launch_stage(c->y, k[1], 0.5 * dt, c->ys, n, st));
-      PW_TRY(rhs(c, kind, c->ys, k[2]));
-      PW_HIP(pw::launch_stage(c->y, k[2], dt, c->ys, n, st));
-      PW_TRY(rhs(c, kind, c->ys, k[3]));
-      PW_HIP(pw::launch_rk4_final(c->y, k[0], k[1], k[2], k[3], dt / 6., dt / 3., n, st));
-      c->rates = k[3];
-    } else if (scheme == PW_RK3) {
-      // rk3.cpp:20-55
-      PW_TRY(rhs(c, kind, c->y, k[0]));
-      PW_HIP(pw::launch_stage(c->y, k[0], 0.5 * dt, c->ys, n, st));
-      PW_TRY(rhs(c, kind, c->ys, k[1]));
-      PW_HIP(pw::launch_rk3_mid(c->y, k[0], k[1], dt, 2. * dt, c->ys, n, st));
-      PW_TRY(rhs(c, kind, c->ys, k[2]));
-      PW_HIP(pw::launch_rk3_final(c->y, k[0], k[1], k[2], dt / 6.0, dt * 2.0 / 3.0, n, st));
-      c->rates = k[2];
-    } else {
-      // integration.cpp:126-136
-      PW_TRY(rhs(c, kind, c->y, k[0]));
-      PW_HIP(pw::launch_euler(c->y, k[0], dt, n, st));
-      c->rates = k[0];
-    }
+  PW_TRY(ensure_kind(c, kind));
+  PW_HIP(pw::launch_fill_int(c->d_flag, 1, 0, c->stream));
+  if (c->use_graph && !c->profiling && nsteps > 0) {
+    hipGraphExec_t exec = nullptr;
+    PW_TRY(step_graph(c, scheme, kind, dt, &exec));
+    for (int64_t it = 0; it < nsteps; ++it) PW_HIP(hipGraphLaunch(exec, c->stream));
+  } else {
+    for (int64_t it = 0; it < nsteps; ++it) PW_TRY(enqueue_step(c, scheme, kind, dt));
   }
+  if (nsteps > 0) c->rates = last_rates(c, scheme);
   return check_flag(c, "pw_step");
 }
 
@@ -642,6 +798,193 @@ int pw_get_pair_timing(pw_ctx* c, double* total_ms, int64_t* launches) {
   PW_TRY(collect_profile(c));
   if (total_ms) *total_ms = c->prof_ms;
   if (launches) *launches = c->prof_launches;
+  return PW_OK;
+}
+
+int pw_set_pair_config(pw_ctx* c, int precision, int parts, int persistent, int waves8) {
+  PW_CTX(c);
+  int kind = 0;
+  PW_TRY(kind_of(precision, &kind));
+  if (parts < 1 || parts > 256) return fail(PW_ERR_ARG, "parts must be in [1, 256]");
+  auto& k = c->ks[kind];
+  k.cfg.parts = parts;
+  k.cfg.persistent = persistent != 0;
+  k.cfg.waves8 = waves8 != 0 && kind == pw::kPairFP32C;
+  k.tuned = true;
+  drop_graphs(c);
+  return PW_OK;
+}
+
+int pw_get_pair_config(pw_ctx* c, int precision, int* parts, int* persistent, int* waves8) {
+  PW_CTX(c);
+  int kind = 0;
+  PW_TRY(kind_of(precision, &kind));
+  const auto& k = c->ks[kind];
+  if (parts) *parts = k.cfg.parts;
+  if (persistent) *persistent = k.cfg.persistent ? 1 : 0;
+  if (waves8) *waves8 = k.cfg.waves8 ? 1 : 0;
+  return PW_OK;
+}
+
+int pw_set_graph(pw_ctx* c, int enable) {
+  PW_CTX(c);
+  c->use_graph = enable != 0;
+  if (!c->use_graph) drop_graphs(c);
+  return PW_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Tuning results shared by every context of the process, keyed by the
+// problem's shape; optionally also appended to / read from the text file
+// named by $PAWAN_TUNE_CACHE ("kind n nsys largest parts persistent waves8").
+using TuneKey = std::tuple<int, int64_t, int, int64_t>;
+std::map<TuneKey, pw::PairConfig>& tune_cache() {
+  static std::map<TuneKey, pw::PairConfig> m;
+  return m;
+}
+
+void load_tune_file() {
+  static bool loaded = false;
+  if (loaded) return;
+  loaded = true;
+  const char* path = std::getenv("PAWAN_TUNE_CACHE");
+  if (!path) return;
+  FILE* f = std::fopen(path, "r");
+  if (!f) return;
+  int kind, nsys, parts, pers, w8;
+  long long n, largest;
+  while (std::fscanf(f, "%d %lld %d %lld %d %d %d", &kind, &n, &nsys, &largest, &parts, &pers, &w8) == 7) {
+    pw::PairConfig cfg;
+    cfg.parts = parts;
+    cfg.persistent = pers != 0;
+    cfg.waves8 = w8 != 0;
+    tune_cache()[TuneKey(kind, n, nsys, largest)] = cfg;
+  }
+  std::fclose(f);
+}
+
+void save_tune_file(const TuneKey& key, const pw::PairConfig& cfg) {
+  const char* path = std::getenv("PAWAN_TUNE_CACHE");
+  if (!path) return;
+  FILE* f = std::fopen(path, "a");
+  if (!f) return;
+  std::fprintf(f, "%d %lld %d %lld %d %d %d\n", std::get<0>(key), static_cast<long long>(std::get<1>(key)),
+               std::get<2>(key), static_cast<long long>(std::get<3>(key)), cfg.parts,
+               cfg.persistent ? 1 : 0, cfg.waves8 ? 1 : 0);
+  std::fclose(f);
+}
+
+int time_rhs(pw_ctx* c, int kind, double* ms) {
+  hipEvent_t e0, e1;
+  PW_HIP(hipEventCreate(&e0));
+  PW_HIP(hipEventCreate(&e1));
+  float best = 1e30f;
+  for (int rep = 0; rep < 4; ++rep) {
+    PW_HIP(hipEventRecord(e0, c->stream));
+    const int rc = rhs(c, kind, c->y, c->k[0]);
+    if (rc != PW_OK) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      return rc;
+    }
+    PW_HIP(hipEventRecord(e1, c->stream));
+    PW_HIP(hipEventSynchronize(e1));
+    float t = 0.0f;
+    PW_HIP(hipEventElapsedTime(&t, e0, e1));
+    best = std::min(best, t);
+    if (t > 50.0f) break;  // long enough that the first-launch cost is noise
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = best;
+  return PW_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pw_autotune(pw_ctx* c, int precision, double* best_ms) {
+  PW_CTX(c);
+  int kind = 0;
+  PW_TRY(kind_of(precision, &kind));
+  PW_TRY(ensure_built(c));
+  load_tune_file();
+  const int64_t largest = c->systems.empty() ? 0 : c->systems.front().b - c->systems.front().a;
+  const TuneKey key(kind, c->n, c->nsys(), largest);
+  auto& k = c->ks[kind];
+  const bool was_profiling = c->profiling;
+  c->profiling = false;
+  auto hit = tune_cache().find(key);
+  if (hit != tune_cache().end()) {
+    k.cfg = hit->second;
+    k.tuned = true;
+    drop_graphs(c);
+    double ms = 0.0;
+    const int rc = best_ms ? time_rhs(c, kind, &ms) : PW_OK;
+    c->profiling = was_profiling;
+    if (best_ms) *best_ms = ms;
+    c->rates = c->k[0];
+    return rc;
+  }
+  // candidates: split the sources (fills the GPU for few target blocks and
+  // cuts the tail of the last block wave), keep blocks resident, and for
+  // FP32C the 8-wave register budget
+  int64_t blocks = 0, max_tiles = 1;
+  for (const auto& sr : c->systems) {
+    const int64_t nt = (sr.b - sr.a + kTargetsPerItem - 1) / kTargetsPerItem;
+    blocks += nt;
+    max_tiles = std::max(max_tiles, nt);
+  }
+  std::vector<pw::PairConfig> cands;
+  auto add = [&](int parts, bool pers, bool w8) {
+    if (parts > max_tiles) return;
+    pw::PairConfig p;
+    p.parts = parts;
+    p.persistent = pers;
+    p.waves8 = w8;
+    cands.push_back(p);
+  };
+  const bool f32 = kind == pw::kPairFP32C;
+  add(1, false, false);
+  add(1, true, false);
+  add(2, false, false);
+  add(4, true, false);
+  add(8, true, false);
+  if (f32) {
+    add(1, false, true);
+    add(4, true, true);
+    add(8, true, true);
+  }
+  const int64_t resident = static_cast<int64_t>(c->num_cus) * 8;
+  for (int parts = 16; parts <= 64; parts *= 2)
+    if (blocks * parts <= 8 * resident) add(parts, true, f32);
+  pw::PairConfig best_cfg = cands.front();
+  double best = 1e300;
+  for (const auto& cand : cands) {
+    k.cfg = cand;
+    double ms = 0.0;
+    const int rc = time_rhs(c, kind, &ms);
+    if (rc != PW_OK) {
+      c->profiling = was_profiling;
+      return rc;
+    }
+    if (ms < best) {
+      best = ms;
+      best_cfg = cand;
+    }
+  }
+  k.cfg = best_cfg;
+  k.tuned = true;
+  drop_graphs(c);
+  tune_cache()[key] = best_cfg;
+  save_tune_file(key, best_cfg);
+  c->profiling = was_profiling;
+  c->rates = c->k[0];
+  if (best_ms) *best_ms = best;
   return PW_OK;
 }
 

@@ -23,9 +23,33 @@ struct alignas(16) Rate64 {
 };
 
 // One block of work for the pair kernels: targets [t0, t1) against
-// sources [s0, s1).  t1 - t0 is at most the block's target count.
+// sources [s0, s1).  t1 - t0 is at most the block's target count.  part < 0:
+// the item covers all of its targets' sources and writes their rates; part
+// >= 0: it covers one of several source ranges and writes a partial sum to
+// slice `part` of the partial buffer.  (The diagnostics reuse the
+// structure with {first, last, wake, -}.)
 struct WorkItem {
   int32_t t0, t1, s0, s1;
+  int32_t part, pad0, pad1, pad2;
+};
+
+// Arguments of a pair-kernel launch (passed by value).
+struct PairArgs {
+  const Rec64* state;
+  const WorkItem* items;
+  int nitems;
+  int n;              // particles (stride of the partial buffer)
+  Rate64* rates;
+  Rate64* partial;    // parts x n, used by items with part >= 0
+  int* queue;         // work counter of the persistent kernels
+  double nu;
+};
+
+// How a right-hand side is launched (chosen by the autotuner).
+struct PairConfig {
+  int parts = 1;           // source ranges per target block
+  bool persistent = false; // resident blocks pulling items from a queue
+  bool waves8 = false;     // FP32C: register budget of 8 waves/SIMD
 };
 
 // Pair-kernel variants.
@@ -36,8 +60,13 @@ enum PairKind : int {
   kPairFP32C = 2,      // FP32 pair math, compensated FP32 accumulation
 };
 
-hipError_t launch_pair(int kind, const Rec64* state, const WorkItem* items,
-                       int nitems, Rate64* rates, double nu, hipStream_t st);
+// Pair kernel of `kind` over args.items; for parts > 1 also the reduction
+// of the partial sums into args.rates.  grid = resident blocks for the
+// persistent form.
+hipError_t launch_pair(int kind, const PairConfig& cfg, const PairArgs& args, int grid,
+                       hipStream_t st);
+// waves per SIMD the pair kernel is built for (for the persistent grid)
+int pair_kernel_waves(int kind, const PairConfig& cfg);
 
 // Stage arithmetic of the steppers (FP64, no contraction, the reference's
 // operation order).  n = number of particles.

@@ -29,6 +29,8 @@
 // take the full kernel.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "pw_internal.h"
 
 namespace pw {
@@ -94,11 +96,15 @@ __device__ __forceinline__ void pair64_exact(const Rec64& t, const Rec64& s,
   a.wz += (q * cz + fr * dz) + coef * (t.vol * s.az - s.vol * t.az);
 }
 
-__global__ __launch_bounds__(kTB) void k_pair_fp64_ref(
-    const Rec64* __restrict__ state, const WorkItem* __restrict__ items,
-    Rate64* __restrict__ rates, double nu) {
-  __shared__ Rec64 tile[kTB];
-  const WorkItem it = items[blockIdx.x];
+// Where an item's rates go: straight to the rates, or to its part's slice
+// of the partial-sum buffer (source-split items, summed by k_reduce_parts).
+__device__ __forceinline__ Rate64* out_rates(const PairArgs& a, const WorkItem& it) {
+  return it.part < 0 ? a.rates : a.partial + static_cast<size_t>(it.part) * a.n;
+}
+
+__device__ __forceinline__ void item_fp64_ref(const PairArgs& pa, const WorkItem& it,
+                                              Rec64* tile) {
+  const Rec64* __restrict__ state = pa.state;
   const int t = it.t0 + static_cast<int>(threadIdx.x);
   const bool active = t < it.t1;
   const Rec64 me = state[active ? t : it.t0];
@@ -111,7 +117,7 @@ __global__ __launch_bounds__(kTB) void k_pair_fp64_ref(
     if (active) {
       for (int l = 0; l < cnt; ++l) {
         if (base + l == t) continue;  // self pair: never evaluated
-        pair64_exact(me, tile[l], nu, a);
+        pair64_exact(me, tile[l], pa.nu, a);
       }
     }
   }
@@ -119,7 +125,7 @@ __global__ __launch_bounds__(kTB) void k_pair_fp64_ref(
     Rate64 r;
     r.ux = a.ux; r.uy = a.uy; r.uz = a.uz;
     r.wx = a.wx; r.wy = a.wy; r.wz = a.wz;
-    rates[t] = r;
+    out_rates(pa, it)[t] = r;
   }
 }
 
@@ -165,18 +171,15 @@ __device__ __forceinline__ void pair64(const T64& t, const Rec64& s, Acc64& a) {
   a.wz += q * cz + fr * dz;
 }
 
-__global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_pair_fp64(
-    const Rec64* __restrict__ state, const WorkItem* __restrict__ items,
-    Rate64* __restrict__ rates, double nu) {
-  __shared__ Rec64 tile[kTB];
-  const WorkItem it = items[blockIdx.x];
+__device__ __forceinline__ void item_fp64(const PairArgs& pa, const WorkItem& it, Rec64* tile) {
+  const Rec64* __restrict__ state = pa.state;
   const int t = it.t0 + static_cast<int>(threadIdx.x);
   const bool active = t < it.t1;
   const Rec64 m = state[active ? t : it.t0];
   T64 me;
   me.x = m.x; me.y = m.y; me.z = m.z; me.hs2 = 0.5 * m.s2;
   me.ax = m.ax; me.ay = m.ay; me.az = m.az; me.vol = m.vol;
-  me.nu2 = 2.0 * nu;
+  me.nu2 = 2.0 * pa.nu;
   Acc64 a;
   for (int base = it.s0; base < it.s1; base += kTB) {
     const int cnt = min(kTB, it.s1 - base);
@@ -197,7 +200,7 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     Rate64 r;
     r.ux = a.ux; r.uy = a.uy; r.uz = a.uz;
     r.wx = a.wx; r.wy = a.wy; r.wz = a.wz;
-    rates[t] = r;
+    out_rates(pa, it)[t] = r;
   }
 }
 
@@ -325,11 +328,9 @@ __device__ __forceinline__ void two_sum(float& hi, float& lo, float p) {
   lo += err;
 }
 
-__global__ __launch_bounds__(kTB) void k_pair_fp32c(
-    const Rec64* __restrict__ state, const WorkItem* __restrict__ items,
-    Rate64* __restrict__ rates, float nu) {
-  __shared__ SrcT tile[kTB];
-  const WorkItem it = items[blockIdx.x];
+__device__ __forceinline__ void item_fp32c(const PairArgs& pa, const WorkItem& it, SrcT* tile) {
+  const Rec64* __restrict__ state = pa.state;
+  const float nu = static_cast<float>(pa.nu);
   const int t = it.t0 + static_cast<int>(threadIdx.x);
   const bool active = t < it.t1;
   const Rec64 m = state[active ? t : it.t0];
@@ -393,31 +394,126 @@ __global__ __launch_bounds__(kTB) void k_pair_fp32c(
     r.wx = kQ * acc[3] + static_cast<double>(pse.x);
     r.wy = kQ * acc[4] + static_cast<double>(pse.y);
     r.wz = kQ * acc[5] + static_cast<double>(pse.z);
-    rates[t] = r;
+    out_rates(pa, it)[t] = r;
   }
+}
+
+// ---------------------------------------------------------------------
+// Entry kernels.  A grid kernel runs item blockIdx.x; a persistent kernel
+// keeps its blocks resident and pulls items from an atomic queue (items
+// are ordered largest first), so uneven items leave no idle tail.
+// ---------------------------------------------------------------------
+template <int Kind>
+struct TileOf {
+  using type = Rec64;
+};
+template <>
+struct TileOf<kPairFP32C> {
+  using type = SrcT;
+};
+
+template <int Kind>
+__device__ __forceinline__ void run_item(const PairArgs& a, const WorkItem& it,
+                                         typename TileOf<Kind>::type* tile) {
+  if constexpr (Kind == kPairFP64Exact)
+    item_fp64_ref(a, it, tile);
+  else if constexpr (Kind == kPairFP64)
+    item_fp64(a, it, tile);
+  else
+    item_fp32c(a, it, tile);
+}
+
+template <int Kind, bool Persistent>
+__device__ __forceinline__ void pair_entry(const PairArgs& a) {
+  __shared__ typename TileOf<Kind>::type tile[kTB];
+  if (!Persistent) {
+    run_item<Kind>(a, a.items[blockIdx.x], tile);
+    return;
+  }
+  __shared__ int s_idx;
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) s_idx = atomicAdd(a.queue, 1);
+    __syncthreads();
+    const int idx = s_idx;
+    if (idx >= a.nitems) break;  // every lane sees the same idx: uniform exit
+    run_item<Kind>(a, a.items[idx], tile);
+  }
+}
+
+template <bool P>
+__global__ __launch_bounds__(kTB) void k_pair_fp64_ref(PairArgs a) { pair_entry<kPairFP64Exact, P>(a); }
+template <bool P>
+__global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_pair_fp64(PairArgs a) {
+  pair_entry<kPairFP64, P>(a);
+}
+// FP32C at the register budget of 7 waves/SIMD (no spills) and of 8
+// waves/SIMD (one spill slot, touched once per tile)
+template <bool P>
+__global__ __launch_bounds__(kTB) void k_pair_fp32c(PairArgs a) { pair_entry<kPairFP32C, P>(a); }
+template <bool P>
+__global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_pair_fp32c_w8(PairArgs a) {
+  pair_entry<kPairFP32C, P>(a);
+}
+
+// rates[t] = sum over parts of partial[p][t], parts in order (FP64).
+__global__ __launch_bounds__(256) void k_reduce_parts(const Rate64* __restrict__ partial,
+                                                      int parts, int n, Rate64* __restrict__ rates) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  Rate64 s = partial[t];
+  for (int p = 1; p < parts; ++p) {
+    const Rate64 q = partial[static_cast<size_t>(p) * n + t];
+    s.ux += q.ux; s.uy += q.uy; s.uz += q.uz;
+    s.wx += q.wx; s.wy += q.wy; s.wz += q.wz;
+  }
+  rates[t] = s;
 }
 
 }  // namespace
 
-hipError_t launch_pair(int kind, const Rec64* state, const WorkItem* items,
-                       int nitems, Rate64* rates, double nu, hipStream_t st) {
-  if (nitems <= 0) return hipSuccess;
-  const dim3 grid(nitems), block(kTB);
+hipError_t launch_pair(int kind, const PairConfig& cfg, const PairArgs& args, int grid,
+                       hipStream_t st) {
+  if (args.nitems <= 0) return hipSuccess;
+  const dim3 block(kTB);
+  const dim3 g(cfg.persistent ? static_cast<unsigned>(std::min(grid, args.nitems))
+                              : static_cast<unsigned>(args.nitems));
+  if (cfg.persistent) {
+    hipError_t e = launch_fill_int(args.queue, 1, 0, st);
+    if (e != hipSuccess) return e;
+  }
+const bool P = cfg.persistent;
   switch (kind) {
     case kPairFP64Exact:
-      hipLaunchKernelGGL(k_pair_fp64_ref, grid, block, 0, st, state, items, rates, nu);
+      if (P) hipLaunchKernelGGL(k_pair_fp64_ref<true>, g, block, 0, st, args);
+      else hipLaunchKernelGGL(k_pair_fp64_ref<false>, g, block, 0, st, args);
       break;
     case kPairFP64:
-      hipLaunchKernelGGL(k_pair_fp64, grid, block, 0, st, state, items, rates, nu);
+      if (P) hipLaunchKernelGGL(k_pair_fp64<true>, g, block, 0, st, args);
+      else hipLaunchKernelGGL(k_pair_fp64<false>, g, block, 0, st, args);
       break;
     case kPairFP32C:
-      hipLaunchKernelGGL(k_pair_fp32c, grid, block, 0, st, state, items, rates,
-                         static_cast<float>(nu));
+      if (cfg.waves8) {
+        if (P) hipLaunchKernelGGL(k_pair_fp32c_w8<true>, g, block, 0, st, args);
+        else hipLaunchKernelGGL(k_pair_fp32c_w8<false>, g, block, 0, st, args);
+      } else {
+        if (P) hipLaunchKernelGGL(k_pair_fp32c<true>, g, block, 0, st, args);
+        else hipLaunchKernelGGL(k_pair_fp32c<false>, g, block, 0, st, args);
+      }
       break;
     default:
       return hipErrorInvalidValue;
   }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || cfg.parts <= 1) return e;
+  hipLaunchKernelGGL(k_reduce_parts, dim3((args.n + 255) / 256), dim3(256), 0, st, args.partial,
+                     cfg.parts, args.n, args.rates);
   return hipGetLastError();
+}
+
+int pair_kernel_waves(int kind, const PairConfig& cfg) {
+  if (kind == kPairFP32C) return cfg.waves8 ? 8 : 7;
+  return 4;
 }
 
 }  // namespace pw

@@ -1,0 +1,107 @@
+"""Launch shapes of the pair kernels (source-split parts, persistent work
+queue, FP32C register budget), the autotuner, and HIP-graph replay of steps,
+on coupled and independent-wake systems, against the oracle."""
+import numpy as np
+import pytest
+
+from conftest import normwise
+from pawan_amd import capi, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+CONFIGS = [(1, False, False), (3, False, False), (1, True, False), (8, True, False),
+           (4, True, True), (2, False, True)]
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def batched():
+    return W.batched_rings(nwakes=24, nmin=64, nmax=3000, seed=5)
+
+
+def _check_wakes(oracle, s, u, w, tol_u, tol_w, sample=64):
+    for k in range(s.nw):
+        a, b = int(s.off[k]), int(s.off[k + 1])
+        wk = s.wake(k)
+        idx = np.unique(np.linspace(0, wk.n - 1, min(sample, wk.n)).astype(np.int64))
+        ru, rw = oracle.rates_targets(wk, idx)
+        assert normwise(u[a:b][idx], ru) < tol_u, (k, normwise(u[a:b][idx], ru))
+        if tol_w is not None:
+            assert normwise(w[a:b][idx], rw) < tol_w, (k, normwise(w[a:b][idx], rw))
+
+
+@pytest.mark.parametrize("parts,persistent,waves8", CONFIGS)
+def test_launch_shapes_coupled(oracle, parts, persistent, waves8):
+    s = W.leapfrog_64k(nst=1024)  # 16,384 particles, two coupled wakes
+    idx = np.unique(np.linspace(0, s.n - 1, 300).astype(np.int64))
+    ru, rw = oracle.rates_targets(s, idx)
+    _say(f"leapfrog N={s.n} parts={parts} persistent={persistent} waves8={waves8}")
+    with capi.Solver(s) as g:
+        for prec, tu, tw in (("fp64", 1e-12, 1e-12), ("fp32c", 1e-6, 1e-5)):
+            g.set_pair_config(prec, parts, persistent, waves8)
+            u, w = g.solve(prec)
+            assert normwise(u[idx], ru) < tu
+            assert normwise(w[idx], rw) < tw
+
+
+@pytest.mark.parametrize("parts,persistent,waves8", CONFIGS)
+def test_launch_shapes_independent_wakes(oracle, batched, parts, persistent, waves8):
+    _say(f"batched {batched.nw} wakes N={batched.n} parts={parts} persistent={persistent}")
+    with capi.Solver(batched) as g:
+        g.set_pair_config("fp64", parts, persistent, waves8)
+        u, _ = g.solve("fp64")
+        _check_wakes(oracle, batched, u, None, 1e-12, None)
+        g.set_pair_config("fp32c", parts, persistent, waves8)
+        u, _ = g.solve("fp32c")
+        _check_wakes(oracle, batched, u, None, 1e-6, None)
+
+
+def test_autotune_keeps_results(oracle, batched):
+    with capi.Solver(batched) as g:
+        for prec, tol in (("fp32c", 1e-6), ("fp64", 1e-12)):
+            ms = g.autotune(prec)
+            cfg = g.pair_config(prec)
+            _say(f"autotune {prec}: {cfg} {ms:.3f} ms")
+            assert ms > 0
+            u, _ = g.solve(prec)
+            _check_wakes(oracle, batched, u, None, tol, None, sample=16)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32c"])
+def test_graph_replay_equals_eager_steps(batched, prec):
+    """Steps replayed from a captured HIP graph give bit-identical states to
+    eagerly launched steps, on every replay, with eager work (diagnostics,
+    a state read) between the replays."""
+    with capi.Solver(batched) as eager, capi.Solver(batched) as graph:
+        eager.set_pair_config(prec, 4, True, prec == "fp32c")
+        graph.set_pair_config(prec, 4, True, prec == "fp32c")
+        graph.set_graph(True)
+        for rep in range(4):
+            eager.step("rk4", 2e-3, prec, 1)
+            graph.step("rk4", 2e-3, prec, 1)
+            graph.diagnose()
+            pe, ae = eager.state()
+            pg, ag = graph.state()
+            np.testing.assert_array_equal(pg, pe)
+            np.testing.assert_array_equal(ag, ae)
+        eager.step("rk3", 2e-3, prec, 2)
+        graph.step("rk3", 2e-3, prec, 2)
+        np.testing.assert_array_equal(graph.state()[0], eager.state()[0])
+
+
+def test_batched_rk4_against_oracle(oracle):
+    """A few RK4 steps of independent wakes, each wake against the oracle run
+    on that wake alone."""
+    s = W.batched_rings(nwakes=8, nmin=64, nmax=600, seed=9)
+    with capi.Solver(s) as g:
+        g.set_graph(True)
+        g.step("rk4", 2e-3, "fp64", 3)
+        pos, alpha = g.state()
+    for k in range(s.nw):
+        a, b = int(s.off[k]), int(s.off[k + 1])
+        ref = oracle.step(s.wake(k), "rk4", 2e-3, 3)
+        assert normwise(pos[a:b] - s.pos[a:b], ref.pos - s.pos[a:b]) < 1e-11
+        assert normwise(alpha[a:b], ref.alpha) < 1e-12
