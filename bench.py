@@ -72,6 +72,98 @@ def time_path(system, precision: str, dt: float, steps: int, warmup: int, tune: 
     return wall, ms, launches, tuned
 
 
+def train_network(n_train: int, samples: int, steps: int, theta: float, seed: int = 11):
+    """Samples from seeded training wakes (helical wakes of other seeds and
+    size, generated here), then Adam on the GPU.  Returns (weights, report,
+    seconds)."""
+    from pawan_amd import capi, workloads
+
+    t0 = time.perf_counter()
+    seeds = [seed, seed + 1]
+    with capi.Solver(workloads.helix_wake(n_train, seed=seeds[0])) as tr:
+        tr.ff_set_params(theta, 0.0, use_mlp=False)
+        for k, sd in enumerate(seeds):
+            if k > 0:
+                w = workloads.helix_wake(n_train, seed=sd)
+                tr.set_wake_state(0, w.pos, w.alpha)
+            tr.ff_sample(samples // len(seeds), seed=1000 + sd)
+        tgen = time.perf_counter() - t0
+        tr.ff_init_weights(seed)
+        rep = tr.ff_train(steps, batch=1024, lr=2e-3, seed=seed)
+        weights = tr.ff_get_weights()
+    rep["sample_seconds"] = tgen
+    return weights, rep, time.perf_counter() - t0
+
+
+def calibrate_tol(g, target_err: float, ref_u: np.ndarray, use_mlp: bool, theta: float, urms: float):
+    """Largest tolerance (geometric search) whose velocity error against the
+    direct sum is at most target_err; returns (tol, err, info)."""
+    def err_at(tol):
+        g.ff_set_params(theta, tol, use_mlp=use_mlp)
+        u, _ = g.solve("learned")
+        return float(np.linalg.norm(u - ref_u) / np.linalg.norm(ref_u)), g.ff_info()
+
+    lo, hi = None, None
+    tol = target_err * urms
+    best = None
+    for _ in range(14):
+        e, info = err_at(tol)
+        log(f"  {'mlp' if use_mlp else 'md '} tol {tol:.3e}: err {e:.3e} near {info['near_pairs']:.3e} "
+            f"far {info['far_evals']:.3e}")
+        if e <= target_err:
+            lo = tol
+            best = (tol, e, info)
+            if hi is not None and hi / lo < 1.3:
+                break
+            tol = tol * 4 if hi is None else np.sqrt(lo * hi)
+        else:
+            hi = tol
+            if lo is not None and hi / lo < 1.3:
+                break
+            tol = tol / 4 if lo is None else np.sqrt(lo * hi)
+    return best
+
+
+def learned_leg(system, dt: float, steps: int, target_err: float, theta: float, train_n: int,
+                train_samples: int, train_steps: int):
+    from pawan_amd import capi
+
+    weights, rep, ttrain = train_network(train_n, train_samples, train_steps, theta)
+    log(f"network trained in {ttrain:.1f} s: {rep}")
+    out = {"training": dict(rep, seconds=ttrain, samples=train_samples, wake_particles=train_n),
+           "theta": theta, "target_error": target_err}
+    with capi.Solver(system) as g:
+        ref_u, _ = g.solve("fp32c")
+        urms = float(np.sqrt((ref_u * ref_u).sum(axis=1).mean()))
+        g.ff_set_weights(weights)
+        g.ff_set_err_coef(rep["coef_md"], rep["coef_mlp"])
+        for use_mlp in (False, True):
+            key = "mlp" if use_mlp else "analytic"
+            best = calibrate_tol(g, target_err, ref_u, use_mlp, theta, urms)
+            if best is None:
+                out[key] = None
+                continue
+            tol, err, info = best
+            g.ff_set_params(theta, tol, use_mlp=use_mlp)
+            g.step("rk4", dt, "learned", 1)  # warm-up
+            g.synchronize()
+            g.set_profiling(True)
+            t0 = time.perf_counter()
+            for i in range(steps):
+                g.step("rk4", dt, "learned", 1)
+            g.synchronize()
+            wall = time.perf_counter() - t0
+            ms, launches = g.pair_timing()
+            g.set_profiling(False)
+            out[key] = {"steps_per_s": steps / wall, "rhs_ms": ms / max(1, launches), "velocity_error": err,
+                        "tol_abs": tol, "near_pairs": info["near_pairs"], "far_evals": info["far_evals"],
+                        "max_far_nodes": info["max_far"], "max_near_leaves": info["max_near"]}
+            log(f"{key}: {out[key]}")
+            # back to the start state for the next variant
+            g.set_wake_state(0, system.pos, system.alpha)
+    return out
+
+
 def _spread(n: int, count: int) -> np.ndarray:
     """count indices spread over [0, n), in an order whose every prefix
     covers the range evenly."""
@@ -149,6 +241,13 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--learned-steps", type=int, default=3)
+    ap.add_argument("--learned-err", type=float, default=1e-4)
+    ap.add_argument("--theta", type=float, default=0.5)
+    ap.add_argument("--train-n", type=int, default=1 << 18)
+    ap.add_argument("--train-samples", type=int, default=1_000_000)
+    ap.add_argument("--train-steps", type=int, default=4000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
@@ -233,6 +332,17 @@ def main() -> int:
     result.update(out64)
     if out64:
         result["fp32c_over_fp64"] = sps32 / out64["fp64_steps_per_s"]
+    if not args.no_learned and system.coupled and system.nw == 1:
+        lg = learned_leg(system, dt, args.learned_steps, args.learned_err, args.theta, args.train_n,
+                         args.train_samples, args.train_steps)
+        result["learned"] = lg
+        best = max((v for k, v in lg.items() if k in ("analytic", "mlp") and v), default=None,
+                   key=lambda v: v["steps_per_s"])
+        if lg.get("mlp"):
+            result["learned_steps_per_s"] = lg["mlp"]["steps_per_s"]
+            result["learned_over_fp32c"] = lg["mlp"]["steps_per_s"] / sps32
+        if best is not None:
+            result["learned_best_steps_per_s"] = best["steps_per_s"]
     if not args.no_cpu and system.coupled:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(system, args.cpu_seconds)

@@ -49,7 +49,10 @@ enum {
   PW_FP64 = 0,  /* FP64 pair math and accumulation (the plain form)         */
   PW_FP32C = 1, /* FP32 pair math, compensated per-target accumulation,
                    FP64 state, stage combination and diagnostics           */
-  PW_LEARNED = 2, /* learned far field (reserved)                           */
+  PW_LEARNED = 2, /* tree path: direct FP32C near field over Morton-sorted
+                     256-particle leaves, analytic monopole + dipole far
+                     field plus the learned residual network (pw_ff_*);
+                     coupled systems only                                  */
   PW_FP64_REF = 3 /* FP64 with the reference's kernel formula evaluated for
                      every pair, operation by operation (PW_FP64 replaces
                      it by its closed far-field limit where the Gaussian
@@ -181,6 +184,77 @@ int pw_autotune(pw_ctx *ctx, int precision, double *best_ms);
  * precision and dt) and replays it for every step, instead of launching
  * each kernel.  Ignored while kernel profiling is on. */
 int pw_set_graph(pw_ctx *ctx, int enable);
+
+/* ---- learned far field (PW_LEARNED) -------------------------------------
+ * No counterpart in the reference, which has only the direct sum
+ * (interaction.cpp:196-210).  Particles are sorted along a Morton curve
+ * into leaves of 256, grouped 8 to a node; for each target leaf a node is
+ * far when r_node + r_leaf < theta * d, it lies beyond the Gaussian core,
+ * and the self-estimated error of its far evaluation,
+ * coef * s2 / (d - r_leaf)^4 (s2 = sum |alpha| |x - centre|^2, coef the
+ * calibrated coefficient of the evaluation in use), is at most tol.  Far
+ * nodes are evaluated by the analytic monopole + dipole expansion plus,
+ * when use_mlp, the network's residual; every other pair is summed
+ * directly (FP32C).  tol <= 0 disables the error test. */
+int pw_ff_set_params(pw_ctx *ctx, double theta, double tol, int use_mlp);
+int pw_ff_get_params(pw_ctx *ctx, double *theta, double *tol, int *use_mlp);
+/* error coefficients of the analytic expansion and of expansion + network
+ * (set by pw_ff_train from held-out samples; may be overridden) */
+int pw_ff_set_err_coef(pw_ctx *ctx, double coef_md, double coef_mlp);
+int pw_ff_get_err_coef(pw_ctx *ctx, double *coef_md, double *coef_mlp);
+/* build the tree and the lists of the current state (done by every
+ * PW_LEARNED right-hand side; for inspection) */
+int pw_ff_build(pw_ctx *ctx);
+/* out[0..12): n, leaves, nodes, levels, near capacity, far capacity,
+ * training samples held, near pairs, far (target, node) evaluations,
+ * list overflow flag, max near leaves, max far nodes (of the lists last
+ * built) */
+int pw_ff_info(pw_ctx *ctx, int64_t *out, int64_t len);
+/* copy an internal array to host memory (bytes must match exactly):
+ * PW_FF_PERM int32[n] (sorted position -> particle), PW_FF_NODES
+ * double[nodes][24] (centre 3, r, s0, s2, sigma, M 3, D 9 (row-major
+ * sum a_i delta_j), first, last, level, child), PW_FF_NEAR_CNT /
+ * PW_FF_FAR_CNT int32[leaves], PW_FF_NEAR / PW_FF_FAR int32[leaves][cap],
+ * PW_FF_SAMPLES_X float[ns][16], PW_FF_SAMPLES_Y float[ns][12],
+ * PW_FF_SAMPLES_AUX double[ns][10] (u scale, gradient scale, |x - c|, s2,
+ * |expansion error of 4 pi u|, |4 pi u|, node radius, node, target
+ * (sorted index), target leaf) */
+enum {
+  PW_FF_PERM = 0, PW_FF_NODES = 1, PW_FF_NEAR_CNT = 2, PW_FF_NEAR = 3,
+  PW_FF_FAR_CNT = 4, PW_FF_FAR = 5, PW_FF_SAMPLES_X = 6, PW_FF_SAMPLES_Y = 7,
+  PW_FF_SAMPLES_AUX = 8
+};
+int pw_ff_export(pw_ctx *ctx, int what, void *out, int64_t bytes);
+
+/* Training samples from the context's current state: nsamples (target,
+ * far node) pairs drawn with the seed (leaf uniform, far node uniform in its
+ * list, target uniform in the leaf), the node's exact sum at the target
+ * (FP64, the reference's Gaussian kernel) minus the analytic expansion, in
+ * the network's scaled units; appended to the context's sample set. */
+int pw_ff_sample(pw_ctx *ctx, int64_t nsamples, uint64_t seed);
+int pw_ff_clear_samples(pw_ctx *ctx);
+/* network parameters (FP32 master copy, 36,748 floats: W1[16][128] b1
+ * W2[128][128] b2 W3[128][128] b3 W4[128][12] b4) */
+int pw_ff_init_weights(pw_ctx *ctx, uint64_t seed);
+int pw_ff_set_weights(pw_ctx *ctx, const float *params, int64_t n);
+int pw_ff_get_weights(pw_ctx *ctx, float *params, int64_t n);
+/* Adam training on the held samples (the last tenth held out): `steps`
+ * minibatches of `batch`, learning rate lr with cosine decay to lr/20.
+ * Then the BF16 inference network is packed, evaluated on the held-out
+ * samples, and the error coefficients are calibrated (99th percentile of
+ * |error| / (s2/d^4)).  report (may be NULL) gets up to 10 doubles: final
+ * train loss, held-out loss, held-out loss of the zero residual (expansion
+ * only), rms velocity error ratio network/expansion, the same for the
+ * gradient, coef_md, coef_mlp, steps, held-out samples, milliseconds. */
+int pw_ff_train(pw_ctx *ctx, int64_t steps, int batch, double lr, uint64_t seed,
+                double *report, int64_t report_len);
+/* The inference network (BF16 matrix cores) on n scaled inputs: X
+ * float[n][16] -> Y float[n][12]. */
+int pw_ff_mlp_forward(pw_ctx *ctx, const float *X, int64_t n, float *Y);
+/* Loss (1/n) sum (f(x) - y)^2 and its gradient w.r.t. the FP32 master
+ * parameters, by the training kernels (FP32). */
+int pw_ff_mlp_grad(pw_ctx *ctx, const float *X, const float *Y, int64_t n,
+                   float *grad, double *loss);
 
 #ifdef __cplusplus
 }

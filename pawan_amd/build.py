@@ -31,7 +31,8 @@ HIPFLAGS = [
     "-I", os.path.join(ROOT, "include"), "-I", CSRC, "-I", HOST,
 ]
 
-HIP_SOURCES = ["pw_pair.hip", "pw_stage.hip", "pw_diag.hip", "pw_capi.hip"]
+HIP_SOURCES = ["pw_pair.hip", "pw_stage.hip", "pw_diag.hip", "pw_capi.hip", "pw_tree.hip",
+               "pw_far.hip", "pw_mlp_train.hip", "pw_ff.hip"]
 HOST_SOURCES = ["pawan_host.cpp"]
 
 

@@ -31,7 +31,19 @@ EXPORTS = [
     "pw_get_states", "pw_get_rates", "pw_solve", "pw_step", "pw_diagnose",
     "pw_synchronize", "pw_set_profiling", "pw_get_pair_timing",
     "pw_set_pair_config", "pw_get_pair_config", "pw_autotune", "pw_set_graph",
+    "pw_ff_set_params", "pw_ff_get_params", "pw_ff_set_err_coef", "pw_ff_get_err_coef",
+    "pw_ff_build", "pw_ff_info", "pw_ff_export", "pw_ff_sample", "pw_ff_clear_samples",
+    "pw_ff_init_weights", "pw_ff_set_weights", "pw_ff_get_weights", "pw_ff_train",
+    "pw_ff_mlp_forward", "pw_ff_mlp_grad",
 ]
+
+# pw_ff_export selectors
+FF_PERM, FF_NODES, FF_NEAR_CNT, FF_NEAR, FF_FAR_CNT, FF_FAR, FF_SAMPLES_X, FF_SAMPLES_Y, FF_SAMPLES_AUX = range(9)
+MLP_PARAMS = 36748
+FF_INFO_NAMES = ["n", "leaves", "nodes", "levels", "cap_near", "cap_far", "samples", "near_pairs",
+                 "far_evals", "overflow", "max_near", "max_far"]
+FF_TRAIN_NAMES = ["train_loss", "val_loss", "val_loss_zero", "rms_ratio_u", "rms_ratio_grad",
+                  "coef_md", "coef_mlp", "steps", "val_samples", "ms"]
 
 
 class PawanError(RuntimeError):
@@ -41,6 +53,7 @@ class PawanError(RuntimeError):
 
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _LIB: C.CDLL | None = None
 
 
@@ -84,6 +97,21 @@ def load(path: str | None = None) -> C.CDLL:
         "pw_get_pair_config": ([vp, i, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)], i),
         "pw_autotune": ([vp, i, C.POINTER(C.c_double)], i),
         "pw_set_graph": ([vp, i], i),
+        "pw_ff_set_params": ([vp, d, d, i], i),
+        "pw_ff_get_params": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int)], i),
+        "pw_ff_set_err_coef": ([vp, d, d], i),
+        "pw_ff_get_err_coef": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_double)], i),
+        "pw_ff_build": ([vp], i),
+        "pw_ff_info": ([vp, C.POINTER(C.c_int64), i64], i),
+        "pw_ff_export": ([vp, i, vp, i64], i),
+        "pw_ff_sample": ([vp, i64, C.c_uint64], i),
+        "pw_ff_clear_samples": ([vp], i),
+        "pw_ff_init_weights": ([vp, C.c_uint64], i),
+        "pw_ff_set_weights": ([vp, _fp, i64], i),
+        "pw_ff_get_weights": ([vp, _fp, i64], i),
+        "pw_ff_train": ([vp, i64, i, d, C.c_uint64, _dp, i64], i),
+        "pw_ff_mlp_forward": ([vp, _fp, i64, _fp], i),
+        "pw_ff_mlp_grad": ([vp, _fp, _fp, i64, _fp, C.POINTER(C.c_double)], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -265,3 +293,85 @@ class Solver:
         ms, n = C.c_double(), C.c_int64()
         _check(self._lib.pw_get_pair_timing(self._h, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    # -- learned far field (PW_LEARNED) ---------------------------------------
+    def ff_set_params(self, theta: float = 0.5, tol: float = 0.0, use_mlp: bool = True) -> None:
+        _check(self._lib.pw_ff_set_params(self._h, float(theta), float(tol), int(bool(use_mlp))))
+
+    def ff_params(self) -> dict:
+        th, tol, um = C.c_double(), C.c_double(), C.c_int()
+        _check(self._lib.pw_ff_get_params(self._h, C.byref(th), C.byref(tol), C.byref(um)))
+        return {"theta": th.value, "tol": tol.value, "use_mlp": bool(um.value)}
+
+    def ff_set_err_coef(self, coef_md: float, coef_mlp: float) -> None:
+        _check(self._lib.pw_ff_set_err_coef(self._h, float(coef_md), float(coef_mlp)))
+
+    def ff_err_coef(self):
+        a, b = C.c_double(), C.c_double()
+        _check(self._lib.pw_ff_get_err_coef(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def ff_build(self) -> None:
+        _check(self._lib.pw_ff_build(self._h))
+
+    def ff_info(self) -> dict:
+        out = (C.c_int64 * len(FF_INFO_NAMES))()
+        _check(self._lib.pw_ff_info(self._h, out, len(FF_INFO_NAMES)))
+        return dict(zip(FF_INFO_NAMES, [int(v) for v in out]))
+
+    def ff_export(self, what: int) -> np.ndarray:
+        info = self.ff_info()
+        nl = info["leaves"]
+        shapes = {
+            FF_PERM: (np.int32, (info["n"],)),
+            FF_NODES: (np.float64, (info["nodes"], 24)),
+            FF_NEAR_CNT: (np.int32, (nl,)),
+            FF_NEAR: (np.int32, (nl, info["cap_near"])),
+            FF_FAR_CNT: (np.int32, (nl,)),
+            FF_FAR: (np.int32, (nl, info["cap_far"])),
+            FF_SAMPLES_X: (np.float32, (info["samples"], 16)),
+            FF_SAMPLES_Y: (np.float32, (info["samples"], 12)),
+            FF_SAMPLES_AUX: (np.float64, (info["samples"], 10)),
+        }
+        dt, shape = shapes[what]
+        out = np.zeros(shape, dtype=dt)
+        _check(self._lib.pw_ff_export(self._h, what, out.ctypes.data_as(C.c_void_p), out.nbytes))
+        return out
+
+    def ff_sample(self, nsamples: int, seed: int = 1) -> None:
+        _check(self._lib.pw_ff_sample(self._h, int(nsamples), int(seed) & (2**64 - 1)))
+
+    def ff_clear_samples(self) -> None:
+        _check(self._lib.pw_ff_clear_samples(self._h))
+
+    def ff_init_weights(self, seed: int = 1) -> None:
+        _check(self._lib.pw_ff_init_weights(self._h, int(seed) & (2**64 - 1)))
+
+    def ff_set_weights(self, params) -> None:
+        p = np.ascontiguousarray(params, dtype=np.float32).reshape(-1)
+        _check(self._lib.pw_ff_set_weights(self._h, p, len(p)))
+
+    def ff_get_weights(self) -> np.ndarray:
+        out = np.zeros(MLP_PARAMS, dtype=np.float32)
+        _check(self._lib.pw_ff_get_weights(self._h, out, len(out)))
+        return out
+
+    def ff_train(self, steps: int, batch: int = 1024, lr: float = 1e-3, seed: int = 1) -> dict:
+        rep = np.zeros(len(FF_TRAIN_NAMES))
+        _check(self._lib.pw_ff_train(self._h, int(steps), int(batch), float(lr), int(seed) & (2**64 - 1),
+                                     rep, len(rep)))
+        return dict(zip(FF_TRAIN_NAMES, rep.tolist()))
+
+    def ff_mlp_forward(self, X) -> np.ndarray:
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 16)
+        Y = np.zeros((len(X), 12), dtype=np.float32)
+        _check(self._lib.pw_ff_mlp_forward(self._h, X, len(X), Y))
+        return Y
+
+    def ff_mlp_grad(self, X, Y):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 16)
+        Y = np.ascontiguousarray(Y, dtype=np.float32).reshape(-1, 12)
+        g = np.zeros(MLP_PARAMS, dtype=np.float32)
+        loss = C.c_double()
+        _check(self._lib.pw_ff_mlp_grad(self._h, X, Y, len(X), g, C.byref(loss)))
+        return g, loss.value

@@ -14,114 +14,28 @@
 #include <utility>
 #include <vector>
 
-#include "../../include/pawan_gpu.h"
-#include "pw_internal.h"
+#include "pw_ctx.h"
 
 using pw::Rate64;
 using pw::Rec64;
 using pw::WorkItem;
+using pwi::dalloc;
+using pwi::fail;
+using pwi::kKindLearned;
+using pwi::kTargetsPerItem;
+
+namespace pwi {
 
 namespace {
-
 thread_local std::string g_error;
+}  // namespace
 
 int fail(int code, const std::string& msg) {
   g_error = msg;
   return code;
 }
 
-#define PW_HIP(call)                                                        \
-  do {                                                                      \
-    hipError_t e_ = (call);                                                 \
-    if (e_ != hipSuccess)                                                   \
-      return fail(PW_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
-  } while (0)
-
-#define PW_CTX(c)                                                   \
-  do {                                                              \
-    if (!(c)) return fail(PW_ERR_ARG, "null context");              \
-    if (hipSetDevice((c)->device) != hipSuccess)                    \
-      return fail(PW_ERR_HIP, "hipSetDevice failed");               \
-  } while (0)
-
-#define PW_TRY(expr)                     \
-  do {                                   \
-    int rc_ = (expr);                    \
-    if (rc_ != PW_OK) return rc_;        \
-  } while (0)
-
-constexpr int kTargetsPerItem = 256;
-
-template <typename T>
-hipError_t dalloc(T** p, size_t count) {
-  *p = nullptr;
-  if (count == 0) count = 1;
-  return hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
-}
-
-}  // namespace
-
-struct pw_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  double nu = 1.56e-5;  // interaction.cpp:15
-  bool coupled = true;
-
-  // host description of the wakes
-  std::vector<int64_t> wn, wmax, woff{0};
-  std::vector<double> radius, volume;   // per particle
-  std::vector<Rec64> pending;           // state until the device is built
-  bool built = false;
-  int64_t n = 0;
-
-  // device buffers
-  Rec64* y = nullptr;    // current state
-  Rec64* ys = nullptr;   // stage state
-  Rate64* k[4] = {nullptr, nullptr, nullptr, nullptr};
-  Rate64* rates = nullptr;  // rates of the last right-hand side
-  double* d_radius = nullptr;
-  WorkItem* pair_items = nullptr;
-  int npair = 0;
-  WorkItem* lin_items = nullptr;
-  int nlin = 0;
-  std::vector<WorkItem> h_pair_items, h_lin_items;
-  std::vector<int> pair_sys, lin_wake;  // system / wake of each item
-  double* d_lin = nullptr;
-  double* d_quad = nullptr;
-  int* d_flag = nullptr;
-
-  // per pair-kernel kind: launch configuration and its work items
-  struct KindState {
-    pw::PairConfig cfg;
-    WorkItem* d_items = nullptr;
-    int nitems = 0;
-    int built_parts = 0;  // parts the items were built for (0: none yet)
-    bool tuned = false;
-  } ks[3];
-  Rate64* partial = nullptr;  // source-split partial rates, parts x n
-  int partial_parts = 0;
-  int* d_queue = nullptr;     // work counter of the persistent kernels
-  int num_cus = 256;
-  struct SysRange {
-    int64_t a, b;
-    int id;
-  };
-  std::vector<SysRange> systems;  // largest first
-
-  // HIP graphs of one step, per (scheme, kind, dt)
-  bool use_graph = false;
-  std::map<std::tuple<int, int, uint64_t>, hipGraphExec_t> graphs;
-
-  std::vector<double> zc;  // persistent centroid per system
-
-  // profiling
-  bool profiling = false;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_used, ev_free;
-  double prof_ms = 0.0;
-  int64_t prof_launches = 0;
-
-  int nsys() const { return coupled ? 1 : static_cast<int>(wn.size()); }
-};
+}  // namespace pwi
 
 namespace {
 
@@ -138,6 +52,7 @@ void release(pw_ctx* c) {
   for (auto& k : c->ks) (void)hipFree(k.d_items);
   (void)hipFree(c->partial);
   (void)hipFree(c->d_queue);
+  pwi::ff_release(c);
   for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.second);
   c->graphs.clear();
   for (auto& e : c->ev_used) {
@@ -254,8 +169,7 @@ int kind_of(int precision, int* kind) {
     case PW_FP64: *kind = pw::kPairFP64; return PW_OK;
     case PW_FP64_REF: *kind = pw::kPairFP64Exact; return PW_OK;
     case PW_FP32C: *kind = pw::kPairFP32C; return PW_OK;
-    case PW_LEARNED:
-      return fail(PW_ERR_UNSUPPORTED, "PW_LEARNED is not available through pw_solve/pw_step");
+    case PW_LEARNED: *kind = kKindLearned; return PW_OK;
     default:
       return fail(PW_ERR_ARG, "unknown precision " + std::to_string(precision));
   }
@@ -292,6 +206,7 @@ void drop_graphs(pw_ctx* c) {
 }
 
 int ensure_kind(pw_ctx* c, int kind) {
+  if (kind == kKindLearned) return PW_OK;  // the tree path sets itself up
   auto& k = c->ks[kind];
   if (k.built_parts == k.cfg.parts && k.d_items) return PW_OK;
   std::vector<WorkItem> items = make_items(c, k.cfg.parts);
@@ -316,6 +231,20 @@ int ensure_kind(pw_ctx* c, int kind) {
 
 // One right-hand side f(state) -> out, with optional event timing.
 int rhs(pw_ctx* c, int kind, const Rec64* state, Rate64* out) {
+  if (kind == kKindLearned) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->profiling) {
+      PW_TRY(pwi::prof_begin(c, &e0, &e1));
+      PW_HIP(hipEventRecord(e0, c->stream));
+    }
+    PW_TRY(pwi::ff_rhs(c, state, out));
+    if (c->profiling) {
+      PW_HIP(hipEventRecord(e1, c->stream));
+      c->ev_used.emplace_back(e0, e1);
+    }
+    PW_HIP(pw::launch_check_finite(out, static_cast<int>(c->n), c->d_flag, c->stream));
+    return PW_OK;
+  }
   PW_TRY(ensure_kind(c, kind));
   const auto& k = c->ks[kind];
   pw::PairArgs args;
@@ -330,14 +259,7 @@ int rhs(pw_ctx* c, int kind, const Rec64* state, Rate64* out) {
   const int grid = c->num_cus * pw::pair_kernel_waves(kind, k.cfg);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c->profiling) {
-    if (c->ev_free.empty()) {
-      PW_HIP(hipEventCreate(&e0));
-      PW_HIP(hipEventCreate(&e1));
-    } else {
-      e0 = c->ev_free.back().first;
-      e1 = c->ev_free.back().second;
-      c->ev_free.pop_back();
-    }
+    PW_TRY(pwi::prof_begin(c, &e0, &e1));
     PW_HIP(hipEventRecord(e0, c->stream));
   }
   PW_HIP(pw::launch_pair(kind, k.cfg, args, grid, c->stream));
@@ -350,6 +272,7 @@ int rhs(pw_ctx* c, int kind, const Rec64* state, Rate64* out) {
 }
 
 int check_flag(pw_ctx* c, const char* what) {
+  PW_TRY(pwi::ff_check(c));
   int flag = 0;
   PW_HIP(hipMemcpyAsync(&flag, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   PW_HIP(hipStreamSynchronize(c->stream));
@@ -414,11 +337,29 @@ int download_rates(pw_ctx* c, std::vector<Rate64>& h) {
 
 }  // namespace
 
+namespace pwi {
+
+int ensure_built(pw_ctx* c) { return ::ensure_built(c); }
+
+int prof_begin(pw_ctx* c, hipEvent_t* e0, hipEvent_t* e1) {
+  if (c->ev_free.empty()) {
+    PW_HIP(hipEventCreate(e0));
+    PW_HIP(hipEventCreate(e1));
+  } else {
+    *e0 = c->ev_free.back().first;
+    *e1 = c->ev_free.back().second;
+    c->ev_free.pop_back();
+  }
+  return PW_OK;
+}
+
+}  // namespace pwi
+
 extern "C" {
 
 const char* pw_version(void) { return "pawan-mi355x 0.1 (gfx950)"; }
 
-const char* pw_last_error(void) { return g_error.c_str(); }
+const char* pw_last_error(void) { return pwi::g_error.c_str(); }
 
 int pw_device_count(int* count) {
   if (!count) return fail(PW_ERR_ARG, "null count");
@@ -717,7 +658,9 @@ int pw_step(pw_ctx* c, int scheme, double dt, int precision, int64_t nsteps) {
   PW_TRY(ensure_built(c));
   PW_TRY(ensure_kind(c, kind));
   PW_HIP(pw::launch_fill_int(c->d_flag, 1, 0, c->stream));
-  if (c->use_graph && !c->profiling && nsteps > 0) {
+  // the tree path re-sorts the particles every stage (host-side sort
+  // set-up), so its steps are launched eagerly
+  if (c->use_graph && !c->profiling && nsteps > 0 && kind != kKindLearned) {
     hipGraphExec_t exec = nullptr;
     PW_TRY(step_graph(c, scheme, kind, dt, &exec));
     for (int64_t it = 0; it < nsteps; ++it) PW_HIP(hipGraphLaunch(exec, c->stream));
@@ -805,6 +748,7 @@ int pw_set_pair_config(pw_ctx* c, int precision, int parts, int persistent, int 
   PW_CTX(c);
   int kind = 0;
   PW_TRY(kind_of(precision, &kind));
+  if (kind == kKindLearned) return fail(PW_ERR_UNSUPPORTED, "launch configuration applies to the direct-sum kinds");
   if (parts < 1 || parts > 256) return fail(PW_ERR_ARG, "parts must be in [1, 256]");
   auto& k = c->ks[kind];
   k.cfg.parts = parts;
@@ -819,6 +763,7 @@ int pw_get_pair_config(pw_ctx* c, int precision, int* parts, int* persistent, in
   PW_CTX(c);
   int kind = 0;
   PW_TRY(kind_of(precision, &kind));
+  if (kind == kKindLearned) return fail(PW_ERR_UNSUPPORTED, "launch configuration applies to the direct-sum kinds");
   const auto& k = c->ks[kind];
   if (parts) *parts = k.cfg.parts;
   if (persistent) *persistent = k.cfg.persistent ? 1 : 0;
@@ -911,6 +856,7 @@ int pw_autotune(pw_ctx* c, int precision, double* best_ms) {
   PW_CTX(c);
   int kind = 0;
   PW_TRY(kind_of(precision, &kind));
+  if (kind == kKindLearned) return fail(PW_ERR_UNSUPPORTED, "launch configuration applies to the direct-sum kinds");
   PW_TRY(ensure_built(c));
   load_tune_file();
   const int64_t largest = c->systems.empty() ? 0 : c->systems.front().b - c->systems.front().a;

@@ -1,0 +1,600 @@
+// pw_far.hip -- far field of the tree path: analytic monopole + dipole
+// evaluation, the learned residual (MLP on MFMA), training-sample
+// generation and the scatter back to the caller's particle order.
+//
+// Units: inside this file velocities and gradients carry a factor 4 pi
+// (u' = 4 pi u); the kernels multiply by 1/(4 pi) when they add to the
+// rates.
+//
+// Far field of a node (centre c, moments M, D) at a target x, r = x - c:
+//   u'  = M x r / r^3 - w / r^3 + 3 (p x r) / r^5
+//   G'  = [M]x / r^3 - 3 (M x r) r^T / r^5
+//         + 3 w r^T / r^5 + 3 (D_:m x r + p x e_m)_k / r^5 - 15 (p x r) r^T / r^7
+// with w_k = eps_kij D_ij, p = D r, G_km = du_k/dx_m (the singular kernel
+// 1/(4 pi r^3) expanded about c to first order in the particle offsets).
+// The vorticity rate of the target is the transposed contraction
+// dalpha_m = sum_k alpha_k G_km, which is the reference's transposed
+// stretching q (a_t x a_s) + F [d.(a_t x a_s)] d summed over the node
+// (src/interaction/interaction_utils.h:456-501).
+//
+// Learned residual: an MLP 16 -> 128 -> 128 -> 128 -> 12 (GELU, tanh form)
+// maps node/target features to the residual of the expansion (truth -
+// monopole - dipole) in scaled units:
+//   inputs  e = r l / r^2 (3), M / s0 (3), D / (s0 l) (9), sigma / l (1)
+//   outputs u' residual / (s0 l^2 / r^4) (3), G' residual / (s0 l^2 / r^5) (9)
+// with l = r_node + sigma the node's length scale.  Inference runs in BF16
+// on the matrix cores with FP32 accumulation, in the transposed form
+// H_next^T = W^T H^T: the FP32 result of one 16x16 output tile is exactly
+// the B operand of the next layer in the same lanes, so activations never
+// leave registers.
+#include <hip/hip_runtime.h>
+
+#include "pw_internal.h"
+#include "pw_mlp.h"
+
+namespace pw {
+namespace {
+
+constexpr double kQd = 0x1.45f306dc9c883p-4;  // 1/(4 pi)
+constexpr double kKd = 0x1.0411e71d77959p-4;  // (2 pi)^-1.5
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------
+// Monopole + dipole of one node at offset r (FP32, 4 pi factored).
+// ---------------------------------------------------------------------
+__device__ __forceinline__ void md_eval(const float r[3], const float M[3], const float D[9],
+                                        float u[3], float G[9]) {
+  const float r2 = fmaf(r[2], r[2], fmaf(r[1], r[1], r[0] * r[0]));
+  const float ir = __builtin_amdgcn_rsqf(r2);
+  const float ir2 = ir * ir;
+  const float ir3 = ir * ir2;
+  const float ir5 = ir3 * ir2;
+  const float ir7 = ir5 * ir2;
+  // monopole
+  const float mr[3] = {M[1] * r[2] - M[2] * r[1], M[2] * r[0] - M[0] * r[2], M[0] * r[1] - M[1] * r[0]};
+  // dipole
+  const float w[3] = {D[5] - D[7], D[6] - D[2], D[1] - D[3]};
+  const float p[3] = {fmaf(D[2], r[2], fmaf(D[1], r[1], D[0] * r[0])),
+                      fmaf(D[5], r[2], fmaf(D[4], r[1], D[3] * r[0])),
+                      fmaf(D[8], r[2], fmaf(D[7], r[1], D[6] * r[0]))};
+  const float pr[3] = {p[1] * r[2] - p[2] * r[1], p[2] * r[0] - p[0] * r[2], p[0] * r[1] - p[1] * r[0]};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) u[k] = mr[k] * ir3 - w[k] * ir3 + 3.0f * pr[k] * ir5;
+  // [M]x / r^3 + [p]x * 3 / r^5 (the p x e_m term)
+  const float a3 = ir3, b5 = 3.0f * ir5;
+  G[0] = 0.0f;
+  G[1] = -M[2] * a3 - p[2] * b5;
+  G[2] = M[1] * a3 + p[1] * b5;
+  G[3] = M[2] * a3 + p[2] * b5;
+  G[4] = 0.0f;
+  G[5] = -M[0] * a3 - p[0] * b5;
+  G[6] = -M[1] * a3 - p[1] * b5;
+  G[7] = M[0] * a3 + p[0] * b5;
+  G[8] = 0.0f;
+  // D_:m x r
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const float c0 = D[m], c1 = D[3 + m], c2 = D[6 + m];
+    G[0 + m] += b5 * (c1 * r[2] - c2 * r[1]);
+    G[3 + m] += b5 * (c2 * r[0] - c0 * r[2]);
+    G[6 + m] += b5 * (c0 * r[1] - c1 * r[0]);
+  }
+  // rank-one terms: (-3 M x r / r^5 + 3 w / r^5 - 15 p x r / r^7) r^T
+  const float c7 = 15.0f * ir7;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float v = b5 * (w[k] - mr[k]) - c7 * pr[k];
+    G[3 * k + 0] = fmaf(v, r[0], G[3 * k + 0]);
+    G[3 * k + 1] = fmaf(v, r[1], G[3 * k + 1]);
+    G[3 * k + 2] = fmaf(v, r[2], G[3 * k + 2]);
+  }
+}
+
+// Node data staged in LDS for the far kernels.
+struct FarNodeL {
+  double cx, cy, cz;
+  float M[3], D[9];
+  float ell, s0;
+  float feat[16];  // slots 3..15 of the MLP input (node part)
+};
+
+__device__ __forceinline__ void stage_node(const NodeRec& q, FarNodeL& f) {
+  f.cx = q.cx; f.cy = q.cy; f.cz = q.cz;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) f.M[k] = q.M[k];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) f.D[k] = q.D[k];
+  const float ell = q.r + q.sg;
+  const float is0 = q.s0 > 0.0f ? 1.0f / q.s0 : 0.0f;
+  f.ell = ell;
+  f.s0 = q.s0;
+  f.feat[0] = f.feat[1] = f.feat[2] = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) f.feat[3 + k] = q.M[k] * is0;
+  const float isl = is0 / ell;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) f.feat[6 + k] = q.D[k] * isl;
+  f.feat[15] = q.sg / ell;
+}
+
+// ---------------------------------------------------------------------
+// Analytic far field: one thread per target of a leaf, far nodes staged
+// through LDS in chunks.
+// ---------------------------------------------------------------------
+constexpr int kNodeChunk = 64;
+
+struct FarArgs {
+  const Rec64* sorted;
+  const NodeRec* nodes;
+  const int* list;
+  const int* cnt;
+  int cap;
+  Rate64* rates;
+};
+
+__global__ __launch_bounds__(256) void k_far_md(FarArgs a) {
+  __shared__ FarNodeL sn[kNodeChunk];
+  const int T = blockIdx.x;
+  const int t0 = a.nodes[T].first, t1 = a.nodes[T].last;
+  const int t = t0 + static_cast<int>(threadIdx.x);
+  const bool active = t < t1;
+  const Rec64 me = a.sorted[active ? t : t0];
+  float u[3] = {0, 0, 0}, G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const int nf = a.cnt[T];
+  const int* __restrict__ L = a.list + static_cast<size_t>(T) * a.cap;
+  for (int base = 0; base < nf; base += kNodeChunk) {
+    const int c = min(kNodeChunk, nf - base);
+    __syncthreads();
+    if (static_cast<int>(threadIdx.x) < c) stage_node(a.nodes[L[base + threadIdx.x]], sn[threadIdx.x]);
+    __syncthreads();
+    if (!active) continue;
+    for (int k = 0; k < c; ++k) {
+      const float r[3] = {static_cast<float>(me.x - sn[k].cx), static_cast<float>(me.y - sn[k].cy),
+                          static_cast<float>(me.z - sn[k].cz)};
+      float uu[3], gg[9];
+      md_eval(r, sn[k].M, sn[k].D, uu, gg);
+#pragma unroll
+      for (int m = 0; m < 3; ++m) u[m] += uu[m];
+#pragma unroll
+      for (int m = 0; m < 9; ++m) G[m] += gg[m];
+    }
+  }
+  if (!active) return;
+  const float ax = static_cast<float>(me.ax), ay = static_cast<float>(me.ay), az = static_cast<float>(me.az);
+  Rate64 r = a.rates[t];
+  r.ux += kQd * u[0];
+  r.uy += kQd * u[1];
+  r.uz += kQd * u[2];
+  r.wx += kQd * (ax * G[0] + ay * G[3] + az * G[6]);
+  r.wy += kQd * (ax * G[1] + ay * G[4] + az * G[7]);
+  r.wz += kQd * (ax * G[2] + ay * G[5] + az * G[8]);
+  a.rates[t] = r;
+}
+
+// ---------------------------------------------------------------------
+// MLP inference on the matrix cores.
+//
+// Layer 1 (K = 16): v_mfma_f32_16x16x16_bf16.  Layers 2-4 (K = 128):
+// v_mfma_f32_16x16x32_bf16 over four K blocks; K block b pairs the output
+// tiles 2b and 2b+1 of the previous layer (slots 0-3 of a lane from tile
+// 2b, slots 4-7 from tile 2b+1), and the packed weights use the same
+// order of k.  Lane l (c = l % 16, g = l / 16) of an output tile holds rows
+// 4g..4g+3 of sample column c.
+// ---------------------------------------------------------------------
+__device__ __forceinline__ float gelu(float x) {
+  // x * sigmoid(2 z), z = sqrt(2/pi) (x + 0.044715 x^3): the tanh form
+  const float z2 = -2.0f * 0.7978845608f * fmaf(0.044715f * x * x, x, x);
+  return x / (1.0f + __expf(z2));
+}
+
+struct MlpLds {
+  v4bf w1[8][64];          // layer 1: 8 output tiles
+  v8bf w2[8][4][64];       // layer 2: output tile, K block
+  v8bf w3[8][4][64];
+  v8bf w4[4][64];          // layer 4: one output tile (12 used)
+  float b[kMlpBiasLen];    // b1 | b2 | b3 | b4 (padded to 16)
+};
+
+__device__ void load_mlp(const MlpWeights& W, MlpLds& s) {
+  const uint4* src = reinterpret_cast<const uint4*>(W.frag);
+  uint4* dst = reinterpret_cast<uint4*>(&s);
+  constexpr int n16 = kMlpFragBytes / 16;
+  for (int i = threadIdx.x; i < n16; i += blockDim.x) dst[i] = src[i];
+  for (int i = threadIdx.x; i < kMlpBiasLen; i += blockDim.x) s.b[i] = W.bias[i];
+}
+
+// One pass of 32 samples (two 16-column blocks) through the network.
+// x[nb] = this lane's 4 input features of column block nb; out[nb] = rows
+// 4g..4g+3 of the 12 (16) outputs.
+__device__ __forceinline__ void mlp_pass(const MlpLds& s, const v4bf x[2], v4f out[2]) {
+  // The weights are the same for every pass; an opaque lane index keeps the
+  // compiler from hoisting all 73 KB of fragment reads out of a caller's
+  // loop into (spilled) registers: they are read from LDS at their use.
+  int lane = threadIdx.x & 63;
+  asm volatile("" : "+v"(lane));
+  const int g = lane >> 4;
+  v4bf h[8][2];
+  // layer 1
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const v4s a = __builtin_bit_cast(v4s, s.w1[m][lane]);
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb) {
+      v4f acc;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) acc[i] = s.b[16 * m + 4 * g + i];
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, __builtin_bit_cast(v4s, x[nb]), acc, 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) h[m][nb][i] = static_cast<__bf16>(gelu(acc[i]));
+    }
+  }
+  // layers 2 and 3
+#pragma unroll
+  for (int layer = 0; layer < 2; ++layer) {
+    const v8bf(*w)[4][64] = layer == 0 ? s.w2 : s.w3;
+    const float* bias = s.b + 128 * (layer + 1);
+    v4f acc[8][2];
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+#pragma unroll
+      for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[m][nb][i] = bias[16 * m + 4 * g + i];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      v8bf bx[2];
+#pragma unroll
+      for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          bx[nb][i] = h[2 * b][nb][i];
+          bx[nb][4 + i] = h[2 * b + 1][nb][i];
+        }
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        const v8bf a = w[m][b][lane];
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb)
+          acc[m][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bx[nb], acc[m][nb], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+#pragma unroll
+      for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) h[m][nb][i] = static_cast<__bf16>(gelu(acc[m][nb][i]));
+  }
+  // layer 4
+#pragma unroll
+  for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) out[nb][i] = s.b[384 + 4 * g + i];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    const v8bf a = s.w4[b][lane];
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb) {
+      v8bf bx;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        bx[i] = h[2 * b][nb][i];
+        bx[4 + i] = h[2 * b + 1][nb][i];
+      }
+      out[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bx, out[nb], 0, 0, 0);
+    }
+  }
+}
+
+// Output scales of one (target, node) sample: u' scale s0 l^2 / r^4, G'
+// scale s0 l^2 / r^5, and el = l / r^2 (the offset feature is r el).
+__device__ __forceinline__ void sample_scales(const FarNodeL& q, float r0, float r1, float r2, float& su,
+                                              float& sG, float& el) {
+  const float d2 = fmaf(r2, r2, fmaf(r1, r1, r0 * r0));
+  const float id2 = 1.0f / d2;
+  const float id = __builtin_amdgcn_sqrtf(id2);
+  su = q.s0 * q.ell * q.ell * id2 * id2;
+  sG = su * id;
+  el = q.ell * id2;
+}
+
+// Features 4g..4g+3 of one (target, node) sample; also the output scales.
+__device__ __forceinline__ v4bf sample_features(const FarNodeL& q, double x, double y, double z, int g,
+                                                float& su, float& sG) {
+  const float r0 = static_cast<float>(x - q.cx), r1 = static_cast<float>(y - q.cy),
+              r2 = static_cast<float>(z - q.cz);
+  float el;
+  sample_scales(q, r0, r1, r2, su, sG, el);
+  float f[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) f[i] = q.feat[4 * g + i];
+  if (g == 0) {
+    f[0] = r0 * el;
+    f[1] = r1 * el;
+    f[2] = r2 * el;
+  }
+  v4bf v;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) v[i] = static_cast<__bf16>(f[i]);
+  return v;
+}
+
+// Learned residual over the far lists: 8 waves per target leaf, wave w
+// handles targets 32w..32w+31 against every far node of the leaf.
+constexpr int kMlpThreads = 512;
+constexpr int kMlpNodeChunk = 32;
+
+__global__ __launch_bounds__(kMlpThreads) void k_far_mlp(FarArgs a, MlpWeights W) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  MlpLds& s = *reinterpret_cast<MlpLds*>(smem);
+  FarNodeL* sn = reinterpret_cast<FarNodeL*>(smem + sizeof(MlpLds));
+  float* res = reinterpret_cast<float*>(smem + sizeof(MlpLds));  // reused at the end: [256][13]
+  load_mlp(W, s);
+  const int T = blockIdx.x;
+  const int t0 = a.nodes[T].first, t1 = a.nodes[T].last;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int g = lane >> 4, c = lane & 15;
+  double px[2], py[2], pz[2];
+  bool on[2];
+#pragma unroll
+  for (int nb = 0; nb < 2; ++nb) {
+    const int t = t0 + 32 * wave + 16 * nb + c;
+    on[nb] = t < t1;
+    const Rec64 r = a.sorted[on[nb] ? t : t0];
+    px[nb] = r.x; py[nb] = r.y; pz[nb] = r.z;
+  }
+  v4f acc[2];
+#pragma unroll
+  for (int nb = 0; nb < 2; ++nb) acc[nb] = v4f{0, 0, 0, 0};
+  const bool wave_on = t0 + 32 * wave < t1;
+  const int nf = a.cnt[T];
+  const int* __restrict__ L = a.list + static_cast<size_t>(T) * a.cap;
+  for (int base = 0; base < nf; base += kMlpNodeChunk) {
+    const int cn = min(kMlpNodeChunk, nf - base);
+    __syncthreads();
+    if (static_cast<int>(threadIdx.x) < cn) stage_node(a.nodes[L[base + threadIdx.x]], sn[threadIdx.x]);
+    __syncthreads();
+    if (!wave_on) continue;
+    for (int k = 0; k < cn; ++k) {
+      v4bf x[2];
+      float su[2], sG[2];
+#pragma unroll
+      for (int nb = 0; nb < 2; ++nb) x[nb] = sample_features(sn[k], px[nb], py[nb], pz[nb], g, su[nb], sG[nb]);
+      v4f o[2];
+      mlp_pass(s, x, o);
+#pragma unroll
+      for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[nb][i] = fmaf(o[nb][i], (4 * g + i) < 3 ? su[nb] : sG[nb], acc[nb][i]);
+    }
+  }
+  // gather the 12 outputs of each target: res[target][row]
+  __syncthreads();
+  if (wave_on) {
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int row = 4 * g + i;
+        if (row < 12) res[(32 * wave + 16 * nb + c) * 13 + row] = acc[nb][i];
+      }
+  }
+  __syncthreads();
+  if (threadIdx.x < 256) {
+    const int t = t0 + static_cast<int>(threadIdx.x);
+    if (t < t1) {
+      const float* o = res + threadIdx.x * 13;
+      const Rec64 me = a.sorted[t];
+      const double ax = me.ax, ay = me.ay, az = me.az;
+      Rate64 r = a.rates[t];
+      r.ux += kQd * o[0];
+      r.uy += kQd * o[1];
+      r.uz += kQd * o[2];
+      r.wx += kQd * (ax * o[3] + ay * o[6] + az * o[9]);
+      r.wy += kQd * (ax * o[4] + ay * o[7] + az * o[10]);
+      r.wz += kQd * (ax * o[5] + ay * o[8] + az * o[11]);
+      a.rates[t] = r;
+    }
+  }
+}
+
+// Plain inference over given inputs (n x 16 FP32, scaled units) -> n x 12:
+// the same network pass as the far kernel.
+__global__ __launch_bounds__(256) void k_mlp_forward(const float* __restrict__ X, int n,
+                                                     float* __restrict__ Y, MlpWeights W) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  MlpLds& s = *reinterpret_cast<MlpLds*>(smem);
+  load_mlp(W, s);
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int g = lane >> 4, c = lane & 15;
+  const int base = (blockIdx.x * 4 + wave) * 32;
+  if (base >= n) return;
+  v4bf x[2];
+#pragma unroll
+  for (int nb = 0; nb < 2; ++nb) {
+    const int j = base + 16 * nb + c;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) x[nb][i] = static_cast<__bf16>(j < n ? X[16 * j + 4 * g + i] : 0.0f);
+  }
+  v4f o[2];
+  mlp_pass(s, x, o);
+#pragma unroll
+  for (int nb = 0; nb < 2; ++nb) {
+    const int j = base + 16 * nb + c;
+    if (j >= n) continue;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (4 * g + i < 12) Y[12 * j + 4 * g + i] = o[nb][i];
+  }
+}
+
+// ---------------------------------------------------------------------
+// Training samples: sample k = (target t of leaf T, the node at position
+// slot of T's far list).  Exact node sum at the target in FP64 with the
+// reference's Gaussian kernel (gaussian.h:47-60), minus the FP32 analytic
+// expansion above, in the scaled units of the network.
+// ---------------------------------------------------------------------
+__device__ __forceinline__ void exact_pair(double dx, double dy, double dz, double sig2, const Rec64& s,
+                                           double u[3], double G[9]) {
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  const double rho = sqrt(r2);
+  const double sig = sqrt(sig2);
+  const double rb = rho / sig;
+  const double Z = kKd * exp(-0.5 * rb * rb) / (sig2 * sig);
+  const double phi = kQd * erf(rb * 0.70710678118654752440) / (sig2 * sig);
+  const double q = (phi / rb - Z) / (rb * rb);
+  const double F = (Z - 3.0 * q) / r2;
+  const double ax = s.ax, ay = s.ay, az = s.az;
+  const double cx = ay * dz - az * dy, cy = az * dx - ax * dz, cz = ax * dy - ay * dx;  // a x d
+  u[0] += q * cx; u[1] += q * cy; u[2] += q * cz;
+  // G_km = q eps_kim a_i + F d_m (a x d)_k
+  const double d[3] = {dx, dy, dz}, ad[3] = {cx, cy, cz};
+  G[1] += -q * az; G[2] += q * ay;
+  G[3] += q * az;  G[5] += -q * ax;
+  G[6] += -q * ay; G[7] += q * ax;
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int m = 0; m < 3; ++m) G[3 * k + m] += F * d[m] * ad[k];
+}
+
+struct SampleArgs {
+  const Rec64* sorted;
+  const NodeRec* nodes;
+  const int* far_list;
+  int cap;
+  const int* spec;  // per sample: leaf T, list slot, target (sorted index)
+  int nsamples;
+  float* X;         // nsamples x 16
+  float* Y;         // nsamples x 12
+  double* aux;      // nsamples x kSampleAux
+};
+
+__global__ __launch_bounds__(256) void k_ff_samples(SampleArgs a) {
+  __shared__ double red[4 * 12];
+  const int k = blockIdx.x;
+  const int T = a.spec[3 * k], slot = a.spec[3 * k + 1], t = a.spec[3 * k + 2];
+  const int id = a.far_list[static_cast<size_t>(T) * a.cap + slot];
+  const NodeRec q = a.nodes[id];
+  const Rec64 me = a.sorted[t];
+  double v[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int s = q.first + static_cast<int>(threadIdx.x); s < q.last; s += 256) {
+    const Rec64 src = a.sorted[s];
+    exact_pair(me.x - src.x, me.y - src.y, me.z - src.z, 0.5 * (me.s2 + src.s2), src, v, v + 3);
+  }
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int m = 0; m < 12; ++m) v[m] += __shfl_xor(v[m], off, 64);
+  if ((threadIdx.x & 63) == 0)
+    for (int m = 0; m < 12; ++m) red[(threadIdx.x >> 6) * 12 + m] = v[m];
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int m = 0; m < 12; ++m) v[m] = ((red[m] + red[12 + m]) + red[24 + m]) + red[36 + m];
+  FarNodeL f;
+  stage_node(q, f);
+  const float r[3] = {static_cast<float>(me.x - f.cx), static_cast<float>(me.y - f.cy),
+                      static_cast<float>(me.z - f.cz)};
+  float uu[3], gg[9];
+  md_eval(r, f.M, f.D, uu, gg);
+  // features in FP32 (the network rounds them to BF16 itself)
+  float su, sG, el;
+  sample_scales(f, r[0], r[1], r[2], su, sG, el);
+  float feat[16];
+  for (int i = 0; i < 16; ++i) feat[i] = f.feat[i];
+  feat[0] = r[0] * el;
+  feat[1] = r[1] * el;
+  feat[2] = r[2] * el;
+  double err_md = 0.0, un = 0.0;
+  for (int m = 0; m < 12; ++m) {
+    const double fourpi_exact = v[m] * 12.566370614359172954;  // 4 pi
+    const double res = fourpi_exact - static_cast<double>(m < 3 ? uu[m] : gg[m - 3]);
+    a.Y[12 * static_cast<size_t>(k) + m] = static_cast<float>(res / (m < 3 ? su : sG));
+    if (m < 3) {
+      err_md += res * res;
+      un += fourpi_exact * fourpi_exact;
+    }
+  }
+  for (int i = 0; i < 16; ++i) a.X[16 * static_cast<size_t>(k) + i] = feat[i];
+  double* o = a.aux + static_cast<size_t>(k) * kSampleAux;
+  const double d = sqrt(static_cast<double>(r[0]) * r[0] + static_cast<double>(r[1]) * r[1] +
+                        static_cast<double>(r[2]) * r[2]);
+  o[0] = su;               // u scale (4 pi units)
+  o[1] = sG;               // G scale
+  o[2] = d;                // |x - c|
+  o[3] = q.s2;             // node s2
+  o[4] = sqrt(err_md);     // |u' exact - u' expansion|
+  o[5] = sqrt(un);         // |u' exact|
+  o[6] = q.r;              // node radius
+  o[7] = id;               // node id
+  o[8] = t;                // target (Morton-sorted index)
+  o[9] = T;                // target leaf
+}
+
+// rates[perm[i]] = rates_s[i]
+__global__ __launch_bounds__(256) void k_scatter(const Rate64* __restrict__ in, const int* __restrict__ perm,
+                                                 int n, Rate64* __restrict__ out) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) out[perm[i]] = in[i];
+}
+
+}  // namespace
+
+size_t mlp_far_lds_bytes() {
+  const size_t tail_nodes = kMlpNodeChunk * sizeof(FarNodeL), tail_res = 256 * 13 * sizeof(float);
+  return sizeof(MlpLds) + (tail_nodes > tail_res ? tail_nodes : tail_res);
+}
+
+hipError_t launch_far(const Rec64* sorted, const NodeRec* nodes, const int* far_list, const int* far_cnt,
+                      int cap_far, int nleaf, const MlpWeights* mlp, Rate64* rates_s, hipStream_t st) {
+  if (nleaf <= 0) return hipSuccess;
+  FarArgs a{sorted, nodes, far_list, far_cnt, cap_far, rates_s};
+  hipLaunchKernelGGL(k_far_md, dim3(nleaf), dim3(256), 0, st, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || mlp == nullptr) return e;
+  const size_t lds = mlp_far_lds_bytes();
+  static bool attr = false;  // LDS beyond the 64 KB default
+  if (!attr) {
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_far_mlp), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            static_cast<int>(lds));
+    if (e != hipSuccess) return e;
+    attr = true;
+  }
+  hipLaunchKernelGGL(k_far_mlp, dim3(nleaf), dim3(kMlpThreads), lds, st, a, *mlp);
+  return hipGetLastError();
+}
+
+hipError_t launch_scatter_rates(const Rate64* rates_s, const int* perm, int n, Rate64* rates, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scatter, dim3((n + 255) / 256), dim3(256), 0, st, rates_s, perm, n, rates);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_forward(const float* X, int n, float* Y, const MlpWeights& W, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  const int blocks = (n + 127) / 128;
+  static bool attr = false;
+  if (!attr) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_mlp_forward),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(sizeof(MlpLds)));
+    if (e != hipSuccess) return e;
+    attr = true;
+  }
+  hipLaunchKernelGGL(k_mlp_forward, dim3(blocks), dim3(256), sizeof(MlpLds), st, X, n, Y, W);
+  return hipGetLastError();
+}
+
+hipError_t launch_ff_samples(const Rec64* sorted, const NodeRec* nodes, const int* far_list, int cap,
+                             const int* spec, int nsamples, float* X, float* Y, double* aux,
+                             hipStream_t st) {
+  if (nsamples <= 0) return hipSuccess;
+  SampleArgs a{sorted, nodes, far_list, cap, spec, nsamples, X, Y, aux};
+  hipLaunchKernelGGL(k_ff_samples, dim3(nsamples), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+}  // namespace pw

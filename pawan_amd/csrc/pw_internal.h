@@ -107,4 +107,98 @@ hipError_t launch_diag_pairs(const Rec64* y, const double* radius,
                              const WorkItem* items, int nitems, double* out,
                              hipStream_t st);
 
+// ---------------------------------------------------------------------
+// Cluster tree of the learned far field.
+// ---------------------------------------------------------------------
+constexpr int kLeafSize = 256;  // particles per leaf (= one LDS tile)
+constexpr int kFan = 8;         // children per node
+constexpr int kMaxLevels = 12;  // 256 * 8^11 particles
+constexpr int kStack = 96;      // traversal stack (7 per level + 1)
+
+// One node: particle range [first, last) of the Morton-sorted state, centre
+// (particle mean), bounding radius r about it, and the vorticity moments
+//   M = sum a, D[3i+j] = sum a_i delta_j, s0 = sum |a|,
+//   S1 = sum |a| delta, s2 = sum |a| |delta|^2, sg = mean sigma
+// with delta = x - centre.  child = index of the first child (level > 0).
+struct alignas(16) NodeRec {
+  double cx, cy, cz;
+  float r, s0, s2, sg;
+  float M[3];
+  float D[9];
+  float S1[3];
+  int first, last, level, child;
+  int pad[3];
+};
+static_assert(sizeof(NodeRec) == 128, "NodeRec is 128 B");
+
+// Level layout of the node array: level L holds level_count[L] nodes from
+// level_off[L]; level 0 are the leaves (node i = particles [256 i, ...)).
+struct TreeShape {
+  int n, nleaf, nnodes, nlevel, root;
+  int level_off[kMaxLevels];
+  int level_count[kMaxLevels];
+};
+TreeShape tree_shape(int n);
+
+struct TreeBuffers {
+  double* bbox_part = nullptr;  // 6 x 1024 block extremes
+  double* bbox = nullptr;       // lo[3], hi[3]
+  uint32_t* code = nullptr;     // n Morton codes (original order)
+  uint32_t* code_sorted = nullptr;
+  int* idx = nullptr;           // 0..n-1
+  int* perm = nullptr;          // sorted position -> original index
+  void* sort_temp = nullptr;
+  size_t sort_temp_bytes = 0;
+  Rec64* sorted = nullptr;      // state in Morton order
+  NodeRec* nodes = nullptr;
+  int* near_list = nullptr;     // nleaf x cap_near leaf ids
+  int* near_cnt = nullptr;
+  int cap_near = 0;
+  int* far_list = nullptr;      // nleaf x cap_far node ids
+  int* far_cnt = nullptr;
+  int cap_far = 0;
+  int* overflow = nullptr;      // set when a list or the stack overflows
+};
+
+// Traversal parameters: a node is far from target leaf T when
+//   r_node + r_T < theta d,  (d - r_node - r_T)^2 > 44 sigma_max^2
+//   and err_coef * s2 <= tol * (d - r_T)^4
+// (the last is the self-estimated truncation error of the far
+// evaluation, err_coef calibrated for the evaluation in use).
+struct ListParams {
+  float theta;
+  float tol;
+  float err_coef;
+  float sigma_max2;
+};
+
+hipError_t tree_sort_temp_bytes(int n, size_t* bytes);
+// bbox, Morton codes, radix sort, gather into tb.sorted, leaf and parent
+// moments.
+hipError_t build_tree(const Rec64* y, int n, TreeBuffers& tb, const TreeShape& shape,
+                      hipStream_t st);
+// near and far lists of every target leaf
+hipError_t build_lists(const TreeBuffers& tb, const TreeShape& shape, const ListParams& lp,
+                       hipStream_t st);
+
+// per target leaf: out[2T] = near pairs, out[2T+1] = far evaluations
+hipError_t launch_list_stats(const TreeBuffers& tb, const TreeShape& shape, long long* out, hipStream_t st);
+
+// Near field over the lists: rates_s[t] = sum over the near leaves of
+// target t's leaf (FP32C pair math, PSE included), Morton order.
+hipError_t launch_near_fp32c(const Rec64* sorted, const NodeRec* nodes, const int* near_list,
+                             const int* near_cnt, int cap_near, int nleaf, double nu,
+                             Rate64* rates_s, hipStream_t st);
+
+// Far field over the lists, analytic monopole + dipole (FP32), added to
+// rates_s; with mlp != nullptr the learned residual is added as well.
+struct MlpWeights;  // packed BF16 fragments, pw_mlp.hip
+hipError_t launch_far(const Rec64* sorted, const NodeRec* nodes, const int* far_list,
+                      const int* far_cnt, int cap_far, int nleaf, const MlpWeights* mlp,
+                      Rate64* rates_s, hipStream_t st);
+
+// rates[perm[i]] = rates_s[i]
+hipError_t launch_scatter_rates(const Rate64* rates_s, const int* perm, int n, Rate64* rates,
+                                hipStream_t st);
+
 }  // namespace pw

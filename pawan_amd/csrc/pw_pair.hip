@@ -399,6 +399,96 @@ __device__ __forceinline__ void item_fp32c(const PairArgs& pa, const WorkItem& i
 }
 
 // ---------------------------------------------------------------------
+// Near field of the tree path: block = target leaf T of the Morton-sorted
+// state, sources = the leaves of T's near list, one LDS tile each; the
+// pair math and compensation are those of item_fp32c.
+// ---------------------------------------------------------------------
+struct NearArgs {
+  const Rec64* sorted;
+  const NodeRec* nodes;
+  const int* list;
+  const int* cnt;
+  int cap;
+  float nu;
+  Rate64* out;
+};
+
+__global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_near_fp32c(NearArgs a) {
+  __shared__ SrcT tile[kTB];
+  const int T = blockIdx.x;
+  const int t0 = a.nodes[T].first, t1 = a.nodes[T].last;
+  const int t = t0 + static_cast<int>(threadIdx.x);
+  const bool active = t < t1;
+  const Rec64 m = a.sorted[active ? t : t0];
+  T32 me;
+  me.x = m.x; me.y = m.y; me.z = m.z;
+  me.hs2 = static_cast<float>(0.5 * m.s2);
+  me.ax = static_cast<float>(m.ax);
+  me.ay = static_cast<float>(m.ay);
+  me.az = static_cast<float>(m.az);
+  me.vol = static_cast<float>(m.vol);
+  me.nu2 = 2.0f * a.nu;
+  float hi[kAcc], lo[kAcc];
+#pragma unroll
+  for (int k = 0; k < kAcc; ++k) hi[k] = lo[k] = 0.0f;
+  PSE32 pse;
+  const int nl = a.cnt[T];
+  const int* __restrict__ L = a.list + static_cast<size_t>(T) * a.cap;
+  for (int k = 0; k < nl; ++k) {
+    const int id = L[k];
+    const int base = a.nodes[id].first;
+    const int cnt = a.nodes[id].last - base;
+    __syncthreads();
+    if (static_cast<int>(threadIdx.x) < cnt) {
+      const Rec64 r = a.sorted[base + threadIdx.x];
+      SrcT f;
+      f.x = r.x; f.y = r.y; f.z = r.z;
+      f.hs2 = static_cast<float>(0.5 * r.s2);
+      f.ax = static_cast<float>(r.ax);
+      f.ay = static_cast<float>(r.ay);
+      f.az = static_cast<float>(r.az);
+      f.vol = static_cast<float>(r.vol);
+      f.pad = 0.0f;
+      tile[threadIdx.x] = f;
+    }
+    __syncthreads();
+    if (!active) continue;
+    const bool diag = id == T;
+    const int self = static_cast<int>(threadIdx.x);
+    for (int c0 = 0; c0 < cnt; c0 += kChunk) {
+      const int c1 = min(cnt, c0 + kChunk);
+      P32 p;
+      if (diag) {
+        for (int l = c0; l < c1; ++l)
+          if (l != self) pair32(me, tile[l], p, pse);
+      } else {
+#pragma unroll 2
+        for (int l = c0; l < c1; ++l) pair32(me, tile[l], p, pse);
+      }
+      two_sum(hi[0], lo[0], p.ux);
+      two_sum(hi[1], lo[1], p.uy);
+      two_sum(hi[2], lo[2], p.uz);
+      two_sum(hi[3], lo[3], p.wx);
+      two_sum(hi[4], lo[4], p.wy);
+      two_sum(hi[5], lo[5], p.wz);
+    }
+  }
+  if (active) {
+    double acc[kAcc];
+#pragma unroll
+    for (int k = 0; k < kAcc; ++k) acc[k] = static_cast<double>(hi[k]) + static_cast<double>(lo[k]);
+    Rate64 r;
+    r.ux = kQ * acc[0];
+    r.uy = kQ * acc[1];
+    r.uz = kQ * acc[2];
+    r.wx = kQ * acc[3] + static_cast<double>(pse.x);
+    r.wy = kQ * acc[4] + static_cast<double>(pse.y);
+    r.wz = kQ * acc[5] + static_cast<double>(pse.z);
+    a.out[t] = r;
+  }
+}
+
+// ---------------------------------------------------------------------
 // Entry kernels.  A grid kernel runs item blockIdx.x; a persistent kernel
 // keeps its blocks resident and pulls items from an atomic queue (items
 // are ordered largest first), so uneven items leave no idle tail.
@@ -508,6 +598,15 @@ const bool P = cfg.persistent;
   if (e != hipSuccess || cfg.parts <= 1) return e;
   hipLaunchKernelGGL(k_reduce_parts, dim3((args.n + 255) / 256), dim3(256), 0, st, args.partial,
                      cfg.parts, args.n, args.rates);
+  return hipGetLastError();
+}
+
+hipError_t launch_near_fp32c(const Rec64* sorted, const NodeRec* nodes, const int* near_list,
+                             const int* near_cnt, int cap_near, int nleaf, double nu,
+                             Rate64* rates_s, hipStream_t st) {
+  if (nleaf <= 0) return hipSuccess;
+  NearArgs a{sorted, nodes, near_list, near_cnt, cap_near, static_cast<float>(nu), rates_s};
+  hipLaunchKernelGGL(k_near_fp32c, dim3(nleaf), dim3(kTB), 0, st, a);
   return hipGetLastError();
 }
 

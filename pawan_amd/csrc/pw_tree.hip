@@ -1,0 +1,346 @@
+// pw_tree.hip -- the cluster tree of the learned far field.
+//
+// Particles are ordered along a Morton (Z-order) curve of their positions
+// and cut into leaves of 256 consecutive particles (a linear octree with a
+// fixed leaf size); 8 consecutive nodes of one level form a node of the
+// next.  Every node stores its centre (particle mean), a bounding radius,
+// and the vorticity moments the far field needs:
+//   M  = sum a,   D_ij = sum a_i delta_j,   (delta = x - centre)
+//   s0 = sum |a|, S1 = sum |a| delta,  s2 = sum |a| |delta|^2
+// (D, S1, s2 of a parent from its children by the parallel-axis shift).
+// For each target leaf the tree is walked once per right-hand side:
+// nodes that pass the opening-angle test (r_node + r_leaf < theta d), lie
+// beyond the Gaussian core, and whose estimated far-evaluation error
+// c s2 / (d - r_leaf)^4 (the first neglected, quadrupole, order; c is
+// calibrated for the evaluation in use) is within the velocity tolerance
+// go to the far list; leaves that fail go to the near list (evaluated by
+// the direct sum), other nodes that fail are opened.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <cfloat>
+
+#include "pw_internal.h"
+
+namespace pw {
+namespace {
+
+constexpr int kLeaf = kLeafSize;
+// Far nodes keep every pair beyond rho^2 > 44 sigma^2, where the Gaussian
+// terms of the kernel are below FP32 rounding.
+constexpr float kFar32List = 44.0f;
+
+// ---- bounding box -------------------------------------------------------
+__global__ __launch_bounds__(256) void k_bbox_partial(const Rec64* __restrict__ y, int n,
+                                                      double* __restrict__ part) {
+  __shared__ double red[6][256];
+  double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    const Rec64 r = y[i];
+    lo[0] = fmin(lo[0], r.x); lo[1] = fmin(lo[1], r.y); lo[2] = fmin(lo[2], r.z);
+    hi[0] = fmax(hi[0], r.x); hi[1] = fmax(hi[1], r.y); hi[2] = fmax(hi[2], r.z);
+  }
+  for (int k = 0; k < 3; ++k) {
+    red[k][threadIdx.x] = lo[k];
+    red[3 + k][threadIdx.x] = hi[k];
+  }
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (static_cast<int>(threadIdx.x) < s)
+      for (int k = 0; k < 3; ++k) {
+        red[k][threadIdx.x] = fmin(red[k][threadIdx.x], red[k][threadIdx.x + s]);
+        red[3 + k][threadIdx.x] = fmax(red[3 + k][threadIdx.x], red[3 + k][threadIdx.x + s]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 6; ++k) part[6 * blockIdx.x + k] = red[k][0];
+}
+
+__global__ void k_bbox_final(const double* __restrict__ part, int nb, double* __restrict__ box) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double b[6] = {DBL_MAX, DBL_MAX, DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX};
+  for (int i = 0; i < nb; ++i)
+    for (int k = 0; k < 3; ++k) {
+      b[k] = fmin(b[k], part[6 * i + k]);
+      b[3 + k] = fmax(b[3 + k], part[6 * i + 3 + k]);
+    }
+  for (int k = 0; k < 6; ++k) box[k] = b[k];
+}
+
+// ---- Morton codes -------------------------------------------------------
+__device__ __forceinline__ uint32_t spread10(uint32_t v) {
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x30000ffu;
+  v = (v | (v << 8)) & 0x300f00fu;
+  v = (v | (v << 4)) & 0x30c30c3u;
+  v = (v | (v << 2)) & 0x9249249u;
+  return v;
+}
+
+__global__ __launch_bounds__(256) void k_morton(const Rec64* __restrict__ y, int n,
+                                                const double* __restrict__ box,
+                                                uint32_t* __restrict__ code, int* __restrict__ idx) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double span = fmax(fmax(box[3] - box[0], box[4] - box[1]), fmax(box[5] - box[2], 1e-300));
+  const double s = 1023.0 / (span * 1.0000001);
+  const Rec64 r = y[i];
+  const uint32_t qx = static_cast<uint32_t>((r.x - box[0]) * s);
+  const uint32_t qy = static_cast<uint32_t>((r.y - box[1]) * s);
+  const uint32_t qz = static_cast<uint32_t>((r.z - box[2]) * s);
+  code[i] = spread10(qx) | (spread10(qy) << 1) | (spread10(qz) << 2);
+  idx[i] = i;
+}
+
+__global__ __launch_bounds__(256) void k_gather(const Rec64* __restrict__ y, const int* __restrict__ perm,
+                                                int n, Rec64* __restrict__ out) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) out[i] = y[perm[i]];
+}
+
+// ---- leaf moments: one block per leaf -----------------------------------
+// Deterministic block sum of M doubles (wave butterfly, then the 4 wave
+// totals in a fixed order).
+template <int M>
+__device__ __forceinline__ void block_sum_d(double (&v)[M], double* red) {
+  for (int off = 32; off > 0; off >>= 1)
+    for (int m = 0; m < M; ++m) v[m] += __shfl_xor(v[m], off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0)
+    for (int m = 0; m < M; ++m) red[wave * M + m] = v[m];
+  __syncthreads();
+  for (int m = 0; m < M; ++m) v[m] = (red[m] + red[M + m]) + (red[2 * M + m] + red[3 * M + m]);
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(kLeaf) void k_leaf_moments(const Rec64* __restrict__ ys, int n,
+                                                        NodeRec* __restrict__ nodes) {
+  __shared__ double red[4 * 18];
+  const int leaf = blockIdx.x;
+  const int first = leaf * kLeaf, last = min(first + kLeaf, n);
+  const int i = first + static_cast<int>(threadIdx.x);
+  const bool on = i < last;
+  const Rec64 p = ys[on ? i : first];
+  double c[3] = {on ? p.x : 0.0, on ? p.y : 0.0, on ? p.z : 0.0};
+  block_sum_d<3>(c, red);
+  const double cnt = static_cast<double>(last - first);
+  const double cx = c[0] / cnt, cy = c[1] / cnt, cz = c[2] / cnt;
+  const double dx = p.x - cx, dy = p.y - cy, dz = p.z - cz;
+  const double am = sqrt(p.ax * p.ax + p.ay * p.ay + p.az * p.az);
+  double v[18];
+  if (on) {
+    v[0] = p.ax; v[1] = p.ay; v[2] = p.az;
+    v[3] = p.ax * dx; v[4] = p.ax * dy; v[5] = p.ax * dz;
+    v[6] = p.ay * dx; v[7] = p.ay * dy; v[8] = p.ay * dz;
+    v[9] = p.az * dx; v[10] = p.az * dy; v[11] = p.az * dz;
+    v[12] = am; v[13] = am * dx; v[14] = am * dy; v[15] = am * dz;
+    v[16] = am * (dx * dx + dy * dy + dz * dz);
+    v[17] = sqrt(p.s2);
+  } else {
+    for (int m = 0; m < 18; ++m) v[m] = 0.0;
+  }
+  block_sum_d<18>(v, red);
+  // bounding radius about the centre
+  double d = on ? sqrt(dx * dx + dy * dy + dz * dz) : 0.0;
+  for (int off = 32; off > 0; off >>= 1) d = fmax(d, __shfl_xor(d, off, 64));
+  __shared__ double rmax[4];
+  if ((threadIdx.x & 63) == 0) rmax[threadIdx.x >> 6] = d;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    NodeRec nd;
+    nd.cx = cx; nd.cy = cy; nd.cz = cz;
+    // rounded up so that the radius stays a bound in FP32
+    nd.r = static_cast<float>(fmax(fmax(rmax[0], rmax[1]), fmax(rmax[2], rmax[3])) * (1.0 + 1e-6));
+    for (int m = 0; m < 3; ++m) nd.M[m] = static_cast<float>(v[m]);
+    for (int m = 0; m < 9; ++m) nd.D[m] = static_cast<float>(v[3 + m]);
+    nd.s0 = static_cast<float>(v[12]);
+    for (int m = 0; m < 3; ++m) nd.S1[m] = static_cast<float>(v[13 + m]);
+    nd.s2 = static_cast<float>(v[16]);
+    nd.sg = static_cast<float>(v[17] / cnt);
+    nd.first = first; nd.last = last;
+    nd.level = 0; nd.child = -1;
+    nd.pad[0] = nd.pad[1] = nd.pad[2] = 0;
+    nodes[leaf] = nd;
+  }
+}
+
+// ---- parent moments: one thread per node of a level ----------------------
+// Moments about the parent centre from the children's by the parallel-axis
+// shift h = c_child - c_parent:
+//   D += M_c h^T,  S1 += S1_c + s0_c h,  s2 += s2_c + 2 S1_c.h + s0_c |h|^2,
+//   r = max(|h| + r_c).
+__global__ __launch_bounds__(64) void k_parent_moments(NodeRec* __restrict__ nodes, int child_off,
+                                                       int nchild, int off, int count, int level) {
+  const int j = blockIdx.x * 64 + threadIdx.x;
+  if (j >= count) return;
+  const int c0 = child_off + kFan * j, c1 = child_off + min(kFan * (j + 1), nchild);
+  double sx = 0, sy = 0, sz = 0, cnt = 0, sg = 0;
+  for (int c = c0; c < c1; ++c) {
+    const double w = nodes[c].last - nodes[c].first;
+    sx += w * nodes[c].cx; sy += w * nodes[c].cy; sz += w * nodes[c].cz; cnt += w;
+    sg += w * nodes[c].sg;
+  }
+  NodeRec p;
+  p.cx = sx / cnt; p.cy = sy / cnt; p.cz = sz / cnt;
+  double M[3] = {0, 0, 0}, D[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, s0 = 0, S1[3] = {0, 0, 0}, s2 = 0, r = 0;
+  for (int c = c0; c < c1; ++c) {
+    const NodeRec& q = nodes[c];
+    const double h[3] = {q.cx - p.cx, q.cy - p.cy, q.cz - p.cz};
+    double h2 = 0, s1h = 0;
+    for (int a = 0; a < 3; ++a) {
+      M[a] += q.M[a];
+      for (int b = 0; b < 3; ++b) D[3 * a + b] += q.D[3 * a + b] + static_cast<double>(q.M[a]) * h[b];
+      S1[a] += q.S1[a] + static_cast<double>(q.s0) * h[a];
+      h2 += h[a] * h[a];
+      s1h += q.S1[a] * h[a];
+    }
+    s0 += q.s0;
+    s2 += q.s2 + 2.0 * s1h + q.s0 * h2;
+    r = fmax(r, sqrt(h2) + q.r);
+  }
+  p.r = static_cast<float>(r * (1.0 + 1e-6));
+  p.s0 = static_cast<float>(s0);
+  p.s2 = static_cast<float>(s2);
+  p.sg = static_cast<float>(sg / cnt);
+  for (int m = 0; m < 3; ++m) p.M[m] = static_cast<float>(M[m]);
+  for (int m = 0; m < 9; ++m) p.D[m] = static_cast<float>(D[m]);
+  for (int m = 0; m < 3; ++m) p.S1[m] = static_cast<float>(S1[m]);
+  p.first = nodes[c0].first;
+  p.last = nodes[c1 - 1].last;
+  p.level = level;
+  p.child = c0;
+  p.pad[0] = c1 - c0;
+  p.pad[1] = p.pad[2] = 0;
+  nodes[off + j] = p;
+}
+
+// ---- interaction lists: one thread per target leaf -----------------------
+// Depth-first walk from the root; children are pushed last-first so the
+// lists come out in Morton order.
+__global__ __launch_bounds__(64) void k_lists(const NodeRec* __restrict__ nodes, TreeShape shape,
+                                              ListParams lp, int* __restrict__ near_list,
+                                              int* __restrict__ near_cnt, int cap_near,
+                                              int* __restrict__ far_list, int* __restrict__ far_cnt,
+                                              int cap_far, int* __restrict__ overflow) {
+  const int T = blockIdx.x * 64 + threadIdx.x;
+  if (T >= shape.nleaf) return;
+  const NodeRec t = nodes[T];
+  int stack[kStack];
+  int sp = 0;
+  stack[sp++] = shape.root;
+  int nn = 0, nf = 0;
+  bool over = false;
+  while (sp > 0) {
+    const int id = stack[--sp];
+    const NodeRec& q = nodes[id];
+    const double ddx = q.cx - t.cx, ddy = q.cy - t.cy, ddz = q.cz - t.cz;
+    const float d = static_cast<float>(sqrt(ddx * ddx + ddy * ddy + ddz * ddz));
+    const float gap = d - q.r - t.r;
+    bool far = (q.r + t.r) < lp.theta * d && gap > 0.0f && gap * gap > kFar32List * lp.sigma_max2;
+    if (far && lp.tol > 0.0f) {
+      const float dm = d - t.r;
+      far = lp.err_coef * q.s2 <= lp.tol * (dm * dm) * (dm * dm);
+    }
+    if (far) {
+      if (nf < cap_far) far_list[static_cast<size_t>(T) * cap_far + nf] = id;
+      else over = true;
+      ++nf;
+    } else if (q.level == 0) {
+      if (nn < cap_near) near_list[static_cast<size_t>(T) * cap_near + nn] = id;
+      else over = true;
+      ++nn;
+    } else {
+      for (int c = q.child + q.pad[0] - 1; c >= q.child; --c) {
+        if (sp < kStack) stack[sp++] = c;
+        else over = true;
+      }
+    }
+  }
+  near_cnt[T] = min(nn, cap_near);
+  far_cnt[T] = min(nf, cap_far);
+  if (over) atomicOr(overflow, 1);
+}
+
+// Per target leaf: near pairs (targets x near sources, self pairs
+// excluded) and far evaluations (targets x far nodes).
+__global__ __launch_bounds__(64) void k_list_stats(const NodeRec* __restrict__ nodes, int nleaf,
+                                                   const int* __restrict__ near_list,
+                                                   const int* __restrict__ near_cnt, int cap_near,
+                                                   const int* __restrict__ far_cnt, long long* __restrict__ out) {
+  const int T = blockIdx.x * 64 + threadIdx.x;
+  if (T >= nleaf) return;
+  const long long nt = nodes[T].last - nodes[T].first;
+  long long src = 0;
+  for (int k = 0; k < near_cnt[T]; ++k) {
+    const int id = near_list[static_cast<size_t>(T) * cap_near + k];
+    src += nodes[id].last - nodes[id].first;
+  }
+  out[2 * T] = nt * src - nt;  // the leaf itself is on its near list
+  out[2 * T + 1] = nt * far_cnt[T];
+}
+
+}  // namespace
+
+hipError_t launch_list_stats(const TreeBuffers& tb, const TreeShape& shape, long long* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_list_stats, dim3((shape.nleaf + 63) / 64), dim3(64), 0, st, tb.nodes, shape.nleaf,
+                     tb.near_list, tb.near_cnt, tb.cap_near, tb.far_cnt, out);
+  return hipGetLastError();
+}
+
+TreeShape tree_shape(int n) {
+  TreeShape s{};
+  s.n = n;
+  s.nleaf = (n + kLeaf - 1) / kLeaf;
+  int count = s.nleaf, off = 0, L = 0;
+  for (;;) {
+    s.level_off[L] = off;
+    s.level_count[L] = count;
+    off += count;
+    ++L;
+    if (count == 1 || L == kMaxLevels) break;
+    count = (count + kFan - 1) / kFan;
+  }
+  s.nlevel = L;
+  s.nnodes = off;
+  s.root = s.level_off[L - 1];
+  return s;
+}
+
+hipError_t tree_sort_temp_bytes(int n, size_t* bytes) {
+  return hipcub::DeviceRadixSort::SortPairs(nullptr, *bytes, static_cast<uint32_t*>(nullptr),
+                                            static_cast<uint32_t*>(nullptr), static_cast<int*>(nullptr),
+                                            static_cast<int*>(nullptr), n, 0, 30);
+}
+
+hipError_t build_tree(const Rec64* y, int n, TreeBuffers& tb, const TreeShape& shape, hipStream_t st) {
+  const int nb = min(1024, (n + 255) / 256);
+  hipLaunchKernelGGL(k_bbox_partial, dim3(nb), dim3(256), 0, st, y, n, tb.bbox_part);
+  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, st, tb.bbox_part, nb, tb.bbox);
+  hipLaunchKernelGGL(k_morton, dim3((n + 255) / 256), dim3(256), 0, st, y, n, tb.bbox, tb.code,
+                     tb.idx);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  size_t bytes = tb.sort_temp_bytes;
+  e = hipcub::DeviceRadixSort::SortPairs(tb.sort_temp, bytes, tb.code, tb.code_sorted, tb.idx,
+                                         tb.perm, n, 0, 30, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, st, y, tb.perm, n, tb.sorted);
+  hipLaunchKernelGGL(k_leaf_moments, dim3(shape.nleaf), dim3(kLeaf), 0, st, tb.sorted, n, tb.nodes);
+  for (int L = 1; L < shape.nlevel; ++L)
+    hipLaunchKernelGGL(k_parent_moments, dim3((shape.level_count[L] + 63) / 64), dim3(64), 0, st,
+                       tb.nodes, shape.level_off[L - 1], shape.level_count[L - 1], shape.level_off[L],
+                       shape.level_count[L], L);
+  return hipGetLastError();
+}
+
+hipError_t build_lists(const TreeBuffers& tb, const TreeShape& shape, const ListParams& lp,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(k_lists, dim3((shape.nleaf + 63) / 64), dim3(64), 0, st, tb.nodes, shape, lp,
+                     tb.near_list, tb.near_cnt, tb.cap_near, tb.far_list, tb.far_cnt, tb.cap_far,
+                     tb.overflow);
+  return hipGetLastError();
+}
+
+}  // namespace pw

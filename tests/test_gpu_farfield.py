@@ -1,0 +1,254 @@
+"""The tree path (PW_LEARNED) on the GPU against the far-field oracle:
+Morton order, node moments, the interaction lists, the analytic far field
+on the GPU's own lists, training samples, the network's inference (BF16
+matrix cores) and training gradient, and the approximation error against
+the direct sum."""
+import numpy as np
+import pytest
+
+from conftest import normwise
+from oracle import farfield as FF
+from pawan_amd import capi, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def helix():
+    return W.helix_wake(32768, seed=3)
+
+
+@pytest.fixture(scope="module")
+def built(helix):
+    """Tree, lists and exports of the helix at theta 0.5 (no error test)."""
+    with capi.Solver(helix) as g:
+        g.ff_set_params(0.5, 0.0, use_mlp=False)
+        g.ff_build()
+        out = {k: g.ff_export(w) for k, w in (("perm", capi.FF_PERM), ("nodes", capi.FF_NODES),
+                                              ("near_cnt", capi.FF_NEAR_CNT), ("near", capi.FF_NEAR),
+                                              ("far_cnt", capi.FF_FAR_CNT), ("far", capi.FF_FAR))}
+        out["info"] = g.ff_info()
+        u, w = g.solve("learned")
+        out["u"], out["w"] = u, w
+    _say(f"tree {out['info']}")
+    return out
+
+
+def test_morton_order_matches_oracle(helix, built):
+    np.testing.assert_array_equal(built["perm"], FF.morton_perm(helix.pos))
+
+
+def test_node_moments_match_oracle(helix, built):
+    perm = built["perm"]
+    mo = FF.node_moments(helix.pos[perm], helix.alpha[perm], helix.radius[perm])
+    nd = built["nodes"]
+    ext = np.ptp(helix.pos, axis=0).max()
+    np.testing.assert_array_equal(nd[:, 19], mo["first"])
+    np.testing.assert_array_equal(nd[:, 20], mo["last"])
+    np.testing.assert_array_equal(nd[:, 21], mo["level"])
+    np.testing.assert_allclose(nd[:, 0:3], mo["centre"], rtol=0, atol=1e-12 * ext)
+    leaf = mo["level"] == 0
+    # radius: rounded-up FP32 bound; exact for leaves, a bound for parents
+    assert np.all(nd[:, 3] >= mo["r"] * (1 - 1e-7))
+    assert np.all(nd[leaf, 3] <= mo["r"][leaf] * (1 + 3e-6))
+    np.testing.assert_allclose(nd[:, 4], mo["s0"], rtol=2e-6)
+    np.testing.assert_allclose(nd[:, 5], mo["s2"], rtol=2e-5)
+    np.testing.assert_allclose(nd[:, 6], mo["sg"], rtol=2e-6)
+    scale_m = mo["s0"][:, None]
+    assert np.all(np.abs(nd[:, 7:10] - mo["M"]) <= 2e-6 * scale_m)
+    scale_d = (mo["s0"] * mo["r"])[:, None]
+    assert np.all(np.abs(nd[:, 10:19] - mo["D"].reshape(-1, 9)) <= 2e-5 * scale_d)
+
+
+def test_lists_cover_every_pair_once(built):
+    nd, info = built["nodes"], built["info"]
+    first, last = nd[:, 19].astype(np.int64), nd[:, 20].astype(np.int64)
+    nl = info["leaves"]
+    assert info["overflow"] == 0
+    assert FF.list_partition_errors(first, last, nl, built["near_cnt"], built["near"],
+                                    built["far_cnt"], built["far"]) == 0
+    for T in range(nl):
+        assert T in built["near"][T, :built["near_cnt"][T]]
+
+
+def test_far_nodes_pass_the_opening_test(built):
+    nd, nl = built["nodes"], built["info"]["leaves"]
+    for T in range(0, nl, 7):
+        for q in built["far"][T, :built["far_cnt"][T]]:
+            d = np.linalg.norm(nd[q, 0:3] - nd[T, 0:3])
+            assert nd[q, 3] + nd[T, 3] < 0.5 * d * (1 + 1e-5)
+
+
+def _oracle_rates(oracle, helix, built, targets_sorted):
+    """FP64 rates of sorted targets with the GPU's own lists: near leaves
+    by the oracle's pair sums, far nodes by the expansion."""
+    perm, nd = built["perm"], built["nodes"]
+    sp, sa = helix.pos[perm], helix.alpha[perm]
+    sr, sv = helix.radius[perm], helix.volume[perm]
+    us, ws = [], []
+    for t in targets_sorted:
+        T = t // FF.LEAF
+        src = []
+        for q in built["near"][T, :built["near_cnt"][T]]:
+            src.extend(range(int(nd[q, 19]), int(nd[q, 20])))
+        src = [s for s in src if s != t]
+        idx = np.array([t] + src)
+        sub = W.ParticleSystem(off=np.array([0, len(idx)]), pos=sp[idx], alpha=sa[idx], radius=sr[idx],
+                               volume=sv[idx], nu=helix.nu, coupled=True, name="near")
+        u, w = oracle.rates_targets(sub, [0])
+        u, w = u[0].copy(), w[0].copy()
+        for q in built["far"][T, :built["far_cnt"][T]]:
+            uq, Gq = FF.md_eval((sp[t] - nd[q, 0:3])[None], nd[q, 7:10], nd[q, 10:19].reshape(3, 3))
+            u += uq[0] / FF.FOUR_PI
+            w += sa[t] @ Gq[0] / FF.FOUR_PI
+        us.append(u)
+        ws.append(w)
+    return np.array(us), np.array(ws)
+
+
+def test_analytic_far_field_matches_oracle_on_gpu_lists(oracle, helix, built):
+    perm = built["perm"]
+    rng = np.random.default_rng(0)
+    targets = np.sort(rng.choice(helix.n, size=48, replace=False))
+    ru, rw = _oracle_rates(oracle, helix, built, targets)
+    gu, gw = built["u"][perm[targets]], built["w"][perm[targets]]
+    assert normwise(gu, ru) < 1e-5, normwise(gu, ru)
+    assert normwise(gw, rw) < 1e-5, normwise(gw, rw)
+
+
+def test_error_against_direct_sum_falls_with_theta(helix):
+    errs = {}
+    with capi.Solver(helix) as g:
+        ud, wd = g.solve("fp64")
+        for theta in (0.6, 0.45, 0.3):
+            g.ff_set_params(theta, 0.0, use_mlp=False)
+            u, w = g.solve("learned")
+            errs[theta] = (normwise(u, ud), normwise(w, wd))
+            _say(f"theta {theta}: u {errs[theta][0]:.2e} w {errs[theta][1]:.2e} {g.ff_info()}")
+    assert errs[0.6][0] < 3e-2
+    assert errs[0.3][0] < errs[0.45][0] < errs[0.6][0]
+    assert errs[0.3][1] < errs[0.6][1]
+
+
+def test_error_tolerance_opens_nodes(helix):
+    """The error test tightens the lists: a smaller tolerance gives more
+    near pairs and a smaller error."""
+    with capi.Solver(helix) as g:
+        ud, _ = g.solve("fp64")
+        urms = float(np.sqrt((ud * ud).sum(axis=1).mean()))
+        res = []
+        for tol in (0.0, 1e-3, 1e-4):
+            g.ff_set_params(0.5, tol * urms, use_mlp=False)
+            u, _ = g.solve("learned")
+            res.append((normwise(u, ud), g.ff_info()["near_pairs"]))
+            _say(f"tol {tol}: err {res[-1][0]:.2e} near pairs {res[-1][1]}")
+    assert res[1][0] <= res[0][0] and res[2][0] < res[1][0]
+    assert res[2][1] > res[1][1] >= res[0][1]
+
+
+def test_samples_match_oracle(helix, built):
+    with capi.Solver(helix) as g:
+        g.ff_set_params(0.5, 0.0, use_mlp=False)
+        g.ff_sample(40, seed=9)
+        X = g.ff_export(capi.FF_SAMPLES_X)
+        Y = g.ff_export(capi.FF_SAMPLES_Y)
+        A = g.ff_export(capi.FF_SAMPLES_AUX)
+    perm, nd = built["perm"], built["nodes"]
+    sp, sa, sr = helix.pos[perm], helix.alpha[perm], helix.radius[perm]
+    for k in range(len(X)):
+        q, t = int(A[k, 7]), int(A[k, 8])
+        a, b = int(nd[q, 19]), int(nd[q, 20])
+        assert not (a <= t < b)
+        ue, Ge = FF.exact_node(sp[t], sr[t], sp[a:b], sa[a:b], sr[a:b])
+        r = sp[t] - nd[q, 0:3]
+        um, Gm = FF.md_eval(r[None], nd[q, 7:10], nd[q, 10:19].reshape(3, 3))
+        xo, su, sG = FF.features(r[None], nd[q, 3] + nd[q, 6], nd[q, 4], nd[q, 7:10],
+                                 nd[q, 10:19].reshape(3, 3), nd[q, 6])
+        np.testing.assert_allclose(X[k], xo[0], rtol=2e-6, atol=2e-7)
+        # residual in true units against the FP64 residual, relative to the
+        # expansion's size (the GPU expansion is FP32)
+        res_u = Y[k, :3] * su[0]
+        res_G = Y[k, 3:].reshape(3, 3) * sG[0]
+        assert np.linalg.norm(res_u - (ue - um[0])) <= 1e-5 * np.linalg.norm(um[0]), k
+        assert np.linalg.norm(res_G - (Ge - Gm[0])) <= 1e-5 * np.linalg.norm(Gm[0]), k
+        assert A[k, 4] == pytest.approx(np.linalg.norm(ue - um[0]), rel=1e-3, abs=1e-6 * np.linalg.norm(um[0]))
+
+
+def _rand_params(seed, scale=1.0):
+    rng = np.random.default_rng(seed)
+    p = np.zeros(FF.N_PARAMS, dtype=np.float32)
+    for name, (o, size) in FF.OFF.items():
+        fan_in = {"W1": 16, "W2": 128, "W3": 128, "W4": 128}.get(name, 128)
+        p[o:o + size] = rng.normal(size=size) * scale / np.sqrt(fan_in) if name[0] == "W" else rng.normal(size=size) * 0.1
+    return p
+
+
+def test_mlp_inference_matches_bf16_oracle(helix):
+    """The matrix-core network against the oracle with the same BF16
+    roundings (inputs, weights, activations); the FP64 network without them
+    differs visibly, so the comparison is sensitive to the roundings, and a
+    wrong operand layout would be an O(1) error."""
+    params = _rand_params(4)
+    rng = np.random.default_rng(5)
+    X = rng.normal(size=(1000, 16)).astype(np.float32)
+    with capi.Solver(helix) as g:
+        g.ff_set_weights(params)
+        Yg = g.ff_mlp_forward(X)
+        np.testing.assert_array_equal(g.ff_get_weights(), params)
+    Ye = FF.mlp_forward(params, X, bf16_emulate=True)
+    Yf = FF.mlp_forward(params, X, bf16_emulate=False)
+    e_emu, e_fp64 = normwise(Yg, Ye), normwise(Yg, Yf)
+    _say(f"mlp forward: vs bf16 oracle {e_emu:.2e}, vs fp64 oracle {e_fp64:.2e}")
+    assert e_emu < 3e-3
+    assert e_emu < 0.5 * e_fp64
+
+
+def test_mlp_training_gradient_matches_oracle(helix):
+    params = _rand_params(6, scale=0.8)
+    rng = np.random.default_rng(7)
+    n = 100  # not a multiple of the 32-row block
+    X = rng.normal(size=(n, 16)).astype(np.float32)
+    Y = rng.normal(size=(n, 12)).astype(np.float32)
+    with capi.Solver(helix) as g:
+        g.ff_set_weights(params)
+        grad, loss = g.ff_mlp_grad(X, Y)
+    lo, go = FF.mlp_loss_grad(params.astype(np.float64), X.astype(np.float64), Y.astype(np.float64))
+    assert loss == pytest.approx(lo, rel=1e-5)
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    assert rel(grad, go) < 1e-5, rel(grad, go)
+    for name, (o, size) in FF.OFF.items():
+        assert rel(grad[o:o + size], go[o:o + size]) < 1e-4, (name, rel(grad[o:o + size], go[o:o + size]))
+
+
+def test_learned_rk4_step_runs_and_tracks_direct(helix):
+    with capi.Solver(helix) as a, capi.Solver(helix) as b:
+        a.ff_set_params(0.4, 0.0, use_mlp=False)
+        a.step("rk4", 1e-3, "learned", 2)
+        b.step("rk4", 1e-3, "fp32c", 2)
+        pa, aa = a.state()
+        pb, ab = b.state()
+    disp = pb - helix.pos
+    assert np.all(np.isfinite(pa)) and np.all(np.isfinite(aa))
+    assert normwise(pa - helix.pos, disp) < 3e-2
+    assert normwise(aa, ab) < 1e-3
+
+
+def test_learned_requires_weights_and_coupling(helix):
+    with capi.Solver(helix) as g:
+        g.ff_set_params(0.5, 0.0, use_mlp=True)
+        with pytest.raises(capi.PawanError) as e:
+            g.solve("learned")
+        assert e.value.code == capi.PW_ERR_STATE
+    s = W.batched_rings(nwakes=3, nmin=300, nmax=600, seed=1)
+    with capi.Solver(s) as g:
+        g.ff_set_params(0.5, 0.0, use_mlp=False)
+        with pytest.raises(capi.PawanError) as e:
+            g.solve("learned")
+        assert e.value.code == capi.PW_ERR_UNSUPPORTED
