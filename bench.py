@@ -79,7 +79,7 @@ def train_network(n_train: int, samples: int, steps: int, theta: float, seed: in
     from pawan_amd import capi, workloads
 
     t0 = time.perf_counter()
-    seeds = [seed, seed + 1]
+    seeds = [seed, seed + 1, seed + 2, seed + 3]
     with capi.Solver(workloads.helix_wake(n_train, seed=seeds[0])) as tr:
         tr.ff_set_params(theta, 0.0, use_mlp=False)
         for k, sd in enumerate(seeds):
@@ -156,11 +156,60 @@ def learned_leg(system, dt: float, steps: int, target_err: float, theta: float, 
             ms, launches = g.pair_timing()
             g.set_profiling(False)
             out[key] = {"steps_per_s": steps / wall, "rhs_ms": ms / max(1, launches), "velocity_error": err,
-                        "tol_abs": tol, "near_pairs": info["near_pairs"], "far_evals": info["far_evals"],
+                        "tol_abs": float(tol), "near_pairs": info["near_pairs"], "far_evals": info["far_evals"],
                         "max_far_nodes": info["max_far"], "max_near_leaves": info["max_near"]}
             log(f"{key}: {out[key]}")
+            if use_mlp:
+                out["_drift_cfg"] = {"weights": weights, "coef_md": rep["coef_md"], "coef_mlp": rep["coef_mlp"],
+                                     "theta": theta, "tol_rel": tol / urms}
             # back to the start state for the next variant
             g.set_wake_state(0, system.pos, system.alpha)
+    return out
+
+
+def drift_leg(steps: int, every: int, learned_cfg: dict | None, target_err: float):
+    """Conserved-quantity drift over `steps` RK4 steps of the leapfrogging
+    rings (two coupled wakes, N = 65,536, dt = 0.002) for FP64, FP32C and
+    (when configured) the tree path with the network: the maximum over the
+    run of |Omega(t) - Omega(0)| / sum|alpha|, |I(t) - I(0)| / |I(0)| and
+    |KE(t) - KE(0)| / |KE(0)|, diagnosed every `every` steps in FP64.  The
+    tree path's tolerance is calibrated on this system to the same velocity
+    error target as at 1M."""
+    from pawan_amd import capi, workloads
+
+    spec = workloads.WORKLOADS["leapfrog_64k"]
+    system, dt = spec["make"](), spec["dt"]
+    s_alpha = float(np.linalg.norm(system.alpha, axis=1).sum())
+    out = {"workload": "leapfrog_64k", "particles": system.n, "steps": steps, "dt": dt, "every": every}
+    variants = [("fp64", None), ("fp32c", None)]
+    if learned_cfg is not None:
+        variants.append(("learned", learned_cfg))
+    for prec, cfg in variants:
+        t0 = time.perf_counter()
+        with capi.Solver(system) as g:
+            if cfg is not None:
+                g.ff_set_weights(cfg["weights"])
+                g.ff_set_err_coef(cfg["coef_md"], cfg["coef_mlp"])
+                ref, _ = g.solve("fp32c")
+                urms = float(np.sqrt((ref * ref).sum(axis=1).mean()))
+                best = calibrate_tol(g, target_err, ref, True, cfg["theta"], urms)
+                if best is None:
+                    out[prec] = None
+                    continue
+                g.ff_set_params(cfg["theta"], best[0], use_mlp=True)
+                out["learned_velocity_error"] = best[1]
+            else:
+                g.set_graph(True)
+            d0 = g.diagnose()
+            dom = di = dke = 0.0
+            for k in range(0, steps, every):
+                g.step("rk4", dt, prec, min(every, steps - k))
+                d = g.diagnose()
+                dom = max(dom, float(np.linalg.norm(d[0:3] - d0[0:3])) / s_alpha)
+                di = max(di, float(np.linalg.norm(d[3:6] - d0[3:6]) / np.linalg.norm(d0[3:6])))
+                dke = max(dke, float(abs(d[12] - d0[12]) / abs(d0[12])))
+        out[prec] = {"omega": dom, "impulse": di, "ke": dke, "seconds": time.perf_counter() - t0}
+        log(f"drift {prec}: {out[prec]}")
     return out
 
 
@@ -246,8 +295,10 @@ def main() -> int:
     ap.add_argument("--learned-err", type=float, default=1e-4)
     ap.add_argument("--theta", type=float, default=0.5)
     ap.add_argument("--train-n", type=int, default=1 << 18)
-    ap.add_argument("--train-samples", type=int, default=1_000_000)
-    ap.add_argument("--train-steps", type=int, default=4000)
+    ap.add_argument("--train-samples", type=int, default=1_200_000)
+    ap.add_argument("--train-steps", type=int, default=15000)
+    ap.add_argument("--no-drift", action="store_true")
+    ap.add_argument("--drift-steps", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
@@ -343,6 +394,9 @@ def main() -> int:
             result["learned_over_fp32c"] = lg["mlp"]["steps_per_s"] / sps32
         if best is not None:
             result["learned_best_steps_per_s"] = best["steps_per_s"]
+        cfg = lg.pop("_drift_cfg", None)
+        if not args.no_drift:
+            result["drift"] = drift_leg(args.drift_steps, 20, cfg, args.learned_err)
     if not args.no_cpu and system.coupled:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(system, args.cpu_seconds)

@@ -71,6 +71,10 @@ class __interaction : public __system {
   //! Pair arithmetic of solve() and the device steps (PW_FP64 default).
   void setPrecision(int precision) { _precision = precision; }
   int getPrecision() const { return _precision; }
+  //! The device context (created on first use and loaded with the current
+  //! host state), for the pw_* calls without a class-API counterpart, such
+  //! as the tree path's training and parameters.
+  pw_ctx *context();
 
   // diagnostics of the last diagnose() call
   const vec *totalVorticity() const { return _totalVorticity; }

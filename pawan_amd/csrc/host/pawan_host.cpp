@@ -254,6 +254,11 @@ void __interaction::ensureDevice() {
   _deviceStale = false;
 }
 
+pw_ctx *__interaction::context() {
+  pushState();
+  return _ctx;
+}
+
 void __interaction::pushState() {
   ensureDevice();
   if (pw_set_nu(_ctx, _nu) != PW_OK) fail("pw_set_nu");

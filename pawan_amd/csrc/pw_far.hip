@@ -22,7 +22,9 @@
 // monopole - dipole) in scaled units:
 //   inputs  e = r l / r^2 (3), M / s0 (3), D / (s0 l) (9), sigma / l (1)
 //   outputs u' residual / (s0 l^2 / r^4) (3), G' residual / (s0 l^2 / r^5) (9)
-// with l = r_node + sigma the node's length scale.  Inference runs in BF16
+// with l = r_node + sigma the node's length scale.  It is evaluated at the
+// centre of each target leaf and carried to the leaf's targets with its own
+// gradient output.  Inference runs in BF16
 // on the matrix cores with FP32 accumulation, in the transposed form
 // H_next^T = W^T H^T: the FP32 result of one 16x16 output tile is exactly
 // the B operand of the next layer in the same lanes, so activations never
@@ -302,20 +304,39 @@ __device__ __forceinline__ void sample_scales(const FarNodeL& q, float r0, float
   el = q.ell * id2;
 }
 
-// Features 4g..4g+3 of one (target, node) sample; also the output scales.
-__device__ __forceinline__ v4bf sample_features(const FarNodeL& q, double x, double y, double z, int g,
-                                                float& su, float& sG) {
+// Learned residual over the far lists, evaluated once per (target leaf,
+// far node) at the leaf centre c_T and carried to the leaf's targets with
+// the gradient the network also predicts:
+//   u_res(x_t) = U + G (x_t - c_T),   G_res(x_t) = G,
+// U, G = sum over the far nodes of the scaled outputs.  The residual falls
+// off as d^-4, so over a leaf (r_T < theta d / 2) the first-order transfer
+// keeps it to O((r_T/d)^2) relative to itself, while the network runs 256
+// times less often than per target.  4 waves per leaf; wave w takes the
+// 32-node passes w, w+4, ...; node k of a pass is sample column k.
+constexpr int kMlpThreads = 256;
+
+__device__ __forceinline__ v4bf node_features(const NodeRec& q, double x, double y, double z, int g, float& su,
+                                              float& sG) {
   const float r0 = static_cast<float>(x - q.cx), r1 = static_cast<float>(y - q.cy),
               r2 = static_cast<float>(z - q.cz);
-  float el;
-  sample_scales(q, r0, r1, r2, su, sG, el);
+  const float ell = q.r + q.sg;
+  const float d2 = fmaf(r2, r2, fmaf(r1, r1, r0 * r0));
+  const float id2 = 1.0f / d2;
+  const float id = __builtin_amdgcn_sqrtf(id2);
+  su = q.s0 * ell * ell * id2 * id2;
+  sG = su * id;
+  const float is0 = q.s0 > 0.0f ? 1.0f / q.s0 : 0.0f;
+  const float isl = is0 / ell;
   float f[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) f[i] = q.feat[4 * g + i];
   if (g == 0) {
-    f[0] = r0 * el;
-    f[1] = r1 * el;
-    f[2] = r2 * el;
+    const float el = ell * id2;
+    f[0] = r0 * el; f[1] = r1 * el; f[2] = r2 * el; f[3] = q.M[0] * is0;
+  } else if (g == 1) {
+    f[0] = q.M[1] * is0; f[1] = q.M[2] * is0; f[2] = q.D[0] * isl; f[3] = q.D[1] * isl;
+  } else if (g == 2) {
+    f[0] = q.D[2] * isl; f[1] = q.D[3] * isl; f[2] = q.D[4] * isl; f[3] = q.D[5] * isl;
+  } else {
+    f[0] = q.D[6] * isl; f[1] = q.D[7] * isl; f[2] = q.D[8] * isl; f[3] = q.sg / ell;
   }
   v4bf v;
 #pragma unroll
@@ -323,83 +344,65 @@ __device__ __forceinline__ v4bf sample_features(const FarNodeL& q, double x, dou
   return v;
 }
 
-// Learned residual over the far lists: 8 waves per target leaf, wave w
-// handles targets 32w..32w+31 against every far node of the leaf.
-constexpr int kMlpThreads = 512;
-constexpr int kMlpNodeChunk = 32;
-
 __global__ __launch_bounds__(kMlpThreads) void k_far_mlp(FarArgs a, MlpWeights W) {
   extern __shared__ __align__(16) unsigned char smem[];
   MlpLds& s = *reinterpret_cast<MlpLds*>(smem);
-  FarNodeL* sn = reinterpret_cast<FarNodeL*>(smem + sizeof(MlpLds));
-  float* res = reinterpret_cast<float*>(smem + sizeof(MlpLds));  // reused at the end: [256][13]
+  float* red = reinterpret_cast<float*>(smem + sizeof(MlpLds));  // [4 waves][16], then [16]
   load_mlp(W, s);
+  __syncthreads();
   const int T = blockIdx.x;
-  const int t0 = a.nodes[T].first, t1 = a.nodes[T].last;
+  const NodeRec leaf = a.nodes[T];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = lane >> 4, c = lane & 15;
-  double px[2], py[2], pz[2];
-  bool on[2];
-#pragma unroll
-  for (int nb = 0; nb < 2; ++nb) {
-    const int t = t0 + 32 * wave + 16 * nb + c;
-    on[nb] = t < t1;
-    const Rec64 r = a.sorted[on[nb] ? t : t0];
-    px[nb] = r.x; py[nb] = r.y; pz[nb] = r.z;
-  }
-  v4f acc[2];
-#pragma unroll
-  for (int nb = 0; nb < 2; ++nb) acc[nb] = v4f{0, 0, 0, 0};
-  const bool wave_on = t0 + 32 * wave < t1;
   const int nf = a.cnt[T];
   const int* __restrict__ L = a.list + static_cast<size_t>(T) * a.cap;
-  for (int base = 0; base < nf; base += kMlpNodeChunk) {
-    const int cn = min(kMlpNodeChunk, nf - base);
-    __syncthreads();
-    if (static_cast<int>(threadIdx.x) < cn) stage_node(a.nodes[L[base + threadIdx.x]], sn[threadIdx.x]);
-    __syncthreads();
-    if (!wave_on) continue;
-    for (int k = 0; k < cn; ++k) {
-      v4bf x[2];
-      float su[2], sG[2];
+  v4f acc = v4f{0, 0, 0, 0};
+  for (int base = 32 * wave; base < nf; base += 32 * 4) {
+    v4bf x[2];
+    float sc[2][2];
 #pragma unroll
-      for (int nb = 0; nb < 2; ++nb) x[nb] = sample_features(sn[k], px[nb], py[nb], pz[nb], g, su[nb], sG[nb]);
-      v4f o[2];
-      mlp_pass(s, x, o);
-#pragma unroll
-      for (int nb = 0; nb < 2; ++nb)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) acc[nb][i] = fmaf(o[nb][i], (4 * g + i) < 3 ? su[nb] : sG[nb], acc[nb][i]);
+    for (int nb = 0; nb < 2; ++nb) {
+      const int k = base + 16 * nb + c;
+      if (k < nf) {
+        const NodeRec& q = a.nodes[L[k]];
+        x[nb] = node_features(q, leaf.cx, leaf.cy, leaf.cz, g, sc[nb][0], sc[nb][1]);
+      } else {
+        x[nb] = v4bf{0, 0, 0, 0};
+        sc[nb][0] = sc[nb][1] = 0.0f;  // padding column contributes nothing
+      }
     }
-  }
-  // gather the 12 outputs of each target: res[target][row]
-  __syncthreads();
-  if (wave_on) {
+    v4f o[2];
+    mlp_pass(s, x, o);
 #pragma unroll
     for (int nb = 0; nb < 2; ++nb)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int row = 4 * g + i;
-        if (row < 12) res[(32 * wave + 16 * nb + c) * 13 + row] = acc[nb][i];
-      }
+      for (int i = 0; i < 4; ++i) acc[i] = fmaf(o[nb][i], (4 * g + i) < 3 ? sc[nb][0] : sc[nb][1], acc[i]);
   }
+  // sum over the 16 columns of each lane group, then over the waves (in order)
+#pragma unroll
+  for (int off = 1; off < 16; off <<= 1)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) acc[i] += __shfl_xor(acc[i], off, 64);
+  if (c == 0)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) red[wave * 16 + 4 * g + i] = acc[i];
   __syncthreads();
-  if (threadIdx.x < 256) {
-    const int t = t0 + static_cast<int>(threadIdx.x);
-    if (t < t1) {
-      const float* o = res + threadIdx.x * 13;
-      const Rec64 me = a.sorted[t];
-      const double ax = me.ax, ay = me.ay, az = me.az;
-      Rate64 r = a.rates[t];
-      r.ux += kQd * o[0];
-      r.uy += kQd * o[1];
-      r.uz += kQd * o[2];
-      r.wx += kQd * (ax * o[3] + ay * o[6] + az * o[9]);
-      r.wy += kQd * (ax * o[4] + ay * o[7] + az * o[10]);
-      r.wz += kQd * (ax * o[5] + ay * o[8] + az * o[11]);
-      a.rates[t] = r;
-    }
-  }
+  if (threadIdx.x < 16)
+    red[64 + threadIdx.x] = ((red[threadIdx.x] + red[16 + threadIdx.x]) + red[32 + threadIdx.x]) + red[48 + threadIdx.x];
+  __syncthreads();
+  const int t = leaf.first + static_cast<int>(threadIdx.x);
+  if (t >= leaf.last) return;
+  const float* S = red + 64;  // U (0..2), G (3..11), row-major G[k][m] = S[3 + 3k + m]
+  const Rec64 me = a.sorted[t];
+  const double dx = me.x - leaf.cx, dy = me.y - leaf.cy, dz = me.z - leaf.cz;
+  Rate64 r = a.rates[t];
+  r.ux += kQd * (S[0] + S[3] * dx + S[4] * dy + S[5] * dz);
+  r.uy += kQd * (S[1] + S[6] * dx + S[7] * dy + S[8] * dz);
+  r.uz += kQd * (S[2] + S[9] * dx + S[10] * dy + S[11] * dz);
+  r.wx += kQd * (me.ax * S[3] + me.ay * S[6] + me.az * S[9]);
+  r.wy += kQd * (me.ax * S[4] + me.ay * S[7] + me.az * S[10]);
+  r.wz += kQd * (me.ax * S[5] + me.ay * S[8] + me.az * S[11]);
+  a.rates[t] = r;
 }
 
 // Plain inference over given inputs (n x 16 FP32, scaled units) -> n x 12:
@@ -544,10 +547,7 @@ __global__ __launch_bounds__(256) void k_scatter(const Rate64* __restrict__ in, 
 
 }  // namespace
 
-size_t mlp_far_lds_bytes() {
-  const size_t tail_nodes = kMlpNodeChunk * sizeof(FarNodeL), tail_res = 256 * 13 * sizeof(float);
-  return sizeof(MlpLds) + (tail_nodes > tail_res ? tail_nodes : tail_res);
-}
+size_t mlp_far_lds_bytes() { return sizeof(MlpLds) + 80 * sizeof(float); }
 
 hipError_t launch_far(const Rec64* sorted, const NodeRec* nodes, const int* far_list, const int* far_cnt,
                       int cap_far, int nleaf, const MlpWeights* mlp, Rate64* rates_s, hipStream_t st) {

@@ -51,7 +51,13 @@ int ensure_ff(pw_ctx* c) {
       dalloc(&tb.sorted, n) != hipSuccess || dalloc(&tb.nodes, f.shape.nnodes) != hipSuccess ||
       dalloc(&tb.near_list, nl * tb.cap_near) != hipSuccess || dalloc(&tb.near_cnt, nl) != hipSuccess ||
       dalloc(&tb.far_list, nl * tb.cap_far) != hipSuccess || dalloc(&tb.far_cnt, nl) != hipSuccess ||
-      dalloc(&f.d_overflow, 1) != hipSuccess || dalloc(&f.rates_s, n) != hipSuccess) {
+      dalloc(&f.d_overflow, 1) != hipSuccess || dalloc(&f.rates_s, n) != hipSuccess ||
+      dalloc(&tb.near_parts, nl) != hipSuccess || dalloc(&tb.near_offs, nl) != hipSuccess ||
+      dalloc(&tb.near_items, nl * pw::kNearMaxParts) != hipSuccess || dalloc(&tb.near_total, 1) != hipSuccess ||
+      dalloc(&tb.near_queue, 1) != hipSuccess ||
+      dalloc(&tb.near_partial, nl * pw::kNearMaxParts * pw::kLeafSize) != hipSuccess ||
+      pw::near_scan_temp_bytes(f.shape.nleaf, &tb.scan_temp_bytes) != hipSuccess ||
+      dalloc(reinterpret_cast<unsigned char**>(&tb.scan_temp), tb.scan_temp_bytes) != hipSuccess) {
     (void)hipGetLastError();
     pwi::ff_release(c);
     return fail(PW_ERR_NOMEM, "tree buffers");
@@ -151,6 +157,13 @@ void ff_release(pw_ctx* c) {
   (void)hipFree(tb.near_cnt);
   (void)hipFree(tb.far_list);
   (void)hipFree(tb.far_cnt);
+  (void)hipFree(tb.near_parts);
+  (void)hipFree(tb.near_offs);
+  (void)hipFree(tb.near_items);
+  (void)hipFree(tb.near_total);
+  (void)hipFree(tb.near_queue);
+  (void)hipFree(tb.near_partial);
+  (void)hipFree(tb.scan_temp);
   (void)hipFree(f.d_overflow);
   (void)hipFree(f.rates_s);
   (void)hipFree(f.d_params);
@@ -180,8 +193,9 @@ int ff_rhs(pw_ctx* c, const Rec64* state, Rate64* out) {
   PW_TRY(build(c, state, false));
   const auto& tb = f.tb;
   const int n = static_cast<int>(c->n);
-  PW_HIP(pw::launch_near_fp32c(tb.sorted, tb.nodes, tb.near_list, tb.near_cnt, tb.cap_near, f.shape.nleaf,
-                               c->nu, f.rates_s, c->stream));
+  PW_HIP(pw::build_near_items(tb, f.shape.nleaf, c->stream));
+  PW_HIP(pw::launch_near_fp32c(tb.sorted, tb.nodes, tb, f.shape.nleaf, c->nu, f.sigma_max2, c->num_cus * 6,
+                               f.rates_s, c->stream));
   pw::MlpWeights W{f.d_frag, f.d_bias};
   PW_HIP(pw::launch_far(tb.sorted, tb.nodes, tb.far_list, tb.far_cnt, tb.cap_far, f.shape.nleaf,
                         f.use_mlp ? &W : nullptr, f.rates_s, c->stream));

@@ -140,6 +140,14 @@ struct TreeShape {
 };
 TreeShape tree_shape(int n);
 
+// Near-field work item: targets = leaf T, sources = T's near list [k0, k1);
+// slot < 0: writes the rates, else partial-sum row `slot`.
+struct NearItem {
+  int T, k0, k1, slot;
+};
+constexpr int kNearChunk = 128;   // near leaves per work item
+constexpr int kNearMaxParts = 32; // work items per leaf at most
+
 struct TreeBuffers {
   double* bbox_part = nullptr;  // 6 x 1024 block extremes
   double* bbox = nullptr;       // lo[3], hi[3]
@@ -158,6 +166,15 @@ struct TreeBuffers {
   int* far_cnt = nullptr;
   int cap_far = 0;
   int* overflow = nullptr;      // set when a list or the stack overflows
+  // near-field work items
+  int* near_parts = nullptr;    // items per leaf
+  int* near_offs = nullptr;     // first item of each leaf
+  NearItem* near_items = nullptr;  // nleaf x kNearMaxParts at most
+  int* near_total = nullptr;
+  int* near_queue = nullptr;
+  Rate64* near_partial = nullptr;  // item x 256 partial rates
+  void* scan_temp = nullptr;
+  size_t scan_temp_bytes = 0;
 };
 
 // Traversal parameters: a node is far from target leaf T when
@@ -186,9 +203,16 @@ hipError_t launch_list_stats(const TreeBuffers& tb, const TreeShape& shape, long
 
 // Near field over the lists: rates_s[t] = sum over the near leaves of
 // target t's leaf (FP32C pair math, PSE included), Morton order.
-hipError_t launch_near_fp32c(const Rec64* sorted, const NodeRec* nodes, const int* near_list,
-                             const int* near_cnt, int cap_near, int nleaf, double nu,
-                             Rate64* rates_s, hipStream_t st);
+// Near-field work items from the near lists (device only: parts per leaf,
+// exclusive scan, items, total).
+hipError_t near_scan_temp_bytes(int nleaf, size_t* bytes);
+hipError_t build_near_items(const TreeBuffers& tb, int nleaf, hipStream_t st);
+// Near field over the work items (persistent, `grid` resident blocks),
+// then the reduction of split leaves.  Leaf pairs whose bounding spheres
+// are more than sqrt(44) sigma_max apart skip the per-pair near-field test
+// (no pair of them can need it).
+hipError_t launch_near_fp32c(const Rec64* sorted, const NodeRec* nodes, const TreeBuffers& tb, int nleaf,
+                             double nu, float sigma_max2, int grid, Rate64* rates_s, hipStream_t st);
 
 // Far field over the lists, analytic monopole + dipole (FP32), added to
 // rates_s; with mlp != nullptr the learned residual is added as well.

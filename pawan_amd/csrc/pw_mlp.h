@@ -43,7 +43,7 @@ struct MlpWeights {
 uint16_t bf16_bits(float x);
 // master parameters -> fragments + biases
 void pack_mlp(const float* params, std::vector<uint16_t>& frag, std::vector<float>& bias);
-// Glorot-uniform weights, zero biases, from a seed
+// weights and biases uniform in +-1/sqrt(fan_in), from a seed
 void init_mlp(uint64_t seed, float* params);
 
 // Per-sample side data written with the training samples (doubles).

@@ -275,19 +275,24 @@ void pack_mlp(const float* P, std::vector<uint16_t>& frag, std::vector<float>& b
 }
 
 void init_mlp(uint64_t seed, float* P) {
-  std::memset(P, 0, sizeof(float) * kMlpParams);
+  // weights and biases of a layer uniform in +-1/sqrt(fan_in); trains
+  // markedly faster on the far-field samples than Glorot with zero biases
   uint64_t s = seed;
-  auto fill = [&](int off, int fin, int fout) {
-    const double lim = std::sqrt(6.0 / (fin + fout));
-    for (int i = 0; i < fin * fout; ++i) {
+  auto fill = [&](int off, int count, int fan_in) {
+    const double lim = 1.0 / std::sqrt(static_cast<double>(fan_in));
+    for (int i = 0; i < count; ++i) {
       const double u = (splitmix(s) >> 11) * 0x1.0p-53;
       P[off + i] = static_cast<float>((2.0 * u - 1.0) * lim);
     }
   };
-  fill(kOffW1, kMlpIn, H);
-  fill(kOffW2, H, H);
-  fill(kOffW3, H, H);
-  fill(kOffW4, H, kMlpOut);
+  fill(kOffW1, kMlpIn * H, kMlpIn);
+  fill(kOffB1, H, kMlpIn);
+  fill(kOffW2, H * H, H);
+  fill(kOffB2, H, H);
+  fill(kOffW3, H * H, H);
+  fill(kOffB3, H, H);
+  fill(kOffW4, H * kMlpOut, H);
+  fill(kOffB4, kMlpOut, H);
 }
 
 }  // namespace pw

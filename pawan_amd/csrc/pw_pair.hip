@@ -319,6 +319,29 @@ __device__ __forceinline__ void pair32(const T32& t, const SrcT& s, P32& p,
   p.wz = fmaf(q, cz, fmaf(fd, dz, p.wz));
 }
 
+// pair32 for a source known to be beyond 44 sigma^2 of the target (the
+// near branch could not be taken): the same arithmetic without the test.
+__device__ __forceinline__ void pair32_far(const T32& t, const SrcT& s, P32& p) {
+  const float dx = static_cast<float>(t.x - s.x);
+  const float dy = static_cast<float>(t.y - s.y);
+  const float dz = static_cast<float>(t.z - s.z);
+  const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+  const float ir = __builtin_amdgcn_rsqf(r2);
+  const float ir2 = ir * ir;
+  const float q = ir * ir2;
+  const float f = q * ir2;
+  p.ux = fmaf(q, s.ay * dz - s.az * dy, p.ux);
+  p.uy = fmaf(q, s.az * dx - s.ax * dz, p.uy);
+  p.uz = fmaf(q, s.ax * dy - s.ay * dx, p.uz);
+  const float cx = t.ay * s.az - t.az * s.ay;
+  const float cy = t.az * s.ax - t.ax * s.az;
+  const float cz = t.ax * s.ay - t.ay * s.ax;
+  const float fd = -3.0f * f * fmaf(dz, cz, fmaf(dy, cy, dx * cx));
+  p.wx = fmaf(q, cx, fmaf(fd, dx, p.wx));
+  p.wy = fmaf(q, cy, fmaf(fd, dy, p.wy));
+  p.wz = fmaf(q, cz, fmaf(fd, dz, p.wz));
+}
+
 // Knuth's two-sum: (hi, lo) += p with the rounding error of hi + p kept.
 __device__ __forceinline__ void two_sum(float& hi, float& lo, float p) {
   const float s = hi + p;
@@ -399,24 +422,32 @@ __device__ __forceinline__ void item_fp32c(const PairArgs& pa, const WorkItem& i
 }
 
 // ---------------------------------------------------------------------
-// Near field of the tree path: block = target leaf T of the Morton-sorted
-// state, sources = the leaves of T's near list, one LDS tile each; the
-// pair math and compensation are those of item_fp32c.
+// Near field of the tree path.  A work item is (target leaf T, a range of
+// T's near list): up to 256 targets, one per lane, against the listed
+// leaves, one LDS tile each, with the pair math and compensation of
+// item_fp32c.  Items are pulled from an atomic queue by resident blocks;
+// a leaf whose list is split writes per-item partial sums that
+// k_near_reduce adds in item order (deterministic).
 // ---------------------------------------------------------------------
 struct NearArgs {
   const Rec64* sorted;
   const NodeRec* nodes;
   const int* list;
-  const int* cnt;
   int cap;
+  const NearItem* items;
+  const int* total;  // items in use (device)
+  int* queue;
   float nu;
+  float gap2;  // 44 sigma_max^2: leaves further apart take the far-only loop
   Rate64* out;
+  Rate64* partial;  // item x 256
 };
 
-__global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_near_fp32c(NearArgs a) {
-  __shared__ SrcT tile[kTB];
-  const int T = blockIdx.x;
+__device__ __forceinline__ void near_item(const NearArgs& a, const NearItem& it, SrcT* tile) {
+  const int T = it.T;
   const int t0 = a.nodes[T].first, t1 = a.nodes[T].last;
+  const double tcx = a.nodes[T].cx, tcy = a.nodes[T].cy, tcz = a.nodes[T].cz;
+  const float tr = a.nodes[T].r;
   const int t = t0 + static_cast<int>(threadIdx.x);
   const bool active = t < t1;
   const Rec64 m = a.sorted[active ? t : t0];
@@ -432,12 +463,15 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
 #pragma unroll
   for (int k = 0; k < kAcc; ++k) hi[k] = lo[k] = 0.0f;
   PSE32 pse;
-  const int nl = a.cnt[T];
   const int* __restrict__ L = a.list + static_cast<size_t>(T) * a.cap;
-  for (int k = 0; k < nl; ++k) {
+  for (int k = it.k0; k < it.k1; ++k) {
     const int id = L[k];
     const int base = a.nodes[id].first;
     const int cnt = a.nodes[id].last - base;
+    // every pair of the two leaves beyond the Gaussian core: no near test
+    const double ddx = a.nodes[id].cx - tcx, ddy = a.nodes[id].cy - tcy, ddz = a.nodes[id].cz - tcz;
+    const float gap = static_cast<float>(sqrt(ddx * ddx + ddy * ddy + ddz * ddz)) - tr - a.nodes[id].r;
+    const bool all_far = gap > 0.0f && gap * gap > a.gap2 * 1.0001f;
     __syncthreads();
     if (static_cast<int>(threadIdx.x) < cnt) {
       const Rec64 r = a.sorted[base + threadIdx.x];
@@ -458,7 +492,10 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
     for (int c0 = 0; c0 < cnt; c0 += kChunk) {
       const int c1 = min(cnt, c0 + kChunk);
       P32 p;
-      if (diag) {
+      if (all_far) {
+#pragma unroll 2
+        for (int l = c0; l < c1; ++l) pair32_far(me, tile[l], p);
+      } else if (diag) {
         for (int l = c0; l < c1; ++l)
           if (l != self) pair32(me, tile[l], p, pse);
       } else {
@@ -484,8 +521,42 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
     r.wx = kQ * acc[3] + static_cast<double>(pse.x);
     r.wy = kQ * acc[4] + static_cast<double>(pse.y);
     r.wz = kQ * acc[5] + static_cast<double>(pse.z);
-    a.out[t] = r;
+    if (it.slot < 0) a.out[t] = r;
+    else a.partial[static_cast<size_t>(it.slot) * kTB + threadIdx.x] = r;
   }
+}
+
+__global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_near_fp32c(NearArgs a) {
+  __shared__ SrcT tile[kTB];
+  __shared__ int s_idx;
+  const int total = *a.total;
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) s_idx = atomicAdd(a.queue, 1);
+    __syncthreads();
+    const int idx = s_idx;
+    if (idx >= total) break;  // uniform: every lane reads the same s_idx
+    near_item(a, a.items[idx], tile);
+  }
+}
+
+// rates of a split leaf: sum of its items' partial sums, in item order
+__global__ __launch_bounds__(kTB) void k_near_reduce(const NodeRec* __restrict__ nodes, const int* __restrict__ parts,
+                                                     const int* __restrict__ offs, const Rate64* __restrict__ partial,
+                                                     Rate64* __restrict__ out) {
+  const int T = blockIdx.x;
+  const int P = parts[T];
+  if (P <= 1) return;
+  const int t = nodes[T].first + static_cast<int>(threadIdx.x);
+  if (t >= nodes[T].last) return;
+  const size_t base = static_cast<size_t>(offs[T]) * kTB + threadIdx.x;
+  Rate64 s = partial[base];
+  for (int p = 1; p < P; ++p) {
+    const Rate64 q = partial[base + static_cast<size_t>(p) * kTB];
+    s.ux += q.ux; s.uy += q.uy; s.uz += q.uz;
+    s.wx += q.wx; s.wy += q.wy; s.wz += q.wz;
+  }
+  out[t] = s;
 }
 
 // ---------------------------------------------------------------------
@@ -601,12 +672,18 @@ const bool P = cfg.persistent;
   return hipGetLastError();
 }
 
-hipError_t launch_near_fp32c(const Rec64* sorted, const NodeRec* nodes, const int* near_list,
-                             const int* near_cnt, int cap_near, int nleaf, double nu,
-                             Rate64* rates_s, hipStream_t st) {
+hipError_t launch_near_fp32c(const Rec64* sorted, const NodeRec* nodes, const TreeBuffers& tb, int nleaf,
+                             double nu, float sigma_max2, int grid, Rate64* rates_s, hipStream_t st) {
   if (nleaf <= 0) return hipSuccess;
-  NearArgs a{sorted, nodes, near_list, near_cnt, cap_near, static_cast<float>(nu), rates_s};
-  hipLaunchKernelGGL(k_near_fp32c, dim3(nleaf), dim3(kTB), 0, st, a);
+  hipError_t e = launch_fill_int(tb.near_queue, 1, 0, st);
+  if (e != hipSuccess) return e;
+  NearArgs a{sorted, nodes, tb.near_list, tb.cap_near, tb.near_items, tb.near_total, tb.near_queue,
+             static_cast<float>(nu), kFar32 * sigma_max2, rates_s, tb.near_partial};
+  hipLaunchKernelGGL(k_near_fp32c, dim3(grid), dim3(kTB), 0, st, a);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_near_reduce, dim3(nleaf), dim3(kTB), 0, st, nodes, tb.near_parts, tb.near_offs,
+                     tb.near_partial, rates_s);
   return hipGetLastError();
 }
 

@@ -263,6 +263,32 @@ __global__ __launch_bounds__(64) void k_lists(const NodeRec* __restrict__ nodes,
   if (over) atomicOr(overflow, 1);
 }
 
+// Near-field work items: parts per leaf, then the items.
+__global__ __launch_bounds__(256) void k_near_parts(const int* __restrict__ near_cnt, int nleaf,
+                                                    int* __restrict__ parts) {
+  const int T = blockIdx.x * 256 + threadIdx.x;
+  if (T >= nleaf) return;
+  const int nc = near_cnt[T];
+  parts[T] = min(kNearMaxParts, max(1, (nc + kNearChunk - 1) / kNearChunk));
+}
+
+__global__ __launch_bounds__(256) void k_near_emit(const int* __restrict__ near_cnt, const int* __restrict__ parts,
+                                                   const int* __restrict__ offs, int nleaf,
+                                                   NearItem* __restrict__ items, int* __restrict__ total) {
+  const int T = blockIdx.x * 256 + threadIdx.x;
+  if (T >= nleaf) return;
+  const int nc = near_cnt[T], P = parts[T], o = offs[T];
+  for (int p = 0; p < P; ++p) {
+    NearItem it;
+    it.T = T;
+    it.k0 = static_cast<int>(static_cast<long long>(nc) * p / P);
+    it.k1 = static_cast<int>(static_cast<long long>(nc) * (p + 1) / P);
+    it.slot = P == 1 ? -1 : o + p;
+    items[o + p] = it;
+  }
+  if (T == nleaf - 1) *total = o + P;
+}
+
 // Per target leaf: near pairs (targets x near sources, self pairs
 // excluded) and far evaluations (targets x far nodes).
 __global__ __launch_bounds__(64) void k_list_stats(const NodeRec* __restrict__ nodes, int nleaf,
@@ -286,6 +312,24 @@ __global__ __launch_bounds__(64) void k_list_stats(const NodeRec* __restrict__ n
 hipError_t launch_list_stats(const TreeBuffers& tb, const TreeShape& shape, long long* out, hipStream_t st) {
   hipLaunchKernelGGL(k_list_stats, dim3((shape.nleaf + 63) / 64), dim3(64), 0, st, tb.nodes, shape.nleaf,
                      tb.near_list, tb.near_cnt, tb.cap_near, tb.far_cnt, out);
+  return hipGetLastError();
+}
+
+hipError_t near_scan_temp_bytes(int nleaf, size_t* bytes) {
+  return hipcub::DeviceScan::ExclusiveSum(nullptr, *bytes, static_cast<int*>(nullptr), static_cast<int*>(nullptr),
+                                          nleaf);
+}
+
+hipError_t build_near_items(const TreeBuffers& tb, int nleaf, hipStream_t st) {
+  const int nb = (nleaf + 255) / 256;
+  hipLaunchKernelGGL(k_near_parts, dim3(nb), dim3(256), 0, st, tb.near_cnt, nleaf, tb.near_parts);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  size_t bytes = tb.scan_temp_bytes;
+  e = hipcub::DeviceScan::ExclusiveSum(tb.scan_temp, bytes, tb.near_parts, tb.near_offs, nleaf, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_near_emit, dim3(nb), dim3(256), 0, st, tb.near_cnt, tb.near_parts, tb.near_offs, nleaf,
+                     tb.near_items, tb.near_total);
   return hipGetLastError();
 }
 

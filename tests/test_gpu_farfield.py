@@ -120,6 +120,52 @@ def test_analytic_far_field_matches_oracle_on_gpu_lists(oracle, helix, built):
     assert normwise(gw, rw) < 1e-5, normwise(gw, rw)
 
 
+def _residual_at_leaf(params, nd, T, far_ids):
+    """U, G of the network summed over T's far nodes at T's centre (oracle,
+    BF16 roundings emulated), 4 pi factored."""
+    c = nd[T, 0:3]
+    if len(far_ids) == 0:
+        return np.zeros(3), np.zeros((3, 3))
+    X, su, sG = [], [], []
+    for q in far_ids:
+        x, a, b = FF.features((c - nd[q, 0:3])[None], nd[q, 3] + nd[q, 6], nd[q, 4], nd[q, 7:10],
+                              nd[q, 10:19].reshape(3, 3), nd[q, 6])
+        X.append(x[0])
+        su.append(a[0])
+        sG.append(b[0])
+    Y = FF.mlp_forward(params, np.array(X, dtype=np.float32), bf16_emulate=True)
+    U = (Y[:, :3] * np.array(su)[:, None]).sum(axis=0)
+    G = (Y[:, 3:] * np.array(sG)[:, None]).sum(axis=0).reshape(3, 3)
+    return U, G
+
+
+def test_learned_far_field_matches_oracle_on_gpu_lists(oracle, helix, built):
+    """The network's residual at the leaf centres, carried to the targets
+    by its gradient output, on top of the analytic far field."""
+    params = _rand_params(12, scale=0.5)
+    perm, nd = built["perm"], built["nodes"]
+    with capi.Solver(helix) as g:
+        g.ff_set_weights(params)
+        g.ff_set_params(0.5, 0.0, use_mlp=True)
+        u, w = g.solve("learned")
+    rng = np.random.default_rng(1)
+    targets = np.sort(rng.choice(helix.n, size=32, replace=False))
+    ru, rw = _oracle_rates(oracle, helix, built, targets)
+    sp, sa = helix.pos[perm], helix.alpha[perm]
+    du = []
+    for k, t in enumerate(targets):
+        T = t // FF.LEAF
+        U, G = _residual_at_leaf(params, nd, T, built["far"][T, :built["far_cnt"][T]])
+        du.append(np.linalg.norm(U) / np.linalg.norm(ru[k] * FF.FOUR_PI))
+        ru[k] += (U + G @ (sp[t] - nd[T, 0:3])) / FF.FOUR_PI
+        rw[k] += sa[t] @ G / FF.FOUR_PI
+    _say(f"network residual / velocity: median {np.median(du):.2e}, max {np.max(du):.2e}")
+    assert np.median(du) > 1e-4  # the residual is large enough to be checked
+    gu, gw = u[perm[targets]], w[perm[targets]]
+    assert normwise(gu, ru) < 2e-5, normwise(gu, ru)
+    assert normwise(gw, rw) < 2e-5, normwise(gw, rw)
+
+
 def test_error_against_direct_sum_falls_with_theta(helix):
     errs = {}
     with capi.Solver(helix) as g:
