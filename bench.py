@@ -73,9 +73,9 @@ def time_path(system, precision: str, dt: float, steps: int, warmup: int, tune: 
 
 
 def train_network(n_train: int, samples: int, steps: int, theta: float, seed: int = 11):
-    """Samples from seeded training wakes (helical wakes of other seeds and
-    size, generated here), then Adam on the GPU.  Returns (weights, report,
-    seconds)."""
+    """Samples from seeded training wakes (helical wakes of n_train
+    particles and seeds other than the benchmark's, generated here), then
+    Adam on the GPU.  Returns (weights, report, seconds)."""
     from pawan_amd import capi, workloads
 
     t0 = time.perf_counter()
@@ -294,7 +294,8 @@ def main() -> int:
     ap.add_argument("--learned-steps", type=int, default=3)
     ap.add_argument("--learned-err", type=float, default=1e-4)
     ap.add_argument("--theta", type=float, default=0.5)
-    ap.add_argument("--train-n", type=int, default=1 << 18)
+    ap.add_argument("--train-n", type=int, default=0,
+                    help="particles of the training wakes (default: the benchmark's N)")
     ap.add_argument("--train-samples", type=int, default=1_200_000)
     ap.add_argument("--train-steps", type=int, default=15000)
     ap.add_argument("--no-drift", action="store_true")
@@ -384,7 +385,7 @@ def main() -> int:
     if out64:
         result["fp32c_over_fp64"] = sps32 / out64["fp64_steps_per_s"]
     if not args.no_learned and system.coupled and system.nw == 1:
-        lg = learned_leg(system, dt, args.learned_steps, args.learned_err, args.theta, args.train_n,
+        lg = learned_leg(system, dt, args.learned_steps, args.learned_err, args.theta, args.train_n or system.n,
                          args.train_samples, args.train_steps)
         result["learned"] = lg
         best = max((v for k, v in lg.items() if k in ("analytic", "mlp") and v), default=None,

@@ -200,18 +200,40 @@ def exact_node(x: np.ndarray, sig_t: float, pos: np.ndarray, alpha: np.ndarray, 
     return FOUR_PI * u, FOUR_PI * G
 
 
+def frame(r: np.ndarray, M: np.ndarray, s0: float) -> np.ndarray:
+    """Rows e1, e2, e3 of the network's frame: e1 = M/|M| (r/|r| when
+    |M| < 1e-3 s0), e2 = the part of r normal to e1 (any normal if r || e1),
+    e3 = e1 x e2."""
+    r = np.asarray(r, dtype=np.float64)
+    if M @ M > 1e-6 * s0 * s0:
+        e1 = M / np.linalg.norm(M)
+    else:
+        e1 = r / np.linalg.norm(r)
+    v = r - (r @ e1) * e1
+    if not (v @ v > 1e-12 * (r @ r)):
+        a = np.array([1.0, 0.0, 0.0]) if abs(e1[0]) < 0.9 else np.array([0.0, 1.0, 0.0])
+        v = a - (a @ e1) * e1
+    e2 = v / np.linalg.norm(v)
+    return np.array([e1, e2, np.cross(e1, e2)])
+
+
 def features(r: np.ndarray, ell: float, s0: float, M: np.ndarray, D: np.ndarray, sg: float):
-    """Network inputs and output scales of offsets r (k,3) from a node."""
+    """Network inputs of offsets r (k,3) from a node, in the frame of each
+    offset, with the frames and output scales: (X, su, sG, R)."""
+    r = np.atleast_2d(r)
     d2 = (r * r).sum(axis=1)
-    el = ell / d2
     X = np.zeros((len(r), N_IN))
-    X[:, 0:3] = r * el[:, None]
-    X[:, 3:6] = M / s0
-    X[:, 6:15] = D.reshape(-1) / (s0 * ell)
-    X[:, 15] = sg / ell
+    Rs = np.zeros((len(r), 3, 3))
+    for i in range(len(r)):
+        R = frame(r[i], M, s0)
+        Rs[i] = R
+        X[i, 0:3] = R @ r[i] * (ell / d2[i])
+        X[i, 3:6] = R @ M / s0
+        X[i, 6:15] = (R @ D @ R.T).reshape(-1) / (s0 * ell)
+        X[i, 15] = sg / ell
     su = s0 * ell * ell / (d2 * d2)
     sG = su / np.sqrt(d2)
-    return X, su, sG
+    return X, su, sG, Rs
 
 
 # ---------------------------------------------------------------------------

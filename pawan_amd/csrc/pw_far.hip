@@ -22,6 +22,7 @@
 // monopole - dipole) in scaled units:
 //   inputs  e = r l / r^2 (3), M / s0 (3), D / (s0 l) (9), sigma / l (1)
 //   outputs u' residual / (s0 l^2 / r^4) (3), G' residual / (s0 l^2 / r^5) (9)
+// all expressed in a frame fixed by M and r (make_frame below)
 // with l = r_node + sigma the node's length scale.  It is evaluated at the
 // centre of each target leaf and carried to the leaf's targets with its own
 // gradient output.  Inference runs in BF16
@@ -292,83 +293,150 @@ __device__ __forceinline__ void mlp_pass(const MlpLds& s, const v4bf x[2], v4f o
   }
 }
 
-// Output scales of one (target, node) sample: u' scale s0 l^2 / r^4, G'
-// scale s0 l^2 / r^5, and el = l / r^2 (the offset feature is r el).
-__device__ __forceinline__ void sample_scales(const FarNodeL& q, float r0, float r1, float r2, float& su,
-                                              float& sG, float& el) {
-  const float d2 = fmaf(r2, r2, fmaf(r1, r1, r0 * r0));
+// The network works in a frame attached to the node and the offset:
+//   e1 = M / |M| (r / |r| when |M| < 1e-3 s0),
+//   e2 = the part of r normal to e1, normalised (any normal if r || e1),
+//   e3 = e1 x e2;
+// R has rows e1, e2, e3.  Inputs are rotated into it (vectors R v, the
+// dipole R D R^T) and outputs rotated back (u = R^T u', G = R^T G' R).  The
+// residual of a node depends on its shape relative to M and to the offset,
+// which this frame fixes, so the network does not have to learn rotations.
+__device__ __forceinline__ void make_frame(const float r[3], const float M[3], float s0, float R[9]) {
+  const float m2 = fmaf(M[2], M[2], fmaf(M[1], M[1], M[0] * M[0]));
+  const float r2 = fmaf(r[2], r[2], fmaf(r[1], r[1], r[0] * r[0]));
+  float e1[3];
+  if (m2 > 1e-6f * s0 * s0) {
+    const float im = 1.0f / sqrtf(m2);
+    e1[0] = M[0] * im; e1[1] = M[1] * im; e1[2] = M[2] * im;
+  } else {
+    const float ir = 1.0f / sqrtf(r2);
+    e1[0] = r[0] * ir; e1[1] = r[1] * ir; e1[2] = r[2] * ir;
+  }
+  const float re = fmaf(r[2], e1[2], fmaf(r[1], e1[1], r[0] * e1[0]));
+  float v[3] = {r[0] - re * e1[0], r[1] - re * e1[1], r[2] - re * e1[2]};
+  float v2 = fmaf(v[2], v[2], fmaf(v[1], v[1], v[0] * v[0]));
+  if (!(v2 > 1e-12f * r2)) {
+    const float ax = fabsf(e1[0]) < 0.9f ? 1.0f : 0.0f, ay = 1.0f - ax;
+    const float ae = ax * e1[0] + ay * e1[1];
+    v[0] = ax - ae * e1[0]; v[1] = ay - ae * e1[1]; v[2] = -ae * e1[2];
+    v2 = fmaf(v[2], v[2], fmaf(v[1], v[1], v[0] * v[0]));
+  }
+  const float iv = 1.0f / sqrtf(v2);
+  const float e2[3] = {v[0] * iv, v[1] * iv, v[2] * iv};
+  R[0] = e1[0]; R[1] = e1[1]; R[2] = e1[2];
+  R[3] = e2[0]; R[4] = e2[1]; R[5] = e2[2];
+  R[6] = e1[1] * e2[2] - e1[2] * e2[1];
+  R[7] = e1[2] * e2[0] - e1[0] * e2[2];
+  R[8] = e1[0] * e2[1] - e1[1] * e2[0];
+}
+
+// All 16 inputs of one (point, node) sample at offset r = x - c, the frame,
+// and the output scales: u' scale s0 l^2 / r^4, G' scale s0 l^2 / r^5.
+__device__ __forceinline__ void sample_inputs(const NodeRec& q, const float r[3], float feat[16], float R[9],
+                                              float& su, float& sG) {
+  const float ell = q.r + q.sg;
+  const float d2 = fmaf(r[2], r[2], fmaf(r[1], r[1], r[0] * r[0]));
   const float id2 = 1.0f / d2;
   const float id = __builtin_amdgcn_sqrtf(id2);
-  su = q.s0 * q.ell * q.ell * id2 * id2;
+  su = q.s0 * ell * ell * id2 * id2;
   sG = su * id;
-  el = q.ell * id2;
+  make_frame(r, q.M, q.s0, R);
+  const float el = ell * id2;
+  const float is0 = q.s0 > 0.0f ? 1.0f / q.s0 : 0.0f;
+  const float isl = is0 / ell;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    feat[i] = (R[3 * i] * r[0] + R[3 * i + 1] * r[1] + R[3 * i + 2] * r[2]) * el;
+    feat[3 + i] = (R[3 * i] * q.M[0] + R[3 * i + 1] * q.M[1] + R[3 * i + 2] * q.M[2]) * is0;
+  }
+  // R D R^T
+  float RD[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) RD[3 * i + j] = R[3 * i] * q.D[j] + R[3 * i + 1] * q.D[3 + j] + R[3 * i + 2] * q.D[6 + j];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      feat[6 + 3 * i + j] = (RD[3 * i] * R[3 * j] + RD[3 * i + 1] * R[3 * j + 1] + RD[3 * i + 2] * R[3 * j + 2]) * isl;
+  feat[15] = q.sg / ell;
+}
+
+// u = R^T u', G = R^T G' R, scaled, added to acc (12)
+__device__ __forceinline__ void unrotate_add(const float* y, const float R[9], float su, float sG, float acc[12]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) acc[i] += su * (R[i] * y[0] + R[3 + i] * y[1] + R[6 + i] * y[2]);
+  float GR[9];  // G' R
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int m = 0; m < 3; ++m) GR[3 * a + m] = y[3 + 3 * a] * R[m] + y[4 + 3 * a] * R[3 + m] + y[5 + 3 * a] * R[6 + m];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+      acc[3 + 3 * k + m] += sG * (R[k] * GR[m] + R[3 + k] * GR[3 + m] + R[6 + k] * GR[6 + m]);
 }
 
 // Learned residual over the far lists, evaluated once per (target leaf,
 // far node) at the leaf centre c_T and carried to the leaf's targets with
 // the gradient the network also predicts:
 //   u_res(x_t) = U + G (x_t - c_T),   G_res(x_t) = G,
-// U, G = sum over the far nodes of the scaled outputs.  The residual falls
-// off as d^-4, so over a leaf (r_T < theta d / 2) the first-order transfer
-// keeps it to O((r_T/d)^2) relative to itself, while the network runs 256
-// times less often than per target.  4 waves per leaf; wave w takes the
-// 32-node passes w, w+4, ...; node k of a pass is sample column k.
+// U, G = sum over the far nodes of the outputs, rotated back from each
+// node's frame and scaled.  The residual falls off as d^-4, so over a leaf
+// (r_T < theta d / 2) the first-order transfer keeps it to O((r_T/d)^2)
+// relative to itself, while the network runs 256 times less often than per
+// target.  4 waves per leaf; wave w takes the 32-node passes w, w+4, ...;
+// node k of a pass is sample column k.  The 12 outputs of a sample come out
+// spread over four lanes; they are exchanged through LDS so that one lane
+// per sample rotates them back.
 constexpr int kMlpThreads = 256;
-
-__device__ __forceinline__ v4bf node_features(const NodeRec& q, double x, double y, double z, int g, float& su,
-                                              float& sG) {
-  const float r0 = static_cast<float>(x - q.cx), r1 = static_cast<float>(y - q.cy),
-              r2 = static_cast<float>(z - q.cz);
-  const float ell = q.r + q.sg;
-  const float d2 = fmaf(r2, r2, fmaf(r1, r1, r0 * r0));
-  const float id2 = 1.0f / d2;
-  const float id = __builtin_amdgcn_sqrtf(id2);
-  su = q.s0 * ell * ell * id2 * id2;
-  sG = su * id;
-  const float is0 = q.s0 > 0.0f ? 1.0f / q.s0 : 0.0f;
-  const float isl = is0 / ell;
-  float f[4];
-  if (g == 0) {
-    const float el = ell * id2;
-    f[0] = r0 * el; f[1] = r1 * el; f[2] = r2 * el; f[3] = q.M[0] * is0;
-  } else if (g == 1) {
-    f[0] = q.M[1] * is0; f[1] = q.M[2] * is0; f[2] = q.D[0] * isl; f[3] = q.D[1] * isl;
-  } else if (g == 2) {
-    f[0] = q.D[2] * isl; f[1] = q.D[3] * isl; f[2] = q.D[4] * isl; f[3] = q.D[5] * isl;
-  } else {
-    f[0] = q.D[6] * isl; f[1] = q.D[7] * isl; f[2] = q.D[8] * isl; f[3] = q.sg / ell;
-  }
-  v4bf v;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) v[i] = static_cast<__bf16>(f[i]);
-  return v;
-}
+constexpr int kXch = 24;  // floats exchanged per sample: 12 outputs, frame, 2 scales
 
 __global__ __launch_bounds__(kMlpThreads) void k_far_mlp(FarArgs a, MlpWeights W) {
   extern __shared__ __align__(16) unsigned char smem[];
   MlpLds& s = *reinterpret_cast<MlpLds*>(smem);
-  float* red = reinterpret_cast<float*>(smem + sizeof(MlpLds));  // [4 waves][16], then [16]
+  // per wave: 32 samples x {12 outputs, 9 frame, 2 scales}; then the sums
+  float* xch = reinterpret_cast<float*>(smem + sizeof(MlpLds));
+  float* red = xch + 4 * 32 * kXch;
   load_mlp(W, s);
   __syncthreads();
   const int T = blockIdx.x;
   const NodeRec leaf = a.nodes[T];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = lane >> 4, c = lane & 15;
+  float* mine = xch + wave * 32 * kXch;
   const int nf = a.cnt[T];
   const int* __restrict__ L = a.list + static_cast<size_t>(T) * a.cap;
-  v4f acc = v4f{0, 0, 0, 0};
+  float acc[12];
+#pragma unroll
+  for (int i = 0; i < 12; ++i) acc[i] = 0.0f;
   for (int base = 32 * wave; base < nf; base += 32 * 4) {
     v4bf x[2];
-    float sc[2][2];
 #pragma unroll
     for (int nb = 0; nb < 2; ++nb) {
-      const int k = base + 16 * nb + c;
+      const int j = 16 * nb + c;  // sample column
+      const int k = base + j;
+      float feat[16], R[9], su = 0.0f, sG = 0.0f;
       if (k < nf) {
         const NodeRec& q = a.nodes[L[k]];
-        x[nb] = node_features(q, leaf.cx, leaf.cy, leaf.cz, g, sc[nb][0], sc[nb][1]);
+        const float r[3] = {static_cast<float>(leaf.cx - q.cx), static_cast<float>(leaf.cy - q.cy),
+                            static_cast<float>(leaf.cz - q.cz)};
+        sample_inputs(q, r, feat, R, su, sG);
       } else {
-        x[nb] = v4bf{0, 0, 0, 0};
-        sc[nb][0] = sc[nb][1] = 0.0f;  // padding column contributes nothing
+#pragma unroll
+        for (int i = 0; i < 16; ++i) feat[i] = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) R[i] = 0.0f;  // padding column contributes nothing
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) x[nb][i] = static_cast<__bf16>(feat[4 * g + i]);
+      if (g == 0) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) mine[j * kXch + 12 + i] = R[i];
+        mine[j * kXch + 21] = su;
+        mine[j * kXch + 22] = sG;
       }
     }
     v4f o[2];
@@ -376,18 +444,27 @@ __global__ __launch_bounds__(kMlpThreads) void k_far_mlp(FarArgs a, MlpWeights W
 #pragma unroll
     for (int nb = 0; nb < 2; ++nb)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) acc[i] = fmaf(o[nb][i], (4 * g + i) < 3 ? sc[nb][0] : sc[nb][1], acc[i]);
+      for (int i = 0; i < 4; ++i)
+        if (4 * g + i < 12) mine[(16 * nb + c) * kXch + 4 * g + i] = o[nb][i];
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (lane < 32) {
+      const float* e = mine + lane * kXch;
+      unrotate_add(e, e + 12, e[21], e[22], acc);
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   }
-  // sum over the 16 columns of each lane group, then over the waves (in order)
+  // sum over the 32 sample lanes, then over the waves (in order)
 #pragma unroll
-  for (int off = 1; off < 16; off <<= 1)
+  for (int off = 1; off < 32; off <<= 1)
 #pragma unroll
-    for (int i = 0; i < 4; ++i) acc[i] += __shfl_xor(acc[i], off, 64);
-  if (c == 0)
+    for (int i = 0; i < 12; ++i) acc[i] += __shfl_xor(acc[i], off, 64);
+  if (lane == 0)
 #pragma unroll
-    for (int i = 0; i < 4; ++i) red[wave * 16 + 4 * g + i] = acc[i];
+    for (int i = 0; i < 12; ++i) red[wave * 16 + i] = acc[i];
   __syncthreads();
-  if (threadIdx.x < 16)
+  if (threadIdx.x < 12)
     red[64 + threadIdx.x] = ((red[threadIdx.x] + red[16 + threadIdx.x]) + red[32 + threadIdx.x]) + red[48 + threadIdx.x];
   __syncthreads();
   const int t = leaf.first + static_cast<int>(threadIdx.x);
@@ -498,30 +575,33 @@ __global__ __launch_bounds__(256) void k_ff_samples(SampleArgs a) {
   __syncthreads();
   if (threadIdx.x != 0) return;
   for (int m = 0; m < 12; ++m) v[m] = ((red[m] + red[12 + m]) + red[24 + m]) + red[36 + m];
-  FarNodeL f;
-  stage_node(q, f);
-  const float r[3] = {static_cast<float>(me.x - f.cx), static_cast<float>(me.y - f.cy),
-                      static_cast<float>(me.z - f.cz)};
+  const float r[3] = {static_cast<float>(me.x - q.cx), static_cast<float>(me.y - q.cy),
+                      static_cast<float>(me.z - q.cz)};
   float uu[3], gg[9];
-  md_eval(r, f.M, f.D, uu, gg);
-  // features in FP32 (the network rounds them to BF16 itself)
-  float su, sG, el;
-  sample_scales(f, r[0], r[1], r[2], su, sG, el);
-  float feat[16];
-  for (int i = 0; i < 16; ++i) feat[i] = f.feat[i];
-  feat[0] = r[0] * el;
-  feat[1] = r[1] * el;
-  feat[2] = r[2] * el;
-  double err_md = 0.0, un = 0.0;
+  md_eval(r, q.M, q.D, uu, gg);
+  // inputs in FP32 (the network rounds them to BF16 itself) and the frame
+  float feat[16], R[9], su, sG;
+  sample_inputs(q, r, feat, R, su, sG);
+  double res[12], err_md = 0.0, un = 0.0;
   for (int m = 0; m < 12; ++m) {
     const double fourpi_exact = v[m] * 12.566370614359172954;  // 4 pi
-    const double res = fourpi_exact - static_cast<double>(m < 3 ? uu[m] : gg[m - 3]);
-    a.Y[12 * static_cast<size_t>(k) + m] = static_cast<float>(res / (m < 3 ? su : sG));
+    res[m] = fourpi_exact - static_cast<double>(m < 3 ? uu[m] : gg[m - 3]);
     if (m < 3) {
-      err_md += res * res;
+      err_md += res[m] * res[m];
       un += fourpi_exact * fourpi_exact;
     }
   }
+  // targets in the frame: R res_u / su, R res_G R^T / sG
+  float* y = a.Y + 12 * static_cast<size_t>(k);
+  for (int i = 0; i < 3; ++i)
+    y[i] = static_cast<float>((R[3 * i] * res[0] + R[3 * i + 1] * res[1] + R[3 * i + 2] * res[2]) / su);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double acc = 0.0;
+      for (int p = 0; p < 3; ++p)
+        for (int q2 = 0; q2 < 3; ++q2) acc += R[3 * i + p] * res[3 + 3 * p + q2] * R[3 * j + q2];
+      y[3 + 3 * i + j] = static_cast<float>(acc / sG);
+    }
   for (int i = 0; i < 16; ++i) a.X[16 * static_cast<size_t>(k) + i] = feat[i];
   double* o = a.aux + static_cast<size_t>(k) * kSampleAux;
   const double d = sqrt(static_cast<double>(r[0]) * r[0] + static_cast<double>(r[1]) * r[1] +
@@ -547,7 +627,7 @@ __global__ __launch_bounds__(256) void k_scatter(const Rate64* __restrict__ in, 
 
 }  // namespace
 
-size_t mlp_far_lds_bytes() { return sizeof(MlpLds) + 80 * sizeof(float); }
+size_t mlp_far_lds_bytes() { return sizeof(MlpLds) + (4 * 32 * kXch + 80) * sizeof(float); }
 
 hipError_t launch_far(const Rec64* sorted, const NodeRec* nodes, const int* far_list, const int* far_cnt,
                       int cap_far, int nleaf, const MlpWeights* mlp, Rate64* rates_s, hipStream_t st) {

@@ -319,12 +319,20 @@ __device__ __forceinline__ void pair32(const T32& t, const SrcT& s, P32& p,
   p.wz = fmaf(q, cz, fmaf(fd, dz, p.wz));
 }
 
-// pair32 for a source known to be beyond 44 sigma^2 of the target (the
-// near branch could not be taken): the same arithmetic without the test.
-__device__ __forceinline__ void pair32_far(const T32& t, const SrcT& s, P32& p) {
-  const float dx = static_cast<float>(t.x - s.x);
-  const float dy = static_cast<float>(t.y - s.y);
-  const float dz = static_cast<float>(t.z - s.z);
+// Far-only pair on positions relative to the target leaf's centre, rounded
+// to FP32 once per particle: for sources beyond 44 sigma^2 the rounding
+// (about 6e-8 of the distance to the centre) is a relative displacement
+// error below 1e-6, and the pair needs no FP64 at all.
+struct alignas(16) SrcRel {
+  float x, y, z, hs2;
+  float ax, ay, az, vol;
+};
+
+__device__ __forceinline__ void pair32_rel(float tx, float ty, float tz, float tax, float tay, float taz,
+                                           const SrcRel& s, P32& p) {
+  const float dx = tx - s.x;
+  const float dy = ty - s.y;
+  const float dz = tz - s.z;
   const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
   const float ir = __builtin_amdgcn_rsqf(r2);
   const float ir2 = ir * ir;
@@ -333,9 +341,9 @@ __device__ __forceinline__ void pair32_far(const T32& t, const SrcT& s, P32& p) 
   p.ux = fmaf(q, s.ay * dz - s.az * dy, p.ux);
   p.uy = fmaf(q, s.az * dx - s.ax * dz, p.uy);
   p.uz = fmaf(q, s.ax * dy - s.ay * dx, p.uz);
-  const float cx = t.ay * s.az - t.az * s.ay;
-  const float cy = t.az * s.ax - t.ax * s.az;
-  const float cz = t.ax * s.ay - t.ay * s.ax;
+  const float cx = tay * s.az - taz * s.ay;
+  const float cy = taz * s.ax - tax * s.az;
+  const float cz = tax * s.ay - tay * s.ax;
   const float fd = -3.0f * f * fmaf(dz, cz, fmaf(dy, cy, dx * cx));
   p.wx = fmaf(q, cx, fmaf(fd, dx, p.wx));
   p.wy = fmaf(q, cy, fmaf(fd, dy, p.wy));
@@ -459,6 +467,9 @@ __device__ __forceinline__ void near_item(const NearArgs& a, const NearItem& it,
   me.az = static_cast<float>(m.az);
   me.vol = static_cast<float>(m.vol);
   me.nu2 = 2.0f * a.nu;
+  const float rx = static_cast<float>(m.x - tcx), ry = static_cast<float>(m.y - tcy),
+              rz = static_cast<float>(m.z - tcz);
+  SrcRel* rtile = reinterpret_cast<SrcRel*>(tile);
   float hi[kAcc], lo[kAcc];
 #pragma unroll
   for (int k = 0; k < kAcc; ++k) hi[k] = lo[k] = 0.0f;
@@ -475,15 +486,28 @@ __device__ __forceinline__ void near_item(const NearArgs& a, const NearItem& it,
     __syncthreads();
     if (static_cast<int>(threadIdx.x) < cnt) {
       const Rec64 r = a.sorted[base + threadIdx.x];
-      SrcT f;
-      f.x = r.x; f.y = r.y; f.z = r.z;
-      f.hs2 = static_cast<float>(0.5 * r.s2);
-      f.ax = static_cast<float>(r.ax);
-      f.ay = static_cast<float>(r.ay);
-      f.az = static_cast<float>(r.az);
-      f.vol = static_cast<float>(r.vol);
-      f.pad = 0.0f;
-      tile[threadIdx.x] = f;
+      if (all_far) {
+        SrcRel f;
+        f.x = static_cast<float>(r.x - tcx);
+        f.y = static_cast<float>(r.y - tcy);
+        f.z = static_cast<float>(r.z - tcz);
+        f.hs2 = static_cast<float>(0.5 * r.s2);
+        f.ax = static_cast<float>(r.ax);
+        f.ay = static_cast<float>(r.ay);
+        f.az = static_cast<float>(r.az);
+        f.vol = static_cast<float>(r.vol);
+        rtile[threadIdx.x] = f;
+      } else {
+        SrcT f;
+        f.x = r.x; f.y = r.y; f.z = r.z;
+        f.hs2 = static_cast<float>(0.5 * r.s2);
+        f.ax = static_cast<float>(r.ax);
+        f.ay = static_cast<float>(r.ay);
+        f.az = static_cast<float>(r.az);
+        f.vol = static_cast<float>(r.vol);
+        f.pad = 0.0f;
+        tile[threadIdx.x] = f;
+      }
     }
     __syncthreads();
     if (!active) continue;
@@ -494,7 +518,7 @@ __device__ __forceinline__ void near_item(const NearArgs& a, const NearItem& it,
       P32 p;
       if (all_far) {
 #pragma unroll 2
-        for (int l = c0; l < c1; ++l) pair32_far(me, tile[l], p);
+        for (int l = c0; l < c1; ++l) pair32_rel(rx, ry, rz, me.ax, me.ay, me.az, rtile[l], p);
       } else if (diag) {
         for (int l = c0; l < c1; ++l)
           if (l != self) pair32(me, tile[l], p, pse);

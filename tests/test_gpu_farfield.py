@@ -126,16 +126,19 @@ def _residual_at_leaf(params, nd, T, far_ids):
     c = nd[T, 0:3]
     if len(far_ids) == 0:
         return np.zeros(3), np.zeros((3, 3))
-    X, su, sG = [], [], []
+    X, su, sG, Rs = [], [], [], []
     for q in far_ids:
-        x, a, b = FF.features((c - nd[q, 0:3])[None], nd[q, 3] + nd[q, 6], nd[q, 4], nd[q, 7:10],
-                              nd[q, 10:19].reshape(3, 3), nd[q, 6])
+        x, a, b, R = FF.features((c - nd[q, 0:3])[None], nd[q, 3] + nd[q, 6], nd[q, 4], nd[q, 7:10],
+                                 nd[q, 10:19].reshape(3, 3), nd[q, 6])
         X.append(x[0])
         su.append(a[0])
         sG.append(b[0])
+        Rs.append(R[0])
     Y = FF.mlp_forward(params, np.array(X, dtype=np.float32), bf16_emulate=True)
-    U = (Y[:, :3] * np.array(su)[:, None]).sum(axis=0)
-    G = (Y[:, 3:] * np.array(sG)[:, None]).sum(axis=0).reshape(3, 3)
+    U, G = np.zeros(3), np.zeros((3, 3))
+    for k, R in enumerate(Rs):  # back from each node's frame
+        U += su[k] * (R.T @ Y[k, :3])
+        G += sG[k] * (R.T @ Y[k, 3:].reshape(3, 3) @ R)
     return U, G
 
 
@@ -212,15 +215,16 @@ def test_samples_match_oracle(helix, built):
         ue, Ge = FF.exact_node(sp[t], sr[t], sp[a:b], sa[a:b], sr[a:b])
         r = sp[t] - nd[q, 0:3]
         um, Gm = FF.md_eval(r[None], nd[q, 7:10], nd[q, 10:19].reshape(3, 3))
-        xo, su, sG = FF.features(r[None], nd[q, 3] + nd[q, 6], nd[q, 4], nd[q, 7:10],
-                                 nd[q, 10:19].reshape(3, 3), nd[q, 6])
-        np.testing.assert_allclose(X[k], xo[0], rtol=2e-6, atol=2e-7)
-        # residual in true units against the FP64 residual, relative to the
-        # expansion's size (the GPU expansion is FP32)
+        xo, su, sG, R = FF.features(r[None], nd[q, 3] + nd[q, 6], nd[q, 4], nd[q, 7:10],
+                                    nd[q, 10:19].reshape(3, 3), nd[q, 6])
+        np.testing.assert_allclose(X[k], xo[0], rtol=1e-5, atol=1e-6)
+        # residual in the node frame against the FP64 residual, relative to
+        # the expansion's size (the GPU expansion is FP32)
+        R = R[0]
         res_u = Y[k, :3] * su[0]
         res_G = Y[k, 3:].reshape(3, 3) * sG[0]
-        assert np.linalg.norm(res_u - (ue - um[0])) <= 1e-5 * np.linalg.norm(um[0]), k
-        assert np.linalg.norm(res_G - (Ge - Gm[0])) <= 1e-5 * np.linalg.norm(Gm[0]), k
+        assert np.linalg.norm(res_u - R @ (ue - um[0])) <= 1e-5 * np.linalg.norm(um[0]), k
+        assert np.linalg.norm(res_G - R @ (Ge - Gm[0]) @ R.T) <= 1e-5 * np.linalg.norm(Gm[0]), k
         assert A[k, 4] == pytest.approx(np.linalg.norm(ue - um[0]), rel=1e-3, abs=1e-6 * np.linalg.norm(um[0]))
 
 

@@ -174,3 +174,33 @@ def test_bf16_rounding():
     assert FF.bf16(np.array([1.0 + 2 ** -8]))[0] == 1.0
     assert FF.bf16(np.array([1.0 + 3 * 2 ** -8]))[0] == 1.0 + 2 ** -6
     assert FF.bf16(np.array([-2.0 ** -130]))[0] == -(2.0 ** -130)  # subnormal kept
+
+
+def test_frame_is_a_rotation_fixed_by_moment_and_offset():
+    rng = np.random.default_rng(2)
+    for _ in range(20):
+        r, M = rng.normal(size=3), rng.normal(size=3)
+        R = FF.frame(r, M, 1.0)
+        np.testing.assert_allclose(R @ R.T, np.eye(3), atol=1e-12)
+        assert np.linalg.det(R) == pytest.approx(1.0)
+        np.testing.assert_allclose((R @ M)[1:], 0.0, atol=1e-12)  # M along e1
+        assert abs((R @ r)[2]) < 1e-12 and (R @ r)[1] > 0      # r in the (e1, e2) half-plane
+    # tiny moment: e1 along r; r parallel to M: any normal completes the frame
+    R = FF.frame(np.array([0.0, 2.0, 0.0]), np.array([1e-9, 0, 0]), 1.0)
+    np.testing.assert_allclose(R[0], [0, 1, 0], atol=1e-12)
+    R = FF.frame(np.array([0.0, 0.0, 3.0]), np.array([0.0, 0.0, 1.0]), 1.0)
+    np.testing.assert_allclose(R @ R.T, np.eye(3), atol=1e-12)
+
+
+def test_features_are_rotation_invariant():
+    """Rotating the whole configuration leaves the network inputs unchanged
+    and rotates nothing else: the frame absorbs the rotation."""
+    rng = np.random.default_rng(4)
+    r, M, D = rng.normal(size=(1, 3)), rng.normal(size=3), rng.normal(size=(3, 3))
+    Q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    if np.linalg.det(Q) < 0:
+        Q[:, 0] *= -1
+    X1, su1, sG1, _ = FF.features(r, 0.3, 2.0, M, D, 0.02)
+    X2, su2, sG2, _ = FF.features(r @ Q.T, 0.3, 2.0, Q @ M, Q @ D @ Q.T, 0.02)
+    np.testing.assert_allclose(X1, X2, atol=1e-12)
+    np.testing.assert_allclose(su1, su2)
