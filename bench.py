@@ -1,19 +1,29 @@
 #!/usr/bin/env python3
-"""RK4 steps/s of the direct-sum vortex-particle step on one MI355X.
+"""RK4 steps/s of the vortex-particle step on one MI355X: FP64 direct sum,
+FP32-compensated direct sum, and the learned far field at <= 1e-4 error.
 
 Workload (default): the synthetic rotor-like helical wake, N = 1,048,576
 particles (4 contracting tip helices + jittered inboard sheet, sigma = 0.02,
 nu = 1.56e-5), RK4 dt = 1e-3.  A "step" is one full RK4 step: four
-right-hand sides (all N(N-1) ordered pair interactions each: velocity,
-transposed stretching and PSE diffusion) plus the FP64 stage combination,
-with the state resident in HBM.
+right-hand sides plus the FP64 stage combination, with the state resident
+in HBM.  A direct right-hand side evaluates all N(N-1) ordered pair
+interactions (velocity, transposed stretching, PSE diffusion).
 
-`value` is the FP32-compensated path (FP32 pair math, compensated per-target
-accumulation, FP64 state); the FP64 path is timed beside it.  The pair
-kernel's average launch time is measured with HIP events recorded by the
-library on its own stream.  The CPU baseline is the oracle's restatement of
-the reference's OpenMP loops (parallel.cpp), timed on a sample of source
-rows of the same system and extrapolated to one RK4 step.
+`value` is the FP32-compensated direct path (FP32 pair math, compensated
+per-target accumulation, FP64 state); the FP64 direct path is timed beside
+it, with the FP32C error against it.  The pair kernel's average launch time
+is measured with HIP events recorded by the library on its own stream.
+
+The learned leg trains the far-field network on samples the solver
+generates from helical wakes of other seeds, then, for the tree path with
+the analytic expansion only and with the network, searches the largest
+tolerance whose velocity error against the direct sum is <= 1e-4 and times
+RK4 steps at it.  The drift leg runs 200 RK4 steps of the leapfrogging rings
+(N = 65,536) with FP64, FP32C and the tree path and reports the drift of
+total vorticity, linear impulse and kinetic energy.  The CPU baseline is the
+oracle's restatement of the reference's OpenMP loops (parallel.cpp), timed
+on a sample of source rows of the same system and extrapolated to one RK4
+step.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -156,6 +166,7 @@ def learned_leg(system, dt: float, steps: int, target_err: float, theta: float, 
             ms, launches = g.pair_timing()
             g.set_profiling(False)
             out[key] = {"steps_per_s": steps / wall, "rhs_ms": ms / max(1, launches), "velocity_error": err,
+                        "near_pairs_per_s": info["near_pairs"] / (ms / max(1, launches) * 1e-3),
                         "tol_abs": float(tol), "near_pairs": info["near_pairs"], "far_evals": info["far_evals"],
                         "max_far_nodes": info["max_far"], "max_near_leaves": info["max_near"]}
             log(f"{key}: {out[key]}")
@@ -211,6 +222,42 @@ def drift_leg(steps: int, every: int, learned_cfg: dict | None, target_err: floa
         out[prec] = {"omega": dom, "impulse": di, "ke": dke, "seconds": time.perf_counter() - t0}
         log(f"drift {prec}: {out[prec]}")
     return out
+
+
+def measured_traffic(workload: str, n: int):
+    """HBM bytes per launch of the FP32C pair kernel from the committed
+    rocprofv3 counter runs (FETCH_SIZE and WRITE_SIZE, scaled by the factor
+    the same counters read for a copy kernel of known size), for the
+    workload they were measured on; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "pair_fp32c_helix1m_traffic.txt")
+    if workload != "helix" or n != 1 << 20 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    return {
+        "bytes": (t["fetch_kib"] * t["fetch_factor"] + t["write_kib"] * t["write_factor"]) * 1024.0,
+        "fetch_bytes": t["fetch_kib"] * t["fetch_factor"] * 1024.0,
+        "write_bytes": t["write_kib"] * t["write_factor"] * 1024.0,
+        "launch_config": t["launch_config"],
+        "summary": t["summary"],
+    }
+
+
+def fp32c_error(system):
+    """Normwise error of one FP32C right-hand side against FP64 on the same
+    state: max_i |x_i - x_ref,i| / max_i |x_ref,i| and the 2-norm ratio."""
+    from pawan_amd import capi
+
+    with capi.Solver(system) as g:
+        u64, w64 = g.solve("fp64")
+        u32, w32 = g.solve("fp32c")
+
+    def nw(a, b):
+        return float(np.max(np.linalg.norm(a - b, axis=1)) / np.max(np.linalg.norm(b, axis=1)))
+
+    return {"velocity_maxnorm": nw(u32, u64), "stretching_maxnorm": nw(w32, w64),
+            "velocity_2norm": float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64)),
+            "stretching_2norm": float(np.linalg.norm(w32 - w64) / np.linalg.norm(w64))}
 
 
 def _spread(n: int, count: int) -> np.ndarray:
@@ -344,8 +391,11 @@ def main() -> int:
             "fp64_frac_of_peak": FLOP_PER_PAIR * pairs / (avg64 * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
         }
         log(f"fp64: {out64['fp64_steps_per_s']:.4f} steps/s, pair kernel avg {avg64:.1f} ms")
+        out64["fp32c_error_vs_fp64"] = fp32c_error(system)
+        log(f"fp32c error vs fp64: {out64['fp32c_error_vs_fp64']}")
 
     achieved = FLOP_PER_PAIR * pairs / (avg32 * 1e-3) / 1e12
+    traffic = measured_traffic(args.workload, system.n)
     result = {
         "metric": METRIC,
         "value": sps32,
@@ -373,7 +423,9 @@ def main() -> int:
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": None,
+            "traffic": traffic["bytes"] if traffic else None,
+            "traffic_detail": traffic,
+            "algorithmic_bytes_per_launch": 112 * system.n,
             "kernel": "k_pair_fp32c" + ("_w8" if tune32 and tune32.get("waves8") else ""),
             "launch_config": tune32,
             "avg_launch_ms": avg32,
