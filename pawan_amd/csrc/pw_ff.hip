@@ -194,7 +194,7 @@ int ff_rhs(pw_ctx* c, const Rec64* state, Rate64* out) {
   const auto& tb = f.tb;
   const int n = static_cast<int>(c->n);
   PW_HIP(pw::build_near_items(tb, f.shape.nleaf, c->stream));
-  PW_HIP(pw::launch_near_fp32c(tb.sorted, tb.nodes, tb, f.shape.nleaf, c->nu, f.sigma_max2, c->num_cus * 6,
+  PW_HIP(pw::launch_near_fp32c(tb.sorted, tb.nodes, tb, f.shape.nleaf, c->nu, f.sigma_max2, c->num_cus * pw::near_kernel_blocks_per_cu(),
                                f.rates_s, c->stream));
   pw::MlpWeights W{f.d_frag, f.d_bias};
   PW_HIP(pw::launch_far(tb.sorted, tb.nodes, tb.far_list, tb.far_cnt, tb.cap_far, f.shape.nleaf,

@@ -65,8 +65,11 @@ enum PairKind : int {
 // persistent form.
 hipError_t launch_pair(int kind, const PairConfig& cfg, const PairArgs& args, int grid,
                        hipStream_t st);
-// waves per SIMD the pair kernel is built for (for the persistent grid)
+// resident blocks per CU of the persistent pair kernel of `kind` (equal to
+// its waves per SIMD: 256-thread blocks), from the occupancy calculator
 int pair_kernel_waves(int kind, const PairConfig& cfg);
+// the same for the tree path's near-field kernel
+int near_kernel_blocks_per_cu();
 
 // Stage arithmetic of the steppers (FP64, no contraction, the reference's
 // operation order).  n = number of particles.

@@ -30,6 +30,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
 
 #include "pw_internal.h"
 
@@ -288,8 +289,8 @@ __device__ __forceinline__ void near32(float r2, float sig2, float nu2, float& q
   c = nu2 * kKf * e * isg3 * isg2;  // 2 nu Z / sigma^2
 }
 
-__device__ __forceinline__ void pair32(const T32& t, const SrcT& s, P32& p,
-                                       PSE32& e) {
+template <typename S>
+__device__ __forceinline__ void pair32(const T32& t, const S& s, P32& p, PSE32& e) {
   const float dx = static_cast<float>(t.x - s.x);
   const float dy = static_cast<float>(t.y - s.y);
   const float dz = static_cast<float>(t.z - s.z);
@@ -324,15 +325,24 @@ __device__ __forceinline__ void pair32(const T32& t, const SrcT& s, P32& p,
 // (about 6e-8 of the distance to the centre) is a relative displacement
 // error below 1e-6, and the pair needs no FP64 at all.
 struct alignas(16) SrcRel {
-  float x, y, z, hs2;
+  float rx, ry, rz, hs2;
   float ax, ay, az, vol;
 };
 
+// Direct-sum source record: the relative FP32 position for separated tiles
+// first (two 16-byte reads), then the FP64 position for the others.
+struct alignas(16) SrcT2 {
+  float rx, ry, rz, hs2;
+  float ax, ay, az, vol;
+  double x, y, z, pad;
+};
+
+template <typename S>
 __device__ __forceinline__ void pair32_rel(float tx, float ty, float tz, float tax, float tay, float taz,
-                                           const SrcRel& s, P32& p) {
-  const float dx = tx - s.x;
-  const float dy = ty - s.y;
-  const float dz = tz - s.z;
+                                           const S& s, P32& p) {
+  const float dx = tx - s.rx;
+  const float dy = ty - s.ry;
+  const float dz = tz - s.rz;
   const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
   const float ir = __builtin_amdgcn_rsqf(r2);
   const float ir2 = ir * ir;
@@ -359,11 +369,51 @@ __device__ __forceinline__ void two_sum(float& hi, float& lo, float p) {
   lo += err;
 }
 
-__device__ __forceinline__ void item_fp32c(const PairArgs& pa, const WorkItem& it, SrcT* tile) {
+// Bounding boxes of a block's targets and of each source tile decide, per
+// tile, whether every pair is far enough apart for the FP32 relative-
+// position loop: the gap between the two bounding spheres must exceed both
+// sqrt(44) sigma (no pair needs the Gaussian kernel) and the sum of the two
+// radii (kSepFP32).  The rounding of the relative positions, at most
+// 2^-24 of their distance to the block's centre, is then below
+// 2^-24 (1 + 3 / kSepFP32) of the displacement: 2.4e-7 at worst, of the
+// order of the FP32 pair arithmetic itself.
+constexpr float kSepFP32 = 1.0f;
+
+__device__ __forceinline__ float wave_max(float v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ float wave_min(float v) {
+  for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_maxd(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_mind(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+  return v;
+}
+// a block-uniform value held in scalar registers
+__device__ __forceinline__ float uniform(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ double uniform(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane(static_cast<int>(b & 0xffffffff));
+  const int hi = __builtin_amdgcn_readfirstlane(static_cast<int>(b >> 32));
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
+__device__ __forceinline__ void item_fp32c(const PairArgs& pa, const WorkItem& it, SrcT2* tile) {
+  __shared__ double s_tb[4][6];
+  __shared__ float s_b[4][8];
   const Rec64* __restrict__ state = pa.state;
   const float nu = static_cast<float>(pa.nu);
   const int t = it.t0 + static_cast<int>(threadIdx.x);
   const bool active = t < it.t1;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const Rec64 m = state[active ? t : it.t0];
   T32 me;
   me.x = m.x; me.y = m.y; me.z = m.z;
@@ -373,6 +423,29 @@ __device__ __forceinline__ void item_fp32c(const PairArgs& pa, const WorkItem& i
   me.az = static_cast<float>(m.az);
   me.vol = static_cast<float>(m.vol);
   me.nu2 = 2.0f * nu;
+  // the targets' bounding box (inactive lanes repeat the first target)
+  {
+    const double v[6] = {wave_mind(m.x), wave_mind(m.y), wave_mind(m.z),
+                         wave_maxd(m.x), wave_maxd(m.y), wave_maxd(m.z)};
+    __syncthreads();
+    if (lane == 0)
+      for (int k = 0; k < 6; ++k) s_tb[wave][k] = v[k];
+  }
+  __syncthreads();
+  double cT[3];
+  float rT2 = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double lo = fmin(fmin(s_tb[0][k], s_tb[1][k]), fmin(s_tb[2][k], s_tb[3][k]));
+    const double hi = fmax(fmax(s_tb[0][3 + k], s_tb[1][3 + k]), fmax(s_tb[2][3 + k], s_tb[3][3 + k]));
+    cT[k] = uniform(0.5 * (lo + hi));
+    const float h = static_cast<float>(0.5 * (hi - lo));
+    rT2 += h * h;
+  }
+  const float rT = uniform(sqrtf(rT2) * 1.00001f);
+  const float rtx = static_cast<float>(m.x - cT[0]), rty = static_cast<float>(m.y - cT[1]),
+              rtz = static_cast<float>(m.z - cT[2]);
+  const float h2T = uniform(wave_max(me.hs2));
   float hi[kAcc], lo[kAcc];
 #pragma unroll
   for (int k = 0; k < kAcc; ++k) hi[k] = lo[k] = 0.0f;
@@ -380,30 +453,61 @@ __device__ __forceinline__ void item_fp32c(const PairArgs& pa, const WorkItem& i
   for (int base = it.s0; base < it.s1; base += kTB) {
     const int cnt = min(kTB, it.s1 - base);
     __syncthreads();
+    float bl[3] = {INFINITY, INFINITY, INFINITY}, bh[3] = {-INFINITY, -INFINITY, -INFINITY}, h2 = 0.0f;
     if (static_cast<int>(threadIdx.x) < cnt) {
       const Rec64 r = state[base + threadIdx.x];
-      SrcT f;
-      f.x = r.x; f.y = r.y; f.z = r.z;
+      SrcT2 f;
+      f.rx = static_cast<float>(r.x - cT[0]);
+      f.ry = static_cast<float>(r.y - cT[1]);
+      f.rz = static_cast<float>(r.z - cT[2]);
       f.hs2 = static_cast<float>(0.5 * r.s2);
       f.ax = static_cast<float>(r.ax);
       f.ay = static_cast<float>(r.ay);
       f.az = static_cast<float>(r.az);
       f.vol = static_cast<float>(r.vol);
-      f.pad = 0.0f;
+      f.x = r.x; f.y = r.y; f.z = r.z;
+      f.pad = 0.0;
       tile[threadIdx.x] = f;
+      bl[0] = bh[0] = f.rx;
+      bl[1] = bh[1] = f.ry;
+      bl[2] = bh[2] = f.rz;
+      h2 = f.hs2;
+    }
+    {
+      const float v[7] = {wave_min(bl[0]), wave_min(bl[1]), wave_min(bl[2]), wave_max(bh[0]),
+                          wave_max(bh[1]), wave_max(bh[2]), wave_max(h2)};
+      if (lane == 0)
+        for (int k = 0; k < 7; ++k) s_b[wave][k] = v[k];
     }
     __syncthreads();
     if (!active) continue;
+    // the source tile's sphere, relative to the targets' centre
+    float cs2 = 0.0f, rs2 = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float l = fminf(fminf(s_b[0][k], s_b[1][k]), fminf(s_b[2][k], s_b[3][k]));
+      const float h = fmaxf(fmaxf(s_b[0][3 + k], s_b[1][3 + k]), fmaxf(s_b[2][3 + k], s_b[3][3 + k]));
+      const float c = 0.5f * (l + h), e = 0.5f * (h - l);
+      cs2 += c * c;
+      rs2 += e * e;
+    }
+    const float h2S = uniform(fmaxf(fmaxf(s_b[0][6], s_b[1][6]), fmaxf(s_b[2][6], s_b[3][6])));
+    const float rS = uniform(sqrtf(rs2) * 1.00001f);
+    const float gap = uniform(sqrtf(cs2) * 0.99999f - rT - rS);
+    const bool sep = gap > 0.0f && gap * gap > kFar32 * (h2T + h2S) * 1.0001f && gap > kSepFP32 * (rT + rS);
     const bool diag = base == it.t0;
     const int self = static_cast<int>(threadIdx.x);
     for (int c0 = 0; c0 < cnt; c0 += kChunk) {
       const int c1 = min(cnt, c0 + kChunk);
       P32 p;
-      if (diag) {
+      if (sep) {
+#pragma unroll 2
+        for (int l = c0; l < c1; ++l) pair32_rel(rtx, rty, rtz, me.ax, me.ay, me.az, tile[l], p);
+      } else if (diag) {
         for (int l = c0; l < c1; ++l)
           if (l != self) pair32(me, tile[l], p, pse);
       } else {
-#pragma unroll 2
+#pragma unroll 1
         for (int l = c0; l < c1; ++l) pair32(me, tile[l], p, pse);
       }
       two_sum(hi[0], lo[0], p.ux);
@@ -488,9 +592,9 @@ __device__ __forceinline__ void near_item(const NearArgs& a, const NearItem& it,
       const Rec64 r = a.sorted[base + threadIdx.x];
       if (all_far) {
         SrcRel f;
-        f.x = static_cast<float>(r.x - tcx);
-        f.y = static_cast<float>(r.y - tcy);
-        f.z = static_cast<float>(r.z - tcz);
+        f.rx = static_cast<float>(r.x - tcx);
+        f.ry = static_cast<float>(r.y - tcy);
+        f.rz = static_cast<float>(r.z - tcz);
         f.hs2 = static_cast<float>(0.5 * r.s2);
         f.ax = static_cast<float>(r.ax);
         f.ay = static_cast<float>(r.ay);
@@ -594,7 +698,7 @@ struct TileOf {
 };
 template <>
 struct TileOf<kPairFP32C> {
-  using type = SrcT;
+  using type = SrcT2;
 };
 
 template <int Kind>
@@ -711,9 +815,32 @@ hipError_t launch_near_fp32c(const Rec64* sorted, const NodeRec* nodes, const Tr
   return hipGetLastError();
 }
 
-int pair_kernel_waves(int kind, const PairConfig& cfg) {
-  if (kind == kPairFP32C) return cfg.waves8 ? 8 : 7;
-  return 4;
+namespace {
+// resident 256-thread blocks per CU of a kernel, from the runtime's
+// occupancy calculator (registers, LDS), cached per kernel
+int blocks_per_cu(const void* kernel) {
+  static std::map<const void*, int> cache;
+  auto it = cache.find(kernel);
+  if (it != cache.end()) return it->second;
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, kTB, 0) != hipSuccess || nb < 1) nb = 4;
+  cache[kernel] = nb;
+  return nb;
 }
+}  // namespace
+
+int pair_kernel_waves(int kind, const PairConfig& cfg) {
+  const void* k = nullptr;
+  if (kind == kPairFP32C)
+    k = cfg.waves8 ? reinterpret_cast<const void*>(k_pair_fp32c_w8<true>)
+                   : reinterpret_cast<const void*>(k_pair_fp32c<true>);
+  else if (kind == kPairFP64)
+    k = reinterpret_cast<const void*>(k_pair_fp64<true>);
+  else
+    k = reinterpret_cast<const void*>(k_pair_fp64_ref<true>);
+  return blocks_per_cu(k);
+}
+
+int near_kernel_blocks_per_cu() { return blocks_per_cu(reinterpret_cast<const void*>(k_near_fp32c)); }
 
 }  // namespace pw
