@@ -198,6 +198,12 @@ int pw_set_graph(pw_ctx *ctx, int enable);
  * directly (FP32C).  tol <= 0 disables the error test. */
 int pw_ff_set_params(pw_ctx *ctx, double theta, double tol, int use_mlp);
 int pw_ff_get_params(pw_ctx *ctx, double *theta, double *tol, int *use_mlp);
+/* enable = 1 (default): remove the mean vorticity rate after every
+ * PW_LEARNED right-hand side.  The direct sum conserves total vorticity to
+ * rounding because its pair terms are antisymmetric
+ * (interaction_utils.h:493-495); one-sided far-field evaluations are not. */
+int pw_ff_set_conserve(pw_ctx *ctx, int enable);
+int pw_ff_get_conserve(pw_ctx *ctx, int *enable);
 /* error coefficients of the analytic expansion and of expansion + network
  * (set by pw_ff_train from held-out samples; may be overridden) */
 int pw_ff_set_err_coef(pw_ctx *ctx, double coef_md, double coef_mlp);

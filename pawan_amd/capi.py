@@ -34,7 +34,7 @@ EXPORTS = [
     "pw_ff_set_params", "pw_ff_get_params", "pw_ff_set_err_coef", "pw_ff_get_err_coef",
     "pw_ff_build", "pw_ff_info", "pw_ff_export", "pw_ff_sample", "pw_ff_clear_samples",
     "pw_ff_init_weights", "pw_ff_set_weights", "pw_ff_get_weights", "pw_ff_train",
-    "pw_ff_mlp_forward", "pw_ff_mlp_grad",
+    "pw_ff_mlp_forward", "pw_ff_mlp_grad", "pw_ff_set_conserve", "pw_ff_get_conserve",
 ]
 
 # pw_ff_export selectors
@@ -100,6 +100,8 @@ def load(path: str | None = None) -> C.CDLL:
         "pw_ff_set_params": ([vp, d, d, i], i),
         "pw_ff_get_params": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int)], i),
         "pw_ff_set_err_coef": ([vp, d, d], i),
+        "pw_ff_set_conserve": ([vp, i], i),
+        "pw_ff_get_conserve": ([vp, C.POINTER(C.c_int)], i),
         "pw_ff_get_err_coef": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_double)], i),
         "pw_ff_build": ([vp], i),
         "pw_ff_info": ([vp, C.POINTER(C.c_int64), i64], i),
@@ -302,6 +304,14 @@ class Solver:
         th, tol, um = C.c_double(), C.c_double(), C.c_int()
         _check(self._lib.pw_ff_get_params(self._h, C.byref(th), C.byref(tol), C.byref(um)))
         return {"theta": th.value, "tol": tol.value, "use_mlp": bool(um.value)}
+
+    def ff_set_conserve(self, on: bool) -> None:
+        _check(self._lib.pw_ff_set_conserve(self._h, int(bool(on))))
+
+    def ff_conserve(self) -> bool:
+        v = C.c_int()
+        _check(self._lib.pw_ff_get_conserve(self._h, C.byref(v)))
+        return bool(v.value)
 
     def ff_set_err_coef(self, coef_md: float, coef_mlp: float) -> None:
         _check(self._lib.pw_ff_set_err_coef(self._h, float(coef_md), float(coef_mlp)))

@@ -44,6 +44,8 @@ struct FarField {
   float theta = 0.5f;
   float tol = 0.0f;                  // absolute velocity tolerance (<= 0: opening angle only)
   bool use_mlp = true;
+  bool conserve = true;              // remove the mean vorticity rate
+  double* d_dpart = nullptr;         // block sums for it
   double coef_md = 0.2387324146;     // 3/(4 pi) until calibrated
   double coef_mlp = 0.2387324146;
   // network

@@ -618,6 +618,42 @@ __global__ __launch_bounds__(256) void k_ff_samples(SampleArgs a) {
   o[9] = T;                // target leaf
 }
 
+// Conservation of total vorticity: the pairwise antisymmetry that makes
+// sum dalpha vanish in the direct sum (interaction_utils.h:493-495) does not
+// hold for one-sided far-field evaluations, so the mean vorticity rate is
+// removed: block sums (FP64, fixed order), their total, the subtraction.
+__global__ __launch_bounds__(256) void k_sum_dalpha(const Rate64* __restrict__ r, int n, double* __restrict__ part) {
+  __shared__ double red[4][3];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  double v[3] = {0.0, 0.0, 0.0};
+  if (i < n) {
+    v[0] = r[i].wx; v[1] = r[i].wy; v[2] = r[i].wz;
+  }
+  for (int off = 32; off > 0; off >>= 1)
+    for (int k = 0; k < 3; ++k) v[k] += __shfl_xor(v[k], off, 64);
+  if ((threadIdx.x & 63) == 0)
+    for (int k = 0; k < 3; ++k) red[threadIdx.x >> 6][k] = v[k];
+  __syncthreads();
+  if (threadIdx.x < 3)
+    part[3 * blockIdx.x + threadIdx.x] =
+        (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
+}
+
+__global__ __launch_bounds__(64) void k_mean_dalpha(double* __restrict__ part, int nb, int n) {
+  if (threadIdx.x >= 3) return;
+  double s = 0.0;
+  for (int b = 0; b < nb; ++b) s += part[3 * b + threadIdx.x];
+  part[3 * nb + threadIdx.x] = s / static_cast<double>(n);
+}
+
+__global__ __launch_bounds__(256) void k_sub_dalpha(Rate64* __restrict__ r, int n, const double* __restrict__ mean) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  r[i].wx -= mean[0];
+  r[i].wy -= mean[1];
+  r[i].wz -= mean[2];
+}
+
 // rates[perm[i]] = rates_s[i]
 __global__ __launch_bounds__(256) void k_scatter(const Rate64* __restrict__ in, const int* __restrict__ perm,
                                                  int n, Rate64* __restrict__ out) {
@@ -645,6 +681,15 @@ hipError_t launch_far(const Rec64* sorted, const NodeRec* nodes, const int* far_
     attr = true;
   }
   hipLaunchKernelGGL(k_far_mlp, dim3(nleaf), dim3(kMlpThreads), lds, st, a, *mlp);
+  return hipGetLastError();
+}
+
+hipError_t launch_remove_mean_dalpha(Rate64* rates, int n, double* part, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  const int nb = (n + 255) / 256;
+  hipLaunchKernelGGL(k_sum_dalpha, dim3(nb), dim3(256), 0, st, rates, n, part);
+  hipLaunchKernelGGL(k_mean_dalpha, dim3(1), dim3(64), 0, st, part, nb, n);
+  hipLaunchKernelGGL(k_sub_dalpha, dim3(nb), dim3(256), 0, st, rates, n, part + 3 * nb);
   return hipGetLastError();
 }
 

@@ -55,6 +55,7 @@ int ensure_ff(pw_ctx* c) {
       dalloc(&tb.near_parts, nl) != hipSuccess || dalloc(&tb.near_offs, nl) != hipSuccess ||
       dalloc(&tb.near_items, nl * pw::kNearMaxParts) != hipSuccess || dalloc(&tb.near_total, 1) != hipSuccess ||
       dalloc(&tb.near_queue, 1) != hipSuccess ||
+      dalloc(&f.d_dpart, 3 * (static_cast<size_t>(n + 255) / 256 + 1)) != hipSuccess ||
       dalloc(&tb.near_partial, nl * pw::kNearMaxParts * pw::kLeafSize) != hipSuccess ||
       pw::near_scan_temp_bytes(f.shape.nleaf, &tb.scan_temp_bytes) != hipSuccess ||
       dalloc(reinterpret_cast<unsigned char**>(&tb.scan_temp), tb.scan_temp_bytes) != hipSuccess) {
@@ -165,6 +166,7 @@ void ff_release(pw_ctx* c) {
   (void)hipFree(tb.near_partial);
   (void)hipFree(tb.scan_temp);
   (void)hipFree(f.d_overflow);
+  (void)hipFree(f.d_dpart);
   (void)hipFree(f.rates_s);
   (void)hipFree(f.d_params);
   (void)hipFree(f.d_m);
@@ -199,6 +201,7 @@ int ff_rhs(pw_ctx* c, const Rec64* state, Rate64* out) {
   pw::MlpWeights W{f.d_frag, f.d_bias};
   PW_HIP(pw::launch_far(tb.sorted, tb.nodes, tb.far_list, tb.far_cnt, tb.cap_far, f.shape.nleaf,
                         f.use_mlp ? &W : nullptr, f.rates_s, c->stream));
+  if (f.conserve) PW_HIP(pw::launch_remove_mean_dalpha(f.rates_s, n, f.d_dpart, c->stream));
   PW_HIP(pw::launch_scatter_rates(f.rates_s, tb.perm, n, out, c->stream));
   return PW_OK;
 }
@@ -236,6 +239,19 @@ int pw_ff_get_params(pw_ctx* c, double* theta, double* tol, int* use_mlp) {
   if (theta) *theta = c->ff.theta;
   if (tol) *tol = c->ff.tol;
   if (use_mlp) *use_mlp = c->ff.use_mlp ? 1 : 0;
+  return PW_OK;
+}
+
+int pw_ff_set_conserve(pw_ctx* c, int enable) {
+  PW_CTX(c);
+  c->ff.conserve = enable != 0;
+  return PW_OK;
+}
+
+int pw_ff_get_conserve(pw_ctx* c, int* enable) {
+  PW_CTX(c);
+  if (!enable) return fail(PW_ERR_ARG, "null output");
+  *enable = c->ff.conserve ? 1 : 0;
   return PW_OK;
 }
 

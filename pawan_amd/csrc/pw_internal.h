@@ -224,6 +224,8 @@ hipError_t launch_far(const Rec64* sorted, const NodeRec* nodes, const int* far_
                       const int* far_cnt, int cap_far, int nleaf, const MlpWeights* mlp,
                       Rate64* rates_s, hipStream_t st);
 
+// dalpha_i -= mean over i of dalpha (part: 3 * (ceil(n/256) + 1) doubles)
+hipError_t launch_remove_mean_dalpha(Rate64* rates, int n, double* part, hipStream_t st);
 // rates[perm[i]] = rates_s[i]
 hipError_t launch_scatter_rates(const Rate64* rates_s, const int* perm, int n, Rate64* rates,
                                 hipStream_t st);

@@ -27,6 +27,7 @@ def built(helix):
     """Tree, lists and exports of the helix at theta 0.5 (no error test)."""
     with capi.Solver(helix) as g:
         g.ff_set_params(0.5, 0.0, use_mlp=False)
+        g.ff_set_conserve(False)  # the oracle restates the raw far field
         g.ff_build()
         out = {k: g.ff_export(w) for k, w in (("perm", capi.FF_PERM), ("nodes", capi.FF_NODES),
                                               ("near_cnt", capi.FF_NEAR_CNT), ("near", capi.FF_NEAR),
@@ -150,6 +151,7 @@ def test_learned_far_field_matches_oracle_on_gpu_lists(oracle, helix, built):
     with capi.Solver(helix) as g:
         g.ff_set_weights(params)
         g.ff_set_params(0.5, 0.0, use_mlp=True)
+        g.ff_set_conserve(False)
         u, w = g.solve("learned")
     rng = np.random.default_rng(1)
     targets = np.sort(rng.choice(helix.n, size=32, replace=False))
@@ -167,6 +169,25 @@ def test_learned_far_field_matches_oracle_on_gpu_lists(oracle, helix, built):
     gu, gw = u[perm[targets]], w[perm[targets]]
     assert normwise(gu, ru) < 2e-5, normwise(gu, ru)
     assert normwise(gw, rw) < 2e-5, normwise(gw, rw)
+
+
+def test_conservation_removes_the_mean_vorticity_rate(helix, built):
+    """With conservation on (the default), sum dalpha vanishes to rounding,
+    as in the direct sum, and each rate differs from the raw one by the
+    same vector, the mean."""
+    with capi.Solver(helix) as g:
+        assert g.ff_conserve()
+        g.ff_set_params(0.5, 0.0, use_mlp=False)
+        u, w = g.solve("learned")
+        ud, wd = g.solve("fp64")
+    scale = np.abs(built["w"]).sum(axis=0)
+    raw_sum = built["w"].sum(axis=0)
+    _say(f"sum dalpha raw {raw_sum}, conserved {w.sum(axis=0)}, direct {wd.sum(axis=0)}")
+    assert np.all(np.abs(w.sum(axis=0)) <= 1e-12 * scale)
+    np.testing.assert_array_equal(u, built["u"])
+    shift = built["w"] - w
+    np.testing.assert_allclose(shift, np.broadcast_to(raw_sum / helix.n, shift.shape), rtol=1e-9,
+                               atol=1e-15 * np.abs(built["w"]).max())
 
 
 def test_error_against_direct_sum_falls_with_theta(helix):
