@@ -343,8 +343,8 @@ def main() -> int:
     ap.add_argument("--theta", type=float, default=0.5)
     ap.add_argument("--train-n", type=int, default=0,
                     help="particles of the training wakes (default: the benchmark's N)")
-    ap.add_argument("--train-samples", type=int, default=1_200_000)
-    ap.add_argument("--train-steps", type=int, default=15000)
+    ap.add_argument("--train-samples", type=int, default=2_400_000)
+    ap.add_argument("--train-steps", type=int, default=40000)
     ap.add_argument("--no-drift", action="store_true")
     ap.add_argument("--drift-steps", type=int, default=200)
     args = ap.parse_args()

@@ -3,8 +3,14 @@
 // __parallel -> __rk4(T, n) -> integrate(S, IO, diagnose).  The wakes are
 // read from a binary file instead of being built by __vring/__resolve.
 //
-//   pawan_run <input> <name> <T> <n> [euler|rk3|rk4] [fp64|fp64_ref|fp32c]
+//   pawan_run <input> <name> <T> <n> [euler|rk3|rk4] [fp64|fp64_ref|fp32c|learned]
 //             [nu] [diagnose 0|1]
+//
+// learned: the tree path.  Its network is read from $PAWAN_FF_WEIGHTS (36,748
+// float32 values) when set, else trained on samples of the initial state
+// ($PAWAN_FF_SAMPLES, default 200,000; $PAWAN_FF_STEPS Adam steps, default
+// 5,000).  $PAWAN_FF_THETA (0.5) and $PAWAN_FF_TOL (0: opening angle only)
+// set the traversal.
 //
 // input: int64 nwakes, then per wake int64 n followed by n x 3 positions,
 // n x 3 strengths, n radii, n volumes (doubles, native endianness).
@@ -26,6 +32,36 @@
 
 static void read_exact(FILE* f, void* p, size_t bytes) {
   if (bytes && std::fread(p, 1, bytes, f) != bytes) throw std::runtime_error("short read");
+}
+
+static double env_or(const char* name, double dflt) {
+  const char* v = std::getenv(name);
+  return v ? std::atof(v) : dflt;
+}
+
+static void check(int rc, const char* what) {
+  if (rc != PW_OK) throw std::runtime_error(std::string(what) + ": " + pw_last_error());
+}
+
+// network and traversal of the tree path on the interaction's context
+static void setup_learned(pawan::__interaction* S) {
+  pw_ctx* ctx = S->context();
+  check(pw_ff_set_params(ctx, env_or("PAWAN_FF_THETA", 0.5), env_or("PAWAN_FF_TOL", 0.0), 1), "pw_ff_set_params");
+  if (const char* path = std::getenv("PAWAN_FF_WEIGHTS")) {
+    std::vector<float> w(36748);
+    FILE* f = std::fopen(path, "rb");
+    if (!f) throw std::runtime_error(std::string("cannot open ") + path);
+    read_exact(f, w.data(), w.size() * sizeof(float));
+    std::fclose(f);
+    check(pw_ff_set_weights(ctx, w.data(), static_cast<int64_t>(w.size())), "pw_ff_set_weights");
+    return;
+  }
+  check(pw_ff_sample(ctx, static_cast<int64_t>(env_or("PAWAN_FF_SAMPLES", 200000)), 1), "pw_ff_sample");
+  double rep[10] = {0};
+  check(pw_ff_train(ctx, static_cast<int64_t>(env_or("PAWAN_FF_STEPS", 5000)), 1024, 2e-3, 1, rep, 10),
+        "pw_ff_train");
+  std::fprintf(stderr, "pawan_run: network trained: held-out loss %.4f (expansion alone %.4f), %.0f ms\n", rep[1],
+               rep[2], rep[9]);
 }
 
 int main(int argc, char** argv) {
@@ -59,7 +95,11 @@ int main(int argc, char** argv) {
     pawan::__interaction* S = new pawan::__parallel(wakes[0].get());
     for (size_t w = 1; w < wakes.size(); ++w) S->addWake(wakes[w].get());
     if (argc > 7) S->setNu(std::atof(argv[7]));
-    S->setPrecision(prec == "fp32c" ? PW_FP32C : prec == "fp64_ref" ? PW_FP64_REF : PW_FP64);
+    S->setPrecision(prec == "fp32c"      ? PW_FP32C
+                    : prec == "fp64_ref" ? PW_FP64_REF
+                    : prec == "learned"  ? PW_LEARNED
+                                         : PW_FP64);
+    if (prec == "learned") setup_learned(S);
     const bool diagnose = argc > 8 ? std::atoi(argv[8]) != 0 : true;
 
     pawan::__io* IO = new pawan::__io(argv[2]);

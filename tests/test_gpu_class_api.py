@@ -82,3 +82,24 @@ def test_driver_matches_oracle(tmp_path, oracle, scheme):
     np.testing.assert_allclose(diag[-1, 13:15], d_ref[12:14], rtol=1e-12)   # KE, KEF
     np.testing.assert_allclose(diag[-1, 4:7], d_ref[3:6], rtol=1e-12, atol=1e-15)  # linear impulse
     assert diag[-1, 15] == pytest.approx(d_ref[14], rel=1e-12)  # centroid of the last wake
+
+
+def test_driver_runs_the_tree_path(tmp_path):
+    """The same driver with PW_LEARNED: it trains the network on its own
+    initial state through the C-ABI, integrates, and stays close to the
+    FP64 run of the same driver."""
+    s = W.helix_wake(16384, seed=5)
+    inp = tmp_path / "in.bin"
+    write_input(inp, s)
+    env = dict(os.environ, PAWAN_DATA_DIR=str(tmp_path), PAWAN_FF_SAMPLES="60000", PAWAN_FF_STEPS="2000")
+    out = {}
+    for prec in ("fp64", "learned"):
+        r = subprocess.run([EXE, str(inp), prec, "0.002", "2", "rk4", prec, repr(s.nu), "0"], env=env,
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        print(r.stderr.strip(), flush=True)
+        t, wakes = read_wake(tmp_path / f"{prec}Wake.wake")[-1]
+        out[prec] = (wakes[0][1], wakes[0][2])
+    disp = out["fp64"][0] - s.pos
+    assert normwise(out["learned"][0] - s.pos, disp) < 1e-2
+    assert normwise(out["learned"][1], out["fp64"][1]) < 1e-3
