@@ -15,6 +15,8 @@ pinned here by mathematics instead (tests/test_oracle_farfield.py):
     alpha^T G equals the reference's pair stretching summed over the cluster
     for a one-particle cluster.
   * morton_perm: the codes of hand-placed points.
+  * leaf_first: hand-made codes (a dense cell cut into runs of 256, separate
+    cells never sharing a leaf).
   * node_moments: computed from the particles directly (not by the
     parallel-axis shift the GPU uses), so agreement checks the shift.
   * mlp_*: forward and backward of the network, the gradient checked
@@ -70,23 +72,55 @@ def morton_perm(pos: np.ndarray) -> np.ndarray:
     return np.argsort(morton_codes(pos), kind="stable")
 
 
-def tree_levels(n: int):
-    counts = [(n + LEAF - 1) // LEAF]
+CELL_TARGET = 320
+
+
+def prefix_levels(codes: np.ndarray) -> np.ndarray:
+    """Length in 3-bit groups of the common Morton prefix of sorted
+    neighbours i-1, i (i >= 1); 10 for identical codes."""
+    c = codes.astype(np.uint64)
+    x = c[1:] ^ c[:-1]
+    lev = np.full(len(x), 10, dtype=np.int64)
+    nz = x != 0
+    # bit length of x over the 30-bit code: common prefix = 30 - bitlength
+    bl = np.floor(np.log2(x[nz].astype(np.float64))).astype(np.int64) + 1
+    lev[nz] = (30 - bl) // 3
+    return lev
+
+
+def leaf_first(codes: np.ndarray) -> np.ndarray:
+    """First sorted index of every leaf, then n.  Leaves are the non-empty
+    cells of the first octree level whose mean occupancy is at most
+    CELL_TARGET, each cut into runs of at most LEAF particles."""
+    n = len(codes)
+    lev = prefix_levels(codes)
+    hist = np.bincount(lev, minlength=11)
+    level, cells = 10, 1
+    for D in range(1, 11):
+        cells += hist[D - 1]
+        if n <= CELL_TARGET * cells:
+            level = D
+            break
+    starts = np.concatenate([[0], np.nonzero(lev < level)[0] + 1])
+    ends = np.concatenate([starts[1:], [n]])
+    firsts = [np.arange(a, b, LEAF) for a, b in zip(starts, ends)]
+    return np.concatenate(firsts + [np.array([n])]).astype(np.int64)
+
+
+def tree_levels(nleaf: int):
+    counts = [nleaf]
     while counts[-1] > 1:
         counts.append((counts[-1] + FAN - 1) // FAN)
     offs = np.concatenate([[0], np.cumsum(counts)[:-1]]).astype(int)
     return counts, offs
 
 
-def node_ranges(n: int):
-    """[first, last) of every node, leaves first, then each level."""
-    counts, offs = tree_levels(n)
-    first, last, level, child = [], [], [], []
-    for i in range(counts[0]):
-        first.append(LEAF * i)
-        last.append(min(LEAF * (i + 1), n))
-        level.append(0)
-        child.append(-1)
+def node_ranges(lf: np.ndarray):
+    """[first, last) of every node from the leaves' firsts (lf, ending in n):
+    leaves first, then each level of 8 consecutive nodes."""
+    nleaf = len(lf) - 1
+    counts, offs = tree_levels(nleaf)
+    first, last, level, child = list(lf[:-1]), list(lf[1:]), [0] * nleaf, [-1] * nleaf
     for L in range(1, len(counts)):
         for j in range(counts[L]):
             c0 = offs[L - 1] + FAN * j
@@ -98,14 +132,14 @@ def node_ranges(n: int):
     return np.array(first), np.array(last), np.array(level), np.array(child)
 
 
-def node_moments(pos: np.ndarray, alpha: np.ndarray, sigma: np.ndarray):
-    """Moments of every node straight from its particles (sorted arrays).
+def node_moments(pos: np.ndarray, alpha: np.ndarray, sigma: np.ndarray, lf: np.ndarray):
+    """Moments of every node straight from its particles (sorted arrays;
+    lf = leaf_first of the sorted codes).
 
     Returns dict of arrays: centre (k,3), r, s0, s2, sg, M (k,3), D (k,3,3)
     with D[i, j] = sum a_i delta_j, plus first/last/level/child.
     """
-    n = len(pos)
-    first, last, level, child = node_ranges(n)
+    first, last, level, child = node_ranges(lf)
     k = len(first)
     out = {"centre": np.zeros((k, 3)), "r": np.zeros(k), "s0": np.zeros(k), "s2": np.zeros(k),
            "sg": np.zeros(k), "M": np.zeros((k, 3)), "D": np.zeros((k, 3, 3)),

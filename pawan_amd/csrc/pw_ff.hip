@@ -29,6 +29,57 @@ uint64_t splitmix(uint64_t& s) {
   return z ^ (z >> 31);
 }
 
+void free_leaf_buffers(pwi::FarField& f) {
+  auto& tb = f.tb;
+  for (void* p : {static_cast<void*>(tb.leaf_first), static_cast<void*>(tb.nodes), static_cast<void*>(tb.near_list),
+                  static_cast<void*>(tb.near_cnt), static_cast<void*>(tb.far_list), static_cast<void*>(tb.far_cnt),
+                  static_cast<void*>(tb.near_parts), static_cast<void*>(tb.near_offs),
+                  static_cast<void*>(tb.near_items), static_cast<void*>(tb.near_partial),
+                  static_cast<void*>(tb.scan_temp)})
+    (void)hipFree(p);
+  tb.leaf_first = nullptr;
+  tb.nodes = nullptr;
+  tb.near_list = tb.near_cnt = tb.far_list = tb.far_cnt = nullptr;
+  tb.near_parts = tb.near_offs = nullptr;
+  tb.near_items = nullptr;
+  tb.near_partial = nullptr;
+  tb.scan_temp = nullptr;
+  tb.leaf_cap = tb.node_cap = tb.cap_near = tb.cap_far = 0;
+}
+
+// buffers indexed by leaf or node, for at least nleaf leaves (grown by a
+// quarter beyond, so that the next states rarely need more)
+int ensure_leaf_capacity(pw_ctx* c, int nleaf) {
+  auto& f = c->ff;
+  auto& tb = f.tb;
+  if (nleaf <= tb.leaf_cap) return PW_OK;
+  PW_HIP(hipStreamSynchronize(c->stream));
+  free_leaf_buffers(f);
+  const int n = static_cast<int>(c->n);
+  const int cap = std::min(n, nleaf + nleaf / 4 + 64);
+  const pw::TreeShape sh = pw::tree_shape(n, cap);
+  if (sh.nlevel >= pw::kMaxLevels) return fail(PW_ERR_UNSUPPORTED, "too many leaves for the tree");
+  const size_t nl = static_cast<size_t>(cap);
+  const int cap_near = std::min(cap, 8192), cap_far = std::min(sh.nnodes, 8192);
+  if (dalloc(&tb.leaf_first, nl + 1) != hipSuccess || dalloc(&tb.nodes, sh.nnodes) != hipSuccess ||
+      dalloc(&tb.near_list, nl * cap_near) != hipSuccess || dalloc(&tb.near_cnt, nl) != hipSuccess ||
+      dalloc(&tb.far_list, nl * cap_far) != hipSuccess || dalloc(&tb.far_cnt, nl) != hipSuccess ||
+      dalloc(&tb.near_parts, nl) != hipSuccess || dalloc(&tb.near_offs, nl) != hipSuccess ||
+      dalloc(&tb.near_items, nl * pw::kNearMaxParts) != hipSuccess ||
+      dalloc(&tb.near_partial, nl * pw::kNearMaxParts * pw::kLeafSize) != hipSuccess ||
+      pw::near_scan_temp_bytes(cap, &tb.scan_temp_bytes) != hipSuccess ||
+      dalloc(reinterpret_cast<unsigned char**>(&tb.scan_temp), tb.scan_temp_bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    free_leaf_buffers(f);
+    return fail(PW_ERR_NOMEM, "tree leaf buffers");
+  }
+  tb.leaf_cap = cap;
+  tb.node_cap = sh.nnodes;
+  tb.cap_near = cap_near;
+  tb.cap_far = cap_far;
+  return PW_OK;
+}
+
 int ensure_ff(pw_ctx* c) {
   PW_TRY(pwi::ensure_built(c));
   if (!c->coupled && c->wn.size() > 1)
@@ -36,29 +87,19 @@ int ensure_ff(pw_ctx* c) {
   auto& f = c->ff;
   if (f.ready) return PW_OK;
   const int n = static_cast<int>(c->n);
-  f.shape = pw::tree_shape(n);
-  if (f.shape.nnodes <= 0 || f.shape.nlevel >= pw::kMaxLevels)
-    return fail(PW_ERR_UNSUPPORTED, "too many particles for the tree");
   auto& tb = f.tb;
-  tb.cap_near = std::min(f.shape.nleaf, 8192);
-  tb.cap_far = std::min(f.shape.nnodes, 8192);
   PW_HIP(pw::tree_sort_temp_bytes(n, &tb.sort_temp_bytes));
-  const size_t nl = static_cast<size_t>(f.shape.nleaf);
+  PW_HIP(pw::tree_scan_temp_bytes(n, &tb.scan_n_bytes));
   if (dalloc(&tb.bbox_part, 6 * 1024) != hipSuccess || dalloc(&tb.bbox, 6) != hipSuccess ||
       dalloc(&tb.code, n) != hipSuccess || dalloc(&tb.code_sorted, n) != hipSuccess ||
       dalloc(&tb.idx, n) != hipSuccess || dalloc(&tb.perm, n) != hipSuccess ||
       dalloc(reinterpret_cast<unsigned char**>(&tb.sort_temp), tb.sort_temp_bytes) != hipSuccess ||
-      dalloc(&tb.sorted, n) != hipSuccess || dalloc(&tb.nodes, f.shape.nnodes) != hipSuccess ||
-      dalloc(&tb.near_list, nl * tb.cap_near) != hipSuccess || dalloc(&tb.near_cnt, nl) != hipSuccess ||
-      dalloc(&tb.far_list, nl * tb.cap_far) != hipSuccess || dalloc(&tb.far_cnt, nl) != hipSuccess ||
+      dalloc(&tb.sorted, n) != hipSuccess || dalloc(&tb.cell_start, n) != hipSuccess ||
+      dalloc(&tb.cut, n) != hipSuccess || dalloc(&tb.leaf_id, n) != hipSuccess || dalloc(&tb.hist, 16) != hipSuccess ||
+      dalloc(reinterpret_cast<unsigned char**>(&tb.scan_n_temp), tb.scan_n_bytes) != hipSuccess ||
       dalloc(&f.d_overflow, 1) != hipSuccess || dalloc(&f.rates_s, n) != hipSuccess ||
-      dalloc(&tb.near_parts, nl) != hipSuccess || dalloc(&tb.near_offs, nl) != hipSuccess ||
-      dalloc(&tb.near_items, nl * pw::kNearMaxParts) != hipSuccess || dalloc(&tb.near_total, 1) != hipSuccess ||
-      dalloc(&tb.near_queue, 1) != hipSuccess ||
-      dalloc(&f.d_dpart, 3 * (static_cast<size_t>(n + 255) / 256 + 1)) != hipSuccess ||
-      dalloc(&tb.near_partial, nl * pw::kNearMaxParts * pw::kLeafSize) != hipSuccess ||
-      pw::near_scan_temp_bytes(f.shape.nleaf, &tb.scan_temp_bytes) != hipSuccess ||
-      dalloc(reinterpret_cast<unsigned char**>(&tb.scan_temp), tb.scan_temp_bytes) != hipSuccess) {
+      dalloc(&tb.near_total, 1) != hipSuccess || dalloc(&tb.near_queue, 1) != hipSuccess ||
+      dalloc(&f.d_dpart, 3 * (static_cast<size_t>(n + 255) / 256 + 1)) != hipSuccess) {
     (void)hipGetLastError();
     pwi::ff_release(c);
     return fail(PW_ERR_NOMEM, "tree buffers");
@@ -81,9 +122,23 @@ pw::ListParams list_params(const pwi::FarField& f, bool mac_only) {
   return lp;
 }
 
+// Tree and lists of `state`.  The leaf count is data dependent: it is read
+// back once (one stream synchronisation per right-hand side), and the
+// leaf-indexed buffers grow when it exceeds them.
 int build(pw_ctx* c, const Rec64* state, bool mac_only) {
   auto& f = c->ff;
-  PW_HIP(pw::build_tree(state, static_cast<int>(c->n), f.tb, f.shape, c->stream));
+  const int n = static_cast<int>(c->n);
+  PW_HIP(pw::build_leaves(state, n, f.tb, c->stream));
+  int nleaf = 0;
+  PW_HIP(hipMemcpyAsync(&nleaf, f.tb.hist + 12, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  PW_HIP(hipStreamSynchronize(c->stream));
+  if (nleaf < 1 || nleaf > n) return fail(PW_ERR_STATE, "tree: bad leaf count");
+  if (nleaf > f.tb.leaf_cap) {
+    PW_TRY(ensure_leaf_capacity(c, nleaf));
+    PW_HIP(pw::write_leaf_first(n, f.tb, c->stream));
+  }
+  f.shape = pw::tree_shape(n, nleaf);
+  PW_HIP(pw::build_moments(n, f.tb, f.shape, c->stream));
   PW_HIP(pw::build_lists(f.tb, f.shape, list_params(f, mac_only), c->stream));
   f.lists_valid = true;
   return PW_OK;
@@ -165,6 +220,12 @@ void ff_release(pw_ctx* c) {
   (void)hipFree(tb.near_queue);
   (void)hipFree(tb.near_partial);
   (void)hipFree(tb.scan_temp);
+  (void)hipFree(tb.leaf_first);
+  (void)hipFree(tb.cell_start);
+  (void)hipFree(tb.cut);
+  (void)hipFree(tb.leaf_id);
+  (void)hipFree(tb.hist);
+  (void)hipFree(tb.scan_n_temp);
   (void)hipFree(f.d_overflow);
   (void)hipFree(f.d_dpart);
   (void)hipFree(f.rates_s);
@@ -374,8 +435,9 @@ int pw_ff_sample(pw_ctx* c, int64_t nsamples, uint64_t seed) {
   PW_TRY(build(c, c->y, true));  // opening angle only: every candidate far node
   f.lists_valid = false;         // the next right-hand side rebuilds with the tolerance
   const int nl = f.shape.nleaf;
-  std::vector<int> fc(nl);
+  std::vector<int> fc(nl), lf(nl + 1);
   PW_HIP(hipMemcpyAsync(fc.data(), f.tb.far_cnt, sizeof(int) * nl, hipMemcpyDeviceToHost, c->stream));
+  PW_HIP(hipMemcpyAsync(lf.data(), f.tb.leaf_first, sizeof(int) * (nl + 1), hipMemcpyDeviceToHost, c->stream));
   PW_HIP(hipStreamSynchronize(c->stream));
   PW_TRY(pwi::ff_check(c));
   std::vector<int> with;
@@ -387,8 +449,8 @@ int pw_ff_sample(pw_ctx* c, int64_t nsamples, uint64_t seed) {
   for (int64_t k = 0; k < nsamples; ++k) {
     const int T = with[splitmix(s) % with.size()];
     const int slot = static_cast<int>(splitmix(s) % static_cast<uint64_t>(fc[T]));
-    const int size = static_cast<int>(std::min<int64_t>(pw::kLeafSize, c->n - int64_t(pw::kLeafSize) * T));
-    const int t = pw::kLeafSize * T + static_cast<int>(splitmix(s) % static_cast<uint64_t>(size));
+    const int size = lf[T + 1] - lf[T];
+    const int t = lf[T] + static_cast<int>(splitmix(s) % static_cast<uint64_t>(size));
     spec[3 * k] = T;
     spec[3 * k + 1] = slot;
     spec[3 * k + 2] = t;

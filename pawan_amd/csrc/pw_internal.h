@@ -113,7 +113,11 @@ hipError_t launch_diag_pairs(const Rec64* y, const double* radius,
 // ---------------------------------------------------------------------
 // Cluster tree of the learned far field.
 // ---------------------------------------------------------------------
-constexpr int kLeafSize = 256;  // particles per leaf (= one LDS tile)
+constexpr int kLeafSize = 256;  // particles per leaf at most (= one LDS tile)
+// Leaves are the cells of one octree level (Morton-code prefix), cut into
+// runs of at most 256: the level is the first whose mean occupancy is at
+// most kCellTarget particles per non-empty cell.
+constexpr int kCellTarget = 320;
 constexpr int kFan = 8;         // children per node
 constexpr int kMaxLevels = 12;  // 256 * 8^11 particles
 constexpr int kStack = 96;      // traversal stack (7 per level + 1)
@@ -135,13 +139,13 @@ struct alignas(16) NodeRec {
 static_assert(sizeof(NodeRec) == 128, "NodeRec is 128 B");
 
 // Level layout of the node array: level L holds level_count[L] nodes from
-// level_off[L]; level 0 are the leaves (node i = particles [256 i, ...)).
+// level_off[L]; level 0 are the leaves, in Morton order.
 struct TreeShape {
   int n, nleaf, nnodes, nlevel, root;
   int level_off[kMaxLevels];
   int level_count[kMaxLevels];
 };
-TreeShape tree_shape(int n);
+TreeShape tree_shape(int n, int nleaf);
 
 // Near-field work item: targets = leaf T, sources = T's near list [k0, k1);
 // slot < 0: writes the rates, else partial-sum row `slot`.
@@ -149,7 +153,7 @@ struct NearItem {
   int T, k0, k1, slot;
 };
 constexpr int kNearChunk = 128;   // near leaves per work item
-constexpr int kNearMaxParts = 32; // work items per leaf at most
+constexpr int kNearMaxParts = 16; // work items per leaf at most
 
 struct TreeBuffers {
   double* bbox_part = nullptr;  // 6 x 1024 block extremes
@@ -161,6 +165,16 @@ struct TreeBuffers {
   void* sort_temp = nullptr;
   size_t sort_temp_bytes = 0;
   Rec64* sorted = nullptr;      // state in Morton order
+  // leaves
+  int* cell_start = nullptr;    // n: first sorted index of the particle's cell
+  int* cut = nullptr;           // n: 1 where a leaf starts (scratch before)
+  int* leaf_id = nullptr;       // n: exclusive scan of cut
+  int* leaf_first = nullptr;    // leaf_cap + 1: first particle of each leaf, then n
+  int* hist = nullptr;          // 16: Morton prefix-length histogram, [11] level, [12] nleaf
+  void* scan_n_temp = nullptr;
+  size_t scan_n_bytes = 0;
+  int leaf_cap = 0;             // leaves the leaf-indexed buffers hold
+  int node_cap = 0;
   NodeRec* nodes = nullptr;
   int* near_list = nullptr;     // nleaf x cap_near leaf ids
   int* near_cnt = nullptr;
@@ -193,10 +207,15 @@ struct ListParams {
 };
 
 hipError_t tree_sort_temp_bytes(int n, size_t* bytes);
-// bbox, Morton codes, radix sort, gather into tb.sorted, leaf and parent
-// moments.
-hipError_t build_tree(const Rec64* y, int n, TreeBuffers& tb, const TreeShape& shape,
-                      hipStream_t st);
+hipError_t tree_scan_temp_bytes(int n, size_t* bytes);
+// bbox, Morton codes, radix sort, gather into tb.sorted; the leaves: octree
+// level, cell starts (max-scan), cuts, leaf ids (exclusive scan),
+// leaf_first (written only below leaf_cap) and the leaf count in hist[12].
+hipError_t build_leaves(const Rec64* y, int n, TreeBuffers& tb, hipStream_t st);
+// leaf_first again (after the leaf buffers have grown)
+hipError_t write_leaf_first(int n, TreeBuffers& tb, hipStream_t st);
+// leaf and parent moments
+hipError_t build_moments(int n, TreeBuffers& tb, const TreeShape& shape, hipStream_t st);
 // near and far lists of every target leaf
 hipError_t build_lists(const TreeBuffers& tb, const TreeShape& shape, const ListParams& lp,
                        hipStream_t st);

@@ -1,9 +1,13 @@
 // pw_tree.hip -- the cluster tree of the learned far field.
 //
-// Particles are ordered along a Morton (Z-order) curve of their positions
-// and cut into leaves of 256 consecutive particles (a linear octree with a
-// fixed leaf size); 8 consecutive nodes of one level form a node of the
-// next.  Every node stores its centre (particle mean), a bounding radius,
+// Particles are ordered along a Morton (Z-order) curve of their positions.
+// Leaves are the non-empty cells of one octree level (the first level whose
+// mean occupancy is at most kCellTarget), each cut into runs of at most 256
+// consecutive particles: a leaf never straddles a jump of the curve, so its
+// radius is bounded by its cell's.  (Fixed runs of 256 sorted particles do
+// straddle such jumps; at N = 1M some then spanned the whole wake and were
+// near to every other leaf.)  8 consecutive nodes of one level form a node
+// of the next.  Every node stores its centre (particle mean), a bounding radius,
 // and the vorticity moments the far field needs:
 //   M  = sum a,   D_ij = sum a_i delta_j,   (delta = x - centre)
 //   s0 = sum |a|, S1 = sum |a| delta,  s2 = sum |a| |delta|^2
@@ -93,6 +97,67 @@ __global__ __launch_bounds__(256) void k_morton(const Rec64* __restrict__ y, int
   idx[i] = i;
 }
 
+// Length, in 3-bit groups, of the common Morton prefix of sorted neighbours
+// i-1 and i (10: same finest cell).
+__device__ __forceinline__ int prefix_level(const uint32_t* __restrict__ code, int i) {
+  const uint32_t x = code[i - 1] ^ code[i];
+  if (x == 0) return 10;
+  return (__clz(x) - 2) / 3;  // 30-bit codes: 2 leading zero bits always
+}
+
+__global__ __launch_bounds__(256) void k_prefix_hist(const uint32_t* __restrict__ code, int n, int* __restrict__ hist) {
+  __shared__ int h[11];
+  if (threadIdx.x < 11) h[threadIdx.x] = 0;
+  __syncthreads();
+  for (int i = 1 + blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) atomicAdd(&h[prefix_level(code, i)], 1);
+  __syncthreads();
+  if (threadIdx.x < 11 && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+}
+
+// the octree level of the leaves -> hist[11]
+__global__ void k_leaf_level(int* __restrict__ hist, int n) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  long long cells = 1;
+  int level = 10;
+  for (int D = 1; D <= 10; ++D) {
+    cells += hist[D - 1];  // boundaries whose common prefix is shorter than D
+    if (static_cast<long long>(n) <= static_cast<long long>(kCellTarget) * cells) {
+      level = D;
+      break;
+    }
+  }
+  hist[11] = level;
+}
+
+// cell_start input: i where a cell of the leaf level starts, else 0
+__global__ __launch_bounds__(256) void k_cell_marks(const uint32_t* __restrict__ code, int n,
+                                                    const int* __restrict__ hist, int* __restrict__ mark) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int level = hist[11];
+  mark[i] = (i == 0 || prefix_level(code, i) < level) ? i : 0;
+}
+
+// a leaf starts at every cell start and every 256 particles into a cell
+__global__ __launch_bounds__(256) void k_cuts(const int* __restrict__ cell_start, int n, int* __restrict__ cut) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  cut[i] = ((i - cell_start[i]) % kLeaf) == 0 ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void k_leaf_first(const int* __restrict__ cut, const int* __restrict__ leaf_id, int n,
+                                                    int leaf_cap, int* __restrict__ leaf_first, int* __restrict__ hist) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int id = leaf_id[i];
+  if (cut[i] && id < leaf_cap) leaf_first[id] = i;
+  if (i == n - 1) {
+    const int nleaf = id + cut[i];
+    hist[12] = nleaf;
+    if (nleaf <= leaf_cap) leaf_first[nleaf] = n;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_gather(const Rec64* __restrict__ y, const int* __restrict__ perm,
                                                 int n, Rec64* __restrict__ out) {
   const int i = blockIdx.x * 256 + threadIdx.x;
@@ -115,11 +180,12 @@ __device__ __forceinline__ void block_sum_d(double (&v)[M], double* red) {
   __syncthreads();
 }
 
-__global__ __launch_bounds__(kLeaf) void k_leaf_moments(const Rec64* __restrict__ ys, int n,
+__global__ __launch_bounds__(kLeaf) void k_leaf_moments(const Rec64* __restrict__ ys,
+                                                        const int* __restrict__ leaf_first,
                                                         NodeRec* __restrict__ nodes) {
   __shared__ double red[4 * 18];
   const int leaf = blockIdx.x;
-  const int first = leaf * kLeaf, last = min(first + kLeaf, n);
+  const int first = leaf_first[leaf], last = leaf_first[leaf + 1];
   const int i = first + static_cast<int>(threadIdx.x);
   const bool on = i < last;
   const Rec64 p = ys[on ? i : first];
@@ -263,6 +329,10 @@ __global__ __launch_bounds__(64) void k_lists(const NodeRec* __restrict__ nodes,
   if (over) atomicOr(overflow, 1);
 }
 
+__global__ void k_fill_zero16(int* __restrict__ p) {
+  if (threadIdx.x < 16) p[threadIdx.x] = 0;
+}
+
 // Near-field work items: parts per leaf, then the items.
 __global__ __launch_bounds__(256) void k_near_parts(const int* __restrict__ near_cnt, int nleaf,
                                                     int* __restrict__ parts) {
@@ -333,10 +403,10 @@ hipError_t build_near_items(const TreeBuffers& tb, int nleaf, hipStream_t st) {
   return hipGetLastError();
 }
 
-TreeShape tree_shape(int n) {
+TreeShape tree_shape(int n, int nleaf) {
   TreeShape s{};
   s.n = n;
-  s.nleaf = (n + kLeaf - 1) / kLeaf;
+  s.nleaf = nleaf;
   int count = s.nleaf, off = 0, L = 0;
   for (;;) {
     s.level_off[L] = off;
@@ -358,20 +428,56 @@ hipError_t tree_sort_temp_bytes(int n, size_t* bytes) {
                                             static_cast<int*>(nullptr), n, 0, 30);
 }
 
-hipError_t build_tree(const Rec64* y, int n, TreeBuffers& tb, const TreeShape& shape, hipStream_t st) {
+hipError_t tree_scan_temp_bytes(int n, size_t* bytes) {
+  size_t a = 0, b = 0;
+  hipError_t e = hipcub::DeviceScan::InclusiveScan(nullptr, a, static_cast<int*>(nullptr), static_cast<int*>(nullptr),
+                                                   hipcub::Max(), n);
+  if (e != hipSuccess) return e;
+  e = hipcub::DeviceScan::ExclusiveSum(nullptr, b, static_cast<int*>(nullptr), static_cast<int*>(nullptr), n);
+  *bytes = a > b ? a : b;
+  return e;
+}
+
+hipError_t build_leaves(const Rec64* y, int n, TreeBuffers& tb, hipStream_t st) {
   const int nb = min(1024, (n + 255) / 256);
+  const int grid = (n + 255) / 256;
   hipLaunchKernelGGL(k_bbox_partial, dim3(nb), dim3(256), 0, st, y, n, tb.bbox_part);
   hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, st, tb.bbox_part, nb, tb.bbox);
-  hipLaunchKernelGGL(k_morton, dim3((n + 255) / 256), dim3(256), 0, st, y, n, tb.bbox, tb.code,
-                     tb.idx);
+  hipLaunchKernelGGL(k_morton, dim3(grid), dim3(256), 0, st, y, n, tb.bbox, tb.code, tb.idx);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   size_t bytes = tb.sort_temp_bytes;
   e = hipcub::DeviceRadixSort::SortPairs(tb.sort_temp, bytes, tb.code, tb.code_sorted, tb.idx,
                                          tb.perm, n, 0, 30, st);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, st, y, tb.perm, n, tb.sorted);
-  hipLaunchKernelGGL(k_leaf_moments, dim3(shape.nleaf), dim3(kLeaf), 0, st, tb.sorted, n, tb.nodes);
+  hipLaunchKernelGGL(k_gather, dim3(grid), dim3(256), 0, st, y, tb.perm, n, tb.sorted);
+  hipLaunchKernelGGL(k_fill_zero16, dim3(1), dim3(64), 0, st, tb.hist);
+  hipLaunchKernelGGL(k_prefix_hist, dim3(nb), dim3(256), 0, st, tb.code_sorted, n, tb.hist);
+  hipLaunchKernelGGL(k_leaf_level, dim3(1), dim3(64), 0, st, tb.hist, n);
+  hipLaunchKernelGGL(k_cell_marks, dim3(grid), dim3(256), 0, st, tb.code_sorted, n, tb.hist, tb.cut);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  bytes = tb.scan_n_bytes;
+  e = hipcub::DeviceScan::InclusiveScan(tb.scan_n_temp, bytes, tb.cut, tb.cell_start, hipcub::Max(), n, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_cuts, dim3(grid), dim3(256), 0, st, tb.cell_start, n, tb.cut);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  bytes = tb.scan_n_bytes;
+  e = hipcub::DeviceScan::ExclusiveSum(tb.scan_n_temp, bytes, tb.cut, tb.leaf_id, n, st);
+  if (e != hipSuccess) return e;
+  return write_leaf_first(n, tb, st);
+}
+
+hipError_t write_leaf_first(int n, TreeBuffers& tb, hipStream_t st) {
+  hipLaunchKernelGGL(k_leaf_first, dim3((n + 255) / 256), dim3(256), 0, st, tb.cut, tb.leaf_id, n, tb.leaf_cap,
+                     tb.leaf_first, tb.hist);
+  return hipGetLastError();
+}
+
+hipError_t build_moments(int n, TreeBuffers& tb, const TreeShape& shape, hipStream_t st) {
+  (void)n;
+  hipLaunchKernelGGL(k_leaf_moments, dim3(shape.nleaf), dim3(kLeaf), 0, st, tb.sorted, tb.leaf_first, tb.nodes);
   for (int L = 1; L < shape.nlevel; ++L)
     hipLaunchKernelGGL(k_parent_moments, dim3((shape.level_count[L] + 63) / 64), dim3(64), 0, st,
                        tb.nodes, shape.level_off[L - 1], shape.level_count[L - 1], shape.level_off[L],

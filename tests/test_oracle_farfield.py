@@ -124,12 +124,37 @@ def test_morton_perm_is_stable_on_ties():
 
 
 def test_node_ranges_cover_levels():
-    first, last, level, child = FF.node_ranges(5000)
-    counts, offs = FF.tree_levels(5000)
+    lf = np.array(list(range(0, 5000, 256)) + [5000])  # 20 leaves
+    first, last, level, child = FF.node_ranges(lf)
+    counts, offs = FF.tree_levels(len(lf) - 1)
     assert counts == [20, 3, 1]
     assert first[offs[1]] == 0 and last[offs[1]] == 2048
     assert first[offs[1] + 2] == 4096 and last[offs[1] + 2] == 5000
     assert last[-1] == 5000 and first[-1] == 0 and level[-1] == 2
+
+
+def test_leaves_follow_octree_cells():
+    """Codes of two far-apart clusters (600 and 300 particles, one cell each
+    at level 1) and 20 singletons: every leaf lies in one cell, cells are cut
+    into runs of 256."""
+    a = np.full(600, 0b001 << 27, dtype=np.uint64) + np.arange(600, dtype=np.uint64)
+    b = np.full(300, 0b110 << 27, dtype=np.uint64) + np.arange(300, dtype=np.uint64)
+    c = (np.arange(20, dtype=np.uint64) + 2) << np.uint64(24)
+    codes = np.sort(np.concatenate([a, b, c]))
+    lf = FF.leaf_first(codes)
+    assert lf[-1] == len(codes) and np.all(np.diff(lf) > 0) and np.all(np.diff(lf) <= 256)
+    # no leaf mixes codes of different level-1 cells
+    for x, y in zip(lf[:-1], lf[1:]):
+        assert len(set((codes[x:y] >> np.uint64(27)).tolist())) == 1
+    # level 1 is chosen (920 particles, 4 occupied cells); cell 1 holds the
+    # 600-cluster and 8 of the singletons: 608 = 256 + 256 + 96; cell 6 holds
+    # the 300-cluster: 256 + 44; cells 0 and 2 hold 6 singletons each
+    np.testing.assert_array_equal(np.diff(lf), [6, 256, 256, 96, 6, 256, 44])
+
+
+def test_prefix_levels():
+    codes = np.array([0, 1, 8, 1 << 27, (1 << 27) + 1, (1 << 27) + 1], dtype=np.uint64)
+    np.testing.assert_array_equal(FF.prefix_levels(codes), [9, 8, 0, 9, 10])
 
 
 def _finite_diff(params, X, Y, idx, h=1e-6):
