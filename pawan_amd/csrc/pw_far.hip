@@ -387,11 +387,14 @@ __device__ __forceinline__ void unrotate_add(const float* y, const float R[9], f
 // node's frame and scaled.  The residual falls off as d^-4, so over a leaf
 // (r_T < theta d / 2) the first-order transfer keeps it to O((r_T/d)^2)
 // relative to itself, while the network runs 256 times less often than per
-// target.  4 waves per leaf; wave w takes the 32-node passes w, w+4, ...;
+// target.  8 waves per leaf (one block per CU fits the LDS, so the waves
+// of one block hide each other's matrix-core latency); wave w takes the
+// 32-node passes w, w+8, ...;
 // node k of a pass is sample column k.  The 12 outputs of a sample come out
 // spread over four lanes; they are exchanged through LDS so that one lane
 // per sample rotates them back.
-constexpr int kMlpThreads = 256;
+constexpr int kMlpThreads = 512;
+constexpr int kMlpWaves = kMlpThreads / 64;
 constexpr int kXch = 24;  // floats exchanged per sample: 12 outputs, frame, 2 scales
 
 __global__ __launch_bounds__(kMlpThreads) void k_far_mlp(FarArgs a, MlpWeights W) {
@@ -399,7 +402,7 @@ __global__ __launch_bounds__(kMlpThreads) void k_far_mlp(FarArgs a, MlpWeights W
   MlpLds& s = *reinterpret_cast<MlpLds*>(smem);
   // per wave: 32 samples x {12 outputs, 9 frame, 2 scales}; then the sums
   float* xch = reinterpret_cast<float*>(smem + sizeof(MlpLds));
-  float* red = xch + 4 * 32 * kXch;
+  float* red = xch + kMlpWaves * 32 * kXch;
   load_mlp(W, s);
   __syncthreads();
   const int T = blockIdx.x;
@@ -412,7 +415,7 @@ __global__ __launch_bounds__(kMlpThreads) void k_far_mlp(FarArgs a, MlpWeights W
   float acc[12];
 #pragma unroll
   for (int i = 0; i < 12; ++i) acc[i] = 0.0f;
-  for (int base = 32 * wave; base < nf; base += 32 * 4) {
+  for (int base = 32 * wave; base < nf; base += 32 * kMlpWaves) {
     v4bf x[2];
 #pragma unroll
     for (int nb = 0; nb < 2; ++nb) {
@@ -464,12 +467,16 @@ __global__ __launch_bounds__(kMlpThreads) void k_far_mlp(FarArgs a, MlpWeights W
 #pragma unroll
     for (int i = 0; i < 12; ++i) red[wave * 16 + i] = acc[i];
   __syncthreads();
-  if (threadIdx.x < 12)
-    red[64 + threadIdx.x] = ((red[threadIdx.x] + red[16 + threadIdx.x]) + red[32 + threadIdx.x]) + red[48 + threadIdx.x];
+  if (threadIdx.x < 12) {
+    float tot = 0.0f;
+    for (int w = 0; w < kMlpWaves; ++w) tot += red[16 * w + threadIdx.x];
+    red[16 * kMlpWaves + threadIdx.x] = tot;
+  }
   __syncthreads();
   const int t = leaf.first + static_cast<int>(threadIdx.x);
   if (t >= leaf.last) return;
-  const float* S = red + 64;  // U (0..2), G (3..11), row-major G[k][m] = S[3 + 3k + m]
+  if (threadIdx.x >= kLeafSize) return;
+  const float* S = red + 16 * kMlpWaves;  // U (0..2), G (3..11), row-major G[k][m] = S[3 + 3k + m]
   const Rec64 me = a.sorted[t];
   const double dx = me.x - leaf.cx, dy = me.y - leaf.cy, dz = me.z - leaf.cz;
   Rate64 r = a.rates[t];
@@ -663,7 +670,7 @@ __global__ __launch_bounds__(256) void k_scatter(const Rate64* __restrict__ in, 
 
 }  // namespace
 
-size_t mlp_far_lds_bytes() { return sizeof(MlpLds) + (4 * 32 * kXch + 80) * sizeof(float); }
+size_t mlp_far_lds_bytes() { return sizeof(MlpLds) + (kMlpWaves * 32 * kXch + 16 * kMlpWaves + 16) * sizeof(float); }
 
 hipError_t launch_far(const Rec64* sorted, const NodeRec* nodes, const int* far_list, const int* far_cnt,
                       int cap_far, int nleaf, const MlpWeights* mlp, Rate64* rates_s, hipStream_t st) {
