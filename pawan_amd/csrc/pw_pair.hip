@@ -220,9 +220,15 @@ struct alignas(16) SrcT {
 
 // Partial sums, with 1/(4 pi) factored out of q and F:
 // q' = 4 pi q, f' = -(4 pi / 3) F  (far field: q' = rho^-3, f' = rho^-5).
+// The stretching of a pair, q' (a_t x a_s) - 3 f' [d.(a_t x a_s)] d, is
+// summed with the target's strength factored out: d.(a_t x a_s) =
+// a_t.(a_s x d) reuses the velocity's cross product v = a_s x d, and
+// sum q' (a_t x a_s) = a_t x sum q' a_s, so a pair costs no cross product
+// of its own; fold() forms w = a_t x A - 3 B once per chunk.
 struct P32 {
-  float ux = 0, uy = 0, uz = 0;  // sum q' (a_s x d)
-  float wx = 0, wy = 0, wz = 0;  // sum q' (a_t x a_s) - 3 f' [d.(a_t x a_s)] d
+  float ux = 0, uy = 0, uz = 0;  // U = sum q' v
+  float ax = 0, ay = 0, az = 0;  // A = sum q' a_s
+  float bx = 0, by = 0, bz = 0;  // B = sum f' (a_t.v) d
 };
 // PSE sums collect only the near pairs (about 1 % of them) and are summed
 // directly.
@@ -289,6 +295,25 @@ __device__ __forceinline__ void near32(float r2, float sig2, float nu2, float& q
   c = nu2 * kKf * e * isg3 * isg2;  // 2 nu Z / sigma^2
 }
 
+// One pair's velocity and stretching into the partial sums (v = a_s x d).
+template <typename S>
+__device__ __forceinline__ void accumulate32(float dx, float dy, float dz, float q, float f, float tax,
+                                             float tay, float taz, const S& s, P32& p) {
+  const float vx = s.ay * dz - s.az * dy;
+  const float vy = s.az * dx - s.ax * dz;
+  const float vz = s.ax * dy - s.ay * dx;
+  p.ux = fmaf(q, vx, p.ux);
+  p.uy = fmaf(q, vy, p.uy);
+  p.uz = fmaf(q, vz, p.uz);
+  p.ax = fmaf(q, s.ax, p.ax);
+  p.ay = fmaf(q, s.ay, p.ay);
+  p.az = fmaf(q, s.az, p.az);
+  const float g = f * fmaf(taz, vz, fmaf(tay, vy, tax * vx));
+  p.bx = fmaf(g, dx, p.bx);
+  p.by = fmaf(g, dy, p.by);
+  p.bz = fmaf(g, dz, p.bz);
+}
+
 template <typename S>
 __device__ __forceinline__ void pair32(const T32& t, const S& s, P32& p, PSE32& e) {
   const float dx = static_cast<float>(t.x - s.x);
@@ -307,17 +332,7 @@ __device__ __forceinline__ void pair32(const T32& t, const S& s, P32& p, PSE32& 
     e.y = fmaf(c, t.vol * s.ay - s.vol * t.ay, e.y);
     e.z = fmaf(c, t.vol * s.az - s.vol * t.az, e.z);
   }
-  p.ux = fmaf(q, s.ay * dz - s.az * dy, p.ux);
-  p.uy = fmaf(q, s.az * dx - s.ax * dz, p.uy);
-  p.uz = fmaf(q, s.ax * dy - s.ay * dx, p.uz);
-  // a_trg x a_src
-  const float cx = t.ay * s.az - t.az * s.ay;
-  const float cy = t.az * s.ax - t.ax * s.az;
-  const float cz = t.ax * s.ay - t.ay * s.ax;
-  const float fd = -3.0f * f * fmaf(dz, cz, fmaf(dy, cy, dx * cx));
-  p.wx = fmaf(q, cx, fmaf(fd, dx, p.wx));
-  p.wy = fmaf(q, cy, fmaf(fd, dy, p.wy));
-  p.wz = fmaf(q, cz, fmaf(fd, dz, p.wz));
+  accumulate32(dx, dy, dz, q, f, t.ax, t.ay, t.az, s, p);
 }
 
 // Far-only pair on positions relative to the target leaf's centre, rounded
@@ -347,17 +362,7 @@ __device__ __forceinline__ void pair32_rel(float tx, float ty, float tz, float t
   const float ir = __builtin_amdgcn_rsqf(r2);
   const float ir2 = ir * ir;
   const float q = ir * ir2;
-  const float f = q * ir2;
-  p.ux = fmaf(q, s.ay * dz - s.az * dy, p.ux);
-  p.uy = fmaf(q, s.az * dx - s.ax * dz, p.uy);
-  p.uz = fmaf(q, s.ax * dy - s.ay * dx, p.uz);
-  const float cx = tay * s.az - taz * s.ay;
-  const float cy = taz * s.ax - tax * s.az;
-  const float cz = tax * s.ay - tay * s.ax;
-  const float fd = -3.0f * f * fmaf(dz, cz, fmaf(dy, cy, dx * cx));
-  p.wx = fmaf(q, cx, fmaf(fd, dx, p.wx));
-  p.wy = fmaf(q, cy, fmaf(fd, dy, p.wy));
-  p.wz = fmaf(q, cz, fmaf(fd, dz, p.wz));
+  accumulate32(dx, dy, dz, q, q * ir2, tax, tay, taz, s, p);
 }
 
 // Knuth's two-sum: (hi, lo) += p with the rounding error of hi + p kept.
@@ -367,6 +372,16 @@ __device__ __forceinline__ void two_sum(float& hi, float& lo, float p) {
   const float err = (hi - (s - bb)) + (p - bb);
   hi = s;
   lo += err;
+}
+
+// A chunk's partial sums into the compensated totals (u, then w).
+__device__ __forceinline__ void fold(const P32& p, float tax, float tay, float taz, float* hi, float* lo) {
+  two_sum(hi[0], lo[0], p.ux);
+  two_sum(hi[1], lo[1], p.uy);
+  two_sum(hi[2], lo[2], p.uz);
+  two_sum(hi[3], lo[3], fmaf(tay, p.az, fmaf(-taz, p.ay, -3.0f * p.bx)));
+  two_sum(hi[4], lo[4], fmaf(taz, p.ax, fmaf(-tax, p.az, -3.0f * p.by)));
+  two_sum(hi[5], lo[5], fmaf(tax, p.ay, fmaf(-tay, p.ax, -3.0f * p.bz)));
 }
 
 // Bounding boxes of a block's targets and of each source tile decide, per
@@ -510,12 +525,7 @@ __device__ __forceinline__ void item_fp32c(const PairArgs& pa, const WorkItem& i
 #pragma unroll 1
         for (int l = c0; l < c1; ++l) pair32(me, tile[l], p, pse);
       }
-      two_sum(hi[0], lo[0], p.ux);
-      two_sum(hi[1], lo[1], p.uy);
-      two_sum(hi[2], lo[2], p.uz);
-      two_sum(hi[3], lo[3], p.wx);
-      two_sum(hi[4], lo[4], p.wy);
-      two_sum(hi[5], lo[5], p.wz);
+      fold(p, me.ax, me.ay, me.az, hi, lo);
     }
   }
   if (active) {
@@ -630,12 +640,7 @@ __device__ __forceinline__ void near_item(const NearArgs& a, const NearItem& it,
 #pragma unroll 2
         for (int l = c0; l < c1; ++l) pair32(me, tile[l], p, pse);
       }
-      two_sum(hi[0], lo[0], p.ux);
-      two_sum(hi[1], lo[1], p.uy);
-      two_sum(hi[2], lo[2], p.uz);
-      two_sum(hi[3], lo[3], p.wx);
-      two_sum(hi[4], lo[4], p.wy);
-      two_sum(hi[5], lo[5], p.wz);
+      fold(p, me.ax, me.ay, me.az, hi, lo);
     }
   }
   if (active) {

@@ -23,6 +23,8 @@ def main() -> int:
     ap.add_argument("--reps", type=int, default=1)
     ap.add_argument("--copy-mb", type=int, default=256)
     ap.add_argument("--theta", type=float, default=0.5)
+    ap.add_argument("--tol", type=float, default=0.0,
+                    help="tree path: absolute far-field error tolerance (0: opening angle only)")
     ap.add_argument("--config", default="",
                     help="pair-kernel launch shape 'parts,persistent,waves8' (e.g. 8,1,1); default: untuned")
     args = ap.parse_args()
@@ -43,7 +45,7 @@ def main() -> int:
         system = workloads.WORKLOADS[args.workload]["make"]()
     with capi.Solver(system) as g:
         if args.precision == "learned":
-            g.ff_set_params(args.theta, 0.0, use_mlp=False)
+            g.ff_set_params(args.theta, args.tol, use_mlp=False)
         elif args.config:
             parts, pers, w8 = (int(v) for v in args.config.split(","))
             g.set_pair_config(args.precision, parts, bool(pers), bool(w8))
