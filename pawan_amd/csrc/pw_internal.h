@@ -68,7 +68,7 @@ hipError_t launch_pair(int kind, const PairConfig& cfg, const PairArgs& args, in
 // resident blocks per CU of the persistent pair kernel of `kind` (equal to
 // its waves per SIMD: 256-thread blocks), from the occupancy calculator
 int pair_kernel_waves(int kind, const PairConfig& cfg);
-// the same for the tree path's near-field kernel
+// resident one-wave blocks per CU of the tree path's near-field kernel
 int near_kernel_blocks_per_cu();
 
 // Stage arithmetic of the steppers (FP64, no contraction, the reference's

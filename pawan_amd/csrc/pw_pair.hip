@@ -257,13 +257,26 @@ constexpr float kQf = 0x1.45f306p-4f;        // 1/(4 pi)
 // for s < 4 (the incomplete-gamma series of erf, A&S 6.5.29: every term
 // positive, so no cancellation as rho -> 0), and the closed erf form for
 // 4 <= s <= 44 where its cancellation is mild.
+// e = exp(-s/2) as 2^(-s log2(e) / 2) on the hardware exp2: the rounding of
+// the argument (|arg| <= 32) costs at most 2^-24 |arg| ln 2 relative, and
+// at large s the Gaussian terms are a vanishing part of q.
+constexpr float kMHalfLog2e = -0x1.715476p-1f;  // -log2(e) / 2
+// erfc(x) = exp(-x^2) R(w) for x = rho / (sqrt(2) sigma), w = sigma / rho:
+// R = exp(x^2) erfc(x) (the scaled complementary error function) as a
+// degree-6 polynomial in w on 4 <= rho^2/sigma^2 <= 44, a Chebyshev least-
+// squares fit (fit_erfcx.py beside this file); largest relative error of
+// the FP32 Horner evaluation 3.8e-7, i.e. below 2e-8 absolute in erf.  The
+// exponential is the e above, since x^2 = s/2.
+constexpr float kErfcx[7] = {0x1.c20ea2p-13f, 0x1.9559b4p-1f, 0x1.354a9p-4f, -0x1.4d4cdcp+0f,
+                             0x1.efececp+0f, -0x1.65f268p+0f, 0x1.b22f5p-2f};
+
 __device__ __forceinline__ void near32(float r2, float sig2, float nu2, float& qp,
                                        float& fp, float& c) {
   const float isg = __builtin_amdgcn_rsqf(sig2);
   const float isg2 = isg * isg;
   const float isg3 = isg2 * isg;
   const float sb = r2 * isg2;
-  const float e = expf(-0.5f * sb);
+  const float e = __builtin_amdgcn_exp2f(kMHalfLog2e * sb);
   if (sb < kSeries32) {
     float B = 0x1.519298p-48f;                  // 1/27!!
     B = fmaf(B, sb, 0x1.1cd3b0p-43f);           // 1/25!!
@@ -283,10 +296,12 @@ __device__ __forceinline__ void near32(float r2, float sig2, float nu2, float& q
     fp = ke * B * isg2;  // -(4 pi/3) F = 4 pi K e B / sigma^5
   } else {
     const float ir = __builtin_amdgcn_rsqf(r2);
-    const float rb = r2 * ir * isg;     // rho / sigma
-    const float irb = sig2 * isg * ir;  // sigma / rho
+    const float irb = sig2 * isg * ir;  // w = sigma / rho
     const float Z = kKf * e * isg3;
-    const float phi = kQf * erff(0.70710678f * rb) * isg3;
+    float R = kErfcx[6];
+#pragma unroll
+    for (int k = 5; k >= 0; --k) R = fmaf(R, irb, kErfcx[k]);
+    const float phi = kQf * fmaf(-e, R, 1.0f) * isg3;  // erf(x) / (4 pi sigma^3)
     const float q = (phi * irb - Z) * (irb * irb);
     const float F = (Z - 3.0f * q) * (ir * ir);
     qp = kFourPi * q;
@@ -545,11 +560,21 @@ __device__ __forceinline__ void item_fp32c(const PairArgs& pa, const WorkItem& i
 
 // ---------------------------------------------------------------------
 // Near field of the tree path.  A work item is (target leaf T, a range of
-// T's near list): up to 256 targets, one per lane, against the listed
-// leaves, one LDS tile each, with the pair math and compensation of
-// item_fp32c.  Items are pulled from an atomic queue by resident blocks;
-// a leaf whose list is split writes per-item partial sums that
-// k_near_reduce adds in item order (deterministic).
+// T's near list); its targets are cut into 64-lane slices, and one slice of
+// one item is what a wave pulls from the atomic queue (queue index =
+// 4 item + slice; slices past the leaf's end are skipped).  Each wave is a
+// block of its own: it stages the listed leaves' sources 64 at a time into
+// its own LDS tile and waits on no other wave, so leaves of any size keep
+// every resident wave busy.  A leaf whose list is split writes per-item
+// partial sums that k_near_reduce adds in item order (deterministic).
+//
+// Positions are FP32 relative to T's centre for every listed leaf.  The
+// rounding is 2^-24 of the distance to the centre, which for near pairs is
+// a few leaf radii: the displacement of the closest pairs carries a relative
+// error of about 2^-24 (leaf radius / spacing), some 1e-6 on these wakes,
+// far below the error of the far field the near field is paired with.
+// Listed leaves whose every pair is beyond 44 sigma^2 skip the Gaussian
+// core test.
 // ---------------------------------------------------------------------
 struct NearArgs {
   const Rec64* sorted;
@@ -558,19 +583,47 @@ struct NearArgs {
   int cap;
   const NearItem* items;
   const int* total;  // items in use (device)
-  int* queue;
+  int* queue;        // slices handed out
   float nu;
   float gap2;  // 44 sigma_max^2: leaves further apart take the far-only loop
   Rate64* out;
   Rate64* partial;  // item x 256
 };
 
-__device__ __forceinline__ void near_item(const NearArgs& a, const NearItem& it, SrcT* tile) {
+constexpr int kNearSlice = 64;                       // targets per wave
+static_assert(kTB == kLeafSize, "partial-sum rows hold one leaf");
+constexpr int kNearSlices = kLeafSize / kNearSlice;  // slices per leaf at most
+
+// pair on relative FP32 positions, Gaussian core included
+__device__ __forceinline__ void pair32_rel_core(float tx, float ty, float tz, const T32& t, const SrcRel& s,
+                                                P32& p, PSE32& e) {
+  const float dx = tx - s.rx;
+  const float dy = ty - s.ry;
+  const float dz = tz - s.rz;
+  const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+  const float sig2 = s.hs2 + t.hs2;
+  const float ir = __builtin_amdgcn_rsqf(r2);
+  const float ir2 = ir * ir;
+  float q = ir * ir2;
+  float f = q * ir2;
+  if (r2 <= kFar32 * sig2) {
+    float c;
+    near32(r2, sig2, t.nu2, q, f, c);
+    e.x = fmaf(c, t.vol * s.ax - s.vol * t.ax, e.x);
+    e.y = fmaf(c, t.vol * s.ay - s.vol * t.ay, e.y);
+    e.z = fmaf(c, t.vol * s.az - s.vol * t.az, e.z);
+  }
+  accumulate32(dx, dy, dz, q, f, t.ax, t.ay, t.az, s, p);
+}
+
+__device__ __forceinline__ void near_slice(const NearArgs& a, const NearItem& it, int slice, SrcRel* tile) {
   const int T = it.T;
+  const int lane = static_cast<int>(threadIdx.x);
   const int t0 = a.nodes[T].first, t1 = a.nodes[T].last;
   const double tcx = a.nodes[T].cx, tcy = a.nodes[T].cy, tcz = a.nodes[T].cz;
   const float tr = a.nodes[T].r;
-  const int t = t0 + static_cast<int>(threadIdx.x);
+  const int self = kNearSlice * slice + lane;  // index of this lane's target in T
+  const int t = t0 + self;
   const bool active = t < t1;
   const Rec64 m = a.sorted[active ? t : t0];
   T32 me;
@@ -583,7 +636,6 @@ __device__ __forceinline__ void near_item(const NearArgs& a, const NearItem& it,
   me.nu2 = 2.0f * a.nu;
   const float rx = static_cast<float>(m.x - tcx), ry = static_cast<float>(m.y - tcy),
               rz = static_cast<float>(m.z - tcz);
-  SrcRel* rtile = reinterpret_cast<SrcRel*>(tile);
   float hi[kAcc], lo[kAcc];
 #pragma unroll
   for (int k = 0; k < kAcc; ++k) hi[k] = lo[k] = 0.0f;
@@ -593,14 +645,16 @@ __device__ __forceinline__ void near_item(const NearArgs& a, const NearItem& it,
     const int id = L[k];
     const int base = a.nodes[id].first;
     const int cnt = a.nodes[id].last - base;
-    // every pair of the two leaves beyond the Gaussian core: no near test
+    // every pair of the two leaves beyond the Gaussian core: no core test
     const double ddx = a.nodes[id].cx - tcx, ddy = a.nodes[id].cy - tcy, ddz = a.nodes[id].cz - tcz;
     const float gap = static_cast<float>(sqrt(ddx * ddx + ddy * ddy + ddz * ddz)) - tr - a.nodes[id].r;
     const bool all_far = gap > 0.0f && gap * gap > a.gap2 * 1.0001f;
-    __syncthreads();
-    if (static_cast<int>(threadIdx.x) < cnt) {
-      const Rec64 r = a.sorted[base + threadIdx.x];
-      if (all_far) {
+    const bool diag = id == T;
+    for (int c0 = 0; c0 < cnt; c0 += kNearSlice) {
+      const int nc = min(kNearSlice, cnt - c0);
+      __syncthreads();  // the previous tile is read
+      if (lane < nc) {
+        const Rec64 r = a.sorted[base + c0 + lane];
         SrcRel f;
         f.rx = static_cast<float>(r.x - tcx);
         f.ry = static_cast<float>(r.y - tcy);
@@ -610,35 +664,21 @@ __device__ __forceinline__ void near_item(const NearArgs& a, const NearItem& it,
         f.ay = static_cast<float>(r.ay);
         f.az = static_cast<float>(r.az);
         f.vol = static_cast<float>(r.vol);
-        rtile[threadIdx.x] = f;
-      } else {
-        SrcT f;
-        f.x = r.x; f.y = r.y; f.z = r.z;
-        f.hs2 = static_cast<float>(0.5 * r.s2);
-        f.ax = static_cast<float>(r.ax);
-        f.ay = static_cast<float>(r.ay);
-        f.az = static_cast<float>(r.az);
-        f.vol = static_cast<float>(r.vol);
-        f.pad = 0.0f;
-        tile[threadIdx.x] = f;
+        tile[lane] = f;
       }
-    }
-    __syncthreads();
-    if (!active) continue;
-    const bool diag = id == T;
-    const int self = static_cast<int>(threadIdx.x);
-    for (int c0 = 0; c0 < cnt; c0 += kChunk) {
-      const int c1 = min(cnt, c0 + kChunk);
+      __syncthreads();
+      if (!active) continue;
       P32 p;
       if (all_far) {
 #pragma unroll 2
-        for (int l = c0; l < c1; ++l) pair32_rel(rx, ry, rz, me.ax, me.ay, me.az, rtile[l], p);
+        for (int l = 0; l < nc; ++l) pair32_rel(rx, ry, rz, me.ax, me.ay, me.az, tile[l], p);
       } else if (diag) {
-        for (int l = c0; l < c1; ++l)
-          if (l != self) pair32(me, tile[l], p, pse);
+        const int own = self - c0;  // this target's slot in the tile, if there
+        for (int l = 0; l < nc; ++l)
+          if (l != own) pair32_rel_core(rx, ry, rz, me, tile[l], p, pse);
       } else {
-#pragma unroll 2
-        for (int l = c0; l < c1; ++l) pair32(me, tile[l], p, pse);
+#pragma unroll 1
+        for (int l = 0; l < nc; ++l) pair32_rel_core(rx, ry, rz, me, tile[l], p, pse);
       }
       fold(p, me.ax, me.ay, me.az, hi, lo);
     }
@@ -655,21 +695,22 @@ __device__ __forceinline__ void near_item(const NearArgs& a, const NearItem& it,
     r.wy = kQ * acc[4] + static_cast<double>(pse.y);
     r.wz = kQ * acc[5] + static_cast<double>(pse.z);
     if (it.slot < 0) a.out[t] = r;
-    else a.partial[static_cast<size_t>(it.slot) * kTB + threadIdx.x] = r;
+    else a.partial[static_cast<size_t>(it.slot) * kLeafSize + self] = r;
   }
 }
 
-__global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_near_fp32c(NearArgs a) {
-  __shared__ SrcT tile[kTB];
-  __shared__ int s_idx;
-  const int total = *a.total;
+__global__ __launch_bounds__(kNearSlice) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_near_fp32c(NearArgs a) {
+  __shared__ SrcRel tile[kNearSlice];
+  const int slices = kNearSlices * *a.total;
   for (;;) {
-    __syncthreads();
-    if (threadIdx.x == 0) s_idx = atomicAdd(a.queue, 1);
-    __syncthreads();
-    const int idx = s_idx;
-    if (idx >= total) break;  // uniform: every lane reads the same s_idx
-    near_item(a, a.items[idx], tile);
+    int idx = 0;
+    if (threadIdx.x == 0) idx = atomicAdd(a.queue, 1);
+    idx = __builtin_amdgcn_readfirstlane(idx);  // lane 0's: every lane is active here
+    if (idx >= slices) break;
+    const NearItem it = a.items[idx / kNearSlices];
+    const int slice = idx % kNearSlices;
+    if (a.nodes[it.T].first + kNearSlice * slice >= a.nodes[it.T].last) continue;
+    near_slice(a, it, slice, tile);
   }
 }
 
@@ -812,7 +853,7 @@ hipError_t launch_near_fp32c(const Rec64* sorted, const NodeRec* nodes, const Tr
   if (e != hipSuccess) return e;
   NearArgs a{sorted, nodes, tb.near_list, tb.cap_near, tb.near_items, tb.near_total, tb.near_queue,
              static_cast<float>(nu), kFar32 * sigma_max2, rates_s, tb.near_partial};
-  hipLaunchKernelGGL(k_near_fp32c, dim3(grid), dim3(kTB), 0, st, a);
+  hipLaunchKernelGGL(k_near_fp32c, dim3(grid), dim3(kNearSlice), 0, st, a);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_near_reduce, dim3(nleaf), dim3(kTB), 0, st, nodes, tb.near_parts, tb.near_offs,
@@ -821,14 +862,15 @@ hipError_t launch_near_fp32c(const Rec64* sorted, const NodeRec* nodes, const Tr
 }
 
 namespace {
-// resident 256-thread blocks per CU of a kernel, from the runtime's
+// resident blocks per CU of a kernel (256 threads unless given), from the runtime's
 // occupancy calculator (registers, LDS), cached per kernel
-int blocks_per_cu(const void* kernel) {
+int blocks_per_cu(const void* kernel, int threads = kTB) {
   static std::map<const void*, int> cache;
   auto it = cache.find(kernel);
   if (it != cache.end()) return it->second;
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, kTB, 0) != hipSuccess || nb < 1) nb = 4;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, threads, 0) != hipSuccess || nb < 1)
+    nb = kTB / threads * 4;
   cache[kernel] = nb;
   return nb;
 }
@@ -846,6 +888,6 @@ int pair_kernel_waves(int kind, const PairConfig& cfg) {
   return blocks_per_cu(k);
 }
 
-int near_kernel_blocks_per_cu() { return blocks_per_cu(reinterpret_cast<const void*>(k_near_fp32c)); }
+int near_kernel_blocks_per_cu() { return blocks_per_cu(reinterpret_cast<const void*>(k_near_fp32c), kNearSlice); }
 
 }  // namespace pw
