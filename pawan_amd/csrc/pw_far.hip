@@ -98,37 +98,67 @@ __device__ __forceinline__ void md_eval(const float r[3], const float M[3], cons
 }
 
 // Node data staged in LDS for the far kernels.
-struct FarNodeL {
-  double cx, cy, cz;
-  float M[3], D[9];
-  float ell, s0;
-  float feat[16];  // slots 3..15 of the MLP input (node part)
+// ---------------------------------------------------------------------
+// Analytic far field: one thread per target of a leaf, far nodes staged
+// through LDS in chunks, positions FP32 relative to the leaf's centre (the
+// rounding is 2^-24 of a far node's distance, 1e-7 of the offset).  The
+// stretching is contracted with the target's strength node by node,
+// dalpha_m = sum_k a_k G[k][m], which for the monopole + dipole field is
+//   a x c + (3/r^5) D^T (r x a) + (a.v) r,  c = M/r^3 + 3 (D r)/r^5,
+//   v = (3/r^5)(W - M x r) - (15/r^7)((D r) x r),
+// W = (D21 - D12, D02 - D20, D10 - D01): the rows of G are never formed.
+// ---------------------------------------------------------------------
+constexpr int kNodeChunk = 64;
+
+struct FarNodeMD {
+  float rc[3];       // centre relative to the target leaf's centre
+  float M[3], W[3];  // monopole, dipole's antisymmetric part
+  float D[9];
 };
 
-__device__ __forceinline__ void stage_node(const NodeRec& q, FarNodeL& f) {
-  f.cx = q.cx; f.cy = q.cy; f.cz = q.cz;
+__device__ __forceinline__ void stage_md(const NodeRec& q, double tcx, double tcy, double tcz, FarNodeMD& f) {
+  f.rc[0] = static_cast<float>(q.cx - tcx);
+  f.rc[1] = static_cast<float>(q.cy - tcy);
+  f.rc[2] = static_cast<float>(q.cz - tcz);
 #pragma unroll
   for (int k = 0; k < 3; ++k) f.M[k] = q.M[k];
 #pragma unroll
   for (int k = 0; k < 9; ++k) f.D[k] = q.D[k];
-  const float ell = q.r + q.sg;
-  const float is0 = q.s0 > 0.0f ? 1.0f / q.s0 : 0.0f;
-  f.ell = ell;
-  f.s0 = q.s0;
-  f.feat[0] = f.feat[1] = f.feat[2] = 0.0f;
-#pragma unroll
-  for (int k = 0; k < 3; ++k) f.feat[3 + k] = q.M[k] * is0;
-  const float isl = is0 / ell;
-#pragma unroll
-  for (int k = 0; k < 9; ++k) f.feat[6 + k] = q.D[k] * isl;
-  f.feat[15] = q.sg / ell;
+  f.W[0] = q.D[5] - q.D[7];
+  f.W[1] = q.D[6] - q.D[2];
+  f.W[2] = q.D[1] - q.D[3];
 }
 
-// ---------------------------------------------------------------------
-// Analytic far field: one thread per target of a leaf, far nodes staged
-// through LDS in chunks.
-// ---------------------------------------------------------------------
-constexpr int kNodeChunk = 64;
+// u += field of node n at offset r, w += its stretching of strength a
+// (4 pi factored; the same terms as md_eval)
+__device__ __forceinline__ void md_accumulate(const float r[3], const FarNodeMD& n, const float a[3], float u[3],
+                                              float w[3]) {
+  const float r2 = fmaf(r[2], r[2], fmaf(r[1], r[1], r[0] * r[0]));
+  const float ir = __builtin_amdgcn_rsqf(r2);
+  const float ir2 = ir * ir;
+  const float a3 = ir * ir2;
+  const float b5 = 3.0f * a3 * ir2;
+  const float c7 = 5.0f * b5 * ir2;
+  const float* M = n.M;
+  const float* D = n.D;
+  const float mr[3] = {M[1] * r[2] - M[2] * r[1], M[2] * r[0] - M[0] * r[2], M[0] * r[1] - M[1] * r[0]};
+  const float p[3] = {fmaf(D[2], r[2], fmaf(D[1], r[1], D[0] * r[0])),
+                      fmaf(D[5], r[2], fmaf(D[4], r[1], D[3] * r[0])),
+                      fmaf(D[8], r[2], fmaf(D[7], r[1], D[6] * r[0]))};
+  const float pr[3] = {p[1] * r[2] - p[2] * r[1], p[2] * r[0] - p[0] * r[2], p[0] * r[1] - p[1] * r[0]};
+  float v[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    u[k] = fmaf(mr[k] - n.W[k], a3, fmaf(pr[k], b5, u[k]));
+    v[k] = fmaf(b5, n.W[k] - mr[k], -c7 * pr[k]);
+  }
+  const float c[3] = {fmaf(M[0], a3, p[0] * b5), fmaf(M[1], a3, p[1] * b5), fmaf(M[2], a3, p[2] * b5)};
+  const float sx = r[1] * a[2] - r[2] * a[1], sy = r[2] * a[0] - r[0] * a[2], sz = r[0] * a[1] - r[1] * a[0];
+  const float av = fmaf(a[2], v[2], fmaf(a[1], v[1], a[0] * v[0]));
+  w[0] += fmaf(av, r[0], fmaf(a[1], c[2], fmaf(-a[2], c[1], b5 * fmaf(D[6], sz, fmaf(D[3], sy, D[0] * sx)))));
+  w[1] += fmaf(av, r[1], fmaf(a[2], c[0], fmaf(-a[0], c[2], b5 * fmaf(D[7], sz, fmaf(D[4], sy, D[1] * sx)))));
+  w[2] += fmaf(av, r[2], fmaf(a[0], c[1], fmaf(-a[1], c[0], b5 * fmaf(D[8], sz, fmaf(D[5], sy, D[2] * sx)))));
+}
 
 struct FarArgs {
   const Rec64* sorted;
@@ -140,41 +170,38 @@ struct FarArgs {
 };
 
 __global__ __launch_bounds__(256) void k_far_md(FarArgs a) {
-  __shared__ FarNodeL sn[kNodeChunk];
+  __shared__ FarNodeMD sn[kNodeChunk];
   const int T = blockIdx.x;
   const int t0 = a.nodes[T].first, t1 = a.nodes[T].last;
+  const double tcx = a.nodes[T].cx, tcy = a.nodes[T].cy, tcz = a.nodes[T].cz;
   const int t = t0 + static_cast<int>(threadIdx.x);
   const bool active = t < t1;
   const Rec64 me = a.sorted[active ? t : t0];
-  float u[3] = {0, 0, 0}, G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const float rt[3] = {static_cast<float>(me.x - tcx), static_cast<float>(me.y - tcy),
+                       static_cast<float>(me.z - tcz)};
+  const float at[3] = {static_cast<float>(me.ax), static_cast<float>(me.ay), static_cast<float>(me.az)};
+  float u[3] = {0, 0, 0}, w[3] = {0, 0, 0};
   const int nf = a.cnt[T];
   const int* __restrict__ L = a.list + static_cast<size_t>(T) * a.cap;
   for (int base = 0; base < nf; base += kNodeChunk) {
     const int c = min(kNodeChunk, nf - base);
     __syncthreads();
-    if (static_cast<int>(threadIdx.x) < c) stage_node(a.nodes[L[base + threadIdx.x]], sn[threadIdx.x]);
+    if (static_cast<int>(threadIdx.x) < c) stage_md(a.nodes[L[base + threadIdx.x]], tcx, tcy, tcz, sn[threadIdx.x]);
     __syncthreads();
     if (!active) continue;
     for (int k = 0; k < c; ++k) {
-      const float r[3] = {static_cast<float>(me.x - sn[k].cx), static_cast<float>(me.y - sn[k].cy),
-                          static_cast<float>(me.z - sn[k].cz)};
-      float uu[3], gg[9];
-      md_eval(r, sn[k].M, sn[k].D, uu, gg);
-#pragma unroll
-      for (int m = 0; m < 3; ++m) u[m] += uu[m];
-#pragma unroll
-      for (int m = 0; m < 9; ++m) G[m] += gg[m];
+      const float r[3] = {rt[0] - sn[k].rc[0], rt[1] - sn[k].rc[1], rt[2] - sn[k].rc[2]};
+      md_accumulate(r, sn[k], at, u, w);
     }
   }
   if (!active) return;
-  const float ax = static_cast<float>(me.ax), ay = static_cast<float>(me.ay), az = static_cast<float>(me.az);
   Rate64 r = a.rates[t];
   r.ux += kQd * u[0];
   r.uy += kQd * u[1];
   r.uz += kQd * u[2];
-  r.wx += kQd * (ax * G[0] + ay * G[3] + az * G[6]);
-  r.wy += kQd * (ax * G[1] + ay * G[4] + az * G[7]);
-  r.wz += kQd * (ax * G[2] + ay * G[5] + az * G[8]);
+  r.wx += kQd * w[0];
+  r.wy += kQd * w[1];
+  r.wz += kQd * w[2];
   a.rates[t] = r;
 }
 

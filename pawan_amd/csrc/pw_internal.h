@@ -120,7 +120,6 @@ constexpr int kLeafSize = 256;  // particles per leaf at most (= one LDS tile)
 constexpr int kCellTarget = 320;
 constexpr int kFan = 8;         // children per node
 constexpr int kMaxLevels = 12;  // 256 * 8^11 particles
-constexpr int kStack = 96;      // traversal stack (7 per level + 1)
 
 // One node: particle range [first, last) of the Morton-sorted state, centre
 // (particle mean), bounding radius r about it, and the vorticity moments

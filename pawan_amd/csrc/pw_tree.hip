@@ -282,51 +282,106 @@ __global__ __launch_bounds__(64) void k_parent_moments(NodeRec* __restrict__ nod
   nodes[off + j] = p;
 }
 
-// ---- interaction lists: one thread per target leaf -----------------------
-// Depth-first walk from the root; children are pushed last-first so the
-// lists come out in Morton order.
+// ---- interaction lists: one wave per target leaf --------------------------
+// Breadth-first from the root, one level per pass: the wave tests 64
+// candidates at a time (the children of the nodes opened on the level
+// above, taken in order) and appends far nodes, near leaves and opened
+// nodes in lane order through ballots, so the lists are deterministic: far
+// nodes coarse level first, each level in Morton order, and the near leaves
+// (all on the last level) in Morton order.  Opened nodes of a level wait in
+// LDS (kOpen per level; more set the overflow flag).
+constexpr int kOpen = 2048;
+
+__device__ __forceinline__ bool is_far(const NodeRec& q, double tcx, double tcy, double tcz, float tr,
+                                       const ListParams& lp) {
+  const double ddx = q.cx - tcx, ddy = q.cy - tcy, ddz = q.cz - tcz;
+  const float d = static_cast<float>(sqrt(ddx * ddx + ddy * ddy + ddz * ddz));
+  const float gap = d - q.r - tr;
+  bool far = (q.r + tr) < lp.theta * d && gap > 0.0f && gap * gap > kFar32List * lp.sigma_max2;
+  if (far && lp.tol > 0.0f) {
+    const float dm = d - tr;
+    far = lp.err_coef * q.s2 <= lp.tol * (dm * dm) * (dm * dm);
+  }
+  return far;
+}
+
 __global__ __launch_bounds__(64) void k_lists(const NodeRec* __restrict__ nodes, TreeShape shape,
                                               ListParams lp, int* __restrict__ near_list,
                                               int* __restrict__ near_cnt, int cap_near,
                                               int* __restrict__ far_list, int* __restrict__ far_cnt,
                                               int cap_far, int* __restrict__ overflow) {
-  const int T = blockIdx.x * 64 + threadIdx.x;
-  if (T >= shape.nleaf) return;
-  const NodeRec t = nodes[T];
-  int stack[kStack];
-  int sp = 0;
-  stack[sp++] = shape.root;
+  __shared__ int open[2][kOpen];
+  const int T = blockIdx.x;
+  const int lane = static_cast<int>(threadIdx.x);
+  const double tcx = nodes[T].cx, tcy = nodes[T].cy, tcz = nodes[T].cz;
+  const float tr = nodes[T].r;
+  int* __restrict__ NL = near_list + static_cast<size_t>(T) * cap_near;
+  int* __restrict__ FL = far_list + static_cast<size_t>(T) * cap_far;
+  const unsigned long long below = (1ull << lane) - 1ull;
   int nn = 0, nf = 0;
   bool over = false;
-  while (sp > 0) {
-    const int id = stack[--sp];
-    const NodeRec& q = nodes[id];
-    const double ddx = q.cx - t.cx, ddy = q.cy - t.cy, ddz = q.cz - t.cz;
-    const float d = static_cast<float>(sqrt(ddx * ddx + ddy * ddy + ddz * ddz));
-    const float gap = d - q.r - t.r;
-    bool far = (q.r + t.r) < lp.theta * d && gap > 0.0f && gap * gap > kFar32List * lp.sigma_max2;
-    if (far && lp.tol > 0.0f) {
-      const float dm = d - t.r;
-      far = lp.err_coef * q.s2 <= lp.tol * (dm * dm) * (dm * dm);
-    }
-    if (far) {
-      if (nf < cap_far) far_list[static_cast<size_t>(T) * cap_far + nf] = id;
-      else over = true;
-      ++nf;
-    } else if (q.level == 0) {
-      if (nn < cap_near) near_list[static_cast<size_t>(T) * cap_near + nn] = id;
-      else over = true;
-      ++nn;
+  // level 0 of the walk: the root alone, tested as the only candidate
+  int np = 0, cur = 0;
+  {
+    const int id = shape.root;
+    if (is_far(nodes[id], tcx, tcy, tcz, tr, lp)) {
+      if (lane == 0) FL[0] = id;  // (cap_far >= 1)
+      nf = 1;
+    } else if (nodes[id].level == 0) {
+      if (lane == 0) NL[0] = id;
+      nn = 1;
     } else {
-      for (int c = q.child + q.pad[0] - 1; c >= q.child; --c) {
-        if (sp < kStack) stack[sp++] = c;
-        else over = true;
-      }
+      if (lane == 0) open[0][0] = id;
+      np = 1;
     }
   }
-  near_cnt[T] = min(nn, cap_near);
-  far_cnt[T] = min(nf, cap_far);
-  if (over) atomicOr(overflow, 1);
+  while (np > 0) {
+    __syncthreads();  // the opened nodes of the level above are in open[cur]
+    int nnext = 0;
+    for (int base = 0; base < kFan * np; base += 64) {
+      const int s = base + lane, j = s / kFan, k = s % kFan;
+      int id = -1;
+      if (j < np) {
+        const NodeRec& P = nodes[open[cur][j]];
+        if (k < P.pad[0]) id = P.child + k;
+      }
+      bool far = false, near = false, opn = false;
+      if (id >= 0) {
+        const NodeRec& q = nodes[id];
+        far = is_far(q, tcx, tcy, tcz, tr, lp);
+        near = !far && q.level == 0;
+        opn = !far && !near;
+      }
+      const unsigned long long mf = __ballot(far), mn = __ballot(near), mo = __ballot(opn);
+      if (far) {
+        const int p = nf + __popcll(mf & below);
+        if (p < cap_far) FL[p] = id;
+      }
+      if (near) {
+        const int p = nn + __popcll(mn & below);
+        if (p < cap_near) NL[p] = id;
+      }
+      if (opn) {
+        const int p = nnext + __popcll(mo & below);
+        if (p < kOpen) open[cur ^ 1][p] = id;
+      }
+      nf += __popcll(mf);
+      nn += __popcll(mn);
+      nnext += __popcll(mo);
+    }
+    if (nnext > kOpen) {
+      over = true;
+      nnext = kOpen;
+    }
+    cur ^= 1;
+    np = nnext;
+  }
+  if (nn > cap_near || nf > cap_far) over = true;
+  if (lane == 0) {
+    near_cnt[T] = min(nn, cap_near);
+    far_cnt[T] = min(nf, cap_far);
+    if (over) atomicOr(overflow, 1);
+  }
 }
 
 __global__ void k_fill_zero16(int* __restrict__ p) {
@@ -487,7 +542,8 @@ hipError_t build_moments(int n, TreeBuffers& tb, const TreeShape& shape, hipStre
 
 hipError_t build_lists(const TreeBuffers& tb, const TreeShape& shape, const ListParams& lp,
                        hipStream_t st) {
-  hipLaunchKernelGGL(k_lists, dim3((shape.nleaf + 63) / 64), dim3(64), 0, st, tb.nodes, shape, lp,
+  if (shape.nleaf <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_lists, dim3(shape.nleaf), dim3(64), 0, st, tb.nodes, shape, lp,
                      tb.near_list, tb.near_cnt, tb.cap_near, tb.far_list, tb.far_cnt, tb.cap_far,
                      tb.overflow);
   return hipGetLastError();
