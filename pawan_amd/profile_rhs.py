@@ -54,6 +54,8 @@ def main() -> int:
             g.solve(args.precision)
             print(f"[profile_rhs] {system.name} N={system.n} {args.precision} rhs {k + 1}/{args.reps}: "
                   f"{time.perf_counter() - t:.3f} s", flush=True)
+        if args.precision == "learned":
+            print(f"[profile_rhs] tree: {g.ff_info()}", flush=True)
     return 0
 
 
