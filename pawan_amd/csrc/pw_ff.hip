@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <string>
@@ -60,7 +61,11 @@ int ensure_leaf_capacity(pw_ctx* c, int nleaf) {
   const pw::TreeShape sh = pw::tree_shape(n, cap);
   if (sh.nlevel >= pw::kMaxLevels) return fail(PW_ERR_UNSUPPORTED, "too many leaves for the tree");
   const size_t nl = static_cast<size_t>(cap);
-  const int cap_near = std::min(cap, 8192), cap_far = std::min(sh.nnodes, 8192);
+  // initial list capacities (PAWAN_FF_LIST_CAP, default 8192; lists grow
+  // past them when a tree needs it)
+  int cap0 = 8192;
+  if (const char* e = std::getenv("PAWAN_FF_LIST_CAP")) cap0 = std::max(1, std::atoi(e));
+  const int cap_near = std::min(cap, cap0), cap_far = std::min(sh.nnodes, cap0);
   if (dalloc(&tb.leaf_first, nl + 1) != hipSuccess || dalloc(&tb.nodes, sh.nnodes) != hipSuccess ||
       dalloc(&tb.near_list, nl * cap_near) != hipSuccess || dalloc(&tb.near_cnt, nl) != hipSuccess ||
       dalloc(&tb.far_list, nl * cap_far) != hipSuccess || dalloc(&tb.far_cnt, nl) != hipSuccess ||
@@ -75,6 +80,29 @@ int ensure_leaf_capacity(pw_ctx* c, int nleaf) {
   }
   tb.leaf_cap = cap;
   tb.node_cap = sh.nnodes;
+  tb.cap_near = cap_near;
+  tb.cap_far = cap_far;
+  return PW_OK;
+}
+
+// near and far lists that hold at least need_near and need_far entries per
+// leaf slot (grown by a quarter beyond; at most every leaf or node)
+int ensure_list_capacity(pw_ctx* c, int need_near, int need_far) {
+  auto& tb = c->ff.tb;
+  if (need_near <= tb.cap_near && need_far <= tb.cap_far) return PW_OK;
+  PW_HIP(hipStreamSynchronize(c->stream));
+  const int cap_near = std::max(tb.cap_near, std::min(tb.leaf_cap, need_near + need_near / 4 + 64));
+  const int cap_far = std::max(tb.cap_far, std::min(tb.node_cap, need_far + need_far / 4 + 64));
+  (void)hipFree(tb.near_list);
+  (void)hipFree(tb.far_list);
+  tb.near_list = tb.far_list = nullptr;
+  tb.cap_near = tb.cap_far = 0;
+  const size_t nl = static_cast<size_t>(tb.leaf_cap);
+  if (dalloc(&tb.near_list, nl * cap_near) != hipSuccess || dalloc(&tb.far_list, nl * cap_far) != hipSuccess) {
+    (void)hipGetLastError();
+    free_leaf_buffers(c->ff);
+    return fail(PW_ERR_NOMEM, "interaction lists");
+  }
   tb.cap_near = cap_near;
   tb.cap_far = cap_far;
   return PW_OK;
@@ -139,7 +167,18 @@ int build(pw_ctx* c, const Rec64* state, bool mac_only) {
   }
   f.shape = pw::tree_shape(n, nleaf);
   PW_HIP(pw::build_moments(n, f.tb, f.shape, c->stream));
-  PW_HIP(pw::build_lists(f.tb, f.shape, list_params(f, mac_only), c->stream));
+  const pw::ListParams lp = list_params(f, mac_only);
+  PW_HIP(pw::build_lists(f.tb, f.shape, lp, c->stream));
+  // lists longer than their capacity: grow them and build again (the
+  // lengths are those of this tree, so the second build fits)
+  int need[2] = {0, 0};
+  PW_HIP(hipMemcpyAsync(need, f.tb.hist + pw::kHistNeed, sizeof(need), hipMemcpyDeviceToHost, c->stream));
+  PW_HIP(hipStreamSynchronize(c->stream));
+  if (need[0] > f.tb.cap_near || need[1] > f.tb.cap_far) {
+    PW_TRY(ensure_list_capacity(c, need[0], need[1]));
+    PW_HIP(pw::launch_fill_int(f.d_overflow, 1, 0, c->stream));
+    PW_HIP(pw::build_lists(f.tb, f.shape, lp, c->stream));
+  }
   f.lists_valid = true;
   return PW_OK;
 }

@@ -151,6 +151,8 @@ TreeShape tree_shape(int n, int nleaf);
 struct NearItem {
   int T, k0, k1, slot;
 };
+constexpr int kHistNeed = 13;     // hist slots of the longest near and far lists
+constexpr size_t kListLdsMax = 128 * 1024;  // LDS of the list kernel's level buffers at most
 constexpr int kNearChunk = 128;   // near leaves per work item
 constexpr int kNearMaxParts = 16; // work items per leaf at most
 
@@ -169,7 +171,8 @@ struct TreeBuffers {
   int* cut = nullptr;           // n: 1 where a leaf starts (scratch before)
   int* leaf_id = nullptr;       // n: exclusive scan of cut
   int* leaf_first = nullptr;    // leaf_cap + 1: first particle of each leaf, then n
-  int* hist = nullptr;          // 16: Morton prefix-length histogram, [11] level, [12] nleaf
+  int* hist = nullptr;          // 16: Morton prefix-length histogram, [11] level, [12] nleaf,
+                                //     [kHistNeed], [kHistNeed + 1]: longest near and far lists
   void* scan_n_temp = nullptr;
   size_t scan_n_bytes = 0;
   int leaf_cap = 0;             // leaves the leaf-indexed buffers hold

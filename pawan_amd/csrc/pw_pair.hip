@@ -310,7 +310,9 @@ __device__ __forceinline__ void near32(float r2, float sig2, float nu2, float& q
   c = nu2 * kKf * e * isg3 * isg2;  // 2 nu Z / sigma^2
 }
 
-// One pair's velocity and stretching into the partial sums (v = a_s x d).
+// One pair's velocity and stretching into the partial sums (v = a_s x d),
+// the stretching factored (A and B).  Used where every pair of the tile is
+// beyond the Gaussian core: there q'|a_s| is small against the near pairs'.
 template <typename S>
 __device__ __forceinline__ void accumulate32(float dx, float dy, float dz, float q, float f, float tax,
                                              float tay, float taz, const S& s, P32& p) {
@@ -327,6 +329,28 @@ __device__ __forceinline__ void accumulate32(float dx, float dy, float dz, float
   p.bx = fmaf(g, dx, p.bx);
   p.by = fmaf(g, dy, p.by);
   p.bz = fmaf(g, dz, p.bz);
+}
+
+// The same sums with the stretching formed per pair: q' (a_t x a_s) goes
+// into B as -(q'/3)(a_t x a_s), so that fold() still gives a_t x A - 3 B.
+// Used where pairs may be near: there A would be a large sum of nearly
+// parallel strengths (thin rings) whose cross product with a_t cancels, and
+// forming the small a_t x a_s pair by pair keeps that cancellation out of
+// the sums.
+template <typename S>
+__device__ __forceinline__ void accumulate32_pair(float dx, float dy, float dz, float q, float f, float tax,
+                                                  float tay, float taz, const S& s, P32& p) {
+  const float vx = s.ay * dz - s.az * dy;
+  const float vy = s.az * dx - s.ax * dz;
+  const float vz = s.ax * dy - s.ay * dx;
+  p.ux = fmaf(q, vx, p.ux);
+  p.uy = fmaf(q, vy, p.uy);
+  p.uz = fmaf(q, vz, p.uz);
+  const float g = f * fmaf(taz, vz, fmaf(tay, vy, tax * vx));
+  const float m = -0x1.555556p-2f * q;  // -q'/3
+  p.bx = fmaf(m, tay * s.az - taz * s.ay, fmaf(g, dx, p.bx));
+  p.by = fmaf(m, taz * s.ax - tax * s.az, fmaf(g, dy, p.by));
+  p.bz = fmaf(m, tax * s.ay - tay * s.ax, fmaf(g, dz, p.bz));
 }
 
 template <typename S>
@@ -347,7 +371,7 @@ __device__ __forceinline__ void pair32(const T32& t, const S& s, P32& p, PSE32& 
     e.y = fmaf(c, t.vol * s.ay - s.vol * t.ay, e.y);
     e.z = fmaf(c, t.vol * s.az - s.vol * t.az, e.z);
   }
-  accumulate32(dx, dy, dz, q, f, t.ax, t.ay, t.az, s, p);
+  accumulate32_pair(dx, dy, dz, q, f, t.ax, t.ay, t.az, s, p);
 }
 
 // Far-only pair on positions relative to the target leaf's centre, rounded
@@ -613,7 +637,7 @@ __device__ __forceinline__ void pair32_rel_core(float tx, float ty, float tz, co
     e.y = fmaf(c, t.vol * s.ay - s.vol * t.ay, e.y);
     e.z = fmaf(c, t.vol * s.az - s.vol * t.az, e.z);
   }
-  accumulate32(dx, dy, dz, q, f, t.ax, t.ay, t.az, s, p);
+  accumulate32_pair(dx, dy, dz, q, f, t.ax, t.ay, t.az, s, p);
 }
 
 __device__ __forceinline__ void near_slice(const NearArgs& a, const NearItem& it, int slice, SrcRel* tile) {

@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
 #include <cfloat>
 
 #include "pw_internal.h"
@@ -288,10 +289,11 @@ __global__ __launch_bounds__(64) void k_parent_moments(NodeRec* __restrict__ nod
 // above, taken in order) and appends far nodes, near leaves and opened
 // nodes in lane order through ballots, so the lists are deterministic: far
 // nodes coarse level first, each level in Morton order, and the near leaves
-// (all on the last level) in Morton order.  Opened nodes of a level wait in
-// LDS (kOpen per level; more set the overflow flag).
-constexpr int kOpen = 2048;
-
+// (all on the last level) in Morton order.  The nodes opened on one level
+// wait in LDS, two buffers of max_open entries (the largest internal level,
+// so they cannot overflow).  need[0], need[1] collect the longest near and
+// far lists, which may exceed the capacities: the lists are then cut (and
+// the overflow flag set), and the host grows them and builds again.
 __device__ __forceinline__ bool is_far(const NodeRec& q, double tcx, double tcy, double tcz, float tr,
                                        const ListParams& lp) {
   const double ddx = q.cx - tcx, ddy = q.cy - tcy, ddz = q.cz - tcz;
@@ -309,8 +311,10 @@ __global__ __launch_bounds__(64) void k_lists(const NodeRec* __restrict__ nodes,
                                               ListParams lp, int* __restrict__ near_list,
                                               int* __restrict__ near_cnt, int cap_near,
                                               int* __restrict__ far_list, int* __restrict__ far_cnt,
-                                              int cap_far, int* __restrict__ overflow) {
-  __shared__ int open[2][kOpen];
+                                              int cap_far, int max_open, int* __restrict__ overflow,
+                                              int* __restrict__ need) {
+  extern __shared__ int open_lds[];
+  int* open[2] = {open_lds, open_lds + max_open};
   const int T = blockIdx.x;
   const int lane = static_cast<int>(threadIdx.x);
   const double tcx = nodes[T].cx, tcy = nodes[T].cy, tcz = nodes[T].cz;
@@ -363,15 +367,15 @@ __global__ __launch_bounds__(64) void k_lists(const NodeRec* __restrict__ nodes,
       }
       if (opn) {
         const int p = nnext + __popcll(mo & below);
-        if (p < kOpen) open[cur ^ 1][p] = id;
+        if (p < max_open) open[cur ^ 1][p] = id;
       }
       nf += __popcll(mf);
       nn += __popcll(mn);
       nnext += __popcll(mo);
     }
-    if (nnext > kOpen) {
+    if (nnext > max_open) {  // (cannot happen: a level has at most max_open nodes)
       over = true;
-      nnext = kOpen;
+      nnext = max_open;
     }
     cur ^= 1;
     np = nnext;
@@ -380,6 +384,8 @@ __global__ __launch_bounds__(64) void k_lists(const NodeRec* __restrict__ nodes,
   if (lane == 0) {
     near_cnt[T] = min(nn, cap_near);
     far_cnt[T] = min(nf, cap_far);
+    atomicMax(need, nn);
+    atomicMax(need + 1, nf);
     if (over) atomicOr(overflow, 1);
   }
 }
@@ -543,9 +549,20 @@ hipError_t build_moments(int n, TreeBuffers& tb, const TreeShape& shape, hipStre
 hipError_t build_lists(const TreeBuffers& tb, const TreeShape& shape, const ListParams& lp,
                        hipStream_t st) {
   if (shape.nleaf <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_lists, dim3(shape.nleaf), dim3(64), 0, st, tb.nodes, shape, lp,
-                     tb.near_list, tb.near_cnt, tb.cap_near, tb.far_list, tb.far_cnt, tb.cap_far,
-                     tb.overflow);
+  int max_open = 1;
+  for (int L = 1; L < shape.nlevel; ++L) max_open = std::max(max_open, shape.level_count[L]);
+  const size_t lds = 2 * static_cast<size_t>(max_open) * sizeof(int);
+  if (lds > kListLdsMax) return hipErrorInvalidValue;
+  static size_t lds_set = 64 * 1024;  // the default limit
+  if (lds > lds_set) {
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_lists),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kListLdsMax));
+    if (e != hipSuccess) return e;
+    lds_set = kListLdsMax;
+  }
+  hipLaunchKernelGGL(k_lists, dim3(shape.nleaf), dim3(64), lds, st, tb.nodes, shape, lp, tb.near_list,
+                     tb.near_cnt, tb.cap_near, tb.far_list, tb.far_cnt, tb.cap_far, max_open, tb.overflow,
+                     tb.hist + kHistNeed);
   return hipGetLastError();
 }
 

@@ -85,6 +85,31 @@ def test_lists_cover_every_pair_once(built):
         assert T in built["near"][T, :built["near_cnt"][T]]
 
 
+def test_lists_grow_past_a_small_capacity(helix, built, monkeypatch):
+    """Lists longer than their capacity are grown and built again: from an
+    initial capacity of 16 entries the lists and the rates are those built
+    with the default capacity, bit for bit."""
+    monkeypatch.setenv("PAWAN_FF_LIST_CAP", "16")
+    with capi.Solver(helix) as g:
+        g.ff_set_params(0.5, 0.0, use_mlp=False)
+        g.ff_set_conserve(False)
+        g.ff_build()
+        info = g.ff_info()
+        nc, near = g.ff_export(capi.FF_NEAR_CNT), g.ff_export(capi.FF_NEAR)
+        fc, far = g.ff_export(capi.FF_FAR_CNT), g.ff_export(capi.FF_FAR)
+        u, w = g.solve("learned")
+    _say(f"grown lists: cap_near {info['cap_near']} cap_far {info['cap_far']}")
+    assert info["overflow"] == 0
+    assert info["cap_near"] >= info["max_near"] > 16 and info["cap_far"] >= info["max_far"] > 16
+    np.testing.assert_array_equal(nc, built["near_cnt"])
+    np.testing.assert_array_equal(fc, built["far_cnt"])
+    for T in range(info["leaves"]):
+        np.testing.assert_array_equal(near[T, :nc[T]], built["near"][T, :nc[T]])
+        np.testing.assert_array_equal(far[T, :fc[T]], built["far"][T, :fc[T]])
+    np.testing.assert_array_equal(u, built["u"])
+    np.testing.assert_array_equal(w, built["w"])
+
+
 def test_far_nodes_pass_the_opening_test(built):
     nd, nl = built["nodes"], built["info"]["leaves"]
     for T in range(0, nl, 7):
