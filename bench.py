@@ -168,7 +168,8 @@ def learned_leg(system, dt: float, steps: int, target_err: float, theta: float, 
             out[key] = {"steps_per_s": steps / wall, "rhs_ms": ms / max(1, launches), "velocity_error": err,
                         "near_pairs_per_s": info["near_pairs"] / (ms / max(1, launches) * 1e-3),
                         "tol_abs": float(tol), "near_pairs": info["near_pairs"], "far_evals": info["far_evals"],
-                        "max_far_nodes": info["max_far"], "max_near_leaves": info["max_near"]}
+                        "max_far_nodes": info["max_far"], "max_near_leaves": info["max_near"],
+                        "near_core_pairs": info["near_core_pairs"]}
             log(f"{key}: {out[key]}")
             if use_mlp:
                 out["_drift_cfg"] = {"weights": weights, "coef_md": rep["coef_md"], "coef_mlp": rep["coef_mlp"],

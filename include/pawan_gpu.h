@@ -211,10 +211,11 @@ int pw_ff_get_err_coef(pw_ctx *ctx, double *coef_md, double *coef_mlp);
 /* build the tree and the lists of the current state (done by every
  * PW_LEARNED right-hand side; for inspection) */
 int pw_ff_build(pw_ctx *ctx);
-/* out[0..12): n, leaves, nodes, levels, near capacity, far capacity,
+/* out[0..13): n, leaves, nodes, levels, near capacity, far capacity,
  * training samples held, near pairs, far (target, node) evaluations,
- * list overflow flag, max near leaves, max far nodes (of the lists last
- * built) */
+ * list overflow flag, max near leaves, max far nodes, near pairs of leaf
+ * pairs within the Gaussian core range (of the lists last built; a
+ * shorter len takes the leading values) */
 int pw_ff_info(pw_ctx *ctx, int64_t *out, int64_t len);
 /* copy an internal array to host memory (bytes must match exactly):
  * PW_FF_PERM int32[n] (sorted position -> particle), PW_FF_NODES

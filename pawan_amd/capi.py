@@ -41,7 +41,7 @@ EXPORTS = [
 FF_PERM, FF_NODES, FF_NEAR_CNT, FF_NEAR, FF_FAR_CNT, FF_FAR, FF_SAMPLES_X, FF_SAMPLES_Y, FF_SAMPLES_AUX = range(9)
 MLP_PARAMS = 36748
 FF_INFO_NAMES = ["n", "leaves", "nodes", "levels", "cap_near", "cap_far", "samples", "near_pairs",
-                 "far_evals", "overflow", "max_near", "max_far"]
+                 "far_evals", "overflow", "max_near", "max_far", "near_core_pairs"]
 FF_TRAIN_NAMES = ["train_loss", "val_loss", "val_loss_zero", "rms_ratio_u", "rms_ratio_grad",
                   "coef_md", "coef_mlp", "steps", "val_samples", "ms"]
 

@@ -383,7 +383,7 @@ int pw_ff_info(pw_ctx* c, int64_t* out, int64_t len) {
   if (!out || len < 1) return fail(PW_ERR_ARG, "null output");
   PW_TRY(ensure_ff(c));
   auto& f = c->ff;
-  std::vector<int64_t> v(12, 0);
+  std::vector<int64_t> v(13, 0);
   v[0] = c->n;
   v[1] = f.shape.nleaf;
   v[2] = f.shape.nnodes;
@@ -393,10 +393,10 @@ int pw_ff_info(pw_ctx* c, int64_t* out, int64_t len) {
   v[6] = f.ns;
   if (f.lists_valid) {
     long long* d = nullptr;
-    if (dalloc(&d, 2 * static_cast<size_t>(f.shape.nleaf)) != hipSuccess) return fail(PW_ERR_NOMEM, "stats");
-    std::vector<long long> h(2 * static_cast<size_t>(f.shape.nleaf));
+    if (dalloc(&d, 3 * static_cast<size_t>(f.shape.nleaf)) != hipSuccess) return fail(PW_ERR_NOMEM, "stats");
+    std::vector<long long> h(3 * static_cast<size_t>(f.shape.nleaf));
     std::vector<int> nc(f.shape.nleaf), fc(f.shape.nleaf);
-    hipError_t e = pw::launch_list_stats(f.tb, f.shape, d, c->stream);
+    hipError_t e = pw::launch_list_stats(f.tb, f.shape, f.sigma_max2, d, c->stream);
     if (e == hipSuccess)
       e = hipMemcpyAsync(h.data(), d, h.size() * sizeof(long long), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess)
@@ -410,14 +410,15 @@ int pw_ff_info(pw_ctx* c, int64_t* out, int64_t len) {
     (void)hipFree(d);
     if (e != hipSuccess) return fail(PW_ERR_HIP, std::string("list stats: ") + hipGetErrorString(e));
     for (int T = 0; T < f.shape.nleaf; ++T) {
-      v[7] += h[2 * T];
-      v[8] += h[2 * T + 1];
+      v[7] += h[3 * T];
+      v[8] += h[3 * T + 1];
+      v[12] += h[3 * T + 2];
       v[10] = std::max<int64_t>(v[10], nc[T]);
       v[11] = std::max<int64_t>(v[11], fc[T]);
     }
     v[9] = over;
   }
-  for (int64_t i = 0; i < std::min<int64_t>(len, 12); ++i) out[i] = v[i];
+  for (int64_t i = 0; i < std::min<int64_t>(len, 13); ++i) out[i] = v[i];
   return PW_OK;
 }
 

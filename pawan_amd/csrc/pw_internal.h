@@ -223,7 +223,8 @@ hipError_t build_lists(const TreeBuffers& tb, const TreeShape& shape, const List
                        hipStream_t st);
 
 // per target leaf: out[2T] = near pairs, out[2T+1] = far evaluations
-hipError_t launch_list_stats(const TreeBuffers& tb, const TreeShape& shape, long long* out, hipStream_t st);
+hipError_t launch_list_stats(const TreeBuffers& tb, const TreeShape& shape, float sigma_max2, long long* out,
+                             hipStream_t st);
 
 // Near field over the lists: rates_s[t] = sum over the near leaves of
 // target t's leaf (FP32C pair math, PSE included), Morton order.

@@ -311,8 +311,9 @@ __device__ __forceinline__ void near32(float r2, float sig2, float nu2, float& q
 }
 
 // One pair's velocity and stretching into the partial sums (v = a_s x d),
-// the stretching factored (A and B).  Used where every pair of the tile is
-// beyond the Gaussian core: there q'|a_s| is small against the near pairs'.
+// the stretching factored (A and B).  Used where the source is beyond the
+// Gaussian core of every lane's target: there q'|a_s| is small against the
+// core pairs'.
 template <typename S>
 __device__ __forceinline__ void accumulate32(float dx, float dy, float dz, float q, float f, float tax,
                                              float tay, float taz, const S& s, P32& p) {
@@ -333,10 +334,10 @@ __device__ __forceinline__ void accumulate32(float dx, float dy, float dz, float
 
 // The same sums with the stretching formed per pair: q' (a_t x a_s) goes
 // into B as -(q'/3)(a_t x a_s), so that fold() still gives a_t x A - 3 B.
-// Used where pairs may be near: there A would be a large sum of nearly
-// parallel strengths (thin rings) whose cross product with a_t cancels, and
-// forming the small a_t x a_s pair by pair keeps that cancellation out of
-// the sums.
+// Used for a source that is inside the Gaussian core of some lane's target:
+// there q' is large, A would gather large, nearly parallel strengths (thin
+// rings) whose cross product with a_t cancels, and forming the small
+// a_t x a_s pair by pair keeps that cancellation out of the sums.
 template <typename S>
 __device__ __forceinline__ void accumulate32_pair(float dx, float dy, float dz, float q, float f, float tax,
                                                   float tay, float taz, const S& s, P32& p) {
@@ -364,14 +365,19 @@ __device__ __forceinline__ void pair32(const T32& t, const S& s, P32& p, PSE32& 
   const float ir2 = ir * ir;
   float q = ir * ir2;
   float f = q * ir2;
-  if (r2 <= kFar32 * sig2) {
+  const bool core = r2 <= kFar32 * sig2;
+  if (core) {
     float c;
     near32(r2, sig2, t.nu2, q, f, c);
     e.x = fmaf(c, t.vol * s.ax - s.vol * t.ax, e.x);
     e.y = fmaf(c, t.vol * s.ay - s.vol * t.ay, e.y);
     e.z = fmaf(c, t.vol * s.az - s.vol * t.az, e.z);
   }
-  accumulate32_pair(dx, dy, dz, q, f, t.ax, t.ay, t.az, s, p);
+  // wave-uniform choice: per pair if the source is in any lane's core
+  if (__ballot(core))
+    accumulate32_pair(dx, dy, dz, q, f, t.ax, t.ay, t.az, s, p);
+  else
+    accumulate32(dx, dy, dz, q, f, t.ax, t.ay, t.az, s, p);
 }
 
 // Far-only pair on positions relative to the target leaf's centre, rounded
@@ -630,14 +636,19 @@ __device__ __forceinline__ void pair32_rel_core(float tx, float ty, float tz, co
   const float ir2 = ir * ir;
   float q = ir * ir2;
   float f = q * ir2;
-  if (r2 <= kFar32 * sig2) {
+  const bool core = r2 <= kFar32 * sig2;
+  if (core) {
     float c;
     near32(r2, sig2, t.nu2, q, f, c);
     e.x = fmaf(c, t.vol * s.ax - s.vol * t.ax, e.x);
     e.y = fmaf(c, t.vol * s.ay - s.vol * t.ay, e.y);
     e.z = fmaf(c, t.vol * s.az - s.vol * t.az, e.z);
   }
-  accumulate32_pair(dx, dy, dz, q, f, t.ax, t.ay, t.az, s, p);
+  // wave-uniform choice: per pair if the source is in any lane's core
+  if (__ballot(core))
+    accumulate32_pair(dx, dy, dz, q, f, t.ax, t.ay, t.az, s, p);
+  else
+    accumulate32(dx, dy, dz, q, f, t.ax, t.ay, t.az, s, p);
 }
 
 __device__ __forceinline__ void near_slice(const NearArgs& a, const NearItem& it, int slice, SrcRel* tile) {

@@ -421,28 +421,37 @@ __global__ __launch_bounds__(256) void k_near_emit(const int* __restrict__ near_
 }
 
 // Per target leaf: near pairs (targets x near sources, self pairs
-// excluded) and far evaluations (targets x far nodes).
+// excluded), far evaluations (targets x far nodes), and the near pairs of
+// listed leaves within the Gaussian core range (bounding spheres at most
+// sqrt(44) sigma_max apart: the near kernel tests these pair by pair).
 __global__ __launch_bounds__(64) void k_list_stats(const NodeRec* __restrict__ nodes, int nleaf,
                                                    const int* __restrict__ near_list,
                                                    const int* __restrict__ near_cnt, int cap_near,
-                                                   const int* __restrict__ far_cnt, long long* __restrict__ out) {
+                                                   const int* __restrict__ far_cnt, float gap2,
+                                                   long long* __restrict__ out) {
   const int T = blockIdx.x * 64 + threadIdx.x;
   if (T >= nleaf) return;
   const long long nt = nodes[T].last - nodes[T].first;
-  long long src = 0;
+  long long src = 0, core = 0;
   for (int k = 0; k < near_cnt[T]; ++k) {
     const int id = near_list[static_cast<size_t>(T) * cap_near + k];
-    src += nodes[id].last - nodes[id].first;
+    const long long ns = nodes[id].last - nodes[id].first;
+    src += ns;
+    const double ddx = nodes[id].cx - nodes[T].cx, ddy = nodes[id].cy - nodes[T].cy, ddz = nodes[id].cz - nodes[T].cz;
+    const float gap = static_cast<float>(sqrt(ddx * ddx + ddy * ddy + ddz * ddz)) - nodes[T].r - nodes[id].r;
+    if (!(gap > 0.0f && gap * gap > gap2 * 1.0001f)) core += ns;
   }
-  out[2 * T] = nt * src - nt;  // the leaf itself is on its near list
-  out[2 * T + 1] = nt * far_cnt[T];
+  out[3 * T] = nt * src - nt;  // the leaf itself is on its near list
+  out[3 * T + 1] = nt * far_cnt[T];
+  out[3 * T + 2] = nt * core - nt;
 }
 
 }  // namespace
 
-hipError_t launch_list_stats(const TreeBuffers& tb, const TreeShape& shape, long long* out, hipStream_t st) {
+hipError_t launch_list_stats(const TreeBuffers& tb, const TreeShape& shape, float sigma_max2, long long* out,
+                             hipStream_t st) {
   hipLaunchKernelGGL(k_list_stats, dim3((shape.nleaf + 63) / 64), dim3(64), 0, st, tb.nodes, shape.nleaf,
-                     tb.near_list, tb.near_cnt, tb.cap_near, tb.far_cnt, out);
+                     tb.near_list, tb.near_cnt, tb.cap_near, tb.far_cnt, kFar32List * sigma_max2, out);
   return hipGetLastError();
 }
 

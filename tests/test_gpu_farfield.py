@@ -79,6 +79,7 @@ def test_lists_cover_every_pair_once(built):
     first, last = nd[:, 19].astype(np.int64), nd[:, 20].astype(np.int64)
     nl = info["leaves"]
     assert info["overflow"] == 0
+    assert 0 <= info["near_core_pairs"] <= info["near_pairs"]
     assert FF.list_partition_errors(first, last, nl, built["near_cnt"], built["near"],
                                     built["far_cnt"], built["far"]) == 0
     for T in range(nl):
