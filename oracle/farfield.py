@@ -72,7 +72,7 @@ def morton_perm(pos: np.ndarray) -> np.ndarray:
     return np.argsort(morton_codes(pos), kind="stable")
 
 
-CELL_TARGET = 320
+CELL_TARGET = 512
 
 
 def prefix_levels(codes: np.ndarray) -> np.ndarray:

@@ -117,7 +117,7 @@ constexpr int kLeafSize = 256;  // particles per leaf at most (= one LDS tile)
 // Leaves are the cells of one octree level (Morton-code prefix), cut into
 // runs of at most 256: the level is the first whose mean occupancy is at
 // most kCellTarget particles per non-empty cell.
-constexpr int kCellTarget = 320;
+constexpr int kCellTarget = 512;
 constexpr int kFan = 8;         // children per node
 constexpr int kMaxLevels = 12;  // 256 * 8^11 particles
 
