@@ -32,6 +32,8 @@
 // leave registers.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "pw_internal.h"
 #include "pw_mlp.h"
 
@@ -661,12 +663,15 @@ __global__ __launch_bounds__(256) void k_ff_samples(SampleArgs a) {
 // sum dalpha vanish in the direct sum (interaction_utils.h:493-495) does not
 // hold for one-sided far-field evaluations, so the mean vorticity rate is
 // removed: block sums (FP64, fixed order), their total, the subtraction.
+// block b sums particles b*256 + t + k*(grid*256) (each lane in k order),
+// then its lanes in a fixed order
 __global__ __launch_bounds__(256) void k_sum_dalpha(const Rate64* __restrict__ r, int n, double* __restrict__ part) {
   __shared__ double red[4][3];
-  const int i = blockIdx.x * 256 + threadIdx.x;
   double v[3] = {0.0, 0.0, 0.0};
-  if (i < n) {
-    v[0] = r[i].wx; v[1] = r[i].wy; v[2] = r[i].wz;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    v[0] += r[i].wx;
+    v[1] += r[i].wy;
+    v[2] += r[i].wz;
   }
   for (int off = 32; off > 0; off >>= 1)
     for (int k = 0; k < 3; ++k) v[k] += __shfl_xor(v[k], off, 64);
@@ -678,11 +683,21 @@ __global__ __launch_bounds__(256) void k_sum_dalpha(const Rate64* __restrict__ r
         (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
 }
 
-__global__ __launch_bounds__(64) void k_mean_dalpha(double* __restrict__ part, int nb, int n) {
-  if (threadIdx.x >= 3) return;
-  double s = 0.0;
-  for (int b = 0; b < nb; ++b) s += part[3 * b + threadIdx.x];
-  part[3 * nb + threadIdx.x] = s / static_cast<double>(n);
+// the mean of the nb block sums: one block, lane t takes blocks t, t+256,
+// ..., then the lanes in a fixed order
+__global__ __launch_bounds__(256) void k_mean_dalpha(double* __restrict__ part, int nb, int n) {
+  __shared__ double red[4][3];
+  double v[3] = {0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < nb; b += 256)
+    for (int k = 0; k < 3; ++k) v[k] += part[3 * b + k];
+  for (int off = 32; off > 0; off >>= 1)
+    for (int k = 0; k < 3; ++k) v[k] += __shfl_xor(v[k], off, 64);
+  if ((threadIdx.x & 63) == 0)
+    for (int k = 0; k < 3; ++k) red[threadIdx.x >> 6][k] = v[k];
+  __syncthreads();
+  if (threadIdx.x < 3)
+    part[3 * nb + threadIdx.x] = ((red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x])) /
+                                 static_cast<double>(n);
 }
 
 __global__ __launch_bounds__(256) void k_sub_dalpha(Rate64* __restrict__ r, int n, const double* __restrict__ mean) {
@@ -723,12 +738,14 @@ hipError_t launch_far(const Rec64* sorted, const NodeRec* nodes, const int* far_
   return hipGetLastError();
 }
 
+constexpr int kDalphaBlocks = 1024;  // block sums of the mean vorticity rate at most
+
 hipError_t launch_remove_mean_dalpha(Rate64* rates, int n, double* part, hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  const int nb = (n + 255) / 256;
+  const int nb = std::min((n + 255) / 256, kDalphaBlocks);  // a function of n only: a fixed summation order
   hipLaunchKernelGGL(k_sum_dalpha, dim3(nb), dim3(256), 0, st, rates, n, part);
-  hipLaunchKernelGGL(k_mean_dalpha, dim3(1), dim3(64), 0, st, part, nb, n);
-  hipLaunchKernelGGL(k_sub_dalpha, dim3(nb), dim3(256), 0, st, rates, n, part + 3 * nb);
+  hipLaunchKernelGGL(k_mean_dalpha, dim3(1), dim3(256), 0, st, part, nb, n);
+  hipLaunchKernelGGL(k_sub_dalpha, dim3((n + 255) / 256), dim3(256), 0, st, rates, n, part + 3 * nb);
   return hipGetLastError();
 }
 

@@ -62,15 +62,27 @@ __global__ __launch_bounds__(256) void k_bbox_partial(const Rec64* __restrict__ 
     for (int k = 0; k < 6; ++k) part[6 * blockIdx.x + k] = red[k][0];
 }
 
-__global__ void k_bbox_final(const double* __restrict__ part, int nb, double* __restrict__ box) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ __launch_bounds__(256) void k_bbox_final(const double* __restrict__ part, int nb, double* __restrict__ box) {
+  __shared__ double red[6][4];
   double b[6] = {DBL_MAX, DBL_MAX, DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX};
-  for (int i = 0; i < nb; ++i)
+  for (int i = threadIdx.x; i < nb; i += 256)
     for (int k = 0; k < 3; ++k) {
       b[k] = fmin(b[k], part[6 * i + k]);
       b[3 + k] = fmax(b[3 + k], part[6 * i + 3 + k]);
     }
-  for (int k = 0; k < 6; ++k) box[k] = b[k];
+  for (int off = 32; off > 0; off >>= 1)
+    for (int k = 0; k < 3; ++k) {
+      b[k] = fmin(b[k], __shfl_xor(b[k], off, 64));
+      b[3 + k] = fmax(b[3 + k], __shfl_xor(b[3 + k], off, 64));
+    }
+  if ((threadIdx.x & 63) == 0)
+    for (int k = 0; k < 6; ++k) red[k][threadIdx.x >> 6] = b[k];
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int k = threadIdx.x;
+    box[k] = fmin(fmin(red[k][0], red[k][1]), fmin(red[k][2], red[k][3]));
+    box[3 + k] = fmax(fmax(red[3 + k][0], red[3 + k][1]), fmax(red[3 + k][2], red[3 + k][3]));
+  }
 }
 
 // ---- Morton codes -------------------------------------------------------
@@ -512,7 +524,7 @@ hipError_t build_leaves(const Rec64* y, int n, TreeBuffers& tb, hipStream_t st) 
   const int nb = min(1024, (n + 255) / 256);
   const int grid = (n + 255) / 256;
   hipLaunchKernelGGL(k_bbox_partial, dim3(nb), dim3(256), 0, st, y, n, tb.bbox_part);
-  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, st, tb.bbox_part, nb, tb.bbox);
+  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(256), 0, st, tb.bbox_part, nb, tb.bbox);
   hipLaunchKernelGGL(k_morton, dim3(grid), dim3(256), 0, st, y, n, tb.bbox, tb.code, tb.idx);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
