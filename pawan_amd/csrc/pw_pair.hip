@@ -137,7 +137,20 @@ struct T64 {
   double x, y, z, hs2, ax, ay, az, vol, nu2;
 };
 
-__device__ __forceinline__ void pair64(const T64& t, const Rec64& s, Acc64& a) {
+// Sums of the FP64 kernel.  A source beyond the Gaussian core of every
+// lane's target adds its stretching with the target's strength factored
+// out (d.(a_t x a_s) = a_t.(a_s x d), v = a_s x d shared with the velocity;
+// sum q (a_t x a_s) = a_t x A); a source in some lane's core adds
+// q (a_t x a_s) pair by pair into B, where A would gather large, nearly
+// parallel strengths (thin rings) and a_t x A would cancel.
+//   w = a_t x A + B,  A = sum q a_s,  B = sum F (a_t.v) d + per-pair terms + PSE.
+struct AccF64 {
+  double ux = 0, uy = 0, uz = 0;  // sum q v
+  double ax = 0, ay = 0, az = 0;  // A
+  double bx = 0, by = 0, bz = 0;  // B
+};
+
+__device__ __forceinline__ void pair64(const T64& t, const Rec64& s, AccF64& a) {
   const double dx = t.x - s.x, dy = t.y - s.y, dz = t.z - s.z;
   const double r2 = dx * dx + dy * dy + dz * dz;
   const double sig2 = 0.5 * s.s2 + t.hs2;
@@ -145,11 +158,8 @@ __device__ __forceinline__ void pair64(const T64& t, const Rec64& s, Acc64& a) {
   const double ir2 = ir * ir;
   double q = kQ * (ir * ir2);
   double F = -3.0 * q * ir2;
-  // a_trg x a_src
-  const double cx = t.ay * s.az - t.az * s.ay;
-  const double cy = t.az * s.ax - t.ax * s.az;
-  const double cz = t.ax * s.ay - t.ay * s.ax;
-  if (r2 <= kFar64 * sig2) {
+  const bool core = r2 <= kFar64 * sig2;
+  if (core) {
     const double isg = rsqrt(sig2);
     const double isg3 = isg * isg * isg;
     const double rb = (r2 * ir) * isg;    // rho / sigma
@@ -159,17 +169,29 @@ __device__ __forceinline__ void pair64(const T64& t, const Rec64& s, Acc64& a) {
     q = (phi * irb - Z) * (irb * irb);
     F = (Z - 3.0 * q) * ir2;
     const double c = t.nu2 * Z * (isg * isg);  // 2 nu Z / sigma^2
-    a.wx += c * (t.vol * s.ax - s.vol * t.ax);
-    a.wy += c * (t.vol * s.ay - s.vol * t.ay);
-    a.wz += c * (t.vol * s.az - s.vol * t.az);
+    a.bx += c * (t.vol * s.ax - s.vol * t.ax);
+    a.by += c * (t.vol * s.ay - s.vol * t.ay);
+    a.bz += c * (t.vol * s.az - s.vol * t.az);
   }
-  a.ux += q * (s.ay * dz - s.az * dy);
-  a.uy += q * (s.az * dx - s.ax * dz);
-  a.uz += q * (s.ax * dy - s.ay * dx);
-  const double fr = F * (dx * cx + dy * cy + dz * cz);
-  a.wx += q * cx + fr * dx;
-  a.wy += q * cy + fr * dy;
-  a.wz += q * cz + fr * dz;
+  const double vx = s.ay * dz - s.az * dy;
+  const double vy = s.az * dx - s.ax * dz;
+  const double vz = s.ax * dy - s.ay * dx;
+  a.ux += q * vx;
+  a.uy += q * vy;
+  a.uz += q * vz;
+  const double fr = F * (t.ax * vx + t.ay * vy + t.az * vz);
+  if (__ballot(core)) {
+    a.bx += q * (t.ay * s.az - t.az * s.ay) + fr * dx;
+    a.by += q * (t.az * s.ax - t.ax * s.az) + fr * dy;
+    a.bz += q * (t.ax * s.ay - t.ay * s.ax) + fr * dz;
+  } else {
+    a.ax += q * s.ax;
+    a.ay += q * s.ay;
+    a.az += q * s.az;
+    a.bx += fr * dx;
+    a.by += fr * dy;
+    a.bz += fr * dz;
+  }
 }
 
 __device__ __forceinline__ void item_fp64(const PairArgs& pa, const WorkItem& it, Rec64* tile) {
@@ -181,7 +203,7 @@ __device__ __forceinline__ void item_fp64(const PairArgs& pa, const WorkItem& it
   me.x = m.x; me.y = m.y; me.z = m.z; me.hs2 = 0.5 * m.s2;
   me.ax = m.ax; me.ay = m.ay; me.az = m.az; me.vol = m.vol;
   me.nu2 = 2.0 * pa.nu;
-  Acc64 a;
+  AccF64 a;
   for (int base = it.s0; base < it.s1; base += kTB) {
     const int cnt = min(kTB, it.s1 - base);
     __syncthreads();
@@ -200,7 +222,9 @@ __device__ __forceinline__ void item_fp64(const PairArgs& pa, const WorkItem& it
   if (active) {
     Rate64 r;
     r.ux = a.ux; r.uy = a.uy; r.uz = a.uz;
-    r.wx = a.wx; r.wy = a.wy; r.wz = a.wz;
+    r.wx = (me.ay * a.az - me.az * a.ay) + a.bx;
+    r.wy = (me.az * a.ax - me.ax * a.az) + a.by;
+    r.wz = (me.ax * a.ay - me.ay * a.ax) + a.bz;
     out_rates(pa, it)[t] = r;
   }
 }
