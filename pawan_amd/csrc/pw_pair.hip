@@ -1,7 +1,7 @@
 // pw_pair.hip -- tiled direct-sum pair kernels for gfx950.
 //
-// One work item = one block of 256 lanes (four wave64): up to 256 targets,
-// one per lane, against a contiguous source range.  Sources are staged
+// Direct sum: one work item = one block of 256 lanes (four wave64): up to
+// 256 targets, one per lane, against a contiguous source range.  Sources are staged
 // through LDS in tiles of 256 records, loaded with coalesced 16-byte global
 // loads, and read back as wave-uniform (broadcast) LDS reads.  Every lane
 // accumulates the rates of its own target in source order: the results are
@@ -18,10 +18,13 @@
 // the two particles is the target (the source-side update is the same
 // expression with the roles swapped and the sign flipped).
 //
-// The pair terms are kept per pair (a_t x a_s and V_t a_s - V_s a_t are
-// formed for each pair, as in the reference): factoring them out of the
-// source sum (a_t x sum q a_s) accumulates large, nearly parallel terms and
-// cancels only at the end, which on thin rings costs several digits.
+// The stretching q (a_t x a_s) + F [d . (a_t x a_s)] d is summed with the
+// target's strength factored out, a_t x sum q a_s and d.(a_t x a_s) =
+// a_t.(a_s x d) (the velocity's cross product), for every source that is
+// beyond the Gaussian core of all the wave's targets; a source inside some
+// lane's core adds a_t x a_s pair by pair, as the reference does.  Factoring
+// those too gathers large, nearly parallel strengths into sum q a_s (thin
+// rings), whose cross product with a_t then cancels several digits.
 // Where rho^2 > kFar sigma^2 the Gaussian terms (exp(-rho_bar^2/2) and
 // erfc(rho_bar/sqrt 2)) are below the arithmetic's rounding, and
 // q = 1/(4 pi rho^3), F = -3 q / rho^2, Z = 0 exactly as far as the result
