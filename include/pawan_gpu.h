@@ -123,6 +123,15 @@ int pw_set_states(pw_ctx *ctx, const double *state, int64_t len);
 int pw_get_states(pw_ctx *ctx, double *state, int64_t len);
 int pw_get_rates(pw_ctx *ctx, double *rate, int64_t len);
 
+/* Velocity at m points (points: m x 3 AoS, host) induced by every particle
+ * of the context (all wakes), FP64, with each particle's own radius as
+ * sigma; written to u (m x 3, host).  Replaces __interaction::getVi
+ * (src/interaction/interaction.cpp:453-476), which adds the same sum to its
+ * output vector for one point (its KERNEL call at :465 takes the source's
+ * radius).  A point coinciding with a particle gets no contribution from it.
+ * Blocking; allocates its point buffers per call. */
+int pw_velocity_at(pw_ctx *ctx, const double *points, int64_t m, double *u);
+
 /* One right-hand-side evaluation: zero the rates, then all self and (when
  * coupled) cross-wake pair interactions, leaving velocity and vorticity
  * rate of every particle on the device (__interaction::solve ->

@@ -48,6 +48,7 @@ def _load() -> C.CDLL:
     lib.or_step_generic.argtypes = [C.c_int, d, i64, i64, dp, RHS_FN, C.c_void_p]
     lib.or_step_generic.restype = C.c_int
     lib.or_diagnose.argtypes = [C.c_int, ip, dp, dp, dp, d, dp, C.c_int]
+    lib.or_velocity_at.argtypes = [i64, dp, dp, dp, i64, dp, dp]
     return lib
 
 
@@ -107,6 +108,17 @@ def solve(s, nu=None):
     vel, ret = np.zeros((s.n, 3)), np.zeros((s.n, 3))
     lib().or_solve(*_arrs(s), nu, vel, ret)
     return vel, ret
+
+
+def velocity_at(s, points):
+    """getVi (interaction.cpp:453-476): velocity at each point from every
+    particle of every wake, sigma = the particle's own radius."""
+    pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+    u = np.zeros_like(pts)
+    lib().or_velocity_at(s.n, np.ascontiguousarray(s.pos, dtype=np.float64),
+                         np.ascontiguousarray(s.alpha, dtype=np.float64),
+                         np.ascontiguousarray(s.radius, dtype=np.float64), len(pts), pts, u)
+    return u
 
 
 def solve_omp(s, nu=None):

@@ -160,6 +160,37 @@ void or_interact(double nu, double s_src, double s_trg, const double *r_src,
   }
 }
 
+/* getVi, src/interaction/interaction.cpp:453-476: the velocity induced at a
+ * point r by every particle of every wake, added to vi.  The kernel is
+ * evaluated with the source's own radius as sigma (the call at
+ * interaction.cpp:465 passes it, with 5 arguments that match no KERNEL
+ * overload; the 6-argument KERNEL of gaussian.h:47-60 is used with the same
+ * arguments), and VELOCITY (interaction_utils.h:55-62) with
+ * d = r - x_particle.  A point that coincides with a particle gets no
+ * contribution from it (alpha x d = 0 there; the reference's kernel would
+ * give 0/0).
+ * Pinned: the singular Biot-Savart law far from the particles
+ * (tests/test_oracle_point.py), and, for wakes of one radius, the velocity
+ * or_solve gives the particles themselves (there the pair sigma
+ * sqrt((s^2 + s^2)/2) is the particle's own). */
+void or_velocity_at(int64_t n, const double *pos, const double *alpha, const double *radius,
+                    int64_t m, const double *pts, double *u) {
+  for (int64_t k = 0; k < m; ++k) {
+    double vi[3] = {u[3 * k], u[3 * k + 1], u[3 * k + 2]};
+    for (int64_t i = 0; i < n; ++i) {
+      double d[3] = {pts[3 * k] - pos[3 * i], pts[3 * k + 1] - pos[3 * i + 1], pts[3 * k + 2] - pos[3 * i + 2]};
+      double rho = or_dnrm2(d);
+      if (rho == 0.0) continue;
+      double q, F, Z, nn;
+      or_kernel(rho, radius[i], &q, &F, &Z, &nn);
+      double c[3];
+      cross3(&alpha[3 * i], d, c);
+      for (int j = 0; j < 3; ++j) vi[j] += q * c[j];
+    }
+    for (int j = 0; j < 3; ++j) u[3 * k + j] = vi[j];
+  }
+}
+
 /* Target-side half of INTERACT: the contribution of source j to target t.
  * Written with t in the reference's "target" role.  Because every
  * operation in INTERACT is sign-symmetric under swapping the roles

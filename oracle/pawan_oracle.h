@@ -36,6 +36,9 @@ void or_kernel_q128(double rho, double sigma, double *q, double *F,
 
 /* One symmetric pair, INTERACT (src/interaction/interaction_utils.h:456-501).
  * Accumulates into dr_src, dr_trg, da_src, da_trg (each a 3-vector). */
+/* getVi (interaction.cpp:453-476): velocity at m points, added to u (m x 3). */
+void or_velocity_at(int64_t n, const double *pos, const double *alpha, const double *radius,
+                    int64_t m, const double *pts, double *u);
 void or_interact(double nu, double s_src, double s_trg, const double *r_src,
                  const double *r_trg, const double *a_src,
                  const double *a_trg, double v_src, double v_trg,

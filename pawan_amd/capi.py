@@ -35,6 +35,7 @@ EXPORTS = [
     "pw_ff_build", "pw_ff_info", "pw_ff_export", "pw_ff_sample", "pw_ff_clear_samples",
     "pw_ff_init_weights", "pw_ff_set_weights", "pw_ff_get_weights", "pw_ff_train",
     "pw_ff_mlp_forward", "pw_ff_mlp_grad", "pw_ff_set_conserve", "pw_ff_get_conserve",
+    "pw_velocity_at",
 ]
 
 # pw_ff_export selectors
@@ -88,6 +89,7 @@ def load(path: str | None = None) -> C.CDLL:
         "pw_get_states": ([vp, _dp, i64], i),
         "pw_get_rates": ([vp, _dp, i64], i),
         "pw_solve": ([vp, i], i),
+        "pw_velocity_at": ([vp, _dp, i64, _dp], i),
         "pw_step": ([vp, i, d, i, i64], i),
         "pw_diagnose": ([vp, _dp, i64], i),
         "pw_synchronize": ([vp], i),
@@ -256,6 +258,14 @@ class Solver:
     def solve(self, precision: str = "fp64"):
         _check(self._lib.pw_solve(self._h, PRECISIONS[precision]))
         return self.rates()
+
+    def velocity_at(self, points) -> np.ndarray:
+        """Velocity (FP64) at points (m x 3) from every particle of the
+        context, each with its own radius (getVi, interaction.cpp:453-476)."""
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        u = np.empty_like(pts)
+        _check(self._lib.pw_velocity_at(self._h, pts, len(pts), u))
+        return u
 
     def step(self, scheme: str, dt: float, precision: str = "fp64", nsteps: int = 1) -> None:
         _check(self._lib.pw_step(self._h, SCHEMES[scheme], float(dt), PRECISIONS[precision],

@@ -63,6 +63,10 @@ class __interaction : public __system {
   virtual void getRates(vec *rate) override;
   virtual void getStates(vec *state) override;
   virtual void getIdealRates(vec *rate);
+  //! Velocity induced at point r (3 doubles) by every particle of every
+  //! wake, each with its own radius, added to vi (interaction.cpp:453-476);
+  //! n is unused, as in the reference.  FP64 on the device (pw_velocity_at).
+  virtual void getVi(const vec *r, vec *vi, const size_t &n);
   virtual bool stepOnDevice(int scheme, const double &dt, vec *state) override;
 
   //! The reference hard-codes nu = 1.56e-5 (interaction.cpp:15); settable.

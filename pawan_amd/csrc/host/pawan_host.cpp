@@ -384,6 +384,13 @@ void __interaction::getStates(vec *state) {
 
 void __interaction::getIdealRates(vec *rate) { rate->set_zero(); }
 
+void __interaction::getVi(const vec *r, vec *vi, const size_t & /*n*/) {
+  pushState();
+  double u[3];
+  if (pw_velocity_at(_ctx, r->ptr(), 1, u) != PW_OK) fail("pw_velocity_at");
+  for (int k = 0; k < 3; ++k) vi->data[k] += u[k];
+}
+
 // ---------------------------------------------------------------------------
 // __io (src/io/io.cpp)
 // ---------------------------------------------------------------------------

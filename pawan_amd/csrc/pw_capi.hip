@@ -561,6 +561,30 @@ int pw_get_rates(pw_ctx* c, double* rate, int64_t len) {
   return PW_OK;
 }
 
+int pw_velocity_at(pw_ctx* c, const double* points, int64_t m, double* u) {
+  PW_CTX(c);
+  if (m < 0 || m > (int64_t(1) << 28)) return fail(PW_ERR_ARG, "m must be in [0, 2^28]");
+  if (m == 0) return PW_OK;
+  if (!points || !u) return fail(PW_ERR_ARG, "null points or output");
+  PW_TRY(ensure_built(c));
+  double *d_pts = nullptr, *d_u = nullptr;
+  const size_t len = 3 * static_cast<size_t>(m);
+  if (dalloc(&d_pts, len) != hipSuccess || dalloc(&d_u, len) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipFree(d_pts);
+    return fail(PW_ERR_NOMEM, "point buffers");
+  }
+  hipError_t e = hipMemcpyAsync(d_pts, points, len * sizeof(double), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess)
+    e = pw::launch_point_velocity(c->y, static_cast<int>(c->n), d_pts, static_cast<int>(m), d_u, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(u, d_u, len * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d_pts);
+  (void)hipFree(d_u);
+  if (e != hipSuccess) return fail(PW_ERR_HIP, std::string("pw_velocity_at: ") + hipGetErrorString(e));
+  return PW_OK;
+}
+
 int pw_solve(pw_ctx* c, int precision) {
   PW_CTX(c);
   int kind = 0;

@@ -236,6 +236,8 @@ hipError_t build_near_items(const TreeBuffers& tb, int nleaf, hipStream_t st);
 // then the reduction of split leaves.  Leaf pairs whose bounding spheres
 // are more than sqrt(44) sigma_max apart skip the per-pair near-field test
 // (no pair of them can need it).
+// velocity at m points (AoS doubles, device) from the n particles of state
+hipError_t launch_point_velocity(const Rec64* state, int n, const double* pts, int m, double* u, hipStream_t st);
 hipError_t launch_near_fp32c(const Rec64* sorted, const NodeRec* nodes, const TreeBuffers& tb, int nleaf,
                              double nu, float sigma_max2, int grid, Rate64* rates_s, hipStream_t st);
 

@@ -243,6 +243,58 @@ __device__ __forceinline__ void item_fp64(const PairArgs& pa, const WorkItem& it
 }
 
 // ---------------------------------------------------------------------
+// Point-evaluation velocity (the reference's getVi,
+// src/interaction/interaction.cpp:453-476): the velocity at m arbitrary
+// points from every particle, with each particle's own radius as sigma,
+// in FP64 with the closed far form beyond 90 sigma^2.  One lane per point,
+// sources through LDS in tiles of 256 as in the pair kernels.  A point that
+// coincides with a particle gets no contribution from it (a_s x d = 0).
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(kTB) void k_point_velocity(const Rec64* __restrict__ state, int n,
+                                                        const double* __restrict__ pts, int m,
+                                                        double* __restrict__ u) {
+  __shared__ Rec64 tile[kTB];
+  const int k = blockIdx.x * kTB + static_cast<int>(threadIdx.x);
+  const bool active = k < m;
+  const double px = active ? pts[3 * k] : 0.0, py = active ? pts[3 * k + 1] : 0.0,
+               pz = active ? pts[3 * k + 2] : 0.0;
+  double ux = 0.0, uy = 0.0, uz = 0.0;
+  for (int base = 0; base < n; base += kTB) {
+    const int cnt = min(kTB, n - base);
+    __syncthreads();
+    if (static_cast<int>(threadIdx.x) < cnt) tile[threadIdx.x] = state[base + threadIdx.x];
+    __syncthreads();
+    if (!active) continue;
+    for (int l = 0; l < cnt; ++l) {
+      const Rec64& s = tile[l];
+      const double dx = px - s.x, dy = py - s.y, dz = pz - s.z;
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      if (r2 == 0.0) continue;
+      const double ir = rsqrt_pos(r2);
+      const double ir2 = ir * ir;
+      double q = kQ * (ir * ir2);
+      if (r2 <= kFar64 * s.s2) {
+        const double isg = rsqrt_pos(s.s2);
+        const double isg3 = isg * isg * isg;
+        const double rb = (r2 * ir) * isg;    // rho / sigma
+        const double irb = (s.s2 * isg) * ir;  // sigma / rho
+        const double Z = exp(-0.5 * rb * rb) * isg3 * kK;
+        const double phi = kQ * erf(kSqrtHalf * rb) * isg3;
+        q = (phi * irb - Z) * (irb * irb);
+      }
+      ux += q * (s.ay * dz - s.az * dy);
+      uy += q * (s.az * dx - s.ax * dz);
+      uz += q * (s.ax * dy - s.ay * dx);
+    }
+  }
+  if (active) {
+    u[3 * k] = ux;
+    u[3 * k + 1] = uy;
+    u[3 * k + 2] = uz;
+  }
+}
+
+// ---------------------------------------------------------------------
 // FP32 pair math, compensated FP32 accumulation, FP64 state (PW_FP32C).
 // ---------------------------------------------------------------------
 // Source record in LDS: FP64 position (the displacement is formed in FP64
@@ -915,6 +967,12 @@ const bool P = cfg.persistent;
   if (e != hipSuccess || cfg.parts <= 1) return e;
   hipLaunchKernelGGL(k_reduce_parts, dim3((args.n + 255) / 256), dim3(256), 0, st, args.partial,
                      cfg.parts, args.n, args.rates);
+  return hipGetLastError();
+}
+
+hipError_t launch_point_velocity(const Rec64* state, int n, const double* pts, int m, double* u, hipStream_t st) {
+  if (m <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_point_velocity, dim3((m + kTB - 1) / kTB), dim3(kTB), 0, st, state, n, pts, m, u);
   return hipGetLastError();
 }
 
