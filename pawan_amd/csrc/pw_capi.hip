@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <new>
 #include <tuple>
 #include <string>
@@ -814,6 +815,11 @@ std::map<TuneKey, pw::PairConfig>& tune_cache() {
   static std::map<TuneKey, pw::PairConfig> m;
   return m;
 }
+// guards the cache and its file (contexts may be driven from several threads)
+std::mutex& tune_mutex() {
+  static std::mutex m;
+  return m;
+}
 
 void load_tune_file() {
   static bool loaded = false;
@@ -882,15 +888,24 @@ int pw_autotune(pw_ctx* c, int precision, double* best_ms) {
   PW_TRY(kind_of(precision, &kind));
   if (kind == kKindLearned) return fail(PW_ERR_UNSUPPORTED, "launch configuration applies to the direct-sum kinds");
   PW_TRY(ensure_built(c));
-  load_tune_file();
   const int64_t largest = c->systems.empty() ? 0 : c->systems.front().b - c->systems.front().a;
   const TuneKey key(kind, c->n, c->nsys(), largest);
   auto& k = c->ks[kind];
   const bool was_profiling = c->profiling;
   c->profiling = false;
-  auto hit = tune_cache().find(key);
-  if (hit != tune_cache().end()) {
-    k.cfg = hit->second;
+  bool found = false;
+  pw::PairConfig cached;
+  {
+    std::lock_guard<std::mutex> lock(tune_mutex());
+    load_tune_file();
+    auto hit = tune_cache().find(key);
+    if (hit != tune_cache().end()) {
+      found = true;
+      cached = hit->second;
+    }
+  }
+  if (found) {
+    k.cfg = cached;
     k.tuned = true;
     drop_graphs(c);
     double ms = 0.0;
@@ -950,8 +965,11 @@ int pw_autotune(pw_ctx* c, int precision, double* best_ms) {
   k.cfg = best_cfg;
   k.tuned = true;
   drop_graphs(c);
-  tune_cache()[key] = best_cfg;
-  save_tune_file(key, best_cfg);
+  {
+    std::lock_guard<std::mutex> lock(tune_mutex());
+    tune_cache()[key] = best_cfg;
+    save_tune_file(key, best_cfg);
+  }
   c->profiling = was_profiling;
   c->rates = c->k[0];
   if (best_ms) *best_ms = best;

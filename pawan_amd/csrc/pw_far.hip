@@ -727,13 +727,9 @@ hipError_t launch_far(const Rec64* sorted, const NodeRec* nodes, const int* far_
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || mlp == nullptr) return e;
   const size_t lds = mlp_far_lds_bytes();
-  static bool attr = false;  // LDS beyond the 64 KB default
-  if (!attr) {
-    e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_far_mlp), hipFuncAttributeMaxDynamicSharedMemorySize,
-                            static_cast<int>(lds));
-    if (e != hipSuccess) return e;
-    attr = true;
-  }
+  static std::atomic<unsigned long long> lds_raised{0};  // LDS beyond the 64 KB default
+  e = ensure_lds_limit(reinterpret_cast<const void*>(k_far_mlp), lds, lds_raised);
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_far_mlp, dim3(nleaf), dim3(kMlpThreads), lds, st, a, *mlp);
   return hipGetLastError();
 }
@@ -758,13 +754,9 @@ hipError_t launch_scatter_rates(const Rate64* rates_s, const int* perm, int n, R
 hipError_t launch_mlp_forward(const float* X, int n, float* Y, const MlpWeights& W, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   const int blocks = (n + 127) / 128;
-  static bool attr = false;
-  if (!attr) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_mlp_forward),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(sizeof(MlpLds)));
-    if (e != hipSuccess) return e;
-    attr = true;
-  }
+  static std::atomic<unsigned long long> lds_raised{0};
+  const hipError_t e = ensure_lds_limit(reinterpret_cast<const void*>(k_mlp_forward), sizeof(MlpLds), lds_raised);
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_mlp_forward, dim3(blocks), dim3(256), sizeof(MlpLds), st, X, n, Y, W);
   return hipGetLastError();
 }

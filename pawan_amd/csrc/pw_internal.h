@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include <cstdint>
 
 namespace pw {
@@ -236,6 +238,20 @@ hipError_t build_near_items(const TreeBuffers& tb, int nleaf, hipStream_t st);
 // then the reduction of split leaves.  Leaf pairs whose bounding spheres
 // are more than sqrt(44) sigma_max apart skip the per-pair near-field test
 // (no pair of them can need it).
+// Raise a kernel's dynamic-LDS limit to `bytes` on the current device, once
+// per device (hipFuncSetAttribute applies to the current device); `done`
+// holds one bit per device id (ids >= 64 set the attribute on every call).
+inline hipError_t ensure_lds_limit(const void* kernel, size_t bytes, std::atomic<unsigned long long>& done) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  const unsigned long long bit = dev < 64 ? (1ull << dev) : 0ull;
+  if (bit && (done.load(std::memory_order_acquire) & bit)) return hipSuccess;
+  e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes));
+  if (e == hipSuccess && bit) done.fetch_or(bit, std::memory_order_release);
+  return e;
+}
+
 // velocity at m points (AoS doubles, device) from the n particles of state
 hipError_t launch_point_velocity(const Rec64* state, int n, const double* pts, int m, double* u, hipStream_t st);
 hipError_t launch_near_fp32c(const Rec64* sorted, const NodeRec* nodes, const TreeBuffers& tb, int nleaf,

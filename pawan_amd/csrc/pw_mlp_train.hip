@@ -201,16 +201,12 @@ hipError_t launch_mlp_grad(const float* params, const float* X, const float* Y, 
                            float* gpart, float* lpart, float* grad, float* loss_dev, hipStream_t st) {
   if (n <= 0) return hipErrorInvalidValue;
   const int blocks = train_blocks(n);
-  static bool attr = false;
-  if (!attr) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_mlp_grad),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, sizeof(TrainLds));
-    if (e != hipSuccess) return e;
-    attr = true;
-  }
+  static std::atomic<unsigned long long> lds_raised{0};
+  hipError_t e = ensure_lds_limit(reinterpret_cast<const void*>(k_mlp_grad), sizeof(TrainLds), lds_raised);
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_mlp_grad, dim3(blocks), dim3(256), sizeof(TrainLds), st, params, X, Y, idx, n, gpart,
                      lpart);
-  hipError_t e = hipGetLastError();
+  e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_grad_reduce, dim3((kMlpParams + 255) / 256), dim3(256), 0, st, gpart, lpart, blocks,
                      n, grad, loss_dev);

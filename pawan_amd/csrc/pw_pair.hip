@@ -34,6 +34,7 @@
 
 #include <algorithm>
 #include <map>
+#include <mutex>
 
 #include "pw_internal.h"
 
@@ -996,6 +997,8 @@ namespace {
 // occupancy calculator (registers, LDS), cached per kernel
 int blocks_per_cu(const void* kernel, int threads = kTB) {
   static std::map<const void*, int> cache;
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lock(mu);
   auto it = cache.find(kernel);
   if (it != cache.end()) return it->second;
   int nb = 0;

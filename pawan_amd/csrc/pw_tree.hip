@@ -574,12 +574,10 @@ hipError_t build_lists(const TreeBuffers& tb, const TreeShape& shape, const List
   for (int L = 1; L < shape.nlevel; ++L) max_open = std::max(max_open, shape.level_count[L]);
   const size_t lds = 2 * static_cast<size_t>(max_open) * sizeof(int);
   if (lds > kListLdsMax) return hipErrorInvalidValue;
-  static size_t lds_set = 64 * 1024;  // the default limit
-  if (lds > lds_set) {
-    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_lists),
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kListLdsMax));
+  if (lds > 64 * 1024) {  // beyond the default limit
+    static std::atomic<unsigned long long> lds_raised{0};
+    const hipError_t e = ensure_lds_limit(reinterpret_cast<const void*>(k_lists), kListLdsMax, lds_raised);
     if (e != hipSuccess) return e;
-    lds_set = kListLdsMax;
   }
   hipLaunchKernelGGL(k_lists, dim3(shape.nleaf), dim3(64), lds, st, tb.nodes, shape, lp, tb.near_list,
                      tb.near_cnt, tb.cap_near, tb.far_list, tb.far_cnt, tb.cap_far, max_open, tb.overflow,
