@@ -49,3 +49,13 @@ def test_points_in_the_helical_wake(oracle):
         assert g.velocity_at(np.zeros((0, 3))).shape == (0, 3)
     _say(f"helix 1M, 16 points: {normwise(u, ref):.2e}")
     assert normwise(u, ref) < 1e-12
+
+
+def test_bad_arguments_are_errors():
+    s = W.ring_1k()
+    with capi.Solver(s) as g:
+        lib = capi.load()
+        u = np.zeros((2, 3))
+        assert lib.pw_velocity_at(g._h, np.zeros((2, 3)), -1, u) != 0
+        assert "m must be" in lib.pw_last_error().decode()
+        assert lib.pw_velocity_at(g._h, np.zeros((2, 3)), 2, u) == 0
