@@ -62,6 +62,13 @@ enum {
 /* time integration schemes (the value is the reference's order) */
 enum { PW_EULER = 1, PW_RK3 = 3, PW_RK4 = 4 };
 
+/* Smoothing kernel (the reference selects one header when it is compiled:
+ * interaction_utils.h:22-26, -DGAUSSIAN or -DHIGHORDER). */
+enum {
+  PW_KERNEL_GAUSSIAN = 0,  /* gaussian.h:20-92 (default)                     */
+  PW_KERNEL_HIGHORDER = 1  /* high-order algebraic, highorder.h:20-106       */
+};
+
 /* Library version string. */
 const char *pw_version(void);
 
@@ -102,6 +109,16 @@ int pw_get_nu(pw_ctx *ctx, double *nu);
  * __interaction object per wake); they are advanced together, each wake's
  * targets seeing only its own wake's sources. */
 int pw_set_coupled(pw_ctx *ctx, int coupled);
+
+/* Smoothing kernel of every later solve, step, diagnosis and point
+ * velocity: PW_KERNEL_GAUSSIAN (default) or PW_KERNEL_HIGHORDER (KERNEL,
+ * QSIG, ENST, ENSTF of highorder.h:20-106 in place of gaussian.h's; the
+ * reference makes this choice at compile time, interaction_utils.h:22-26).
+ * The high-order kernel is computed in FP64: pw_solve/pw_step return
+ * PW_ERR_UNSUPPORTED for PW_FP32C and PW_LEARNED while it is selected, and
+ * PW_FP64 and PW_FP64_REF both run it.  May be changed between steps. */
+int pw_set_kernel(pw_ctx *ctx, int kernel);
+int pw_get_kernel(pw_ctx *ctx, int *kernel);
 
 /* Total particles and wakes in the context. */
 int pw_get_size(pw_ctx *ctx, int64_t *num_particles, int *num_wakes);

@@ -49,6 +49,15 @@ def _load() -> C.CDLL:
     lib.or_step_generic.restype = C.c_int
     lib.or_diagnose.argtypes = [C.c_int, ip, dp, dp, dp, d, dp, C.c_int]
     lib.or_velocity_at.argtypes = [i64, dp, dp, dp, i64, dp, dp]
+    lib.or_set_smoothing.argtypes = [C.c_int]
+    lib.or_get_smoothing.restype = C.c_int
+    lib.or_enst_pair.argtypes = [d, d, pd, pd]
+    lib.or_enstf_pair.argtypes = [d, d]
+    lib.or_enstf_pair.restype = d
+    lib.or_enst_self.argtypes = [d]
+    lib.or_enst_self.restype = d
+    lib.or_enstf_self.argtypes = [d]
+    lib.or_enstf_self.restype = d
     return lib
 
 
@@ -82,6 +91,45 @@ def kernel_q128(rho: float, sigma: float):
 
 def qsig(rho: float, sigma: float) -> float:
     return lib().or_qsig(rho, sigma)
+
+
+GAUSSIAN, HIGHORDER = 0, 1
+
+
+class smoothing:
+    """Context manager: every oracle function inside uses the given
+    smoothing kernel (GAUSSIAN = gaussian.h, HIGHORDER = highorder.h; the
+    reference picks one at compile time, interaction_utils.h:22-26)."""
+
+    def __init__(self, kind: int):
+        self.kind = kind
+
+    def __enter__(self):
+        self.prev = lib().or_get_smoothing()
+        lib().or_set_smoothing(self.kind)
+        return self
+
+    def __exit__(self, *exc):
+        lib().or_set_smoothing(self.prev)
+        return False
+
+
+def enst_pair(rho: float, sigma: float):
+    F1, F2 = C.c_double(), C.c_double()
+    lib().or_enst_pair(rho, sigma, F1, F2)
+    return F1.value, F2.value
+
+
+def enstf_pair(rho: float, sigma: float) -> float:
+    return lib().or_enstf_pair(rho, sigma)
+
+
+def enst_self(sigma: float) -> float:
+    return lib().or_enst_self(sigma)
+
+
+def enstf_self(sigma: float) -> float:
+    return lib().or_enstf_self(sigma)
 
 
 def dnrm2(x) -> float:

@@ -90,8 +90,8 @@ static inline double zetasig(double rho, double sigma) {
  * F against the finite difference q'(rho)/rho, Z against its normalisation
  * int Z 4 pi r^2 dr = 1, all three against the __float128 twin where no
  * cancellation occurs, and the small/large rho_bar series limits. */
-void or_kernel(double rho, double sigma, double *q, double *F, double *Z,
-               double *n) {
+static void gaussian_kernel(double rho, double sigma, double *q, double *F,
+                            double *Z, double *n) {
   double rho_bar = rho / sigma;
   double sig3 = sigma * sigma * sigma;
   *Z = zetasig(rho, sigma);
@@ -102,12 +102,112 @@ void or_kernel(double rho, double sigma, double *q, double *F, double *Z,
 }
 
 /* QSIG, gaussian.h:30-37 (used by HELICITY). */
-double or_qsig(double rho, double sigma) {
+static double gaussian_qsig(double rho, double sigma) {
   double rho_bar = rho / sigma;
   double sig3 = sigma * sigma * sigma;
   double Z = zetasig(rho, sigma);
   double phi = 0.25 * M_1_PI * erf(M_SQRT1_2 * rho_bar) / sig3;
   return (phi / rho_bar - Z) / (rho_bar * rho_bar);
+}
+
+/* ---------------------------------------------------------------------- */
+/* High-order algebraic smoothing kernel, src/interaction/highorder.h      */
+/* ---------------------------------------------------------------------- */
+/* The reference chooses highorder.h or gaussian.h when it is compiled
+ * (HIGHORDER / GAUSSIAN, interaction_utils.h:22-26); the oracle takes the
+ * same choice at run time through or_set_smoothing.
+ * Pinned (tests/test_oracle_highorder.py): Z by its normalisation
+ * int Z 4 pi r^2 dr = 1; q against (1/rho^3) int_0^rho Z(r) r^2 dr by
+ * quadrature; F against q'(rho)/rho by finite differences; the PSE kernel
+ * eta by the second-moment condition int eta rho_bar^2 dV = 3 that makes
+ * (2/sigma^2) int (a_s - a_t) eta the Laplacian; ENSTF's pair factor
+ * equals ZETASIG; the enstrophy pair factors tend to the self terms as
+ * rho -> 0. */
+static int smoothing = OR_GAUSSIAN;
+
+void or_set_smoothing(int kind) { smoothing = kind; }
+int or_get_smoothing(void) { return smoothing; }
+
+static inline double cube(double x) { return x * x * x; } /* gsl_pow_3 */
+static inline double pow7(double x) {                      /* gsl_pow_7 */
+  double x2 = x * x, x4 = x2 * x2;
+  return x * x2 * x4;
+}
+
+/* ZETASIG, highorder.h:20-24. */
+static inline double ho_zetasig(double rho, double sigma) {
+  double rho_bar = rho / sigma;
+  return 1.875 * M_1_PI / pow(rho_bar * rho_bar + 1.0, 3.5) / pow(sigma, 3);
+}
+
+/* QSIG, highorder.h:31-39. */
+static double ho_qsig(double rho, double sigma) {
+  double rho_bar = rho / sigma;
+  double rho_bar2 = rho_bar * rho_bar;
+  double rho3 = rho * rho * rho;
+  return 0.25 * M_1_PI * rho_bar2 * rho_bar * (rho_bar2 + 2.5) /
+         pow(rho_bar2 + 1.0, 2.5) / rho3;
+}
+
+/* ETASIG, highorder.h:41-44. */
+static inline double ho_etasig(double rho, double sigma) {
+  double rho_bar = rho / sigma;
+  return 13.125 * M_1_PI / pow(rho_bar * rho_bar + 1.0, 4.5) / pow(sigma, 3);
+}
+
+/* KERNEL, highorder.h:56-67. */
+static void ho_kernel(double rho, double sigma, double *q, double *F,
+                      double *Z, double *n) {
+  *Z = ho_zetasig(rho, sigma);
+  *q = ho_qsig(rho, sigma);
+  *n = ho_etasig(rho, sigma);
+  *F = (*Z - 3.0 * *q) / (rho * rho);
+}
+
+void or_kernel(double rho, double sigma, double *q, double *F, double *Z,
+               double *n) {
+  if (smoothing == OR_HIGHORDER)
+    ho_kernel(rho, sigma, q, F, Z, n);
+  else
+    gaussian_kernel(rho, sigma, q, F, Z, n);
+}
+
+double or_qsig(double rho, double sigma) {
+  return smoothing == OR_HIGHORDER ? ho_qsig(rho, sigma)
+                                   : gaussian_qsig(rho, sigma);
+}
+
+/* ENST(rho, sigma, F1, F2), highorder.h:77-87; zero for the Gaussian
+ * (gaussian.h:69-75). */
+void or_enst_pair(double rho, double sigma, double *F1, double *F2) {
+  if (smoothing != OR_HIGHORDER) {
+    *F1 = 0.0;
+    *F2 = 0.0;
+    return;
+  }
+  double rho2 = rho * rho;
+  double sig2 = sigma * sigma;
+  double factor = (M_1_PI / 8.0) / sqrt(pow7(rho2 + sig2));
+  *F1 = factor * (2.0 * rho2 * rho2 + 7.0 * sig2 * rho2 + 20.0 * sig2 * sig2);
+  *F2 = factor * (-6.0 * rho2 * rho2 - 27.0 * sig2 * rho2 - 21.0 * sig2 * sig2) /
+        (rho2 + sig2);
+}
+
+/* ENSTF(rho, sigma, F1), highorder.h:89-96; zero for the Gaussian
+ * (gaussian.h:77-81). */
+double or_enstf_pair(double rho, double sigma) {
+  if (smoothing != OR_HIGHORDER) return 0.0;
+  double rho2 = rho * rho;
+  double sig2 = sigma * sigma;
+  return (15.0 * M_1_PI * sig2 * sig2 / 8.0) / sqrt(pow7(rho2 + sig2));
+}
+
+/* ENST(sigma) and ENSTF(sigma), highorder.h:98-106 and gaussian.h:87-92. */
+double or_enst_self(double sigma) {
+  return smoothing == OR_HIGHORDER ? 5.0 * M_1_PI / 2.0 / cube(sigma) : 0.0;
+}
+double or_enstf_self(double sigma) {
+  return smoothing == OR_HIGHORDER ? 15.0 * M_1_PI / 8.0 / cube(sigma) : 0.0;
 }
 
 /* ---------------------------------------------------------------------- */
@@ -533,7 +633,55 @@ static inline double kef_self(double s, const double *a) {
   return 3 * M_1_PI * ddot3(a, a) / (s * 16.0);
 }
 
-enum { Q_H = 0, Q_KE = 1, Q_KEF = 2 };
+/* ENSTROPHY / ENSTROPHYF pair, interaction_utils.h:141-202. */
+static inline double enst_pair(double s1, double s2, const double *x1,
+                               const double *x2, const double *a1,
+                               const double *a2) {
+  double x12[3] = {x2[0] - x1[0], x2[1] - x1[1], x2[2] - x1[2]};
+  double rho = or_dnrm2(x12);
+  double sigma = sqrt(0.5 * (s1 * s1 + s2 * s2));
+  double F1, F2;
+  or_enst_pair(rho, sigma, &F1, &F2);
+  double a1a2 = ddot3(a1, a2);
+  double a1x12 = ddot3(a1, x12);
+  double a2x12 = ddot3(a2, x12);
+  return F1 * a1a2 + F2 * a1x12 * a2x12;
+}
+static inline double enstf_pair(double s1, double s2, const double *x1,
+                                const double *x2, const double *a1,
+                                const double *a2) {
+  double x12[3] = {x2[0] - x1[0], x2[1] - x1[1], x2[2] - x1[2]};
+  double rho = or_dnrm2(x12);
+  double sigma = sqrt(0.5 * (s1 * s1 + s2 * s2));
+  return or_enstf_pair(rho, sigma) * ddot3(a1, a2);
+}
+
+enum { Q_H = 0, Q_KE = 1, Q_KEF = 2, Q_E = 3, Q_EF = 4 };
+
+/* One pair term of quadratic diagnostic `which` (unscaled). */
+static inline double quad_pair(int which, double s1, double s2,
+                               const double *x1, const double *x2,
+                               const double *a1, const double *a2) {
+  switch (which) {
+    case Q_H: return helicity_pair(s1, s2, x1, x2, a1, a2);
+    case Q_KE: return ke_pair(s1, s2, x1, x2, a1, a2);
+    case Q_KEF: return kef_pair(s1, s2, x1, x2, a1, a2);
+    case Q_E: return enst_pair(s1, s2, x1, x2, a1, a2);
+    default: return enstf_pair(s1, s2, x1, x2, a1, a2);
+  }
+}
+
+/* Self term of quadratic diagnostic `which` (ENSTROPHY(s, a) etc.,
+ * interaction_utils.h:112-130, 249-267); helicity has none. */
+static inline double quad_self(int which, double s, const double *a) {
+  switch (which) {
+    case Q_KE: return ke_self(s, a);
+    case Q_KEF: return kef_self(s, a);
+    case Q_E: return or_enst_self(s) * ddot3(a, a);
+    case Q_EF: return or_enstf_self(s) * ddot3(a, a);
+    default: return 0.0;
+  }
+}
 
 /* Sum over J of one row I of a quadratic diagnostic (the J loop of
  * interaction.cpp:608-718 / parallel.cpp:196-317). */
@@ -544,14 +692,7 @@ static double quad_row(int which, int64_t I, int64_t j0, int64_t j1,
   const double *rI = pos + 3 * I, *aI = alpha + 3 * I;
   for (int64_t J = j0; J < j1; ++J) {
     const double *rJ = pos + 3 * J, *aJ = alpha + 3 * J;
-    double v;
-    if (which == Q_H)
-      v = helicity_pair(radius[I], radius[J], rI, rJ, aI, aJ);
-    else if (which == Q_KE)
-      v = ke_pair(radius[I], radius[J], rI, rJ, aI, aJ);
-    else
-      v = kef_pair(radius[I], radius[J], rI, rJ, aI, aJ);
-    acc += 2.0 * v;
+    acc += 2.0 * quad_pair(which, radius[I], radius[J], rI, rJ, aI, aJ);
   }
   return acc;
 }
@@ -568,19 +709,11 @@ static double quad_block(int which, int64_t i0, int64_t i1, int64_t j0,
   if (!use_omp) {
     for (int64_t I = i0; I < i1; ++I) {
       const double *rI = pos + 3 * I, *aI = alpha + 3 * I;
-      if (self) {
-        if (which == Q_KE) tot += ke_self(radius[I], aI);
-        if (which == Q_KEF) tot += kef_self(radius[I], aI);
-      }
+      if (self && which != Q_H) tot += quad_self(which, radius[I], aI);
       int64_t js = self ? I + 1 : j0, je = self ? i1 : j1;
       for (int64_t J = js; J < je; ++J) {
         const double *rJ = pos + 3 * J, *aJ = alpha + 3 * J;
-        if (which == Q_H)
-          tot += 2.0 * helicity_pair(radius[I], radius[J], rI, rJ, aI, aJ);
-        else if (which == Q_KE)
-          tot += 2.0 * ke_pair(radius[I], radius[J], rI, rJ, aI, aJ);
-        else
-          tot += 2.0 * kef_pair(radius[I], radius[J], rI, rJ, aI, aJ);
+        tot += 2.0 * quad_pair(which, radius[I], radius[J], rI, rJ, aI, aJ);
       }
     }
     return tot;
@@ -595,10 +728,7 @@ static double quad_block(int which, int64_t i0, int64_t i1, int64_t j0,
   }
   for (int64_t k = 0; k < n; ++k) {
     int64_t I = i0 + k;
-    if (self) {
-      if (which == Q_KE) tot += ke_self(radius[I], alpha + 3 * I);
-      if (which == Q_KEF) tot += kef_self(radius[I], alpha + 3 * I);
-    }
+    if (self && which != Q_H) tot += quad_self(which, radius[I], alpha + 3 * I);
     tot += row[k];
   }
   free(row);
@@ -647,9 +777,13 @@ void or_diagnose(int nw, const int64_t *off, const double *pos,
     }
   }
   double H = quad_total(Q_H, nw, off, pos, alpha, radius, use_omp);
-  /* Gaussian ENST/ENSTF are identically zero (gaussian.h:69-92), so the
-   * enstrophy sums (interaction.cpp:101-116) are zero. */
+  /* calculateEnstrophy(F), interaction.cpp:528-606: identically zero for
+   * the Gaussian kernel (gaussian.h:69-92) */
   double E = 0.0, EF = 0.0;
+  if (smoothing == OR_HIGHORDER) {
+    E = quad_total(Q_E, nw, off, pos, alpha, radius, use_omp);
+    EF = quad_total(Q_EF, nw, off, pos, alpha, radius, use_omp);
+  }
   double KE = quad_total(Q_KE, nw, off, pos, alpha, radius, use_omp);
   double KEF = quad_total(Q_KEF, nw, off, pos, alpha, radius, use_omp);
   /* Centroid, interaction.cpp:135-157: the last wake with a non-zero

@@ -31,6 +31,20 @@ double or_dnrm2(const double *x);
 void or_kernel(double rho, double sigma, double *q, double *F, double *Z,
                double *n);
 double or_qsig(double rho, double sigma);
+
+/* Smoothing kernel of every or_* function below (except the __float128
+ * twins, which are Gaussian only): gaussian.h or highorder.h, the choice the
+ * reference makes at compile time (interaction_utils.h:22-26).  Global to
+ * the process; OR_GAUSSIAN at load. */
+enum { OR_GAUSSIAN = 0, OR_HIGHORDER = 1 };
+void or_set_smoothing(int kind);
+int or_get_smoothing(void);
+/* Enstrophy factors: ENST(rho, sigma, F1, F2), ENSTF(rho, sigma, F1),
+ * ENST(sigma), ENSTF(sigma) (highorder.h:77-106; zero for the Gaussian). */
+void or_enst_pair(double rho, double sigma, double *F1, double *F2);
+double or_enstf_pair(double rho, double sigma);
+double or_enst_self(double sigma);
+double or_enstf_self(double sigma);
 void or_kernel_q128(double rho, double sigma, double *q, double *F,
                     double *Z);
 

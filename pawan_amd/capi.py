@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(PKG, "lib", "libpawan_gpu.so")
 PW_OK, PW_ERR_ARG, PW_ERR_HIP, PW_ERR_STATE, PW_ERR_NONFINITE, PW_ERR_UNSUPPORTED, PW_ERR_NOMEM = range(7)
 PW_FP64, PW_FP32C, PW_LEARNED, PW_FP64_REF = 0, 1, 2, 3
 PW_EULER, PW_RK3, PW_RK4 = 1, 3, 4
+PW_KERNEL_GAUSSIAN, PW_KERNEL_HIGHORDER = 0, 1
 
 PRECISIONS = {"fp64": PW_FP64, "fp64_ref": PW_FP64_REF, "fp32c": PW_FP32C, "learned": PW_LEARNED}
 SCHEMES = {"euler": PW_EULER, "rk3": PW_RK3, "rk4": PW_RK4}
@@ -35,7 +36,7 @@ EXPORTS = [
     "pw_ff_build", "pw_ff_info", "pw_ff_export", "pw_ff_sample", "pw_ff_clear_samples",
     "pw_ff_init_weights", "pw_ff_set_weights", "pw_ff_get_weights", "pw_ff_train",
     "pw_ff_mlp_forward", "pw_ff_mlp_grad", "pw_ff_set_conserve", "pw_ff_get_conserve",
-    "pw_velocity_at",
+    "pw_velocity_at", "pw_set_kernel", "pw_get_kernel",
 ]
 
 # pw_ff_export selectors
@@ -81,6 +82,8 @@ def load(path: str | None = None) -> C.CDLL:
         "pw_set_nu": ([vp, d], i),
         "pw_get_nu": ([vp, C.POINTER(C.c_double)], i),
         "pw_set_coupled": ([vp, i], i),
+        "pw_set_kernel": ([vp, i], i),
+        "pw_get_kernel": ([vp, C.POINTER(C.c_int)], i),
         "pw_get_size": ([vp, C.POINTER(C.c_int64), C.POINTER(C.c_int)], i),
         "pw_set_wake_state": ([vp, i, _dp, _dp], i),
         "pw_get_wake_state": ([vp, i, _dp, _dp], i),
@@ -202,6 +205,16 @@ class Solver:
 
     def set_coupled(self, coupled: bool) -> None:
         _check(self._lib.pw_set_coupled(self._h, int(bool(coupled))))
+
+    def set_kernel(self, kernel: int) -> None:
+        """PW_KERNEL_GAUSSIAN or PW_KERNEL_HIGHORDER (FP64 only)."""
+        _check(self._lib.pw_set_kernel(self._h, int(kernel)))
+
+    @property
+    def kernel(self) -> int:
+        v = C.c_int()
+        _check(self._lib.pw_get_kernel(self._h, C.byref(v)))
+        return v.value
 
     @property
     def n(self) -> int:

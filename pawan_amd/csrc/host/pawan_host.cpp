@@ -245,6 +245,10 @@ void __interaction::addWake(__wake *W) {
 void __interaction::ensureDevice() {
   if (_ctx) return;
   if (pw_create(&_ctx, 0) != PW_OK) fail("pw_create");
+#if defined(HIGHORDER) && HIGHORDER
+  // the reference's compile-time kernel choice (interaction_utils.h:22-26)
+  if (pw_set_kernel(_ctx, PW_KERNEL_HIGHORDER) != PW_OK) fail("pw_set_kernel");
+#endif
   for (__wake *w : _W) {
     if (pw_add_wake(_ctx, static_cast<int64_t>(w->_numParticles),
                     static_cast<int64_t>(w->_maxnumParticles), w->_position->ptr(),

@@ -123,7 +123,7 @@ int ensure_built(pw_ctx* c) {
       dalloc(&c->pair_items, c->h_pair_items.size()) != hipSuccess ||
       dalloc(&c->lin_items, c->h_lin_items.size()) != hipSuccess ||
       dalloc(&c->d_lin, 11 * c->h_lin_items.size()) != hipSuccess ||
-      dalloc(&c->d_quad, 3 * c->h_pair_items.size()) != hipSuccess ||
+      dalloc(&c->d_quad, pw::kDiagQuad * c->h_pair_items.size()) != hipSuccess ||
       dalloc(&c->d_flag, 1) != hipSuccess || dalloc(&c->d_queue, 1) != hipSuccess) {
     release(c);
     c->y = c->ys = nullptr;
@@ -174,6 +174,14 @@ int kind_of(int precision, int* kind) {
     default:
       return fail(PW_ERR_ARG, "unknown precision " + std::to_string(precision));
   }
+}
+
+// The high-order algebraic kernel is built in FP64 only.
+int check_smoothing(const pw_ctx* c, int kind) {
+  if (c->smoothing == PW_KERNEL_HIGHORDER && kind != pw::kPairFP64 && kind != pw::kPairFP64Exact)
+    return fail(PW_ERR_UNSUPPORTED, "the high-order algebraic kernel is computed in FP64 only "
+                                    "(PW_FP64 or PW_FP64_REF)");
+  return PW_OK;
 }
 
 // Work items for a launch split into `parts` source ranges per target
@@ -257,13 +265,15 @@ int rhs(pw_ctx* c, int kind, const Rec64* state, Rate64* out) {
   args.partial = c->partial;
   args.queue = c->d_queue;
   args.nu = c->nu;
-  const int grid = c->num_cus * pw::pair_kernel_waves(kind, k.cfg);
+  // the high-order kernel replaces the Gaussian FP64 kinds (pw_set_kernel)
+  const int launch_kind = c->smoothing == PW_KERNEL_HIGHORDER ? pw::kPairFP64HO : kind;
+  const int grid = c->num_cus * pw::pair_kernel_waves(launch_kind, k.cfg);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c->profiling) {
     PW_TRY(pwi::prof_begin(c, &e0, &e1));
     PW_HIP(hipEventRecord(e0, c->stream));
   }
-  PW_HIP(pw::launch_pair(kind, k.cfg, args, grid, c->stream));
+  PW_HIP(pw::launch_pair(launch_kind, k.cfg, args, grid, c->stream));
   if (c->profiling) {
     PW_HIP(hipEventRecord(e1, c->stream));
     c->ev_used.emplace_back(e0, e1);
@@ -453,6 +463,25 @@ int pw_set_coupled(pw_ctx* c, int coupled) {
   return PW_OK;
 }
 
+int pw_set_kernel(pw_ctx* c, int kernel) {
+  PW_CTX(c);
+  if (kernel != PW_KERNEL_GAUSSIAN && kernel != PW_KERNEL_HIGHORDER)
+    return fail(PW_ERR_ARG, "kernel must be PW_KERNEL_GAUSSIAN or PW_KERNEL_HIGHORDER");
+  if (kernel != c->smoothing) {
+    PW_HIP(hipStreamSynchronize(c->stream));
+    drop_graphs(c);  // captured steps hold the other kernel's launches
+    c->smoothing = kernel;
+  }
+  return PW_OK;
+}
+
+int pw_get_kernel(pw_ctx* c, int* kernel) {
+  PW_CTX(c);
+  if (!kernel) return fail(PW_ERR_ARG, "null output");
+  *kernel = c->smoothing;
+  return PW_OK;
+}
+
 int pw_get_size(pw_ctx* c, int64_t* num_particles, int* num_wakes) {
   PW_CTX(c);
   if (num_particles) *num_particles = c->n;
@@ -577,7 +606,8 @@ int pw_velocity_at(pw_ctx* c, const double* points, int64_t m, double* u) {
   }
   hipError_t e = hipMemcpyAsync(d_pts, points, len * sizeof(double), hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess)
-    e = pw::launch_point_velocity(c->y, static_cast<int>(c->n), d_pts, static_cast<int>(m), d_u, c->stream);
+    e = pw::launch_point_velocity(c->y, static_cast<int>(c->n), d_pts, static_cast<int>(m), d_u,
+                                  c->smoothing == PW_KERNEL_HIGHORDER, c->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(u, d_u, len * sizeof(double), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   (void)hipFree(d_pts);
@@ -590,6 +620,7 @@ int pw_solve(pw_ctx* c, int precision) {
   PW_CTX(c);
   int kind = 0;
   PW_TRY(kind_of(precision, &kind));
+  PW_TRY(check_smoothing(c, kind));
   PW_TRY(ensure_built(c));
   PW_HIP(pw::launch_fill_int(c->d_flag, 1, 0, c->stream));
   PW_TRY(rhs(c, kind, c->y, c->k[0]));
@@ -676,6 +707,7 @@ int pw_step(pw_ctx* c, int scheme, double dt, int precision, int64_t nsteps) {
   PW_CTX(c);
   int kind = 0;
   PW_TRY(kind_of(precision, &kind));
+  PW_TRY(check_smoothing(c, kind));
   if (scheme != PW_EULER && scheme != PW_RK3 && scheme != PW_RK4)
     return fail(PW_ERR_ARG, "scheme must be PW_EULER, PW_RK3 or PW_RK4");
   if (!std::isfinite(dt)) return fail(PW_ERR_ARG, "dt must be finite");
@@ -704,8 +736,10 @@ int pw_diagnose(pw_ctx* c, double* out, int64_t out_len) {
   if (out_len < 15 * static_cast<int64_t>(ns))
     return fail(PW_ERR_ARG, "output needs 15 doubles per system");
   PW_HIP(pw::launch_diag_linear(c->y, c->lin_items, c->nlin, c->d_lin, c->stream));
-  PW_HIP(pw::launch_diag_pairs(c->y, c->d_radius, c->pair_items, c->npair, c->d_quad, c->stream));
-  std::vector<double> lin(11 * static_cast<size_t>(c->nlin)), quad(3 * static_cast<size_t>(c->npair));
+  const bool ho = c->smoothing == PW_KERNEL_HIGHORDER;
+  constexpr int Q = pw::kDiagQuad;
+  PW_HIP(pw::launch_diag_pairs(c->y, c->d_radius, c->pair_items, c->npair, c->d_quad, ho, c->stream));
+  std::vector<double> lin(11 * static_cast<size_t>(c->nlin)), quad(Q * static_cast<size_t>(c->npair));
   PW_HIP(hipMemcpyAsync(lin.data(), c->d_lin, lin.size() * sizeof(double), hipMemcpyDeviceToHost,
                         c->stream));
   PW_HIP(hipMemcpyAsync(quad.data(), c->d_quad, quad.size() * sizeof(double),
@@ -718,9 +752,9 @@ int pw_diagnose(pw_ctx* c, double* out, int64_t out_len) {
     for (int m = 0; m < 11; ++m) wl[11 * c->lin_wake[b] + m] += lin[11 * b + m];
   // per-system quadratic sums in item order (items of one system are
   // contiguous and in target order)
-  std::vector<double> sq(3 * static_cast<size_t>(ns), 0.0);
+  std::vector<double> sq(Q * static_cast<size_t>(ns), 0.0);
   for (int b = 0; b < c->npair; ++b)
-    for (int m = 0; m < 3; ++m) sq[3 * c->pair_sys[b] + m] += quad[3 * b + m];
+    for (int m = 0; m < Q; ++m) sq[Q * c->pair_sys[b] + m] += quad[Q * b + m];
   for (int s = 0; s < ns; ++s) {
     double* o = out + 15 * s;
     for (int m = 0; m < 9; ++m) o[m] = 0.0;
@@ -736,11 +770,11 @@ int pw_diagnose(pw_ctx* c, double* out, int64_t out_len) {
       }
       if (v[10] != 0.0) c->zc[s] = v[9] / v[10];  // interaction.cpp:153-155
     }
-    o[9] = sq[3 * s + 0];   // helicity
-    o[10] = 0.0;            // enstrophy: ENST == 0 for the Gaussian kernel
-    o[11] = 0.0;            // div-free enstrophy: ENSTF == 0
-    o[12] = sq[3 * s + 1];  // kinetic energy
-    o[13] = sq[3 * s + 2];  // div-free kinetic energy
+    o[9] = sq[Q * s + 0];   // helicity
+    o[10] = sq[Q * s + 3];  // enstrophy (ENST == 0 for the Gaussian kernel)
+    o[11] = sq[Q * s + 4];  // div-free enstrophy (ENSTF == 0 likewise)
+    o[12] = sq[Q * s + 1];  // kinetic energy
+    o[13] = sq[Q * s + 2];  // div-free kinetic energy
     o[14] = c->zc[s];
   }
   return PW_OK;
@@ -887,6 +921,8 @@ int pw_autotune(pw_ctx* c, int precision, double* best_ms) {
   int kind = 0;
   PW_TRY(kind_of(precision, &kind));
   if (kind == kKindLearned) return fail(PW_ERR_UNSUPPORTED, "launch configuration applies to the direct-sum kinds");
+  if (c->smoothing != PW_KERNEL_GAUSSIAN)
+    return fail(PW_ERR_UNSUPPORTED, "the autotuner times the Gaussian kernels");
   PW_TRY(ensure_built(c));
   const int64_t largest = c->systems.empty() ? 0 : c->systems.front().b - c->systems.front().a;
   const TuneKey key(kind, c->n, c->nsys(), largest);

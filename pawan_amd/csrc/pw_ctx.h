@@ -72,6 +72,7 @@ struct pw_ctx {
   hipStream_t stream = nullptr;
   double nu = 1.56e-5;  // interaction.cpp:15
   bool coupled = true;
+  int smoothing = PW_KERNEL_GAUSSIAN;  // PW_KERNEL_*
 
   // host description of the wakes
   std::vector<int64_t> wn, wmax, woff{0};

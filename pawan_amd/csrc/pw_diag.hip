@@ -6,9 +6,10 @@
 //     angular impulse sum(x x (x x a))/3 and the centroid sums
 //     |a| z (x^2+y^2) and |a| (x^2+y^2)  (interaction.cpp:489-527, 135-157);
 //   * O(N^2), over every unordered pair counted twice plus self terms:
-//     helicity, kinetic energy and its div-free variant
-//     (interaction_utils.h:214-331; enstrophy is identically zero for the
-//     Gaussian kernel, gaussian.h:69-92).
+//     helicity, kinetic energy and its div-free variant, enstrophy and its
+//     div-free variant (interaction_utils.h:107-331; the enstrophies are
+//     identically zero for the Gaussian kernel, gaussian.h:69-92, and
+//     nonzero for the high-order algebraic one, highorder.h:77-106).
 // Both are computed here as per-block partial sums in a fixed tree order;
 // the host adds the partials in block order, so results are deterministic.
 // Pair terms are symmetric in the two particles, so the ordered sum over
@@ -89,19 +90,33 @@ __device__ __forceinline__ double qsig(double rho, double sigma) {
   return (phi / rho_bar - Z) / (rho_bar * rho_bar);
 }
 
-// Per-target rows of helicity, KE, KEF over one work item's sources; the
-// block total goes to out[3 * b].  radius is needed for the self terms.
+// QSIG of the high-order algebraic kernel (highorder.h:31-39), same
+// operation order, pow(x, 2.5) as x^2 sqrt(x).
+__device__ __forceinline__ double qsig_ho(double rho, double sigma) {
+#pragma clang fp contract(off)
+  const double rho_bar = rho / sigma;
+  const double rho_bar2 = rho_bar * rho_bar;
+  const double rho3 = rho * rho * rho;
+  const double b = rho_bar2 + 1.0;
+  return 0.25 * k1Pi * rho_bar2 * rho_bar * (rho_bar2 + 2.5) / (b * b * sqrt(b)) / rho3;
+}
+
+// Per-target rows of helicity, KE, KEF (and, for the high-order kernel,
+// enstrophy and its div-free variant) over one work item's sources; the
+// block total goes to out[kDiagQuad * b].  radius is needed for the self
+// terms.
+template <bool HighOrder>
 __global__ __launch_bounds__(kTB) void k_diag_pairs(
     const Rec64* __restrict__ y, const double* __restrict__ radius,
     const WorkItem* __restrict__ items, double* __restrict__ out) {
 #pragma clang fp contract(off)
   __shared__ Rec64 tile[kTB];
-  __shared__ double red[(kTB / 64) * 3];
+  __shared__ double red[(kTB / 64) * kDiagQuad];
   const WorkItem it = items[blockIdx.x];
   const int t = it.t0 + static_cast<int>(threadIdx.x);
   const bool active = t < it.t1;
   const Rec64 me = y[active ? t : it.t0];
-  double h = 0.0, ke = 0.0, kef = 0.0;
+  double h = 0.0, ke = 0.0, kef = 0.0, en = 0.0, enf = 0.0;
   for (int base = it.s0; base < it.s1; base += kTB) {
     const int cnt = min(kTB, it.s1 - base);
     __syncthreads();
@@ -120,7 +135,7 @@ __global__ __launch_bounds__(kTB) void k_diag_pairs(
         // HELICITY (interaction_utils.h:214-242)
         const double rho = sqrt(rho2);
         const double sigma = sqrt(0.5 * (me.s2 + s.s2));
-        const double q = qsig(rho, sigma);
+        const double q = HighOrder ? qsig_ho(rho, sigma) : qsig(rho, sigma);
         const double c0 = me.ay * s.az - me.az * s.ay;
         const double c1 = me.az * s.ax - me.ax * s.az;
         const double c2 = me.ax * s.ay - me.ay * s.ax;
@@ -145,6 +160,18 @@ __global__ __launch_bounds__(kTB) void k_diag_pairs(
         const double p15 = b * sqrt(b);  // pow(b, 1.5)
         ke += ((rho2 + 2.0 * sigma2) * a1a2 + (x12a1 * x12a2)) / p15 / 16.0 / kPi;
         kef += (rho2 + 1.5 * sigma2) * a1a2 / p15 / 8.0 / kPi;
+        if (HighOrder) {
+          // ENSTROPHY / ENSTROPHYF pairs (interaction_utils.h:141-202) with
+          // ENST / ENSTF (highorder.h:77-96); sqrt(b^7) as b^3 sqrt(b)
+          const double sig2 = sigma * sigma;
+          const double bb = rho2 + sig2;
+          const double s7 = bb * bb * bb * sqrt(bb);
+          const double factor = (k1Pi / 8.0) / s7;
+          const double F1 = factor * (2.0 * rho2 * rho2 + 7.0 * sig2 * rho2 + 20.0 * sig2 * sig2);
+          const double F2 = factor * (-6.0 * rho2 * rho2 - 27.0 * sig2 * rho2 - 21.0 * sig2 * sig2) / bb;
+          en += F1 * a1a2 + F2 * x12a1 * x12a2;
+          enf += (15.0 * k1Pi * sig2 * sig2 / 8.0) / s7 * a1a2;
+        }
       }
     }
   }
@@ -157,14 +184,17 @@ __global__ __launch_bounds__(kTB) void k_diag_pairs(
     a2 += me.az * me.az;
     ke += k1Pi * a2 / sg / 8.0;
     kef += 3 * k1Pi * a2 / (sg * 16.0);
+    if (HighOrder) {
+      // ENST(sigma), ENSTF(sigma) (highorder.h:98-106)
+      en += 5.0 * k1Pi / 2.0 / (sg * sg * sg) * a2;
+      enf += 15.0 * k1Pi / 8.0 / (sg * sg * sg) * a2;
+    }
   }
-  double v[3] = {active ? h : 0.0, active ? ke : 0.0, active ? kef : 0.0};
-  block_sum<3>(v, red);
-  if (threadIdx.x == 0) {
-    out[3 * blockIdx.x + 0] = v[0];
-    out[3 * blockIdx.x + 1] = v[1];
-    out[3 * blockIdx.x + 2] = v[2];
-  }
+  double v[kDiagQuad] = {active ? h : 0.0, active ? ke : 0.0, active ? kef : 0.0,
+                         active ? en : 0.0, active ? enf : 0.0};
+  block_sum<kDiagQuad>(v, red);
+  if (threadIdx.x == 0)
+    for (int m = 0; m < kDiagQuad; ++m) out[kDiagQuad * blockIdx.x + m] = v[m];
 }
 
 }  // namespace
@@ -178,9 +208,12 @@ hipError_t launch_diag_linear(const Rec64* y, const WorkItem* items,
 
 hipError_t launch_diag_pairs(const Rec64* y, const double* radius,
                              const WorkItem* items, int nitems, double* out,
-                             hipStream_t st) {
+                             bool high_order, hipStream_t st) {
   if (nitems <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_diag_pairs, dim3(nitems), dim3(kTB), 0, st, y, radius, items, out);
+  if (high_order)
+    hipLaunchKernelGGL(k_diag_pairs<true>, dim3(nitems), dim3(kTB), 0, st, y, radius, items, out);
+  else
+    hipLaunchKernelGGL(k_diag_pairs<false>, dim3(nitems), dim3(kTB), 0, st, y, radius, items, out);
   return hipGetLastError();
 }
 

@@ -60,6 +60,9 @@ enum PairKind : int {
   kPairFP64 = 1,       // same, closed far-field form where exp/erf terms
                        // are below FP64 rounding (rho^2 > 90 sigma^2)
   kPairFP32C = 2,      // FP32 pair math, compensated FP32 accumulation
+  kPairFP64HO = 4,     // FP64, high-order algebraic kernel (highorder.h);
+                       // launched in place of kinds 0 and 1 when a context
+                       // uses that kernel (3 is the tree path's kind)
 };
 
 // Pair kernel of `kind` over args.items; for parts > 1 also the reduction
@@ -106,11 +109,13 @@ hipError_t launch_fill_int(int* p, int n, int value, hipStream_t st);
 // [a (3), x x a (3), x x (x x a) (3), |a| z (x^2+y^2), |a| (x^2+y^2)].
 hipError_t launch_diag_linear(const Rec64* y, const WorkItem* items,
                               int nitems, double* out, hipStream_t st);
-// Quadratic sums over the pair work items: out[3 * b + m] gets the block's
-// total of [helicity, KE, KEF] rows (self terms included).
+// Quadratic sums over the pair work items: out[kDiagQuad * b + m] gets the
+// block's total of [helicity, KE, KEF, enstrophy, div-free enstrophy] rows
+// (self terms included; the enstrophies are zero for the Gaussian kernel).
+constexpr int kDiagQuad = 5;
 hipError_t launch_diag_pairs(const Rec64* y, const double* radius,
                              const WorkItem* items, int nitems, double* out,
-                             hipStream_t st);
+                             bool high_order, hipStream_t st);
 
 // ---------------------------------------------------------------------
 // Cluster tree of the learned far field.
@@ -253,7 +258,8 @@ inline hipError_t ensure_lds_limit(const void* kernel, size_t bytes, std::atomic
 }
 
 // velocity at m points (AoS doubles, device) from the n particles of state
-hipError_t launch_point_velocity(const Rec64* state, int n, const double* pts, int m, double* u, hipStream_t st);
+hipError_t launch_point_velocity(const Rec64* state, int n, const double* pts, int m, double* u,
+                                 bool high_order, hipStream_t st);
 hipError_t launch_near_fp32c(const Rec64* sorted, const NodeRec* nodes, const TreeBuffers& tb, int nleaf,
                              double nu, float sigma_max2, int grid, Rate64* rates_s, hipStream_t st);
 

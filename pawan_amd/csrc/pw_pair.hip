@@ -244,13 +244,113 @@ __device__ __forceinline__ void item_fp64(const PairArgs& pa, const WorkItem& it
 }
 
 // ---------------------------------------------------------------------
+// FP64 with the high-order algebraic kernel (src/interaction/highorder.h,
+// the reference's compile-time alternative to the Gaussian).  With
+// b = rho^2 + sigma^2 its KERNEL (highorder.h:56-67) is, in closed form,
+//   q   = (1/4pi) (rho^2 + 5/2 sigma^2) b^-5/2,
+//   F   = (Z - 3q)/rho^2 = -(3/8pi) (2 rho^2 + 7 sigma^2) b^-7/2,
+//   eta = (105/8pi) sigma^6 b^-9/2,
+// which the reference evaluates through rho_bar = rho/sigma and pow(); here
+// one reciprocal square root of b gives all three, F without the
+// cancellation of Z - 3q at small rho.  The kernel has no far cutoff (it
+// decays algebraically), so every pair evaluates it.  The sums take the
+// FP64 Gaussian kernel's per-(wave, source) choice: a source within
+// rho^2 <= 90 sigma^2 of some lane's target adds its stretching
+// q (a_t x a_s) and PSE c (V_t a_s - V_s a_t) pair by pair; any other source
+// adds them factored, A += q a_s, C += c a_s, D += c V_s, and the item ends
+// with w = a_t x A + B + V_t C - D a_t.
+// ---------------------------------------------------------------------
+constexpr double kHoF = -0x1.e8ec8a4aeacc4p-4;   // -3/(8 pi)
+constexpr double kHoEta = 0x1.0b615ba0f867bp+2;  // 105/(8 pi)
+
+struct AccHO {
+  double ux = 0, uy = 0, uz = 0;  // sum q v
+  double ax = 0, ay = 0, az = 0;  // A = sum q a_s (factored sources)
+  double bx = 0, by = 0, bz = 0;  // B
+  double cx = 0, cy = 0, cz = 0;  // C = sum c a_s (factored sources)
+  double d = 0;                   // D = sum c V_s (factored sources)
+};
+
+__device__ __forceinline__ void pair64_ho(const T64& t, const Rec64& s, AccHO& a) {
+  const double dx = t.x - s.x, dy = t.y - s.y, dz = t.z - s.z;
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  const double sig2 = 0.5 * s.s2 + t.hs2;
+  const double ib = rsqrt_pos(r2 + sig2);
+  const double ib2 = ib * ib;
+  const double ib5 = ib2 * ib2 * ib;
+  const double ib7 = ib5 * ib2;
+  const double q = kQ * fma(2.5, sig2, r2) * ib5;
+  const double F = kHoF * fma(7.0, sig2, 2.0 * r2) * ib7;
+  const double c = t.nu2 * kHoEta * (sig2 * sig2) * (ib7 * ib2);  // 2 nu eta / sigma^2
+  const double vx = s.ay * dz - s.az * dy;
+  const double vy = s.az * dx - s.ax * dz;
+  const double vz = s.ax * dy - s.ay * dx;
+  a.ux += q * vx;
+  a.uy += q * vy;
+  a.uz += q * vz;
+  const double fr = F * (t.ax * vx + t.ay * vy + t.az * vz);
+  if (__ballot(r2 <= kFar64 * sig2)) {
+    a.bx += q * (t.ay * s.az - t.az * s.ay) + fr * dx + c * (t.vol * s.ax - s.vol * t.ax);
+    a.by += q * (t.az * s.ax - t.ax * s.az) + fr * dy + c * (t.vol * s.ay - s.vol * t.ay);
+    a.bz += q * (t.ax * s.ay - t.ay * s.ax) + fr * dz + c * (t.vol * s.az - s.vol * t.az);
+  } else {
+    a.ax += q * s.ax;
+    a.ay += q * s.ay;
+    a.az += q * s.az;
+    a.bx += fr * dx;
+    a.by += fr * dy;
+    a.bz += fr * dz;
+    a.cx += c * s.ax;
+    a.cy += c * s.ay;
+    a.cz += c * s.az;
+    a.d += c * s.vol;
+  }
+}
+
+__device__ __forceinline__ void item_fp64_ho(const PairArgs& pa, const WorkItem& it, Rec64* tile) {
+  const Rec64* __restrict__ state = pa.state;
+  const int t = it.t0 + static_cast<int>(threadIdx.x);
+  const bool active = t < it.t1;
+  const Rec64 m = state[active ? t : it.t0];
+  T64 me;
+  me.x = m.x; me.y = m.y; me.z = m.z; me.hs2 = 0.5 * m.s2;
+  me.ax = m.ax; me.ay = m.ay; me.az = m.az; me.vol = m.vol;
+  me.nu2 = 2.0 * pa.nu;
+  AccHO a;
+  for (int base = it.s0; base < it.s1; base += kTB) {
+    const int cnt = min(kTB, it.s1 - base);
+    __syncthreads();
+    if (static_cast<int>(threadIdx.x) < cnt) tile[threadIdx.x] = state[base + threadIdx.x];
+    __syncthreads();
+    if (!active) continue;
+    if (base == it.t0) {
+      const int self = static_cast<int>(threadIdx.x);
+      for (int l = 0; l < cnt; ++l)
+        if (l != self) pair64_ho(me, tile[l], a);
+    } else {
+      for (int l = 0; l < cnt; ++l) pair64_ho(me, tile[l], a);
+    }
+  }
+  if (active) {
+    Rate64 r;
+    r.ux = a.ux; r.uy = a.uy; r.uz = a.uz;
+    r.wx = ((me.ay * a.az - me.az * a.ay) + a.bx) + (me.vol * a.cx - a.d * me.ax);
+    r.wy = ((me.az * a.ax - me.ax * a.az) + a.by) + (me.vol * a.cy - a.d * me.ay);
+    r.wz = ((me.ax * a.ay - me.ay * a.ax) + a.bz) + (me.vol * a.cz - a.d * me.az);
+    out_rates(pa, it)[t] = r;
+  }
+}
+
+// ---------------------------------------------------------------------
 // Point-evaluation velocity (the reference's getVi,
 // src/interaction/interaction.cpp:453-476): the velocity at m arbitrary
 // points from every particle, with each particle's own radius as sigma,
 // in FP64 with the closed far form beyond 90 sigma^2.  One lane per point,
 // sources through LDS in tiles of 256 as in the pair kernels.  A point that
 // coincides with a particle gets no contribution from it (a_s x d = 0).
+// HighOrder: the algebraic kernel's q (highorder.h:31-39) instead.
 // ---------------------------------------------------------------------
+template <bool HighOrder>
 __global__ __launch_bounds__(kTB) void k_point_velocity(const Rec64* __restrict__ state, int n,
                                                         const double* __restrict__ pts, int m,
                                                         double* __restrict__ u) {
@@ -271,9 +371,15 @@ __global__ __launch_bounds__(kTB) void k_point_velocity(const Rec64* __restrict_
       const double dx = px - s.x, dy = py - s.y, dz = pz - s.z;
       const double r2 = dx * dx + dy * dy + dz * dz;
       if (r2 == 0.0) continue;
+      double q;
+      if (HighOrder) {
+        const double ib = rsqrt_pos(r2 + s.s2);
+        const double ib2 = ib * ib;
+        q = kQ * fma(2.5, s.s2, r2) * (ib2 * ib2 * ib);
+      } else {
       const double ir = rsqrt_pos(r2);
       const double ir2 = ir * ir;
-      double q = kQ * (ir * ir2);
+      q = kQ * (ir * ir2);
       if (r2 <= kFar64 * s.s2) {
         const double isg = rsqrt_pos(s.s2);
         const double isg3 = isg * isg * isg;
@@ -282,6 +388,7 @@ __global__ __launch_bounds__(kTB) void k_point_velocity(const Rec64* __restrict_
         const double Z = exp(-0.5 * rb * rb) * isg3 * kK;
         const double phi = kQ * erf(kSqrtHalf * rb) * isg3;
         q = (phi * irb - Z) * (irb * irb);
+      }
       }
       ux += q * (s.ay * dz - s.az * dy);
       uy += q * (s.az * dx - s.ax * dz);
@@ -879,6 +986,8 @@ __device__ __forceinline__ void run_item(const PairArgs& a, const WorkItem& it,
     item_fp64_ref(a, it, tile);
   else if constexpr (Kind == kPairFP64)
     item_fp64(a, it, tile);
+  else if constexpr (Kind == kPairFP64HO)
+    item_fp64_ho(a, it, tile);
   else
     item_fp32c(a, it, tile);
 }
@@ -906,6 +1015,10 @@ __global__ __launch_bounds__(kTB) void k_pair_fp64_ref(PairArgs a) { pair_entry<
 template <bool P>
 __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_pair_fp64(PairArgs a) {
   pair_entry<kPairFP64, P>(a);
+}
+template <bool P>
+__global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_pair_fp64_ho(PairArgs a) {
+  pair_entry<kPairFP64HO, P>(a);
 }
 // FP32C at the register budget of 7 waves/SIMD (no spills) and of 8
 // waves/SIMD (one spill slot, touched once per tile)
@@ -952,6 +1065,10 @@ const bool P = cfg.persistent;
       if (P) hipLaunchKernelGGL(k_pair_fp64<true>, g, block, 0, st, args);
       else hipLaunchKernelGGL(k_pair_fp64<false>, g, block, 0, st, args);
       break;
+    case kPairFP64HO:
+      if (P) hipLaunchKernelGGL(k_pair_fp64_ho<true>, g, block, 0, st, args);
+      else hipLaunchKernelGGL(k_pair_fp64_ho<false>, g, block, 0, st, args);
+      break;
     case kPairFP32C:
       if (cfg.waves8) {
         if (P) hipLaunchKernelGGL(k_pair_fp32c_w8<true>, g, block, 0, st, args);
@@ -971,9 +1088,13 @@ const bool P = cfg.persistent;
   return hipGetLastError();
 }
 
-hipError_t launch_point_velocity(const Rec64* state, int n, const double* pts, int m, double* u, hipStream_t st) {
+hipError_t launch_point_velocity(const Rec64* state, int n, const double* pts, int m, double* u,
+                                 bool high_order, hipStream_t st) {
   if (m <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_point_velocity, dim3((m + kTB - 1) / kTB), dim3(kTB), 0, st, state, n, pts, m, u);
+  if (high_order)
+    hipLaunchKernelGGL(k_point_velocity<true>, dim3((m + kTB - 1) / kTB), dim3(kTB), 0, st, state, n, pts, m, u);
+  else
+    hipLaunchKernelGGL(k_point_velocity<false>, dim3((m + kTB - 1) / kTB), dim3(kTB), 0, st, state, n, pts, m, u);
   return hipGetLastError();
 }
 
@@ -1016,6 +1137,8 @@ int pair_kernel_waves(int kind, const PairConfig& cfg) {
                    : reinterpret_cast<const void*>(k_pair_fp32c<true>);
   else if (kind == kPairFP64)
     k = reinterpret_cast<const void*>(k_pair_fp64<true>);
+  else if (kind == kPairFP64HO)
+    k = reinterpret_cast<const void*>(k_pair_fp64_ho<true>);
   else
     k = reinterpret_cast<const void*>(k_pair_fp64_ref<true>);
   return blocks_per_cu(k);

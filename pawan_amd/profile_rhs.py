@@ -27,6 +27,8 @@ def main() -> int:
                     help="tree path: absolute far-field error tolerance (0: opening angle only)")
     ap.add_argument("--config", default="",
                     help="pair-kernel launch shape 'parts,persistent,waves8' (e.g. 8,1,1); default: untuned")
+    ap.add_argument("--kernel", default="gaussian", choices=["gaussian", "highorder"],
+                    help="smoothing kernel (highorder: FP64 precisions only)")
     args = ap.parse_args()
 
     import torch
@@ -44,6 +46,8 @@ def main() -> int:
     else:
         system = workloads.WORKLOADS[args.workload]["make"]()
     with capi.Solver(system) as g:
+        if args.kernel == "highorder":
+            g.set_kernel(capi.PW_KERNEL_HIGHORDER)
         if args.precision == "learned":
             g.ff_set_params(args.theta, args.tol, use_mlp=False)
         elif args.config:
