@@ -120,6 +120,29 @@ int pw_set_coupled(pw_ctx *ctx, int coupled);
 int pw_set_kernel(pw_ctx *ctx, int kernel);
 int pw_get_kernel(pw_ctx *ctx, int *kernel);
 
+/* ---- vortex ring initial condition -------------------------------------
+ * pw_vring_particles: the particles of __vring(R, core, nlayers, nazi, sig)
+ * (src/wake/vring.cpp:50-100), nazi (2 nlayers + 1)^2 of them, station by
+ * station: pos (3 per particle), radius (= sig) and volume.  Their
+ * vorticity is zero until resolved (pass zeros to pw_add_wake).
+ * nlayers in [0, 15]. */
+int pw_vring_particles(double R, double core, int nlayers, int nazi, double sig, double *pos,
+                       double *radius, double *volume);
+/* pw_vring_resolve: __resolve::rebuild (src/resolve/resolve.cpp:12-63) for
+ * a ring held by wake `wake` of the context: the (2L+1)^2 cell strengths
+ * whose smoothed vorticity field at the first station's particles matches
+ * the Gaussian core profile of __vring::getIdealRates (vring.cpp:223-242),
+ * found with the influence matrix built on the device and an LU solve with
+ * partial pivoting (gsl_linalg_LU_decomp/solve).  The strengths are written
+ * into the wake's vorticity on the device (setStates, vring.cpp:201-205) and
+ * returned in strength[(2L+1)^2] if non-null; influence, if non-null,
+ * receives the matrix, row-major.  Uses the context's smoothing kernel.
+ * PW_ERR_NONFINITE if the matrix is singular (the wake is then unchanged).
+ * The .wakeinfluence file the reference writes along the way is not
+ * produced. */
+int pw_vring_resolve(pw_ctx *ctx, int wake, int nlayers, int nazi, double R, double core,
+                     double *strength, double *influence);
+
 /* Total particles and wakes in the context. */
 int pw_get_size(pw_ctx *ctx, int64_t *num_particles, int *num_wakes);
 

@@ -49,6 +49,11 @@ def _load() -> C.CDLL:
     lib.or_step_generic.restype = C.c_int
     lib.or_diagnose.argtypes = [C.c_int, ip, dp, dp, dp, d, dp, C.c_int]
     lib.or_velocity_at.argtypes = [i64, dp, dp, dp, i64, dp, dp]
+    lib.or_vring.argtypes = [d, d, C.c_int, C.c_int, d, dp, dp, dp, dp]
+    lib.or_vring_ideal.argtypes = [d, d, C.c_int, dp]
+    lib.or_vring_influence.argtypes = [C.c_int, C.c_int, dp, dp, dp]
+    lib.or_lu_solve.argtypes = [C.c_int, dp, dp, dp]
+    lib.or_lu_solve.restype = C.c_int
     lib.or_set_smoothing.argtypes = [C.c_int]
     lib.or_get_smoothing.restype = C.c_int
     lib.or_enst_pair.argtypes = [d, d, pd, pd]
@@ -258,3 +263,49 @@ def diagnose(s, zc_prev: float = 0.0, use_omp=True) -> np.ndarray:
     nw, off, pos, alpha, rad, _ = _arrs(s)
     lib().or_diagnose(nw, off, pos, alpha, rad, zc_prev, out, int(use_omp))
     return out
+
+
+def vring(R: float, core: float, nlayers: int, nazi: int, sig: float):
+    """__vring(R, core, nLayers, nAzi, sig) (vring.cpp:50-100) as
+    (pos, alpha, radius, volume); alpha is zero until resolved."""
+    m = (2 * nlayers + 1) ** 2
+    n = nazi * m
+    pos, alpha = np.zeros((n, 3)), np.zeros((n, 3))
+    radius, volume = np.zeros(n), np.zeros(n)
+    lib().or_vring(R, core, nlayers, nazi, sig, pos, alpha, radius, volume)
+    return pos, alpha, radius, volume
+
+
+def vring_ideal(R: float, core: float, nlayers: int) -> np.ndarray:
+    out = np.zeros((2 * nlayers + 1) ** 2)
+    lib().or_vring_ideal(R, core, nlayers, out)
+    return out
+
+
+def vring_influence(nlayers: int, nazi: int, pos, radius) -> np.ndarray:
+    m = (2 * nlayers + 1) ** 2
+    M = np.zeros((m, m))
+    lib().or_vring_influence(nlayers, nazi, np.ascontiguousarray(pos, dtype=np.float64),
+                             np.ascontiguousarray(radius, dtype=np.float64), M)
+    return M
+
+
+def lu_solve(A, b) -> np.ndarray:
+    A = np.array(A, dtype=np.float64, order="C")
+    x = np.zeros(len(b))
+    if lib().or_lu_solve(len(b), A, np.ascontiguousarray(b, dtype=np.float64), x) != 0:
+        raise ValueError("singular matrix")
+    return x
+
+
+def vring_resolve(R: float, core: float, nlayers: int, nazi: int, sig: float):
+    """__resolve::rebuild on one ring (resolve.cpp:12-63): the strengths of
+    the (2L+1)^2 cells and the ring's particles carrying them."""
+    pos, alpha, radius, volume = vring(R, core, nlayers, nazi, sig)
+    M = vring_influence(nlayers, nazi, pos, radius)
+    s = lu_solve(M, vring_ideal(R, core, nlayers))
+    m = len(s)
+    psi = np.repeat(np.arange(nazi) * (2.0 * np.pi / nazi), m)
+    strength = np.tile(s, nazi)
+    alpha = np.stack([-np.sin(psi) * strength, np.cos(psi) * strength, np.zeros_like(psi)], axis=1)
+    return s, M, (pos, alpha, radius, volume)

@@ -811,3 +811,160 @@ void or_diagnose(int nw, const int64_t *off, const double *pos,
   out[13] = KEF;
   out[14] = Zc;
 }
+
+/* ---------------------------------------------------------------------- */
+/* Vortex ring initial condition and its strength resolution              */
+/* ---------------------------------------------------------------------- */
+
+/* __vring(radius, core, nLayers, nAzi, sig), src/wake/vring.cpp:50-100:
+ * nAzi azimuthal stations psi_a = a 2pi/nAzi, each carrying the
+ * (2L+1)^2 cells of one cross-section: a centre particle and L layers, the
+ * j-th with 8j particles at r_j = r0 (12 j^2 + 1)/(6 j) around the core
+ * centre, r0 = 0.35/(2L+1).  Vorticity = strength x (-sin psi, cos psi, 0)
+ * with every strength 0 at construction (vring.cpp:25-48, _strength is
+ * calloc'd).  Writes pos, alpha, radius, volume of nAzi (2L+1)^2 particles.
+ * Pinned (tests/test_oracle_vring.py): positions on circles of radius r_j
+ * about the core centre at angle k 2pi/(8j); per-station volume
+ * pi 0.35^2 R dpsi (the cell volumes telescope). */
+void or_vring(double R, double core, int nLayers, int nAzi, double sig,
+              double *pos, double *alpha, double *radius, double *volume) {
+  (void)core;
+  const double r0 = 0.35 / (2 * nLayers + 1);
+  const double dPsi = 2.0 * M_PI / nAzi;
+  int64_t index = 0;
+  for (int i = 0; i < nAzi; ++i) {
+    double psi = i * dPsi;
+    double sPsi = sin(psi), cPsi = cos(psi);
+    pos[3 * index] = R * cPsi;
+    pos[3 * index + 1] = R * sPsi;
+    pos[3 * index + 2] = 0.0;
+    radius[index] = sig;
+    volume[index] = M_PI * (r0 * r0) * R * dPsi;
+    index++;
+    for (int j = 1; j <= nLayers; ++j) {
+      double r = r0 * (12. * (double)(j * j) + 1.0) / (6. * j);
+      int nCore = 8 * j;
+      double dTheta = 2. * M_PI / nCore;
+      for (int k = 0; k < nCore; ++k) {
+        double theta = k * dTheta;
+        pos[3 * index] = (R + r * cos(theta)) * cPsi;
+        pos[3 * index + 1] = (R + r * cos(theta)) * sPsi;
+        pos[3 * index + 2] = r * sin(theta);
+        radius[index] = sig;
+        volume[index] = 2.0 * r0 * dPsi *
+                        (2.0 * dTheta * j * R * r0 +
+                         (12.0 * (double)(j * j) + 1.0) * (r0 * r0) *
+                             (sin(theta + dTheta) - sin(theta)) / 3.0);
+        index++;
+      }
+    }
+  }
+  for (int64_t p = 0; p < index; ++p) alpha[3 * p] = alpha[3 * p + 1] = alpha[3 * p + 2] = 0.0;
+}
+
+/* __vring::getIdealRates, vring.cpp:223-242: the Gaussian core profile at
+ * the (2L+1)^2 cells of one cross-section. */
+void or_vring_ideal(double R, double core, int nLayers, double *rate) {
+  int64_t index = 0;
+  double mag0 = 0.5 / (M_PI * core * core);
+  double r0 = 0.35 / (2 * nLayers + 1);
+  rate[index++] = mag0;
+  for (int j = 1; j <= nLayers; ++j) {
+    double r = r0 * (12. * (double)(j * j) + 1.0) / (6. * j);
+    int nCore = 8 * j;
+    double dTheta = 2. * M_PI / nCore;
+    double decay = exp(-0.5 * r * r / (core * core));
+    for (int k = 0; k < nCore; ++k) {
+      double theta = k * dTheta;
+      rate[index++] = mag0 * (1.0 + r * cos(theta) / R) * decay;
+    }
+  }
+}
+
+static inline double zeta_any(double rho, double sigma) {
+  return smoothing == OR_HIGHORDER ? ho_zetasig(rho, sigma) : zetasig(rho, sigma);
+}
+
+/* The influence matrix of __resolve::rebuild (src/resolve/resolve.cpp:12-40)
+ * for one ring of m = (2L+1)^2 cells over nAzi stations: column i is
+ * getRates (vring.cpp:208-215: |(k_x, k_y)| of the vorticity field at the
+ * first station's m particles) after setStates(e_i) and resolve(), i.e.
+ * influence() (interaction.cpp:212-243) with INFLUENCE / SELFINFLUENCE
+ * (interaction_utils.h:626-703).  The field at particle p collects the
+ * terms Z(rho_pq, sigma_pq) a_q in ascending q, the self term
+ * ZETASIG(0, s_p) a_p at q = p, as the reference's triangular loop adds
+ * them; particles outside cell i carry a = 0 and add exact zeros, so only
+ * cell i's particles are visited.  M is m x m, row-major M[p * m + i].
+ * Pinned: against an independent numpy evaluation of the smoothed field
+ * omega(x_p) = sum_q zeta_sigma(x_p - x_q) a_q (tests/test_oracle_vring.py). */
+void or_vring_influence(int nLayers, int nAzi, const double *pos, const double *radius,
+                        double *M) {
+  const int m = (2 * nLayers + 1) * (2 * nLayers + 1);
+  const double dPsi = 2.0 * M_PI / nAzi;
+  for (int p = 0; p < m; ++p) {
+    for (int i = 0; i < m; ++i) {
+      double k[3] = {0.0, 0.0, 0.0};
+      for (int a = 0; a < nAzi; ++a) {
+        int64_t q = (int64_t)a * m + i;
+        double psi = a * dPsi;
+        double t[3] = {-sin(psi) * 1.0, cos(psi) * 1.0, 0.0};
+        double Z;
+        if (q == p) {
+          Z = zeta_any(0.0, radius[p]);
+        } else {
+          double d[3] = {pos[3 * q] - pos[3 * p], pos[3 * q + 1] - pos[3 * p + 1],
+                         pos[3 * q + 2] - pos[3 * p + 2]};
+          double sigma = sqrt(0.5 * (radius[p] * radius[p] + radius[q] * radius[q]));
+          Z = zeta_any(or_dnrm2(d), sigma);
+        }
+        for (int c = 0; c < 3; ++c) k[c] += Z * t[c];
+      }
+      M[(int64_t)p * m + i] = sqrt(k[0] * k[0] + k[1] * k[1]);
+    }
+  }
+}
+
+/* gsl_linalg_LU_decomp + gsl_linalg_LU_solve (resolve.cpp:45-49): LU with
+ * partial pivoting (the row of largest |a_ij| at or below the diagonal,
+ * first one on ties), unit lower triangle, then forward and back
+ * substitution on the permuted right-hand side.  A (n x n, row-major) is
+ * overwritten by its factors.  Returns 1 if a pivot is zero.
+ * Pinned: numpy.linalg.solve (LAPACK dgesv) on random and influence
+ * matrices (tests/test_oracle_vring.py). */
+int or_lu_solve(int n, double *A, const double *b, double *x) {
+  int *perm = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
+  for (int i = 0; i < n; ++i) perm[i] = i;
+  int singular = 0;
+  for (int j = 0; j < n; ++j) {
+    int ip = j;
+    double mx = fabs(A[(int64_t)j * n + j]);
+    for (int i = j + 1; i < n; ++i) {
+      double v = fabs(A[(int64_t)i * n + j]);
+      if (v > mx) { mx = v; ip = i; }
+    }
+    if (ip != j) {
+      for (int c = 0; c < n; ++c) {
+        double tmp = A[(int64_t)j * n + c];
+        A[(int64_t)j * n + c] = A[(int64_t)ip * n + c];
+        A[(int64_t)ip * n + c] = tmp;
+      }
+      int t = perm[j]; perm[j] = perm[ip]; perm[ip] = t;
+    }
+    double piv = A[(int64_t)j * n + j];
+    if (piv == 0.0) { singular = 1; continue; }
+    for (int i = j + 1; i < n; ++i) {
+      double l = A[(int64_t)i * n + j] / piv;
+      A[(int64_t)i * n + j] = l;
+      for (int c = j + 1; c < n; ++c) A[(int64_t)i * n + c] -= l * A[(int64_t)j * n + c];
+    }
+  }
+  for (int i = 0; i < n; ++i) x[i] = b[perm[i]];
+  for (int i = 0; i < n; ++i)
+    for (int c = 0; c < i; ++c) x[i] -= A[(int64_t)i * n + c] * x[c];
+  for (int i = n - 1; i >= 0; --i) {
+    for (int c = i + 1; c < n; ++c) x[i] -= A[(int64_t)i * n + c] * x[c];
+    x[i] /= A[(int64_t)i * n + i];
+  }
+  free(perm);
+  return singular;
+}

@@ -133,6 +133,16 @@ void or_diagnose(int nw, const int64_t *off, const double *pos,
                  const double *alpha, const double *radius, double zc_prev,
                  double *out, int use_omp);
 
+/* Vortex ring (vring.cpp:50-100, 223-242) and __resolve::rebuild
+ * (resolve.cpp:12-63): particles of nAzi (2L+1)^2, the ideal core profile,
+ * the (2L+1)^2 x (2L+1)^2 influence matrix and the pivoted LU solve. */
+void or_vring(double R, double core, int nLayers, int nAzi, double sig,
+              double *pos, double *alpha, double *radius, double *volume);
+void or_vring_ideal(double R, double core, int nLayers, double *rate);
+void or_vring_influence(int nLayers, int nAzi, const double *pos, const double *radius,
+                        double *M);
+int or_lu_solve(int n, double *A, const double *b, double *x);
+
 #ifdef __cplusplus
 }
 #endif

@@ -36,7 +36,7 @@ EXPORTS = [
     "pw_ff_build", "pw_ff_info", "pw_ff_export", "pw_ff_sample", "pw_ff_clear_samples",
     "pw_ff_init_weights", "pw_ff_set_weights", "pw_ff_get_weights", "pw_ff_train",
     "pw_ff_mlp_forward", "pw_ff_mlp_grad", "pw_ff_set_conserve", "pw_ff_get_conserve",
-    "pw_velocity_at", "pw_set_kernel", "pw_get_kernel",
+    "pw_velocity_at", "pw_set_kernel", "pw_get_kernel", "pw_vring_particles", "pw_vring_resolve",
 ]
 
 # pw_ff_export selectors
@@ -83,6 +83,8 @@ def load(path: str | None = None) -> C.CDLL:
         "pw_get_nu": ([vp, C.POINTER(C.c_double)], i),
         "pw_set_coupled": ([vp, i], i),
         "pw_set_kernel": ([vp, i], i),
+        "pw_vring_particles": ([d, d, i, i, d, _dp, _dp, _dp], i),
+        "pw_vring_resolve": ([vp, i, i, i, d, d, C.c_void_p, C.c_void_p], i),
         "pw_get_kernel": ([vp, C.POINTER(C.c_int)], i),
         "pw_get_size": ([vp, C.POINTER(C.c_int64), C.POINTER(C.c_int)], i),
         "pw_set_wake_state": ([vp, i, _dp, _dp], i),
@@ -137,6 +139,16 @@ def device_count() -> int:
     n = C.c_int(0)
     rc = load().pw_device_count(C.byref(n))
     return n.value if rc == PW_OK else 0
+
+
+def vring_particles(R: float, core: float, nlayers: int, nazi: int, sig: float):
+    """Particles of __vring(R, core, nlayers, nazi, sig) (vring.cpp:50-100):
+    (pos, radius, volume), vorticity zero until resolved."""
+    n = nazi * (2 * nlayers + 1) ** 2
+    pos, radius, volume = np.zeros((n, 3)), np.zeros(n), np.zeros(n)
+    _check(load().pw_vring_particles(float(R), float(core), int(nlayers), int(nazi), float(sig),
+                                     pos, radius, volume))
+    return pos, radius, volume
 
 
 class Solver:
@@ -205,6 +217,16 @@ class Solver:
 
     def set_coupled(self, coupled: bool) -> None:
         _check(self._lib.pw_set_coupled(self._h, int(bool(coupled))))
+
+    def vring_resolve(self, wake: int, nlayers: int, nazi: int, R: float, core: float):
+        """__resolve::rebuild on the ring held by `wake`: returns the cell
+        strengths and the influence matrix; the wake's vorticity is set."""
+        m = (2 * nlayers + 1) ** 2
+        s = np.zeros(m)
+        M = np.zeros((m, m))
+        _check(self._lib.pw_vring_resolve(self._h, int(wake), int(nlayers), int(nazi), float(R), float(core),
+                                          s.ctypes.data_as(C.c_void_p), M.ctypes.data_as(C.c_void_p)))
+        return s, M
 
     def set_kernel(self, kernel: int) -> None:
         """PW_KERNEL_GAUSSIAN or PW_KERNEL_HIGHORDER (FP64 only)."""

@@ -616,6 +616,96 @@ int pw_velocity_at(pw_ctx* c, const double* points, int64_t m, double* u) {
   return PW_OK;
 }
 
+int pw_vring_particles(double R, double core, int nlayers, int nazi, double sig, double* pos,
+                       double* radius, double* volume) {
+  (void)core;  // sets only the ideal profile (pw_vring_resolve)
+  if (nlayers < 0 || nazi < 1 || (2 * nlayers + 1) * (2 * nlayers + 1) > pw::kVringMaxCells)
+    return fail(PW_ERR_ARG, "nlayers must be in [0, 15] and nazi >= 1");
+  if (!pos || !radius || !volume) return fail(PW_ERR_ARG, "null output");
+  // the particle layout of __vring (vring.cpp:50-100): one centre particle
+  // and layers of 8j particles per station, the cell volumes of its
+  // annular sectors
+  const double r0 = 0.35 / (2 * nlayers + 1);
+  const double dpsi = 2.0 * M_PI / nazi;
+  int64_t q = 0;
+  for (int a = 0; a < nazi; ++a) {
+    const double psi = a * dpsi, sp = std::sin(psi), cp = std::cos(psi);
+    pos[3 * q] = R * cp;
+    pos[3 * q + 1] = R * sp;
+    pos[3 * q + 2] = 0.0;
+    radius[q] = sig;
+    volume[q] = M_PI * (r0 * r0) * R * dpsi;
+    ++q;
+    for (int j = 1; j <= nlayers; ++j) {
+      const double jj = static_cast<double>(j) * j;
+      const double r = r0 * (12. * jj + 1.0) / (6. * j);
+      const int ncell = 8 * j;
+      const double dth = 2. * M_PI / ncell;
+      for (int k = 0; k < ncell; ++k) {
+        const double th = k * dth;
+        pos[3 * q] = (R + r * std::cos(th)) * cp;
+        pos[3 * q + 1] = (R + r * std::cos(th)) * sp;
+        pos[3 * q + 2] = r * std::sin(th);
+        radius[q] = sig;
+        volume[q] = 2.0 * r0 * dpsi * (2.0 * dth * j * R * r0 + (12.0 * jj + 1.0) * (r0 * r0) *
+                                                                    (std::sin(th + dth) - std::sin(th)) / 3.0);
+        ++q;
+      }
+    }
+  }
+  return PW_OK;
+}
+
+int pw_vring_resolve(pw_ctx* c, int wake, int nlayers, int nazi, double R, double core, double* strength,
+                     double* influence) {
+  PW_CTX(c);
+  if (wake < 0 || wake >= static_cast<int>(c->wn.size())) return fail(PW_ERR_ARG, "wake index out of range");
+  if (nlayers < 0 || nazi < 1 || (2 * nlayers + 1) * (2 * nlayers + 1) > pw::kVringMaxCells)
+    return fail(PW_ERR_ARG, "nlayers must be in [0, 15] and nazi >= 1");
+  if (!(core > 0.0) || !(R > 0.0)) return fail(PW_ERR_ARG, "radius and core must be positive");
+  const int m = (2 * nlayers + 1) * (2 * nlayers + 1);
+  if (c->wn[wake] != static_cast<int64_t>(m) * nazi)
+    return fail(PW_ERR_ARG, "the wake does not hold nazi (2 nlayers + 1)^2 particles");
+  PW_TRY(ensure_built(c));
+  // __vring::getIdealRates (vring.cpp:223-242): the Gaussian core profile
+  std::vector<double> ideal(m);
+  {
+    const double mag0 = 0.5 / (M_PI * core * core), r0 = 0.35 / (2 * nlayers + 1);
+    int idx = 0;
+    ideal[idx++] = mag0;
+    for (int j = 1; j <= nlayers; ++j) {
+      const double r = r0 * (12. * (static_cast<double>(j) * j) + 1.0) / (6. * j);
+      const int ncell = 8 * j;
+      const double dth = 2. * M_PI / ncell, decay = std::exp(-0.5 * r * r / (core * core));
+      for (int k = 0; k < ncell; ++k) ideal[idx++] = mag0 * (1.0 + r * std::cos(k * dth) / R) * decay;
+    }
+  }
+  double *d_M = nullptr, *d_lu = nullptr, *d_b = nullptr, *d_s = nullptr;
+  int* d_flag = nullptr;
+  const size_t mm = static_cast<size_t>(m) * m;
+  if (dalloc(&d_M, mm) != hipSuccess || dalloc(&d_lu, mm) != hipSuccess || dalloc(&d_b, m) != hipSuccess ||
+      dalloc(&d_s, m) != hipSuccess || dalloc(&d_flag, 1) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipFree(d_M); (void)hipFree(d_lu); (void)hipFree(d_b); (void)hipFree(d_s); (void)hipFree(d_flag);
+    return fail(PW_ERR_NOMEM, "resolve buffers");
+  }
+  int flag = 0;
+  hipError_t e = hipMemcpyAsync(d_b, ideal.data(), m * sizeof(double), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess)
+    e = pw::launch_vring_resolve(c->y, static_cast<int>(c->woff[wake]), m, nazi,
+                                 c->smoothing == PW_KERNEL_HIGHORDER, d_M, d_lu, d_b, d_s, d_flag, c->stream);
+  if (e == hipSuccess && strength)
+    e = hipMemcpyAsync(strength, d_s, m * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess && influence)
+    e = hipMemcpyAsync(influence, d_M, mm * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(&flag, d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d_M); (void)hipFree(d_lu); (void)hipFree(d_b); (void)hipFree(d_s); (void)hipFree(d_flag);
+  if (e != hipSuccess) return fail(PW_ERR_HIP, std::string("pw_vring_resolve: ") + hipGetErrorString(e));
+  if (flag) return fail(PW_ERR_NONFINITE, "pw_vring_resolve: singular influence matrix");
+  return PW_OK;
+}
+
 int pw_solve(pw_ctx* c, int precision) {
   PW_CTX(c);
   int kind = 0;

@@ -258,6 +258,14 @@ inline hipError_t ensure_lds_limit(const void* kernel, size_t bytes, std::atomic
 }
 
 // velocity at m points (AoS doubles, device) from the n particles of state
+// Vortex-ring strength resolution (pw_vring.hip): influence matrix M (m x m)
+// of the ring at `off` (nazi stations of m cells), its LU solve against
+// `ideal` on a copy `lu`, and the strengths s written into the ring's
+// vorticity.  flag[0] = 1 if the matrix is singular.  m <= kVringMaxCells.
+constexpr int kVringMaxCells = 1024;
+hipError_t launch_vring_resolve(Rec64* y, int off, int m, int nazi, bool high_order, double* M,
+                                double* lu, const double* ideal, double* s, int* flag, hipStream_t st);
+
 hipError_t launch_point_velocity(const Rec64* state, int n, const double* pts, int m, double* u,
                                  bool high_order, hipStream_t st);
 hipError_t launch_near_fp32c(const Rec64* sorted, const NodeRec* nodes, const TreeBuffers& tb, int nleaf,
