@@ -19,7 +19,11 @@ struct pw_ctx;
 
 namespace pawan {
 
+class __resolve;
+
 class __interaction : public __system {
+  friend class __resolve;
+
  protected:
   double _nu;                       /*!< kinematic viscosity             */
   size_t _nWake;                    /*!< number of wakes                 */

@@ -1,5 +1,6 @@
 // wake.h -- pawan::__wake, the particle container
-// (reference: src/wake/wake.h:26-172, src/wake/wake.cpp).
+// (reference: src/wake/wake.h:26-172, src/wake/wake.cpp), and the vortex
+// ring pawan::__vring (src/wake/vring.h, src/wake/vring.cpp).
 //
 // Same public members, state-vector layout and constructors as the
 // reference, with the gsl containers replaced by pawan::vec / pawan::mat.
@@ -74,6 +75,25 @@ class __wake {
   virtual void translate(const double *x);
   //! elemental rotation by angle a about axis n (wake.cpp:916-937)
   virtual void rotate(const size_t &n, const double &a);
+};
+
+//! The vortex-ring initial condition (src/wake/vring.h, vring.cpp:50-100):
+//! nAzi azimuthal stations of (2 nLayers + 1)^2 cross-section cells, a
+//! centre particle and layers of 8j particles, with zero vorticity until
+//! __resolve::rebuild gives the cells their strengths.  Its state vector is
+//! the generic wake's (positions and vorticity); the reference's __vring
+//! exposes the cell strengths as its state instead (vring.cpp:201-221).
+class __vring : public __wake {
+ public:
+  double _R;             /*!< ring radius                                  */
+  double _sigmaR;        /*!< core radius of the ideal profile             */
+  size_t _nLayers;       /*!< layers around the centre particle            */
+  size_t _nAzi;          /*!< azimuthal stations                           */
+  vec *_strength;        /*!< (2 nLayers + 1)^2 cell strengths             */
+
+  __vring(const double &radius, const double &core, const int &nLayers, const int &nAzi,
+          const double &sig);
+  virtual ~__vring();
 };
 
 }  // namespace pawan

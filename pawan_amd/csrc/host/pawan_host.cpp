@@ -11,6 +11,7 @@
 #include "pawan_gpu.h"
 #include "pawan/integration.h"
 #include "pawan/interaction.h"
+#include "pawan/resolve.h"
 #include "pawan/wake.h"
 
 namespace pawan {
@@ -533,4 +534,38 @@ void __rk3::step(const double &dt, __system *S, vec *states) {
   S->setStates(states);
 }
 
+// ---------------------------------------------------------------------------
+// __vring, __resolve
+// ---------------------------------------------------------------------------
+__vring::__vring(const double &radius, const double &core, const int &nLayers, const int &nAzi,
+                 const double &sig)
+    : __wake(nAzi * (2 * nLayers + 1) * (2 * nLayers + 1)) {
+  _R = radius;
+  _sigmaR = core;
+  _nLayers = static_cast<size_t>(nLayers);
+  _nAzi = static_cast<size_t>(nAzi);
+  _strength = new vec((2 * _nLayers + 1) * (2 * _nLayers + 1));
+  if (pw_vring_particles(radius, core, nLayers, nAzi, sig, _position->ptr(), _radius->ptr(),
+                         _volume->ptr()) != PW_OK)
+    throw std::runtime_error(std::string("__vring: ") + pw_last_error());
+  for (size_t i = 0; i < _numParticles; ++i) {
+    for (size_t j = 0; j < 3; ++j) _vorticity->set(i, j, 0.0);
+    _birthstrength->set(i, 1.0);  // vring.cpp:81, 98
+  }
+}
+
+__vring::~__vring() { delete _strength; }
+
+void __resolve::rebuild(__interaction *IN, __io *IO) {
+  (void)IO;
+  __vring *ring = IN->_W.size() == 1 ? dynamic_cast<__vring *>(IN->_W[0]) : nullptr;
+  if (!ring) throw std::runtime_error("__resolve::rebuild: the interaction must hold one __vring wake");
+  pw_ctx *ctx = IN->context();
+  if (pw_vring_resolve(ctx, 0, static_cast<int>(ring->_nLayers), static_cast<int>(ring->_nAzi), ring->_R,
+                       ring->_sigmaR, ring->_strength->ptr(), nullptr) != PW_OK)
+    IN->fail("pw_vring_resolve");
+  IN->pullState();
+}
+
 }  // namespace pawan
+

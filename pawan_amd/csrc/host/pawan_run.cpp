@@ -1,7 +1,7 @@
 // pawan_run.cpp -- a driver written against the class API exactly as the
 // reference's validation drivers are (src/pawan.cpp:140-177): wakes ->
 // __parallel -> __rk4(T, n) -> integrate(S, IO, diagnose).  The wakes are
-// read from a binary file instead of being built by __vring/__resolve.
+// read from a binary file, or built as a ring by __vring/__resolve.
 //
 //   pawan_run <input> <name> <T> <n> [euler|rk3|rk4] [fp64|fp64_ref|fp32c|learned]
 //             [nu] [diagnose 0|1]
@@ -13,7 +13,9 @@
 // set the traversal.
 //
 // input: int64 nwakes, then per wake int64 n followed by n x 3 positions,
-// n x 3 strengths, n radii, n volumes (doubles, native endianness).
+// n x 3 strengths, n radii, n volumes (doubles, native endianness); or
+// "vring:R,core,nLayers,nAzi,sig": a __vring resolved by __resolve::rebuild
+// and copied into a plain wake, as src/pawan.cpp:57-84 builds its rings.
 // Writes $PAWAN_DATA_DIR/<name>Wake.wake and .diagnosis in the reference's
 // formats (readable by its scripts/readWake.py and readDiagnostics.py).
 #include <cstdint>
@@ -28,6 +30,7 @@
 #include "pawan_gpu.h"
 #include "pawan/integration.h"
 #include "pawan/interaction.h"
+#include "pawan/resolve.h"
 #include "pawan/wake.h"
 
 static void read_exact(FILE* f, void* p, size_t bytes) {
@@ -74,11 +77,22 @@ int main(int argc, char** argv) {
     const std::string prec = argc > 6 ? argv[6] : "fp64";
     const double T = std::atof(argv[3]);
     const size_t n = static_cast<size_t>(std::atoll(argv[4]));
-    FILE* in = std::fopen(argv[1], "rb");
-    if (!in) throw std::runtime_error(std::string("cannot open ") + argv[1]);
-    int64_t nw = 0;
-    read_exact(in, &nw, sizeof nw);
     std::vector<std::unique_ptr<pawan::__wake>> wakes;
+    if (std::strncmp(argv[1], "vring:", 6) == 0) {
+      double R = 0, core = 0, sig = 0;
+      int nl = 0, nazi = 0;
+      if (std::sscanf(argv[1] + 6, "%lf,%lf,%d,%d,%lf", &R, &core, &nl, &nazi, &sig) != 5)
+        throw std::runtime_error("vring:R,core,nLayers,nAzi,sig expected");
+      pawan::__vring ring(R, core, nl, nazi, sig);
+      pawan::__interaction resolver(&ring);
+      pawan::__io io(std::string(argv[2]) + "_vring");
+      pawan::__resolve().rebuild(&resolver, &io);
+      wakes.emplace_back(new pawan::__wake(&ring));
+    }
+    FILE* in = wakes.empty() ? std::fopen(argv[1], "rb") : nullptr;
+    if (wakes.empty() && !in) throw std::runtime_error(std::string("cannot open ") + argv[1]);
+    int64_t nw = 0;
+    if (in) read_exact(in, &nw, sizeof nw);
     for (int64_t w = 0; w < nw; ++w) {
       int64_t np = 0;
       read_exact(in, &np, sizeof np);
@@ -90,7 +104,7 @@ int main(int argc, char** argv) {
       wakes.emplace_back(new pawan::__wake(static_cast<size_t>(np), pos.data(), alpha.data(),
                                            rad.data(), vol.data()));
     }
-    std::fclose(in);
+    if (in) std::fclose(in);
 
     pawan::__interaction* S = new pawan::__parallel(wakes[0].get());
     for (size_t w = 1; w < wakes.size(); ++w) S->addWake(wakes[w].get());

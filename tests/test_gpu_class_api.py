@@ -103,3 +103,30 @@ def test_driver_runs_the_tree_path(tmp_path):
     disp = out["fp64"][0] - s.pos
     assert normwise(out["learned"][0] - s.pos, disp) < 1e-2
     assert normwise(out["learned"][1], out["fp64"][1]) < 1e-3
+
+
+def test_driver_resolves_a_vring(tmp_path, oracle):
+    """__vring -> __interaction -> __resolve::rebuild -> copy into a __wake ->
+    __rk4, as the reference's main builds its rings (src/pawan.cpp:57-84)."""
+    R, core, nl, nazi, sig = 1.0, 0.1, 2, 30, 0.15
+    env = dict(os.environ, PAWAN_DATA_DIR=str(tmp_path))
+    nsteps, dt, nu = 3, 0.002, 1e-4
+    print(f"pawan_run vring L={nl} nAzi={nazi}, {nsteps} x rk4", flush=True)
+    r = subprocess.run([EXE, f"vring:{R},{core},{nl},{nazi},{sig}", "vr", str(nsteps * dt), str(nsteps),
+                        "rk4", "fp64", repr(nu), "0"], env=env, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    recs = read_wake(tmp_path / "vrWake.wake")
+    _, (pos, alpha, radius, volume) = oracle.vring_resolve(R, core, nl, nazi, sig)[1:]
+    n0, p0, v0 = recs[0][1][0]
+    assert n0 == len(pos)
+    # the two generators call libm's sin/cos in their own ways (a compiler
+    # may merge them into sincos): positions agree to an ulp
+    np.testing.assert_allclose(p0, pos, rtol=0, atol=4e-16)
+    assert normwise(v0, alpha) < 1e-12
+    s = W.ParticleSystem(off=np.array([0, len(pos)], dtype=np.int64), pos=pos, alpha=alpha, radius=radius,
+                         volume=volume, nu=nu)
+    ref = oracle.step(s, "rk4", dt, nsteps)
+    _, p1, v1 = recs[-1][1][0]
+    print(f"  e_pos={normwise(p1, ref.pos):.2e} e_alpha={normwise(v1, ref.alpha):.2e}", flush=True)
+    assert normwise(p1, ref.pos) < 1e-12
+    assert normwise(v1, ref.alpha) < 1e-11
