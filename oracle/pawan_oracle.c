@@ -178,7 +178,9 @@ double or_qsig(double rho, double sigma) {
 }
 
 /* ENST(rho, sigma, F1, F2), highorder.h:77-87; zero for the Gaussian
- * (gaussian.h:69-75). */
+ * (gaussian.h:69-75).  Pinned in part: F1 at rho -> 0 equals ENST(sigma)
+ * and the pair terms enter the enstrophy sums; the rho-dependence of F1
+ * and F2 is restated from highorder.h:83-86 with no independent check. */
 void or_enst_pair(double rho, double sigma, double *F1, double *F2) {
   if (smoothing != OR_HIGHORDER) {
     *F1 = 0.0;
