@@ -110,13 +110,15 @@ int pw_get_nu(pw_ctx *ctx, double *nu);
  * targets seeing only its own wake's sources. */
 int pw_set_coupled(pw_ctx *ctx, int coupled);
 
-/* Smoothing kernel of every later solve, step, diagnosis and point
- * velocity: PW_KERNEL_GAUSSIAN (default) or PW_KERNEL_HIGHORDER (KERNEL,
- * QSIG, ENST, ENSTF of highorder.h:20-106 in place of gaussian.h's; the
- * reference makes this choice at compile time, interaction_utils.h:22-26).
- * The high-order kernel is computed in FP64: pw_solve/pw_step return
- * PW_ERR_UNSUPPORTED for PW_FP32C and PW_LEARNED while it is selected, and
- * PW_FP64 and PW_FP64_REF both run it.  May be changed between steps. */
+/* Smoothing kernel of every later solve, step, diagnosis, point velocity
+ * and ring resolve: PW_KERNEL_GAUSSIAN (default) or PW_KERNEL_HIGHORDER
+ * (KERNEL, QSIG, ENST, ENSTF of highorder.h:20-106 in place of gaussian.h's;
+ * the reference makes this choice at compile time,
+ * interaction_utils.h:22-26).  The high-order kernel is built for the
+ * direct sums: PW_FP64 and PW_FP64_REF both run its FP64 form, PW_FP32C its
+ * compensated FP32 form; pw_solve/pw_step return PW_ERR_UNSUPPORTED for
+ * PW_LEARNED, and pw_autotune for any precision, while it is selected.
+ * May be changed between steps. */
 int pw_set_kernel(pw_ctx *ctx, int kernel);
 int pw_get_kernel(pw_ctx *ctx, int *kernel);
 

@@ -176,11 +176,11 @@ int kind_of(int precision, int* kind) {
   }
 }
 
-// The high-order algebraic kernel is built in FP64 only.
+// The high-order algebraic kernel is built for the direct sums only.
 int check_smoothing(const pw_ctx* c, int kind) {
-  if (c->smoothing == PW_KERNEL_HIGHORDER && kind != pw::kPairFP64 && kind != pw::kPairFP64Exact)
-    return fail(PW_ERR_UNSUPPORTED, "the high-order algebraic kernel is computed in FP64 only "
-                                    "(PW_FP64 or PW_FP64_REF)");
+  if (c->smoothing == PW_KERNEL_HIGHORDER && kind == kKindLearned)
+    return fail(PW_ERR_UNSUPPORTED, "the high-order algebraic kernel is computed by the direct sums only "
+                                    "(PW_FP64, PW_FP64_REF, PW_FP32C)");
   return PW_OK;
 }
 
@@ -265,8 +265,10 @@ int rhs(pw_ctx* c, int kind, const Rec64* state, Rate64* out) {
   args.partial = c->partial;
   args.queue = c->d_queue;
   args.nu = c->nu;
-  // the high-order kernel replaces the Gaussian FP64 kinds (pw_set_kernel)
-  const int launch_kind = c->smoothing == PW_KERNEL_HIGHORDER ? pw::kPairFP64HO : kind;
+  // the high-order kernel replaces the Gaussian kinds (pw_set_kernel)
+  const int launch_kind = c->smoothing != PW_KERNEL_HIGHORDER ? kind
+                          : kind == pw::kPairFP32C            ? pw::kPairFP32CHO
+                                                              : pw::kPairFP64HO;
   const int grid = c->num_cus * pw::pair_kernel_waves(launch_kind, k.cfg);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c->profiling) {

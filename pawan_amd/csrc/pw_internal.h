@@ -63,6 +63,7 @@ enum PairKind : int {
   kPairFP64HO = 4,     // FP64, high-order algebraic kernel (highorder.h);
                        // launched in place of kinds 0 and 1 when a context
                        // uses that kernel (3 is the tree path's kind)
+  kPairFP32CHO = 5,    // FP32C with that kernel, in place of kind 2
 };
 
 // Pair kernel of `kind` over args.items; for parts > 1 also the reduction

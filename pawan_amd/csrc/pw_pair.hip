@@ -577,6 +577,42 @@ __device__ __forceinline__ void pair32(const T32& t, const S& s, P32& p, PSE32& 
     accumulate32(dx, dy, dz, q, f, t.ax, t.ay, t.az, s, p);
 }
 
+// FP32C pair with the high-order algebraic kernel (highorder.h:20-67), in
+// the scaling of the Gaussian FP32 pair (q' = 4 pi q, f' = -(4 pi/3) F):
+// with b = rho^2 + sigma^2, q' = (rho^2 + 5/2 sigma^2) b^-5/2 and
+// f' = (rho^2 + 7/2 sigma^2) b^-7/2, both exact for every pair.  The PSE
+// term 2 nu eta / sigma^2 = 2 nu (105/8pi) sigma^4 b^-9/2 is summed for the
+// pairs within 44 sigma^2 (beyond, each term is below 4e-8 of the nearest
+// one and all of them together about 1e-6 of the sum), as the Gaussian pair
+// does; so is the per-pair stretching choice.
+constexpr float kHoEta32 = 0x1.0b615cp+2f;  // 105/(8 pi)
+
+template <typename S>
+__device__ __forceinline__ void pair32_ho(const T32& t, const S& s, P32& p, PSE32& e) {
+  const float dx = static_cast<float>(t.x - s.x);
+  const float dy = static_cast<float>(t.y - s.y);
+  const float dz = static_cast<float>(t.z - s.z);
+  const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+  const float sig2 = s.hs2 + t.hs2;
+  const float ib = __builtin_amdgcn_rsqf(r2 + sig2);
+  const float ib2 = ib * ib;
+  const float ib5 = ib2 * ib2 * ib;
+  const float ib7 = ib5 * ib2;
+  const float q = fmaf(2.5f, sig2, r2) * ib5;
+  const float f = fmaf(3.5f, sig2, r2) * ib7;
+  const bool core = r2 <= kFar32 * sig2;
+  if (core) {
+    const float c = t.nu2 * kHoEta32 * (sig2 * sig2) * (ib7 * ib2);
+    e.x = fmaf(c, t.vol * s.ax - s.vol * t.ax, e.x);
+    e.y = fmaf(c, t.vol * s.ay - s.vol * t.ay, e.y);
+    e.z = fmaf(c, t.vol * s.az - s.vol * t.az, e.z);
+  }
+  if (__ballot(core))
+    accumulate32_pair(dx, dy, dz, q, f, t.ax, t.ay, t.az, s, p);
+  else
+    accumulate32(dx, dy, dz, q, f, t.ax, t.ay, t.az, s, p);
+}
+
 // Far-only pair on positions relative to the target leaf's centre, rounded
 // to FP32 once per particle: for sources beyond 44 sigma^2 the rounding
 // (about 6e-8 of the distance to the centre) is a relative displacement
@@ -663,6 +699,12 @@ __device__ __forceinline__ double uniform(double v) {
   return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
 }
 
+// HighOrder: the algebraic kernel; separated tiles then need a gap beyond
+// 4400 sigma^2, where its q' and f' are within 1e-7 of the singular law
+// (the relative difference is (15/8)(sigma/rho)^4).
+constexpr float kFarHO32 = 4400.0f;
+
+template <bool HighOrder>
 __device__ __forceinline__ void item_fp32c(const PairArgs& pa, const WorkItem& it, SrcT2* tile) {
   __shared__ double s_tb[4][6];
   __shared__ float s_b[4][8];
@@ -751,7 +793,8 @@ __device__ __forceinline__ void item_fp32c(const PairArgs& pa, const WorkItem& i
     const float h2S = uniform(fmaxf(fmaxf(s_b[0][6], s_b[1][6]), fmaxf(s_b[2][6], s_b[3][6])));
     const float rS = uniform(sqrtf(rs2) * 1.00001f);
     const float gap = uniform(sqrtf(cs2) * 0.99999f - rT - rS);
-    const bool sep = gap > 0.0f && gap * gap > kFar32 * (h2T + h2S) * 1.0001f && gap > kSepFP32 * (rT + rS);
+    const float far_bound = HighOrder ? kFarHO32 : kFar32;
+    const bool sep = gap > 0.0f && gap * gap > far_bound * (h2T + h2S) * 1.0001f && gap > kSepFP32 * (rT + rS);
     const bool diag = base == it.t0;
     const int self = static_cast<int>(threadIdx.x);
     for (int c0 = 0; c0 < cnt; c0 += kChunk) {
@@ -762,10 +805,16 @@ __device__ __forceinline__ void item_fp32c(const PairArgs& pa, const WorkItem& i
         for (int l = c0; l < c1; ++l) pair32_rel(rtx, rty, rtz, me.ax, me.ay, me.az, tile[l], p);
       } else if (diag) {
         for (int l = c0; l < c1; ++l)
-          if (l != self) pair32(me, tile[l], p, pse);
+          if (l != self) {
+            if (HighOrder) pair32_ho(me, tile[l], p, pse);
+            else pair32(me, tile[l], p, pse);
+          }
       } else {
 #pragma unroll 1
-        for (int l = c0; l < c1; ++l) pair32(me, tile[l], p, pse);
+        for (int l = c0; l < c1; ++l) {
+          if (HighOrder) pair32_ho(me, tile[l], p, pse);
+          else pair32(me, tile[l], p, pse);
+        }
       }
       fold(p, me.ax, me.ay, me.az, hi, lo);
     }
@@ -978,6 +1027,10 @@ template <>
 struct TileOf<kPairFP32C> {
   using type = SrcT2;
 };
+template <>
+struct TileOf<kPairFP32CHO> {
+  using type = SrcT2;
+};
 
 template <int Kind>
 __device__ __forceinline__ void run_item(const PairArgs& a, const WorkItem& it,
@@ -988,8 +1041,10 @@ __device__ __forceinline__ void run_item(const PairArgs& a, const WorkItem& it,
     item_fp64(a, it, tile);
   else if constexpr (Kind == kPairFP64HO)
     item_fp64_ho(a, it, tile);
+  else if constexpr (Kind == kPairFP32CHO)
+    item_fp32c<true>(a, it, tile);
   else
-    item_fp32c(a, it, tile);
+    item_fp32c<false>(a, it, tile);
 }
 
 template <int Kind, bool Persistent>
@@ -1027,6 +1082,13 @@ __global__ __launch_bounds__(kTB) void k_pair_fp32c(PairArgs a) { pair_entry<kPa
 template <bool P>
 __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_pair_fp32c_w8(PairArgs a) {
   pair_entry<kPairFP32C, P>(a);
+}
+// the same two builds with the high-order algebraic kernel
+template <bool P>
+__global__ __launch_bounds__(kTB) void k_pair_fp32c_ho(PairArgs a) { pair_entry<kPairFP32CHO, P>(a); }
+template <bool P>
+__global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_pair_fp32c_ho_w8(PairArgs a) {
+  pair_entry<kPairFP32CHO, P>(a);
 }
 
 // rates[t] = sum over parts of partial[p][t], parts in order (FP64).
@@ -1068,6 +1130,15 @@ const bool P = cfg.persistent;
     case kPairFP64HO:
       if (P) hipLaunchKernelGGL(k_pair_fp64_ho<true>, g, block, 0, st, args);
       else hipLaunchKernelGGL(k_pair_fp64_ho<false>, g, block, 0, st, args);
+      break;
+    case kPairFP32CHO:
+      if (cfg.waves8) {
+        if (P) hipLaunchKernelGGL(k_pair_fp32c_ho_w8<true>, g, block, 0, st, args);
+        else hipLaunchKernelGGL(k_pair_fp32c_ho_w8<false>, g, block, 0, st, args);
+      } else {
+        if (P) hipLaunchKernelGGL(k_pair_fp32c_ho<true>, g, block, 0, st, args);
+        else hipLaunchKernelGGL(k_pair_fp32c_ho<false>, g, block, 0, st, args);
+      }
       break;
     case kPairFP32C:
       if (cfg.waves8) {
@@ -1135,6 +1206,9 @@ int pair_kernel_waves(int kind, const PairConfig& cfg) {
   if (kind == kPairFP32C)
     k = cfg.waves8 ? reinterpret_cast<const void*>(k_pair_fp32c_w8<true>)
                    : reinterpret_cast<const void*>(k_pair_fp32c<true>);
+  else if (kind == kPairFP32CHO)
+    k = cfg.waves8 ? reinterpret_cast<const void*>(k_pair_fp32c_ho_w8<true>)
+                   : reinterpret_cast<const void*>(k_pair_fp32c_ho<true>);
   else if (kind == kPairFP64)
     k = reinterpret_cast<const void*>(k_pair_fp64<true>);
   else if (kind == kPairFP64HO)
