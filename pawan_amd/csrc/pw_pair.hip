@@ -643,6 +643,24 @@ __device__ __forceinline__ void pair32_rel(float tx, float ty, float tz, float t
   accumulate32(dx, dy, dz, q, q * ir2, tax, tay, taz, s, p);
 }
 
+// The same with the high-order algebraic kernel, exact at any distance:
+// q' = (rho^2 + 5/2 sigma^2) b^-5/2, f' = (rho^2 + 7/2 sigma^2) b^-7/2 with
+// b = rho^2 + sigma^2, sigma^2 = the source's s^2/2 + th2 (the target's).
+template <typename S>
+__device__ __forceinline__ void pair32_rel_ho(float tx, float ty, float tz, float th2, float tax, float tay,
+                                              float taz, const S& s, P32& p) {
+  const float dx = tx - s.rx;
+  const float dy = ty - s.ry;
+  const float dz = tz - s.rz;
+  const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+  const float sig2 = s.hs2 + th2;
+  const float ib = __builtin_amdgcn_rsqf(r2 + sig2);
+  const float ib2 = ib * ib;
+  const float ib5 = ib2 * ib2 * ib;
+  accumulate32(dx, dy, dz, fmaf(2.5f, sig2, r2) * ib5, fmaf(3.5f, sig2, r2) * (ib5 * ib2), tax, tay, taz, s,
+               p);
+}
+
 // Knuth's two-sum: (hi, lo) += p with the rounding error of hi + p kept.
 __device__ __forceinline__ void two_sum(float& hi, float& lo, float p) {
   const float s = hi + p;
@@ -699,11 +717,10 @@ __device__ __forceinline__ double uniform(double v) {
   return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
 }
 
-// HighOrder: the algebraic kernel; separated tiles then need a gap beyond
-// 4400 sigma^2, where its q' and f' are within 1e-7 of the singular law
-// (the relative difference is (15/8)(sigma/rho)^4).
-constexpr float kFarHO32 = 4400.0f;
-
+// HighOrder: the algebraic kernel (pair32_ho, and pair32_rel_ho on the
+// separated tiles, which need the same gap as the Gaussian's: it keeps the
+// relative positions' rounding small and the PSE terms beyond it are
+// dropped in both kernels).
 template <bool HighOrder>
 __device__ __forceinline__ void item_fp32c(const PairArgs& pa, const WorkItem& it, SrcT2* tile) {
   __shared__ double s_tb[4][6];
@@ -793,16 +810,20 @@ __device__ __forceinline__ void item_fp32c(const PairArgs& pa, const WorkItem& i
     const float h2S = uniform(fmaxf(fmaxf(s_b[0][6], s_b[1][6]), fmaxf(s_b[2][6], s_b[3][6])));
     const float rS = uniform(sqrtf(rs2) * 1.00001f);
     const float gap = uniform(sqrtf(cs2) * 0.99999f - rT - rS);
-    const float far_bound = HighOrder ? kFarHO32 : kFar32;
-    const bool sep = gap > 0.0f && gap * gap > far_bound * (h2T + h2S) * 1.0001f && gap > kSepFP32 * (rT + rS);
+    const bool sep = gap > 0.0f && gap * gap > kFar32 * (h2T + h2S) * 1.0001f && gap > kSepFP32 * (rT + rS);
     const bool diag = base == it.t0;
     const int self = static_cast<int>(threadIdx.x);
     for (int c0 = 0; c0 < cnt; c0 += kChunk) {
       const int c1 = min(cnt, c0 + kChunk);
       P32 p;
       if (sep) {
+        if (HighOrder) {
 #pragma unroll 2
-        for (int l = c0; l < c1; ++l) pair32_rel(rtx, rty, rtz, me.ax, me.ay, me.az, tile[l], p);
+          for (int l = c0; l < c1; ++l) pair32_rel_ho(rtx, rty, rtz, me.hs2, me.ax, me.ay, me.az, tile[l], p);
+        } else {
+#pragma unroll 2
+          for (int l = c0; l < c1; ++l) pair32_rel(rtx, rty, rtz, me.ax, me.ay, me.az, tile[l], p);
+        }
       } else if (diag) {
         for (int l = c0; l < c1; ++l)
           if (l != self) {

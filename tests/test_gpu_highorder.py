@@ -137,8 +137,8 @@ def test_fp32c_rhs(ho):
 
 
 def test_fp32c_separated_tiles_on_the_helix(ho):
-    """On the 1M helix most tile pairs take the separated loop, whose
-    singular-law form needs a gap beyond 4400 sigma^2 for this kernel:
+    """On the 1M helix most tile pairs take the separated loop (FP32
+    positions relative to the targets' centre, the kernel evaluated exactly):
     sampled targets against the oracle."""
     s = W.helix_wake(1 << 20)
     idx = np.random.default_rng(3).choice(s.n, size=48, replace=False)
