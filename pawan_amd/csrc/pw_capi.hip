@@ -612,6 +612,7 @@ int pw_velocity_at(pw_ctx* c, const double* points, int64_t m, double* u) {
                                   c->smoothing == PW_KERNEL_HIGHORDER, c->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(u, d_u, len * sizeof(double), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  else (void)hipStreamSynchronize(c->stream);  // nothing queued may still use the buffers
   (void)hipFree(d_pts);
   (void)hipFree(d_u);
   if (e != hipSuccess) return fail(PW_ERR_HIP, std::string("pw_velocity_at: ") + hipGetErrorString(e));
@@ -702,6 +703,7 @@ int pw_vring_resolve(pw_ctx* c, int wake, int nlayers, int nazi, double R, doubl
     e = hipMemcpyAsync(influence, d_M, mm * sizeof(double), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(&flag, d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  else (void)hipStreamSynchronize(c->stream);  // nothing queued may still use the buffers
   (void)hipFree(d_M); (void)hipFree(d_lu); (void)hipFree(d_b); (void)hipFree(d_s); (void)hipFree(d_flag);
   if (e != hipSuccess) return fail(PW_ERR_HIP, std::string("pw_vring_resolve: ") + hipGetErrorString(e));
   if (flag) return fail(PW_ERR_NONFINITE, "pw_vring_resolve: singular influence matrix");
