@@ -67,13 +67,16 @@ __global__ __launch_bounds__(kTB) void k_vring_influence(const Rec64* __restrict
 
 // Solve A x = b (n x n, row-major, overwritten by its LU factors) with one
 // block: partial pivoting on the largest |a_ij| at or below the diagonal
-// (lowest row on ties), rows updated in parallel, then the two triangular
-// solves by one lane.  flag[0] = 1 if a pivot is zero.
+// (lowest row on ties); each elimination step stores the multipliers in
+// column j, then updates the trailing submatrix with the lanes along its
+// rows (coalesced); the two triangular solves go column by column with the
+// lanes over the rows, x held in LDS.  flag[0] = 1 if a pivot is zero.
 __global__ __launch_bounds__(kTB) void k_lu_solve(double* __restrict__ A, int n,
                                                   const double* __restrict__ b,
                                                   double* __restrict__ x, int* __restrict__ flag) {
 #pragma clang fp contract(off)
   __shared__ int perm[kVringMaxCells];
+  __shared__ double xs[kVringMaxCells];
   __shared__ double red_v[kTB];
   __shared__ int red_i[kTB];
   const int tid = static_cast<int>(threadIdx.x);
@@ -121,24 +124,34 @@ __global__ __launch_bounds__(kTB) void k_lu_solve(double* __restrict__ A, int n,
       __syncthreads();
       continue;
     }
-    for (int r = j + 1 + tid; r < n; r += kTB) {
+    for (int r = j + 1 + tid; r < n; r += kTB) A[static_cast<size_t>(r) * n + j] /= piv;
+    __syncthreads();
+    const int w = n - j - 1;
+    const double* prow = A + static_cast<size_t>(j) * n;
+    for (int e = tid; e < w * w; e += kTB) {
+      const int r = j + 1 + e / w, c = j + 1 + e % w;
       double* row = A + static_cast<size_t>(r) * n;
-      const double* prow = A + static_cast<size_t>(j) * n;
-      const double l = row[j] / piv;
-      row[j] = l;
-      for (int c = j + 1; c < n; ++c) row[c] -= l * prow[c];
+      row[c] -= row[j] * prow[c];
     }
     __syncthreads();
   }
-  if (tid == 0) {
-    for (int r = 0; r < n; ++r) x[r] = b[perm[r]];
-    for (int r = 0; r < n; ++r)
-      for (int c = 0; c < r; ++c) x[r] -= A[static_cast<size_t>(r) * n + c] * x[c];
-    for (int r = n - 1; r >= 0; --r) {
-      for (int c = r + 1; c < n; ++c) x[r] -= A[static_cast<size_t>(r) * n + c] * x[c];
-      x[r] /= A[static_cast<size_t>(r) * n + r];
-    }
+  for (int r = tid; r < n; r += kTB) xs[r] = b[perm[r]];
+  __syncthreads();
+  // L y = P b (unit diagonal), column by column
+  for (int c = 0; c + 1 < n; ++c) {
+    const double xc = xs[c];
+    for (int r = c + 1 + tid; r < n; r += kTB) xs[r] -= A[static_cast<size_t>(r) * n + c] * xc;
+    __syncthreads();
   }
+  // U x = y, column by column from the last
+  for (int c = n - 1; c >= 0; --c) {
+    if (tid == 0) xs[c] /= A[static_cast<size_t>(c) * n + c];
+    __syncthreads();
+    const double xc = xs[c];
+    for (int r = tid; r < c; r += kTB) xs[r] -= A[static_cast<size_t>(r) * n + c] * xc;
+    __syncthreads();
+  }
+  for (int r = tid; r < n; r += kTB) x[r] = xs[r];
 }
 
 // a = s_i (-sin psi_a, cos psi_a, 0) for the ring's nazi * m particles
