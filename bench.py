@@ -11,21 +11,33 @@ interactions (velocity, transposed stretching, PSE diffusion).
 
 `value` is the FP32-compensated direct path (FP32 pair math, compensated
 per-target accumulation, FP64 state); the FP64 direct path is timed beside
-it, with the FP32C error against it.  The pair kernel's average launch time
-is measured with HIP events recorded by the library on its own stream.
+it, with the FP32C error against it.  Kernel times come from HIP events the
+library records on its own stream.
+
+Errors follow one definition throughout: e = max_i |x_i - x_ref,i| /
+max_i |x_ref,i| over particles (with the RMS ratio beside it), against the
+FP64 direct sum on the same state.
 
 The learned leg trains the far-field network on samples the solver
 generates from helical wakes of other seeds, then, for the tree path with
 the analytic expansion only and with the network, searches the largest
-tolerance whose velocity error against the direct sum is <= 1e-4 and times
-RK4 steps at it.  The drift leg runs 200 RK4 steps of the leapfrogging rings
-(N = 65,536) with FP64, FP32C and the tree path and reports the drift of
-total vorticity, linear impulse and kinetic energy.  The CPU baseline is the
-oracle's restatement of the reference's OpenMP loops (parallel.cpp), timed
-on a sample of source rows of the same system and extrapolated to one RK4
-step.
+tolerance whose velocity error against the FP64 direct sum is <= 1e-4 and
+times RK4 steps at it.  The drift leg runs 200 RK4 steps of the
+leapfrogging rings (N = 65,536) at nu = 1e-4 and nu = 0 with FP64, FP32C and
+the tree path side by side: every 10 steps it records each path's
+conserved quantities against FP64's at the same time, and the velocity
+error of its right-hand side against FP64 on its own state.  The CPU
+baseline is the oracle's restatement of the reference's OpenMP loops
+(parallel.cpp), timed on a sample of source rows of the same system and
+extrapolated to one RK4 step.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Other measurements (not part of the default run):
+  --trajectory          FP32C and FP64 RK4 trajectories of the 1M helical
+                        wake side by side (50 steps), compared every 10
+  --profile-rhs         right-hand sides of one workload and a plain copy of
+                        known size, for rocprofv3 counter runs
+  --profile-vring       ring strength resolutions, for rocprofv3 traces
 """
 from __future__ import annotations
 
@@ -45,6 +57,8 @@ METRIC = "RK4 steps/s at N=1M: FP64 direct vs FP32-comp vs learned far-field @1e
 FLOP_PER_PAIR = 88            # algorithmic FLOP per ordered pair (FMA = 2)
 PEAK_FP32_TFLOPS = 157.3      # MI355X published FP32 vector peak
 PEAK_FP64_TFLOPS = 78.6       # MI355X published FP64 vector peak
+PEAK_BF16_TFLOPS = 2500.0     # MI355X published dense BF16 matrix-core peak
+FLOP_PER_MLP_EVAL = 72704     # 16-128-128-128-12 network, FMA = 2
 
 
 def log(msg: str) -> None:
@@ -105,22 +119,38 @@ def train_network(n_train: int, samples: int, steps: int, theta: float, seed: in
     return weights, rep, time.perf_counter() - t0
 
 
-def calibrate_tol(g, target_err: float, ref_u: np.ndarray, use_mlp: bool, theta: float, urms: float):
+def maxnorm(a: np.ndarray, b: np.ndarray) -> float:
+    """max_i |a_i - b_i| / max_i |b_i| over particles (rows)."""
+    return float(np.max(np.linalg.norm(a - b, axis=1)) / np.max(np.linalg.norm(b, axis=1)))
+
+
+def rmsnorm(a: np.ndarray, b: np.ndarray) -> float:
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def path_errors(u, w, ref_u, ref_w) -> dict:
+    return {"velocity_maxnorm": maxnorm(u, ref_u), "velocity_rms": rmsnorm(u, ref_u),
+            "stretching_maxnorm": maxnorm(w, ref_w), "stretching_rms": rmsnorm(w, ref_w)}
+
+
+def calibrate_tol(g, target_err: float, ref_u: np.ndarray, ref_w: np.ndarray, use_mlp: bool, theta: float,
+                  urms: float):
     """Largest tolerance (geometric search) whose velocity error against the
-    direct sum is at most target_err; returns (tol, err, info)."""
+    FP64 direct sum, max_i |du_i| / max_i |u_i|, is at most target_err;
+    returns (tol, errors, info) or None."""
     def err_at(tol):
         g.ff_set_params(theta, tol, use_mlp=use_mlp)
-        u, _ = g.solve("learned")
-        return float(np.linalg.norm(u - ref_u) / np.linalg.norm(ref_u)), g.ff_info()
+        u, w = g.solve("learned")
+        return path_errors(u, w, ref_u, ref_w), g.ff_info()
 
     lo, hi = None, None
     tol = target_err * urms
     best = None
-    for _ in range(14):
+    for _ in range(16):
         e, info = err_at(tol)
-        log(f"  {'mlp' if use_mlp else 'md '} tol {tol:.3e}: err {e:.3e} near {info['near_pairs']:.3e} "
-            f"far {info['far_evals']:.3e}")
-        if e <= target_err:
+        log(f"  {'mlp' if use_mlp else 'md '} tol {tol:.3e}: err {e['velocity_maxnorm']:.3e} "
+            f"(rms {e['velocity_rms']:.2e}) near {info['near_pairs']:.3e} far {info['far_evals']:.3e}")
+        if e["velocity_maxnorm"] <= target_err:
             lo = tol
             best = (tol, e, info)
             if hi is not None and hi / lo < 1.3:
@@ -141,15 +171,16 @@ def learned_leg(system, dt: float, steps: int, target_err: float, theta: float, 
     weights, rep, ttrain = train_network(train_n, train_samples, train_steps, theta)
     log(f"network trained in {ttrain:.1f} s: {rep}")
     out = {"training": dict(rep, seconds=ttrain, samples=train_samples, wake_particles=train_n),
-           "theta": theta, "target_error": target_err}
+           "theta": theta, "target_error": target_err,
+           "error_metric": "max_i |du_i| / max_i |u_i| against the FP64 direct sum on the same state"}
     with capi.Solver(system) as g:
-        ref_u, _ = g.solve("fp32c")
+        ref_u, ref_w = g.solve("fp64")
         urms = float(np.sqrt((ref_u * ref_u).sum(axis=1).mean()))
         g.ff_set_weights(weights)
         g.ff_set_err_coef(rep["coef_md"], rep["coef_mlp"])
         for use_mlp in (False, True):
             key = "mlp" if use_mlp else "analytic"
-            best = calibrate_tol(g, target_err, ref_u, use_mlp, theta, urms)
+            best = calibrate_tol(g, target_err, ref_u, ref_w, use_mlp, theta, urms)
             if best is None:
                 out[key] = None
                 continue
@@ -164,12 +195,28 @@ def learned_leg(system, dt: float, steps: int, target_err: float, theta: float, 
             g.synchronize()
             wall = time.perf_counter() - t0
             ms, launches = g.pair_timing()
+            stages = g.ff_timing()
             g.set_profiling(False)
-            out[key] = {"steps_per_s": steps / wall, "rhs_ms": ms / max(1, launches), "velocity_error": err,
-                        "near_pairs_per_s": info["near_pairs"] / (ms / max(1, launches) * 1e-3),
+            nrhs = max(1, launches)
+            rhs_ms = ms / nrhs
+            st = {k: v / max(1.0, stages["rhs"]) for k, v in stages.items() if k != "rhs"}
+            out[key] = {"steps_per_s": steps / wall, "rhs_ms": rhs_ms,
+                        "velocity_error": err["velocity_maxnorm"],
+                        "velocity_error_maxnorm": err["velocity_maxnorm"], "errors_vs_fp64": err,
+                        "near_pairs_per_s": info["near_pairs"] / (st["near_ms"] * 1e-3),
                         "tol_abs": float(tol), "near_pairs": info["near_pairs"], "far_evals": info["far_evals"],
-                        "max_far_nodes": info["max_far"], "max_near_leaves": info["max_near"],
-                        "near_core_pairs": info["near_core_pairs"]}
+                        "mlp_evals": info["mlp_evals"], "max_far_nodes": info["max_far"],
+                        "max_near_leaves": info["max_near"], "near_core_pairs": info["near_core_pairs"],
+                        "stage_ms_per_rhs": st}
+            if use_mlp and st["far_mlp_ms"] > 0:
+                flop = FLOP_PER_MLP_EVAL * info["mlp_evals"]
+                ach = flop / (st["far_mlp_ms"] * 1e-3) / 1e12
+                out[key]["roofline_mlp"] = {
+                    "bound": "mfma", "kernel": "k_far_mlp", "achieved": ach, "peak": PEAK_BF16_TFLOPS,
+                    "unit": "TFLOP/s", "frac": ach / PEAK_BF16_TFLOPS, "traffic": None,
+                    "flop_per_launch": flop, "avg_launch_ms": st["far_mlp_ms"],
+                    "work": f"{info['mlp_evals']} network evaluations (one per target leaf and far node) "
+                            f"x {FLOP_PER_MLP_EVAL} FLOP"}
             log(f"{key}: {out[key]}")
             if use_mlp:
                 out["_drift_cfg"] = {"weights": weights, "coef_md": rep["coef_md"], "coef_mlp": rep["coef_mlp"],
@@ -179,49 +226,99 @@ def learned_leg(system, dt: float, steps: int, target_err: float, theta: float, 
     return out
 
 
-def drift_leg(steps: int, every: int, learned_cfg: dict | None, target_err: float):
-    """Conserved-quantity drift over `steps` RK4 steps of the leapfrogging
-    rings (two coupled wakes, N = 65,536, dt = 0.002) for FP64, FP32C and
-    (when configured) the tree path with the network: the maximum over the
-    run of |Omega(t) - Omega(0)| / sum|alpha|, |I(t) - I(0)| / |I(0)| and
-    |KE(t) - KE(0)| / |KE(0)|, diagnosed every `every` steps in FP64.  The
-    tree path's tolerance is calibrated on this system to the same velocity
-    error target as at 1M."""
+def _diag_terms(system, u0: np.ndarray) -> dict:
+    """Scale of the terms each conserved quantity sums, for normalising
+    differences of quantities that are ~0 on these rings."""
+    a = np.linalg.norm(system.alpha, axis=1)
+    r = np.linalg.norm(system.pos, axis=1)
+    return {"omega": float(a.sum()), "angular_impulse": float((r * r * a).sum()),
+            "helicity": float((a * np.linalg.norm(u0, axis=1)).sum())}
+
+
+def _quantities(d: np.ndarray) -> dict:
+    return {"omega": d[0:3], "impulse": d[3:6], "angular_impulse": d[6:9], "helicity": d[9:10], "ke": d[12:13]}
+
+
+def drift_leg(steps: int, every: int, learned_cfg: dict | None, target_err: float, nus=(1e-4, 0.0)):
+    """The leapfrogging rings (two coupled wakes, N = 65,536, dt = 0.002),
+    advanced `steps` RK4 steps by FP64, FP32C and (when configured) the tree
+    path with the network, side by side, at each viscosity in `nus`.  Every
+    `every` steps, for each path:
+      * dev_Q: |Q_path(t) - Q_FP64(t)| / S_Q, the distance of its conserved
+        quantities from FP64's at the same time (S_Q = |Q(0)| for the linear
+        impulse and the kinetic energy, and the size of the summed terms for
+        the quantities that are ~0 here: sum|alpha| for the total vorticity,
+        sum|x|^2|alpha| for the angular impulse, sum|alpha||u| for the
+        helicity);
+      * drift_Q: |Q_path(t) - Q_path(0)| / S_Q;
+      * the velocity error of its right-hand side against an FP64 right-hand
+        side on its own state (max-norm and RMS).
+    The tree path's tolerance is calibrated once on this system to the same
+    velocity error target as at 1M."""
     from pawan_amd import capi, workloads
 
     spec = workloads.WORKLOADS["leapfrog_64k"]
-    system, dt = spec["make"](), spec["dt"]
-    s_alpha = float(np.linalg.norm(system.alpha, axis=1).sum())
-    out = {"workload": "leapfrog_64k", "particles": system.n, "steps": steps, "dt": dt, "every": every}
-    variants = [("fp64", None), ("fp32c", None)]
-    if learned_cfg is not None:
-        variants.append(("learned", learned_cfg))
-    for prec, cfg in variants:
+    dt = spec["dt"]
+    out = {"workload": "leapfrog_64k", "steps": steps, "dt": dt, "every": every, "runs": []}
+    tol_abs = None
+    for nu in nus:
+        system = spec["make"]()
+        system.nu = nu
+        run = {"nu": nu, "particles": system.n, "t": [], "paths": {}}
         t0 = time.perf_counter()
-        with capi.Solver(system) as g:
-            if cfg is not None:
-                g.ff_set_weights(cfg["weights"])
-                g.ff_set_err_coef(cfg["coef_md"], cfg["coef_mlp"])
-                ref, _ = g.solve("fp32c")
-                urms = float(np.sqrt((ref * ref).sum(axis=1).mean()))
-                best = calibrate_tol(g, target_err, ref, True, cfg["theta"], urms)
-                if best is None:
-                    out[prec] = None
-                    continue
-                g.ff_set_params(cfg["theta"], best[0], use_mlp=True)
-                out["learned_velocity_error"] = best[1]
-            else:
-                g.set_graph(True)
-            d0 = g.diagnose()
-            dom = di = dke = 0.0
-            for k in range(0, steps, every):
-                g.step("rk4", dt, prec, min(every, steps - k))
-                d = g.diagnose()
-                dom = max(dom, float(np.linalg.norm(d[0:3] - d0[0:3])) / s_alpha)
-                di = max(di, float(np.linalg.norm(d[3:6] - d0[3:6]) / np.linalg.norm(d0[3:6])))
-                dke = max(dke, float(abs(d[12] - d0[12]) / abs(d0[12])))
-        out[prec] = {"omega": dom, "impulse": di, "ke": dke, "seconds": time.perf_counter() - t0}
-        log(f"drift {prec}: {out[prec]}")
+        paths = ["fp64", "fp32c"] + (["learned"] if learned_cfg is not None else [])
+        ctx = {p: capi.Solver(system) for p in paths}
+        try:
+            u0, _ = ctx["fp64"].solve("fp64")
+            scale = _diag_terms(system, u0)
+            for p, g in ctx.items():
+                if p == "learned":
+                    g.ff_set_weights(learned_cfg["weights"])
+                    g.ff_set_err_coef(learned_cfg["coef_md"], learned_cfg["coef_mlp"])
+                    if tol_abs is None:
+                        ref_u, ref_w = g.solve("fp64")
+                        urms = float(np.sqrt((ref_u * ref_u).sum(axis=1).mean()))
+                        best = calibrate_tol(g, target_err, ref_u, ref_w, True, learned_cfg["theta"], urms)
+                        if best is None:
+                            raise RuntimeError("no tolerance meets the target on the rings")
+                        tol_abs = best[0]
+                        out["learned_tol_abs"] = tol_abs
+                        out["learned_velocity_error_t0"] = best[1]
+                    g.ff_set_params(learned_cfg["theta"], tol_abs, use_mlp=True)
+                else:
+                    g.set_graph(True)
+                run["paths"][p] = {"dev": {}, "drift": {}, "e_u_maxnorm": [], "e_u_rms": []}
+            q0 = {p: _quantities(g.diagnose()) for p, g in ctx.items()}
+            for k in range(0, steps + 1, every):
+                if k > 0:
+                    for p, g in ctx.items():
+                        g.step("rk4", dt, p, every)
+                run["t"].append(k * dt)
+                q64 = _quantities(ctx["fp64"].diagnose())
+                for p, g in ctx.items():
+                    q = _quantities(g.diagnose()) if p != "fp64" else q64
+                    rec = run["paths"][p]
+                    for name in q:
+                        sc = scale.get(name, float(np.linalg.norm(q0["fp64"][name])))
+                        rec["dev"].setdefault(name, []).append(float(np.linalg.norm(q[name] - q64[name])) / sc)
+                        rec["drift"].setdefault(name, []).append(float(np.linalg.norm(q[name] - q0[p][name])) / sc)
+                    if p != "fp64":
+                        up, _ = g.solve(p)
+                        ur, _ = g.solve("fp64")
+                        rec["e_u_maxnorm"].append(maxnorm(up, ur))
+                        rec["e_u_rms"].append(rmsnorm(up, ur))
+        finally:
+            for g in ctx.values():
+                g.close()
+        for p, rec in run["paths"].items():
+            rec["max_dev"] = {k: max(v) for k, v in rec["dev"].items()}
+            rec["max_drift"] = {k: max(v) for k, v in rec["drift"].items()}
+            if rec["e_u_maxnorm"]:
+                rec["max_e_u_maxnorm"] = max(rec["e_u_maxnorm"])
+            log(f"drift nu={nu} {p}: max dev {rec['max_dev']} max drift {rec['max_drift']}"
+                + (f" max e_u {rec['max_e_u_maxnorm']:.2e}" if rec["e_u_maxnorm"] else ""))
+        run["seconds"] = time.perf_counter() - t0
+        out["runs"].append(run)
     return out
 
 
@@ -325,6 +422,122 @@ def cpu_baseline(system, seconds: float):
     }
 
 
+def trajectory(steps: int, every: int, n: int, tune: bool) -> dict:
+    """FP32C and FP64 RK4 trajectories of the helical wake (N = n, dt =
+    1e-3) side by side, graph-replayed with autotuned launches.  Every
+    `every` steps: the FP32C state against the FP64 state (positions as the
+    displacement since t = 0, max-norm; strengths max-norm), the FP32C
+    right-hand side against an FP64 one on the FP32C state, and the
+    conserved quantities of both."""
+    from pawan_amd import capi, workloads
+
+    system, dt = workloads.helix_wake(n), 1e-3
+    out = {"workload": f"helix_{n}", "steps": steps, "dt": dt, "every": every, "t": [],
+           "displacement_maxnorm": [], "alpha_maxnorm": [], "rhs_velocity_maxnorm": [],
+           "rhs_stretching_maxnorm": [], "diag_fp64": [], "diag_fp32c": []}
+    t0 = time.perf_counter()
+    with capi.Solver(system) as g64, capi.Solver(system) as g32:
+        if tune:
+            g64.autotune("fp64")
+            g32.autotune("fp32c")
+        g64.set_graph(True)
+        g32.set_graph(True)
+        for k in range(0, steps + 1, every):
+            if k > 0:
+                for j in range(every):
+                    g64.step("rk4", dt, "fp64", 1)
+                    g32.step("rk4", dt, "fp32c", 1)
+                    g32.synchronize()
+                    log(f"trajectory step {k - every + j + 1}/{steps}: {time.perf_counter() - t0:.1f} s")
+            p64, a64 = g64.state()
+            p32, a32 = g32.state()
+            out["t"].append(k * dt)
+            if k > 0:
+                out["displacement_maxnorm"].append(maxnorm(p32 - system.pos, p64 - system.pos))
+            else:
+                out["displacement_maxnorm"].append(0.0)
+            out["alpha_maxnorm"].append(maxnorm(a32, a64))
+            u32, w32 = g32.solve("fp32c")
+            u64, w64 = g32.solve("fp64")
+            out["rhs_velocity_maxnorm"].append(maxnorm(u32, u64))
+            out["rhs_stretching_maxnorm"].append(maxnorm(w32, w64))
+            out["diag_fp64"].append(g64.diagnose()[:15].tolist())
+            out["diag_fp32c"].append(g32.diagnose()[:15].tolist())
+            log(f"trajectory t={k * dt:.3f}: displacement {out['displacement_maxnorm'][-1]:.2e} "
+                f"alpha {out['alpha_maxnorm'][-1]:.2e} rhs u {out['rhs_velocity_maxnorm'][-1]:.2e}")
+    out["seconds"] = time.perf_counter() - t0
+    return out
+
+
+def profile_rhs(args) -> dict:
+    """Right-hand sides of one workload (for rocprofv3 counter and trace
+    runs), after a plain device copy of known size (torch's elementwise copy
+    of --copy-mb MiB of float32): the copy reads and writes a known number
+    of bytes, so counters in KiB per dispatch scale to bytes as factor =
+    counter(copy) / (copy bytes / 1024)."""
+    import torch
+
+    from pawan_amd import capi, workloads
+
+    nbytes = args.copy_mb << 20
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"copy of {nbytes} bytes done")
+    del src, dst
+    system = workloads.helix_wake(args.n) if args.workload == "helix" else workloads.WORKLOADS[args.workload]["make"]()
+    rec = {"mode": "profile-rhs", "workload": system.name, "particles": system.n, "precision": args.precision,
+           "copy_bytes": nbytes, "rhs_s": []}
+    with capi.Solver(system) as g:
+        if args.kernel == "highorder":
+            g.set_kernel(capi.PW_KERNEL_HIGHORDER)
+        if args.precision == "learned":
+            if args.weights:
+                w = np.load(args.weights, allow_pickle=False)
+                g.ff_set_weights(w["weights"])
+                g.ff_set_err_coef(float(w["coef_md"]), float(w["coef_mlp"]))
+            g.ff_set_params(args.theta, args.tol, use_mlp=bool(args.weights))
+        elif args.config:
+            parts, pers, w8 = (int(v) for v in args.config.split(","))
+            g.set_pair_config(args.precision, parts, bool(pers), bool(w8))
+        g.set_profiling(True)
+        for k in range(args.reps):
+            t = time.perf_counter()
+            g.solve(args.precision)
+            rec["rhs_s"].append(time.perf_counter() - t)
+            log(f"{system.name} N={system.n} {args.precision} rhs {k + 1}/{args.reps}: {rec['rhs_s'][-1]:.3f} s")
+        ms, launches = g.pair_timing()
+        rec["kernel_ms_avg"] = ms / max(1, launches)
+        if args.precision == "learned":
+            rec["tree"] = g.ff_info()
+            rec["stages_ms"] = g.ff_timing()
+        g.set_profiling(False)
+    return rec
+
+
+def profile_vring() -> dict:
+    """Strength resolution of two rings, three resolves each: the
+    reference's own (R = 1, core 0.1, 3 layers, 49 stations, sigma 0.1924;
+    49 cells, 2,401 particles) and the largest the entry point takes (15
+    layers, 961 cells, 256 stations: 246,016 particles)."""
+    from pawan_amd import capi
+
+    rec = {"mode": "profile-vring", "rings": []}
+    for nl, nazi, sig in ((3, 49, 0.1924), (15, 256, 0.02)):
+        pos, rad, vol = capi.vring_particles(1.0, 0.1, nl, nazi, sig)
+        times = []
+        with capi.Solver() as g:
+            g.add_wake(pos, np.zeros_like(pos), rad, vol)
+            for k in range(3):
+                t = time.perf_counter()
+                g.vring_resolve(0, nl, nazi, 1.0, 0.1)
+                times.append(time.perf_counter() - t)
+                log(f"vring L={nl} nAzi={nazi} N={len(pos)} resolve {k + 1}/3: {times[-1]:.4f} s")
+        rec["rings"].append({"layers": nl, "stations": nazi, "particles": len(pos), "resolve_s": times})
+    return rec
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -348,6 +561,22 @@ def main() -> int:
     ap.add_argument("--train-steps", type=int, default=40000)
     ap.add_argument("--no-drift", action="store_true")
     ap.add_argument("--drift-steps", type=int, default=200)
+    ap.add_argument("--drift-every", type=int, default=10)
+    ap.add_argument("--save-weights", default="", help="write the trained network (npz) here")
+    # other measurements
+    ap.add_argument("--trajectory", action="store_true", help="FP32C vs FP64 trajectory of the helical wake")
+    ap.add_argument("--trajectory-steps", type=int, default=50)
+    ap.add_argument("--profile-rhs", action="store_true", help="right-hand sides + a copy of known size")
+    ap.add_argument("--profile-vring", action="store_true", help="ring strength resolutions")
+    ap.add_argument("--precision", default="fp32c", choices=["fp64", "fp64_ref", "fp32c", "learned"])
+    ap.add_argument("--reps", type=int, default=1)
+    ap.add_argument("--copy-mb", type=int, default=256)
+    ap.add_argument("--tol", type=float, default=0.0,
+                    help="profile-rhs, tree path: absolute far-field tolerance (0: opening angle only)")
+    ap.add_argument("--weights", default="", help="profile-rhs, tree path: network (npz from --save-weights)")
+    ap.add_argument("--config", default="",
+                    help="profile-rhs: pair-kernel launch shape 'parts,persistent,waves8' (e.g. 8,1,1)")
+    ap.add_argument("--kernel", default="gaussian", choices=["gaussian", "highorder"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
@@ -360,6 +589,16 @@ def main() -> int:
     if capi.device_count() < 1:
         log("no HIP device visible")
         return 1
+    if args.profile_rhs:
+        print(json.dumps(profile_rhs(args)), flush=True)
+        return 0
+    if args.profile_vring:
+        print(json.dumps(profile_vring()), flush=True)
+        return 0
+    if args.trajectory:
+        print(json.dumps({"mode": "trajectory", **trajectory(args.trajectory_steps, 10, args.n, not args.no_tune)}),
+              flush=True)
+        return 0
     t = time.perf_counter()
     if args.workload == "helix":
         system, dt = workloads.helix_wake(args.n), 1e-3
@@ -384,12 +623,17 @@ def main() -> int:
         wall64, ms64, l64, tune64 = time_path(system, "fp64", dt, args.fp64_steps, 0,
                                               not args.no_tune)
         avg64 = ms64 / max(1, l64)
+        ach64 = FLOP_PER_PAIR * pairs / (avg64 * 1e-3) / 1e12
         out64 = {
             "fp64_launch_config": tune64,
             "fp64_steps_per_s": args.fp64_steps / wall64,
             "fp64_pair_kernel_ms": avg64,
-            "fp64_tflops": FLOP_PER_PAIR * pairs / (avg64 * 1e-3) / 1e12,
-            "fp64_frac_of_peak": FLOP_PER_PAIR * pairs / (avg64 * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+            "fp64_tflops": ach64,
+            "fp64_frac_of_peak": ach64 / PEAK_FP64_TFLOPS,
+            "roofline_fp64": {
+                "bound": "valu", "kernel": "k_pair_fp64", "achieved": ach64, "peak": PEAK_FP64_TFLOPS,
+                "unit": "TFLOP/s", "frac": ach64 / PEAK_FP64_TFLOPS, "traffic": None,
+                "avg_launch_ms": avg64, "flop_per_launch": FLOP_PER_PAIR * pairs, "launch_config": tune64},
         }
         log(f"fp64: {out64['fp64_steps_per_s']:.4f} steps/s, pair kernel avg {avg64:.1f} ms")
         out64["fp32c_error_vs_fp64"] = fp32c_error(system)
@@ -449,8 +693,10 @@ def main() -> int:
         if best is not None:
             result["learned_best_steps_per_s"] = best["steps_per_s"]
         cfg = lg.pop("_drift_cfg", None)
+        if args.save_weights and cfg is not None:
+            np.savez(args.save_weights, weights=cfg["weights"], coef_md=cfg["coef_md"], coef_mlp=cfg["coef_mlp"])
         if not args.no_drift:
-            result["drift"] = drift_leg(args.drift_steps, 20, cfg, args.learned_err)
+            result["drift"] = drift_leg(args.drift_steps, args.drift_every, cfg, args.learned_err)
     if not args.no_cpu and system.coupled:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(system, args.cpu_seconds)

@@ -122,6 +122,13 @@ int pw_set_coupled(pw_ctx *ctx, int coupled);
 int pw_set_kernel(pw_ctx *ctx, int kernel);
 int pw_get_kernel(pw_ctx *ctx, int *kernel);
 
+/* enable = 0: PW_FP32C sums every pair of a target in one plain FP32
+ * accumulator, with no chunking and no two-sum (one 256-target block per
+ * work item, grid launch, Gaussian kernel only).  A measuring aid that
+ * shows what the compensated accumulation buys; not for production.
+ * No counterpart in the reference (FP64 only).  Default 1. */
+int pw_set_compensation(pw_ctx *ctx, int enable);
+
 /* ---- vortex ring initial condition -------------------------------------
  * pw_vring_particles: the particles of __vring(R, core, nlayers, nazi, sig)
  * (src/wake/vring.cpp:50-100), nazi (2 nlayers + 1)^2 of them, station by
@@ -265,9 +272,16 @@ int pw_ff_build(pw_ctx *ctx);
 /* out[0..13): n, leaves, nodes, levels, near capacity, far capacity,
  * training samples held, near pairs, far (target, node) evaluations,
  * list overflow flag, max near leaves, max far nodes, near pairs of leaf
- * pairs within the Gaussian core range (of the lists last built; a
- * shorter len takes the leading values) */
+ * pairs within the Gaussian core range, far-list entries (network
+ * evaluations, one per target leaf and far node) (of the lists last built;
+ * a shorter len takes the leading values) */
 int pw_ff_info(pw_ctx *ctx, int64_t *out, int64_t len);
+/* Stage times of the PW_LEARNED right-hand sides evaluated while kernel
+ * timing is on (pw_set_profiling, which also resets them), from HIP events
+ * on the context's stream: out[0] = right-hand sides, out[1..5] = total
+ * milliseconds of tree and lists, near field, analytic far field, network
+ * residual, and the rest (vorticity conservation, scatter). */
+int pw_ff_get_timing(pw_ctx *ctx, double *out, int64_t len);
 /* copy an internal array to host memory (bytes must match exactly):
  * PW_FF_PERM int32[n] (sorted position -> particle), PW_FF_NODES
  * double[nodes][24] (centre 3, r, s0, s2, sigma, M 3, D 9 (row-major

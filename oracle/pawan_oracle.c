@@ -930,7 +930,9 @@ void or_vring_influence(int nLayers, int nAzi, const double *pos, const double *
  * partial pivoting (the row of largest |a_ij| at or below the diagonal,
  * first one on ties), unit lower triangle, then forward and back
  * substitution on the permuted right-hand side.  A (n x n, row-major) is
- * overwritten by its factors.  Returns 1 if a pivot is zero.
+ * overwritten by its factors.  Returns 1 if a pivot is zero or not finite
+ * (GSL flags only zero pivots and carries NaN/Inf into the solution; the
+ * device solve reports both, and so does this restatement). 
  * Pinned: numpy.linalg.solve (LAPACK dgesv) on random and influence
  * matrices (tests/test_oracle_vring.py). */
 int or_lu_solve(int n, double *A, const double *b, double *x) {
@@ -953,7 +955,9 @@ int or_lu_solve(int n, double *A, const double *b, double *x) {
       int t = perm[j]; perm[j] = perm[ip]; perm[ip] = t;
     }
     double piv = A[(int64_t)j * n + j];
-    if (piv == 0.0) { singular = 1; continue; }
+    /* a zero pivot (GSL's singular case) or a non-finite one (a NaN or Inf
+       entry, which GSL would carry into the solution) */
+    if (!(fabs(piv) > 0.0) || !isfinite(piv)) { singular = 1; continue; }
     for (int i = j + 1; i < n; ++i) {
       double l = A[(int64_t)i * n + j] / piv;
       A[(int64_t)i * n + j] = l;

@@ -37,13 +37,15 @@ EXPORTS = [
     "pw_ff_init_weights", "pw_ff_set_weights", "pw_ff_get_weights", "pw_ff_train",
     "pw_ff_mlp_forward", "pw_ff_mlp_grad", "pw_ff_set_conserve", "pw_ff_get_conserve",
     "pw_velocity_at", "pw_set_kernel", "pw_get_kernel", "pw_vring_particles", "pw_vring_resolve",
+    "pw_set_compensation", "pw_ff_get_timing",
 ]
 
 # pw_ff_export selectors
 FF_PERM, FF_NODES, FF_NEAR_CNT, FF_NEAR, FF_FAR_CNT, FF_FAR, FF_SAMPLES_X, FF_SAMPLES_Y, FF_SAMPLES_AUX = range(9)
 MLP_PARAMS = 36748
 FF_INFO_NAMES = ["n", "leaves", "nodes", "levels", "cap_near", "cap_far", "samples", "near_pairs",
-                 "far_evals", "overflow", "max_near", "max_far", "near_core_pairs"]
+                 "far_evals", "overflow", "max_near", "max_far", "near_core_pairs", "mlp_evals"]
+FF_STAGE_NAMES = ["tree_lists_ms", "near_ms", "far_md_ms", "far_mlp_ms", "rest_ms"]
 FF_TRAIN_NAMES = ["train_loss", "val_loss", "val_loss_zero", "rms_ratio_u", "rms_ratio_grad",
                   "coef_md", "coef_mlp", "steps", "val_samples", "ms"]
 
@@ -83,6 +85,7 @@ def load(path: str | None = None) -> C.CDLL:
         "pw_get_nu": ([vp, C.POINTER(C.c_double)], i),
         "pw_set_coupled": ([vp, i], i),
         "pw_set_kernel": ([vp, i], i),
+        "pw_set_compensation": ([vp, i], i),
         "pw_vring_particles": ([d, d, i, i, d, _dp, _dp, _dp], i),
         "pw_vring_resolve": ([vp, i, i, i, d, d, C.c_void_p, C.c_void_p], i),
         "pw_get_kernel": ([vp, C.POINTER(C.c_int)], i),
@@ -112,6 +115,7 @@ def load(path: str | None = None) -> C.CDLL:
         "pw_ff_get_err_coef": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_double)], i),
         "pw_ff_build": ([vp], i),
         "pw_ff_info": ([vp, C.POINTER(C.c_int64), i64], i),
+        "pw_ff_get_timing": ([vp, C.POINTER(C.c_double), i64], i),
         "pw_ff_export": ([vp, i, vp, i64], i),
         "pw_ff_sample": ([vp, i64, C.c_uint64], i),
         "pw_ff_clear_samples": ([vp], i),
@@ -231,6 +235,11 @@ class Solver:
     def set_kernel(self, kernel: int) -> None:
         """PW_KERNEL_GAUSSIAN or PW_KERNEL_HIGHORDER (direct sums only)."""
         _check(self._lib.pw_set_kernel(self._h, int(kernel)))
+
+    def set_compensation(self, on: bool) -> None:
+        """FP32C: compensated per-target sums (default) or plain FP32 sums
+        (a measuring aid)."""
+        _check(self._lib.pw_set_compensation(self._h, 1 if on else 0))
 
     @property
     def kernel(self) -> int:
@@ -373,6 +382,13 @@ class Solver:
         out = (C.c_int64 * len(FF_INFO_NAMES))()
         _check(self._lib.pw_ff_info(self._h, out, len(FF_INFO_NAMES)))
         return dict(zip(FF_INFO_NAMES, [int(v) for v in out]))
+
+    def ff_timing(self) -> dict:
+        """Stage times (ms, totals) of the tree-path right-hand sides since
+        profiling was switched on."""
+        out = (C.c_double * (1 + len(FF_STAGE_NAMES)))()
+        _check(self._lib.pw_ff_get_timing(self._h, out, len(out)))
+        return dict(zip(["rhs"] + FF_STAGE_NAMES, [float(v) for v in out]))
 
     def ff_export(self, what: int) -> np.ndarray:
         info = self.ff_info()

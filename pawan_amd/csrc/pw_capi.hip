@@ -40,7 +40,13 @@ int fail(int code, const std::string& msg) {
 
 namespace {
 
-void release(pw_ctx* c) {
+// Device buffers of the context.  Captured step graphs hold kernel nodes
+// whose arguments point into these buffers, so they are destroyed first,
+// after the stream has drained.
+void free_buffers(pw_ctx* c) {
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.second);
+  c->graphs.clear();
   (void)hipFree(c->y);
   (void)hipFree(c->ys);
   for (auto& p : c->k) (void)hipFree(p);
@@ -53,9 +59,26 @@ void release(pw_ctx* c) {
   for (auto& k : c->ks) (void)hipFree(k.d_items);
   (void)hipFree(c->partial);
   (void)hipFree(c->d_queue);
+  c->y = c->ys = nullptr;
+  for (auto& p : c->k) p = nullptr;
+  c->rates = nullptr;
+  c->d_radius = nullptr;
+  c->pair_items = c->lin_items = nullptr;
+  c->d_lin = c->d_quad = nullptr;
+  c->d_flag = c->d_queue = nullptr;
+  for (auto& k : c->ks) {
+    k.d_items = nullptr;
+    k.nitems = 0;
+    k.built_parts = 0;
+  }
+  c->partial = nullptr;
+  c->partial_parts = 0;
   pwi::ff_release(c);
-  for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.second);
-  c->graphs.clear();
+}
+
+// Everything the context holds on the device: buffers, events, stream.
+void release(pw_ctx* c) {
+  free_buffers(c);
   for (auto& e : c->ev_used) {
     (void)hipEventDestroy(e.first);
     (void)hipEventDestroy(e.second);
@@ -125,14 +148,7 @@ int ensure_built(pw_ctx* c) {
       dalloc(&c->d_lin, 11 * c->h_lin_items.size()) != hipSuccess ||
       dalloc(&c->d_quad, pw::kDiagQuad * c->h_pair_items.size()) != hipSuccess ||
       dalloc(&c->d_flag, 1) != hipSuccess || dalloc(&c->d_queue, 1) != hipSuccess) {
-    release(c);
-    c->y = c->ys = nullptr;
-    for (auto& p : c->k) p = nullptr;
-    c->d_radius = nullptr;
-    c->pair_items = c->lin_items = nullptr;
-    c->d_lin = c->d_quad = nullptr;
-    c->d_flag = nullptr;
-    c->stream = nullptr;
+    free_buffers(c);  // the stream stays: the context remains usable
     (void)hipGetLastError();
     return fail(PW_ERR_NOMEM, "device allocation failed");
   }
@@ -209,7 +225,11 @@ std::vector<WorkItem> make_items(const pw_ctx* c, int parts) {
   return items;
 }
 
+// Captured steps are replayed only while nothing they were captured with
+// has changed; the stream drains first, so no replay is in flight.
 void drop_graphs(pw_ctx* c) {
+  if (c->graphs.empty()) return;
+  (void)hipStreamSynchronize(c->stream);
   for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.second);
   c->graphs.clear();
 }
@@ -220,6 +240,7 @@ int ensure_kind(pw_ctx* c, int kind) {
   if (k.built_parts == k.cfg.parts && k.d_items) return PW_OK;
   std::vector<WorkItem> items = make_items(c, k.cfg.parts);
   PW_HIP(hipStreamSynchronize(c->stream));
+  drop_graphs(c);  // captured steps point at the buffers replaced here
   (void)hipFree(k.d_items);
   k.d_items = nullptr;
   if (dalloc(&k.d_items, items.size()) != hipSuccess) return fail(PW_ERR_NOMEM, "work items");
@@ -234,7 +255,6 @@ int ensure_kind(pw_ctx* c, int kind) {
       return fail(PW_ERR_NOMEM, "partial-sum buffer");
     c->partial_parts = k.cfg.parts;
   }
-  drop_graphs(c);
   return PW_OK;
 }
 
@@ -251,6 +271,13 @@ int rhs(pw_ctx* c, int kind, const Rec64* state, Rate64* out) {
       PW_HIP(hipEventRecord(e1, c->stream));
       c->ev_used.emplace_back(e0, e1);
     }
+    PW_HIP(pw::launch_check_finite(out, static_cast<int>(c->n), c->d_flag, c->stream));
+    return PW_OK;
+  }
+  if (kind == pw::kPairFP32C && !c->compensated && c->smoothing == PW_KERNEL_GAUSSIAN) {
+    // plain FP32 sums: one item per 256-target block over all its sources
+    pw::PairArgs args{state, c->pair_items, c->npair, static_cast<int>(c->n), out, nullptr, c->d_queue, c->nu};
+    PW_HIP(pw::launch_pair(pw::kPairFP32CPlain, pw::PairConfig{}, args, c->npair, c->stream));
     PW_HIP(pw::launch_check_finite(out, static_cast<int>(c->n), c->d_flag, c->stream));
     return PW_OK;
   }
@@ -447,7 +474,10 @@ int pw_add_wake(pw_ctx* c, int64_t n, int64_t maxn, const double* pos, const dou
 int pw_set_nu(pw_ctx* c, double nu) {
   PW_CTX(c);
   if (!std::isfinite(nu) || nu < 0) return fail(PW_ERR_ARG, "nu must be finite and >= 0");
-  c->nu = nu;
+  if (nu != c->nu) {
+    drop_graphs(c);  // captured steps carry nu in their kernel arguments
+    c->nu = nu;
+  }
   return PW_OK;
 }
 
@@ -470,9 +500,17 @@ int pw_set_kernel(pw_ctx* c, int kernel) {
   if (kernel != PW_KERNEL_GAUSSIAN && kernel != PW_KERNEL_HIGHORDER)
     return fail(PW_ERR_ARG, "kernel must be PW_KERNEL_GAUSSIAN or PW_KERNEL_HIGHORDER");
   if (kernel != c->smoothing) {
-    PW_HIP(hipStreamSynchronize(c->stream));
     drop_graphs(c);  // captured steps hold the other kernel's launches
     c->smoothing = kernel;
+  }
+  return PW_OK;
+}
+
+int pw_set_compensation(pw_ctx* c, int enable) {
+  PW_CTX(c);
+  if ((enable != 0) != c->compensated) {
+    drop_graphs(c);  // captured steps hold the other summation's launches
+    c->compensated = enable != 0;
   }
   return PW_OK;
 }
@@ -624,6 +662,8 @@ int pw_vring_particles(double R, double core, int nlayers, int nazi, double sig,
   (void)core;  // sets only the ideal profile (pw_vring_resolve)
   if (nlayers < 0 || nazi < 1 || (2 * nlayers + 1) * (2 * nlayers + 1) > pw::kVringMaxCells)
     return fail(PW_ERR_ARG, "nlayers must be in [0, 15] and nazi >= 1");
+  if (!(std::isfinite(R) && R > 0.0) || !(std::isfinite(sig) && sig > 0.0))
+    return fail(PW_ERR_ARG, "ring radius and smoothing radius must be finite and positive");
   if (!pos || !radius || !volume) return fail(PW_ERR_ARG, "null output");
   // the particle layout of __vring (vring.cpp:50-100): one centre particle
   // and layers of 8j particles per station, the cell volumes of its
@@ -665,10 +705,15 @@ int pw_vring_resolve(pw_ctx* c, int wake, int nlayers, int nazi, double R, doubl
   if (wake < 0 || wake >= static_cast<int>(c->wn.size())) return fail(PW_ERR_ARG, "wake index out of range");
   if (nlayers < 0 || nazi < 1 || (2 * nlayers + 1) * (2 * nlayers + 1) > pw::kVringMaxCells)
     return fail(PW_ERR_ARG, "nlayers must be in [0, 15] and nazi >= 1");
-  if (!(core > 0.0) || !(R > 0.0)) return fail(PW_ERR_ARG, "radius and core must be positive");
+  if (!(std::isfinite(core) && core > 0.0) || !(std::isfinite(R) && R > 0.0))
+    return fail(PW_ERR_ARG, "radius and core must be finite and positive");
   const int m = (2 * nlayers + 1) * (2 * nlayers + 1);
   if (c->wn[wake] != static_cast<int64_t>(m) * nazi)
     return fail(PW_ERR_ARG, "the wake does not hold nazi (2 nlayers + 1)^2 particles");
+  // a zero or non-finite smoothing radius makes every influence entry 0/0
+  for (int64_t q = c->woff[wake]; q < c->woff[wake + 1]; ++q)
+    if (!(std::isfinite(c->radius[q]) && c->radius[q] > 0.0))
+      return fail(PW_ERR_ARG, "the ring's smoothing radii must be finite and positive");
   PW_TRY(ensure_built(c));
   // __vring::getIdealRates (vring.cpp:223-242): the Gaussian core profile
   std::vector<double> ideal(m);
@@ -883,9 +928,11 @@ int pw_synchronize(pw_ctx* c) {
 int pw_set_profiling(pw_ctx* c, int enable) {
   PW_CTX(c);
   PW_TRY(collect_profile(c));
+  PW_TRY(pwi::ff_collect_stages(c));
   c->profiling = enable != 0;
   c->prof_ms = 0.0;
   c->prof_launches = 0;
+  pwi::ff_reset_stages(c);
   return PW_OK;
 }
 
@@ -960,10 +1007,12 @@ void load_tune_file() {
   int kind, nsys, parts, pers, w8;
   long long n, largest;
   while (std::fscanf(f, "%d %lld %d %lld %d %d %d", &kind, &n, &nsys, &largest, &parts, &pers, &w8) == 7) {
+    // the ranges pw_set_pair_config accepts; other lines are skipped
+    if (kind < 0 || kind > 2 || parts < 1 || parts > 256) continue;
     pw::PairConfig cfg;
     cfg.parts = parts;
     cfg.persistent = pers != 0;
-    cfg.waves8 = w8 != 0;
+    cfg.waves8 = w8 != 0 && kind == pw::kPairFP32C;
     tune_cache()[TuneKey(kind, n, nsys, largest)] = cfg;
   }
   std::fclose(f);
@@ -980,12 +1029,12 @@ void save_tune_file(const TuneKey& key, const pw::PairConfig& cfg) {
   std::fclose(f);
 }
 
-int time_rhs(pw_ctx* c, int kind, double* ms) {
+int time_rhs(pw_ctx* c, int kind, double* ms, int reps = 4) {
   hipEvent_t e0, e1;
   PW_HIP(hipEventCreate(&e0));
   PW_HIP(hipEventCreate(&e1));
   float best = 1e30f;
-  for (int rep = 0; rep < 4; ++rep) {
+  for (int rep = 0; rep < reps; ++rep) {
     PW_HIP(hipEventRecord(e0, c->stream));
     const int rc = rhs(c, kind, c->y, c->k[0]);
     if (rc != PW_OK) {
@@ -1038,12 +1087,16 @@ int pw_autotune(pw_ctx* c, int precision, double* best_ms) {
     k.cfg = cached;
     k.tuned = true;
     drop_graphs(c);
+    // one timed right-hand side (or more, when the time is asked for), so
+    // that the rates of the current state are on the device as after a
+    // search
     double ms = 0.0;
-    const int rc = best_ms ? time_rhs(c, kind, &ms) : PW_OK;
+    const int rc = time_rhs(c, kind, &ms, best_ms ? 4 : 1);
     c->profiling = was_profiling;
+    if (rc != PW_OK) return rc;
     if (best_ms) *best_ms = ms;
     c->rates = c->k[0];
-    return rc;
+    return PW_OK;
   }
   // candidates: split the sources (fills the GPU for few target blocks and
   // cuts the tail of the last block wave), keep blocks resident, and for
@@ -1100,7 +1153,13 @@ int pw_autotune(pw_ctx* c, int precision, double* best_ms) {
     tune_cache()[key] = best_cfg;
     save_tune_file(key, best_cfg);
   }
-  c->profiling = was_profiling;
+  // the rates of the chosen configuration (the candidates' reductions of
+  // source-split partial sums differ in rounding)
+  {
+    const int rc = rhs(c, kind, c->y, c->k[0]);
+    c->profiling = was_profiling;
+    if (rc != PW_OK) return rc;
+  }
   c->rates = c->k[0];
   if (best_ms) *best_ms = best;
   return PW_OK;

@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
 #include <map>
 #include <string>
@@ -63,6 +64,14 @@ struct FarField {
   int64_t ns = 0, ns_cap = 0;
   // last lists built
   bool lists_valid = false;
+  // stage timing of the right-hand sides while the context profiles: tree
+  // and lists, near field, analytic far field, network, the rest
+  static constexpr int kStages = 5;
+  std::vector<std::array<hipEvent_t, kStages + 1>> st_used, st_free;
+  double stage_ms[kStages] = {0, 0, 0, 0, 0};
+  int64_t stage_rhs = 0;
+  // network evaluations (far-list entries) of the last lists
+  int64_t mlp_evals = 0;
 };
 
 }  // namespace pwi
@@ -73,6 +82,7 @@ struct pw_ctx {
   double nu = 1.56e-5;  // interaction.cpp:15
   bool coupled = true;
   int smoothing = PW_KERNEL_GAUSSIAN;  // PW_KERNEL_*
+  bool compensated = true;             // FP32C sums (pw_set_compensation)
 
   // host description of the wakes
   std::vector<int64_t> wn, wmax, woff{0};
@@ -142,6 +152,9 @@ int prof_begin(pw_ctx* c, hipEvent_t* e0, hipEvent_t* e1);
 int ff_rhs(pw_ctx* c, const pw::Rec64* state, pw::Rate64* out);
 int ff_check(pw_ctx* c);
 void ff_release(pw_ctx* c);
+// fold the recorded stage events into the totals (synchronises)
+int ff_collect_stages(pw_ctx* c);
+void ff_reset_stages(pw_ctx* c);
 
 }  // namespace pwi
 

@@ -721,16 +721,29 @@ size_t mlp_far_lds_bytes() { return sizeof(MlpLds) + (kMlpWaves * 32 * kXch + 16
 
 hipError_t launch_far(const Rec64* sorted, const NodeRec* nodes, const int* far_list, const int* far_cnt,
                       int cap_far, int nleaf, const MlpWeights* mlp, Rate64* rates_s, hipStream_t st) {
+  hipError_t e = launch_far_md(sorted, nodes, far_list, far_cnt, cap_far, nleaf, rates_s, st);
+  if (e != hipSuccess || mlp == nullptr) return e;
+  return launch_far_mlp(sorted, nodes, far_list, far_cnt, cap_far, nleaf, *mlp, rates_s, st);
+}
+
+hipError_t launch_far_md(const Rec64* sorted, const NodeRec* nodes, const int* far_list, const int* far_cnt,
+                         int cap_far, int nleaf, Rate64* rates_s, hipStream_t st) {
   if (nleaf <= 0) return hipSuccess;
   FarArgs a{sorted, nodes, far_list, far_cnt, cap_far, rates_s};
   hipLaunchKernelGGL(k_far_md, dim3(nleaf), dim3(256), 0, st, a);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess || mlp == nullptr) return e;
+  return hipGetLastError();
+}
+
+hipError_t launch_far_mlp(const Rec64* sorted, const NodeRec* nodes, const int* far_list, const int* far_cnt,
+                          int cap_far, int nleaf, const MlpWeights& mlp, Rate64* rates_s, hipStream_t st) {
+  if (nleaf <= 0) return hipSuccess;
+  FarArgs a{sorted, nodes, far_list, far_cnt, cap_far, rates_s};
+  hipError_t e;
   const size_t lds = mlp_far_lds_bytes();
   static std::atomic<unsigned long long> lds_raised{0};  // LDS beyond the 64 KB default
   e = ensure_lds_limit(reinterpret_cast<const void*>(k_far_mlp), lds, lds_raised);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_far_mlp, dim3(nleaf), dim3(kMlpThreads), lds, st, a, *mlp);
+  hipLaunchKernelGGL(k_far_mlp, dim3(nleaf), dim3(kMlpThreads), lds, st, a, mlp);
   return hipGetLastError();
 }
 

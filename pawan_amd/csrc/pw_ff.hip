@@ -239,6 +239,10 @@ namespace pwi {
 void ff_release(pw_ctx* c) {
   auto& f = c->ff;
   auto& tb = f.tb;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto* v : {&f.st_used, &f.st_free})
+    for (auto& ev : *v)
+      for (hipEvent_t e : ev) (void)hipEventDestroy(e);
   (void)hipFree(tb.bbox_part);
   (void)hipFree(tb.bbox);
   (void)hipFree(tb.code);
@@ -286,24 +290,83 @@ void ff_release(pw_ctx* c) {
   f.h_params = std::move(keep);
 }
 
+namespace {
+
+using StageEvents = std::array<hipEvent_t, pwi::FarField::kStages + 1>;
+
+// one right-hand side of the tree path; ev (may be null) gets the stage
+// boundaries recorded
+int ff_rhs_body(pw_ctx* c, const Rec64* state, Rate64* out, const StageEvents* ev) {
+  auto& f = c->ff;
+  auto mark = [&](int k) -> hipError_t { return ev ? hipEventRecord((*ev)[k], c->stream) : hipSuccess; };
+  PW_HIP(mark(0));
+  PW_TRY(build(c, state, false));
+  PW_HIP(mark(1));
+  const auto& tb = f.tb;
+  const int n = static_cast<int>(c->n);
+  PW_HIP(pw::build_near_items(tb, f.shape.nleaf, c->stream));
+  PW_HIP(pw::launch_near_fp32c(tb.sorted, tb.nodes, tb, f.shape.nleaf, c->nu, f.sigma_max2, c->num_cus * pw::near_kernel_blocks_per_cu(),
+                               f.rates_s, c->stream));
+  PW_HIP(mark(2));
+  PW_HIP(pw::launch_far_md(tb.sorted, tb.nodes, tb.far_list, tb.far_cnt, tb.cap_far, f.shape.nleaf, f.rates_s,
+                           c->stream));
+  PW_HIP(mark(3));
+  if (f.use_mlp) {
+    pw::MlpWeights W{f.d_frag, f.d_bias};
+    PW_HIP(pw::launch_far_mlp(tb.sorted, tb.nodes, tb.far_list, tb.far_cnt, tb.cap_far, f.shape.nleaf, W,
+                              f.rates_s, c->stream));
+  }
+  PW_HIP(mark(4));
+  if (f.conserve) PW_HIP(pw::launch_remove_mean_dalpha(f.rates_s, n, f.d_dpart, c->stream));
+  PW_HIP(pw::launch_scatter_rates(f.rates_s, tb.perm, n, out, c->stream));
+  PW_HIP(mark(5));
+  return PW_OK;
+}
+
+}  // namespace
+
 int ff_rhs(pw_ctx* c, const Rec64* state, Rate64* out) {
   PW_TRY(ensure_ff(c));
   auto& f = c->ff;
   if (f.use_mlp && !f.have_weights)
     return fail(PW_ERR_STATE, "PW_LEARNED with the network on needs weights (pw_ff_train or pw_ff_set_weights), "
                               "or pw_ff_set_params(..., use_mlp = 0)");
-  PW_TRY(build(c, state, false));
-  const auto& tb = f.tb;
-  const int n = static_cast<int>(c->n);
-  PW_HIP(pw::build_near_items(tb, f.shape.nleaf, c->stream));
-  PW_HIP(pw::launch_near_fp32c(tb.sorted, tb.nodes, tb, f.shape.nleaf, c->nu, f.sigma_max2, c->num_cus * pw::near_kernel_blocks_per_cu(),
-                               f.rates_s, c->stream));
-  pw::MlpWeights W{f.d_frag, f.d_bias};
-  PW_HIP(pw::launch_far(tb.sorted, tb.nodes, tb.far_list, tb.far_cnt, tb.cap_far, f.shape.nleaf,
-                        f.use_mlp ? &W : nullptr, f.rates_s, c->stream));
-  if (f.conserve) PW_HIP(pw::launch_remove_mean_dalpha(f.rates_s, n, f.d_dpart, c->stream));
-  PW_HIP(pw::launch_scatter_rates(f.rates_s, tb.perm, n, out, c->stream));
+  if (!c->profiling) return ff_rhs_body(c, state, out, nullptr);
+  // stage events while the context profiles (recycled; a failed right-hand
+  // side returns its events unread)
+  StageEvents ev{};
+  if (f.st_free.empty()) {
+    for (auto& e : ev) PW_HIP(hipEventCreate(&e));
+  } else {
+    ev = f.st_free.back();
+    f.st_free.pop_back();
+  }
+  const int rc = ff_rhs_body(c, state, out, &ev);
+  (rc == PW_OK ? f.st_used : f.st_free).push_back(ev);
+  return rc;
+}
+
+int ff_collect_stages(pw_ctx* c) {
+  auto& f = c->ff;
+  if (f.st_used.empty()) return PW_OK;
+  PW_HIP(hipStreamSynchronize(c->stream));
+  for (auto& ev : f.st_used) {
+    for (int k = 0; k < FarField::kStages; ++k) {
+      float ms = 0.0f;
+      PW_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+      f.stage_ms[k] += ms;
+    }
+    f.stage_rhs += 1;
+    f.st_free.push_back(ev);
+  }
+  f.st_used.clear();
   return PW_OK;
+}
+
+void ff_reset_stages(pw_ctx* c) {
+  auto& f = c->ff;
+  for (double& v : f.stage_ms) v = 0.0;
+  f.stage_rhs = 0;
 }
 
 int ff_check(pw_ctx* c) {
@@ -383,7 +446,7 @@ int pw_ff_info(pw_ctx* c, int64_t* out, int64_t len) {
   if (!out || len < 1) return fail(PW_ERR_ARG, "null output");
   PW_TRY(ensure_ff(c));
   auto& f = c->ff;
-  std::vector<int64_t> v(13, 0);
+  std::vector<int64_t> v(14, 0);
   v[0] = c->n;
   v[1] = f.shape.nleaf;
   v[2] = f.shape.nnodes;
@@ -415,10 +478,23 @@ int pw_ff_info(pw_ctx* c, int64_t* out, int64_t len) {
       v[12] += h[3 * T + 2];
       v[10] = std::max<int64_t>(v[10], nc[T]);
       v[11] = std::max<int64_t>(v[11], fc[T]);
+      v[13] += fc[T];
     }
     v[9] = over;
   }
-  for (int64_t i = 0; i < std::min<int64_t>(len, 13); ++i) out[i] = v[i];
+  for (int64_t i = 0; i < std::min<int64_t>(len, 14); ++i) out[i] = v[i];
+  return PW_OK;
+}
+
+int pw_ff_get_timing(pw_ctx* c, double* out, int64_t len) {
+  PW_CTX(c);
+  if (!out || len < 1) return fail(PW_ERR_ARG, "null output");
+  PW_TRY(pwi::ff_collect_stages(c));
+  const auto& f = c->ff;
+  double v[1 + pwi::FarField::kStages];
+  v[0] = static_cast<double>(f.stage_rhs);
+  for (int k = 0; k < pwi::FarField::kStages; ++k) v[1 + k] = f.stage_ms[k];
+  for (int64_t i = 0; i < std::min<int64_t>(len, 1 + pwi::FarField::kStages); ++i) out[i] = v[i];
   return PW_OK;
 }
 

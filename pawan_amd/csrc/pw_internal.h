@@ -64,6 +64,9 @@ enum PairKind : int {
                        // launched in place of kinds 0 and 1 when a context
                        // uses that kernel (3 is the tree path's kind)
   kPairFP32CHO = 5,    // FP32C with that kernel, in place of kind 2
+  kPairFP32CPlain = 6, // FP32 pairs summed in plain FP32, no compensation
+                       // (pw_set_compensation(ctx, 0); one item per
+                       // 256-target block, grid launch)
 };
 
 // Pair kernel of `kind` over args.items; for parts > 1 also the reduction
@@ -278,6 +281,12 @@ struct MlpWeights;  // packed BF16 fragments, pw_mlp.hip
 hipError_t launch_far(const Rec64* sorted, const NodeRec* nodes, const int* far_list,
                       const int* far_cnt, int cap_far, int nleaf, const MlpWeights* mlp,
                       Rate64* rates_s, hipStream_t st);
+// the two passes of launch_far: the analytic expansion, and the network's
+// residual added to it
+hipError_t launch_far_md(const Rec64* sorted, const NodeRec* nodes, const int* far_list, const int* far_cnt,
+                         int cap_far, int nleaf, Rate64* rates_s, hipStream_t st);
+hipError_t launch_far_mlp(const Rec64* sorted, const NodeRec* nodes, const int* far_list, const int* far_cnt,
+                          int cap_far, int nleaf, const MlpWeights& mlp, Rate64* rates_s, hipStream_t st);
 
 // dalpha_i -= mean over i of dalpha (part: 3 * (ceil(n/256) + 1) doubles)
 hipError_t launch_remove_mean_dalpha(Rate64* rates, int n, double* part, hipStream_t st);

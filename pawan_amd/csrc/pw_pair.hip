@@ -721,7 +721,10 @@ __device__ __forceinline__ double uniform(double v) {
 // separated tiles, which need the same gap as the Gaussian's: it keeps the
 // relative positions' rounding small and the PSE terms beyond it are
 // dropped in both kernels).
-template <bool HighOrder>
+// Comp = false sums every pair of the item into one plain FP32 accumulator
+// (no chunks, no two-sum): the uncompensated sum, kept to measure what the
+// compensation buys (pw_set_compensation).
+template <bool HighOrder, bool Comp = true>
 __device__ __forceinline__ void item_fp32c(const PairArgs& pa, const WorkItem& it, SrcT2* tile) {
   __shared__ double s_tb[4][6];
   __shared__ float s_b[4][8];
@@ -766,6 +769,7 @@ __device__ __forceinline__ void item_fp32c(const PairArgs& pa, const WorkItem& i
 #pragma unroll
   for (int k = 0; k < kAcc; ++k) hi[k] = lo[k] = 0.0f;
   PSE32 pse;
+  P32 whole;  // Comp == false: the item's plain FP32 sums
   for (int base = it.s0; base < it.s1; base += kTB) {
     const int cnt = min(kTB, it.s1 - base);
     __syncthreads();
@@ -815,7 +819,8 @@ __device__ __forceinline__ void item_fp32c(const PairArgs& pa, const WorkItem& i
     const int self = static_cast<int>(threadIdx.x);
     for (int c0 = 0; c0 < cnt; c0 += kChunk) {
       const int c1 = min(cnt, c0 + kChunk);
-      P32 p;
+      P32 chunk;
+      P32& p = Comp ? chunk : whole;
       if (sep) {
         if (HighOrder) {
 #pragma unroll 2
@@ -837,9 +842,10 @@ __device__ __forceinline__ void item_fp32c(const PairArgs& pa, const WorkItem& i
           else pair32(me, tile[l], p, pse);
         }
       }
-      fold(p, me.ax, me.ay, me.az, hi, lo);
+      if (Comp) fold(p, me.ax, me.ay, me.az, hi, lo);
     }
   }
+  if (!Comp) fold(whole, me.ax, me.ay, me.az, hi, lo);
   if (active) {
     double acc[kAcc];
 #pragma unroll
@@ -1052,6 +1058,10 @@ template <>
 struct TileOf<kPairFP32CHO> {
   using type = SrcT2;
 };
+template <>
+struct TileOf<kPairFP32CPlain> {
+  using type = SrcT2;
+};
 
 template <int Kind>
 __device__ __forceinline__ void run_item(const PairArgs& a, const WorkItem& it,
@@ -1064,6 +1074,8 @@ __device__ __forceinline__ void run_item(const PairArgs& a, const WorkItem& it,
     item_fp64_ho(a, it, tile);
   else if constexpr (Kind == kPairFP32CHO)
     item_fp32c<true>(a, it, tile);
+  else if constexpr (Kind == kPairFP32CPlain)
+    item_fp32c<false, false>(a, it, tile);
   else
     item_fp32c<false>(a, it, tile);
 }
@@ -1111,6 +1123,10 @@ template <bool P>
 __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_pair_fp32c_ho_w8(PairArgs a) {
   pair_entry<kPairFP32CHO, P>(a);
 }
+
+// FP32 pairs summed without compensation (grid launch only): a measuring
+// aid, not a production path
+__global__ __launch_bounds__(kTB) void k_pair_fp32c_plain(PairArgs a) { pair_entry<kPairFP32CPlain, false>(a); }
 
 // rates[t] = sum over parts of partial[p][t], parts in order (FP64).
 __global__ __launch_bounds__(256) void k_reduce_parts(const Rate64* __restrict__ partial,
@@ -1160,6 +1176,10 @@ const bool P = cfg.persistent;
         if (P) hipLaunchKernelGGL(k_pair_fp32c_ho<true>, g, block, 0, st, args);
         else hipLaunchKernelGGL(k_pair_fp32c_ho<false>, g, block, 0, st, args);
       }
+      break;
+    case kPairFP32CPlain:
+      if (P || cfg.parts != 1) return hipErrorInvalidValue;
+      hipLaunchKernelGGL(k_pair_fp32c_plain, g, block, 0, st, args);
       break;
     case kPairFP32C:
       if (cfg.waves8) {

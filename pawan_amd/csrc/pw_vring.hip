@@ -70,7 +70,9 @@ __global__ __launch_bounds__(kTB) void k_vring_influence(const Rec64* __restrict
 // (lowest row on ties); each elimination step stores the multipliers in
 // column j, then updates the trailing submatrix with the lanes along its
 // rows (coalesced); the two triangular solves go column by column with the
-// lanes over the rows, x held in LDS.  flag[0] = 1 if a pivot is zero.
+// lanes over the rows, x held in LDS.  flag[0] = 1 if a pivot is zero or
+// not finite (a NaN column keeps the diagonal row as its pivot, so no row
+// outside the matrix is ever selected).
 __global__ __launch_bounds__(kTB) void k_lu_solve(double* __restrict__ A, int n,
                                                   const double* __restrict__ b,
                                                   double* __restrict__ x, int* __restrict__ flag) {
@@ -85,7 +87,7 @@ __global__ __launch_bounds__(kTB) void k_lu_solve(double* __restrict__ A, int n,
   __syncthreads();
   for (int j = 0; j < n; ++j) {
     double best = -1.0;
-    int bi = n;
+    int bi = j;
     for (int r = j + tid; r < n; r += kTB) {
       const double v = fabs(A[static_cast<size_t>(r) * n + j]);
       if (v > best) { best = v; bi = r; }
@@ -119,7 +121,7 @@ __global__ __launch_bounds__(kTB) void k_lu_solve(double* __restrict__ A, int n,
     }
     __syncthreads();
     const double piv = A[static_cast<size_t>(j) * n + j];
-    if (piv == 0.0) {
+    if (!(fabs(piv) > 0.0) || !isfinite(piv)) {
       if (tid == 0) flag[0] = 1;
       __syncthreads();
       continue;

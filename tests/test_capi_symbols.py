@@ -35,3 +35,18 @@ def test_null_context_is_an_argument_error():
     lib = capi.load()
     assert lib.pw_solve(None, capi.PW_FP64) == capi.PW_ERR_ARG
     assert lib.pw_destroy(None) == capi.PW_OK
+
+
+def test_ring_particles_reject_bad_radii_on_the_host():
+    """pw_vring_particles runs on the host: a zero, negative or non-finite
+    smoothing or ring radius is an argument error before anything reaches
+    a device (a zero sigma would make every influence entry 0/0)."""
+    import numpy as np
+    import pytest
+
+    for R, sig in ((1.0, 0.0), (1.0, -0.1), (1.0, float("nan")), (0.0, 0.19), (float("inf"), 0.19)):
+        with pytest.raises(capi.PawanError) as e:
+            capi.vring_particles(R, 0.1, 3, 49, sig)
+        assert e.value.code == capi.PW_ERR_ARG
+    pos, radius, volume = capi.vring_particles(1.0, 0.1, 3, 49, 0.1924)
+    assert pos.shape == (49 * 49, 3) and np.all(radius == 0.1924) and np.all(volume > 0)

@@ -105,3 +105,66 @@ def test_batched_rk4_against_oracle(oracle):
         ref = oracle.step(s.wake(k), "rk4", 2e-3, 3)
         assert normwise(pos[a:b] - s.pos[a:b], ref.pos - s.pos[a:b]) < 1e-11
         assert normwise(alpha[a:b], ref.alpha) < 1e-12
+
+
+def test_config3_batched_256_wakes(oracle):
+    """The batched workload as benchmarked: 256 independent thin rings of
+    512-16,384 particles (log-uniform, seeded; 1.3M particles), FP64 and FP32C
+    pair sums autotuned, persistent and graph-replayed.  Sampled targets of
+    every wake against the oracle run on that wake alone; two graph-replayed
+    RK4 steps bit-identical to eager steps with the same launch shape, and
+    the smallest, a middle and the largest wake against the oracle's steps.
+    Bars: velocity 1e-12 (FP64) and 1e-6 (FP32C).  The vorticity rate of a
+    planar thin ring is a residual of nearly cancelling a_t x a_s terms:
+    FP64 is held to 2e-11 (4x the FP64 oracle's own distance from the
+    __float128 result on such rings), FP32C to 1e-3 max-norm and 1e-4 rms."""
+    from conftest import rmswise
+
+    s = W.batched_rings()
+    sizes = np.diff(s.off)
+    _say(f"batched {s.nw} wakes N={s.n} (sizes {sizes.min()}..{sizes.max()})")
+    idx_w = []
+    for k in range(s.nw):
+        n = int(sizes[k])
+        idx_w.append(np.unique(np.concatenate([[0, n - 1], np.linspace(0, n - 1, 10).astype(np.int64)])))
+    refs = [oracle.rates_targets(s.wake(k), idx_w[k]) for k in range(s.nw)]
+    order = np.argsort(sizes)
+    check_steps = [int(order[0]), int(order[len(order) // 2]), int(order[-1])]
+    dt = 2e-3
+    for prec, tu, tw in (("fp64", 1e-12, 2e-11), ("fp32c", 1e-6, 1e-3)):
+        with capi.Solver(s) as g:
+            ms = g.autotune(prec)
+            cfg = g.pair_config(prec)
+            _say(f"  {prec} autotuned {cfg}: {ms:.2f} ms per right-hand side")
+            u, w = g.solve(prec)
+            worst_u = worst_w = 0.0
+            all_w, all_rw = [], []
+            for k in range(s.nw):
+                a = int(s.off[k])
+                ru, rw = refs[k]
+                worst_u = max(worst_u, normwise(u[a + idx_w[k]], ru))
+                worst_w = max(worst_w, normwise(w[a + idx_w[k]], rw))
+                all_w.append(w[a + idx_w[k]]), all_rw.append(rw)
+            rms_w = rmswise(np.concatenate(all_w), np.concatenate(all_rw))
+            _say(f"  {prec}: worst wake e_u={worst_u:.2e} e_w={worst_w:.2e} (rms over samples {rms_w:.2e})")
+            assert worst_u < tu and worst_w < tw
+            if prec == "fp32c":
+                assert rms_w < 1e-4
+            # graph replay against eager steps of the same launch shape
+            with capi.Solver(s) as e:
+                e.set_pair_config(prec, cfg["parts"], cfg["persistent"], cfg["waves8"])
+                g.set_graph(True)
+                g.step("rk4", dt, prec, 2)
+                e.step("rk4", dt, prec, 2)
+                pg, ag = g.state()
+                pe, ae = e.state()
+            np.testing.assert_array_equal(pg, pe)
+            np.testing.assert_array_equal(ag, ae)
+        if prec == "fp64":
+            for k in check_steps:
+                a, b = int(s.off[k]), int(s.off[k + 1])
+                ref = oracle.step(s.wake(k), "rk4", dt, 2)
+                ep = normwise(pg[a:b] - s.pos[a:b], ref.pos - s.pos[a:b])
+                ea = normwise(ag[a:b], ref.alpha)
+                _say(f"  wake {k} (n={b - a}) 2 x rk4: e_dx={ep:.2e} e_alpha={ea:.2e}")
+                assert ep < 1e-11 and ea < 1e-12

@@ -87,3 +87,16 @@ def test_bad_rings_are_errors():
             assert e.value.code == capi.PW_ERR_ARG
     with pytest.raises(capi.PawanError):
         capi.vring_particles(R, CORE, -1, 8, 0.2)
+
+
+@pytest.mark.parametrize("nl,nazi,sig", [(8, 16, 0.05), (15, 8, 0.02)])
+def test_resolve_rings_beyond_one_block(oracle, nl, nazi, sig):
+    """m = 289 and 961 cells, more rows than the LU block's 256 lanes, so
+    every lane handles several rows in the pivot search, the trailing update
+    and both triangular solves (condition numbers 5e5 and 2e3)."""
+    pos, radius, volume = capi.vring_particles(R, CORE, nl, nazi, sig)
+    _say(f"vring L={nl} nAzi={nazi} ({(2 * nl + 1) ** 2} cells): {len(pos)} particles")
+    with capi.Solver() as g:
+        g.add_wake(pos, np.zeros_like(pos), radius, volume)
+        s, M = g.vring_resolve(0, nl, nazi, R, CORE)
+        _check_resolve(oracle, g, 0, s, M, nl, nazi, sig)

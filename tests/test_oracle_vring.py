@@ -91,6 +91,29 @@ def test_lu_solve_matches_lapack(oracle):
         oracle.lu_solve(np.zeros((3, 3)), np.ones(3))
 
 
+def test_lu_solve_flags_non_finite_pivots(oracle):
+    """A column of NaN (what a zero smoothing radius gives: 0/0 in every
+    entry) keeps the diagonal row as pivot and is reported as singular, as
+    is an infinite pivot; a NaN below a finite pivot is never chosen."""
+    rng = np.random.default_rng(4)
+    A = rng.normal(size=(6, 6)) + 6 * np.eye(6)
+    for col, val in ((0, np.nan), (3, np.nan), (2, np.inf)):
+        B = A.copy()
+        B[:, col] = val
+        with pytest.raises(ValueError):
+            oracle.lu_solve(B, np.ones(6))
+    # the whole matrix NaN: every pivot stays on the diagonal
+    with pytest.raises(ValueError):
+        oracle.lu_solve(np.full((4, 4), np.nan), np.ones(4))
+    # a single NaN below the diagonal loses every pivot search (fabs(NaN) >
+    # best is false) until its row is the last candidate left, whose pivot
+    # is then NaN: reported too
+    B = A.copy()
+    B[4, 0] = np.nan
+    with pytest.raises(ValueError):
+        oracle.lu_solve(B, np.ones(6))
+
+
 def test_resolved_ring_reproduces_the_profile(oracle):
     s, M, (pos, alpha, radius, volume) = oracle.vring_resolve(R, CORE, L, NAZI, SIG)
     ideal = oracle.vring_ideal(R, CORE, L)
