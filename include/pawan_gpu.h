@@ -198,6 +198,43 @@ int pw_solve(pw_ctx *ctx, int precision);
 int pw_step(pw_ctx *ctx, int scheme, double dt, int precision,
             int64_t nsteps);
 
+/* ---- per-wake blocks (the class API's overridable hooks) ---------------
+ * The reference's __interaction::interact() and influence() visit their
+ * wakes block by block through private virtuals that subclasses override
+ * (interact(W) / interact(W1, W2), influence(W) / influence(W1, W2),
+ * interaction.h:133-157; __parallel does, parallel.h:48-70), and diagnose()
+ * calls the protected calculate* virtuals per wake and per wake pair
+ * (interaction.h:41-111, interaction.cpp:64-132).  These entry points give
+ * the same blocks on the device, so that a subclass can replace some
+ * blocks and leave the others to the GPU.  A block is a pair (a, b) of
+ * wake indices, a <= b: a == b the pairs within wake a, a < b the pairs
+ * between wakes a and b (coupled contexts only).
+ *
+ * pw_interact_wakes: the rates of the listed blocks only (each block adds
+ * what interact(W) or interact(W1, W2) adds to its wakes,
+ * interaction.cpp:262-302); particles of wakes in no block get zero.
+ * precision PW_FP64, PW_FP64_REF or PW_FP32C.  pairs holds 2 * npairs
+ * ints.  The rates are read back with pw_get_wake_rates / pw_get_rates. */
+int pw_interact_wakes(pw_ctx *ctx, int precision, const int *pairs, int npairs);
+
+/* pw_vorticity_field: the smoothed vorticity field of the listed blocks
+ * (pairs NULL: every block of the system) at every particle, field[3 N]
+ * (__interaction::influence(), interaction.cpp:212-260, with
+ * INFLUENCE / SELFINFLUENCE, interaction_utils.h:626-703: sum over
+ * sources of ZETASIG(rho, sigma_ts) a_s, and ZETASIG(0, s_t) a_t for the
+ * particle itself). */
+int pw_vorticity_field(pw_ctx *ctx, const int *pairs, int npairs, double *field, int64_t len);
+
+/* pw_diagnose_wakes: the diagnostics of one block, 16 doubles.
+ * wa == wb: calculateTotalVorticity (3), calculateLinearImpulse (3),
+ * calculateAngularImpulse (3), calculateHelicity, calculateEnstrophy,
+ * calculateEnstrophyF, calculateKineticEnergy, calculateKineticEnergyF of
+ * wake wa (interaction.cpp:489-718), then the centroid sums
+ * sum |a| z (x^2 + y^2) and sum |a| (x^2 + y^2) of the wake
+ * (interaction.cpp:135-157).  wa < wb: zeros, then the five
+ * calculate*(W1, W2) cross terms, then zeros. */
+int pw_diagnose_wakes(pw_ctx *ctx, int wa, int wb, double *out, int64_t out_len);
+
 /* FP64 diagnostics of the current state, 15 doubles in the order
  * __interaction::writediagnosis writes them (interaction.cpp:325-335):
  * total vorticity (3), linear impulse (3), angular impulse (3), helicity,

@@ -37,7 +37,8 @@ EXPORTS = [
     "pw_ff_init_weights", "pw_ff_set_weights", "pw_ff_get_weights", "pw_ff_train",
     "pw_ff_mlp_forward", "pw_ff_mlp_grad", "pw_ff_set_conserve", "pw_ff_get_conserve",
     "pw_velocity_at", "pw_set_kernel", "pw_get_kernel", "pw_vring_particles", "pw_vring_resolve",
-    "pw_set_compensation", "pw_ff_get_timing",
+    "pw_set_compensation", "pw_ff_get_timing", "pw_interact_wakes", "pw_vorticity_field",
+    "pw_diagnose_wakes",
 ]
 
 # pw_ff_export selectors
@@ -116,6 +117,9 @@ def load(path: str | None = None) -> C.CDLL:
         "pw_ff_build": ([vp], i),
         "pw_ff_info": ([vp, C.POINTER(C.c_int64), i64], i),
         "pw_ff_get_timing": ([vp, C.POINTER(C.c_double), i64], i),
+        "pw_interact_wakes": ([vp, i, C.c_void_p, i], i),
+        "pw_vorticity_field": ([vp, C.c_void_p, i, _dp, i64], i),
+        "pw_diagnose_wakes": ([vp, i, i, _dp, i64], i),
         "pw_ff_export": ([vp, i, vp, i64], i),
         "pw_ff_sample": ([vp, i64, C.c_uint64], i),
         "pw_ff_clear_samples": ([vp], i),
@@ -302,6 +306,34 @@ class Solver:
     def solve(self, precision: str = "fp64"):
         _check(self._lib.pw_solve(self._h, PRECISIONS[precision]))
         return self.rates()
+
+    @staticmethod
+    def _pairs(pairs):
+        if pairs is None:
+            return None, 0
+        arr = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2))
+        return arr, len(arr)
+
+    def interact_wakes(self, precision: str, pairs):
+        """Rates of the listed wake blocks only [(a, b), a <= b]."""
+        arr, n = self._pairs(pairs)
+        _check(self._lib.pw_interact_wakes(self._h, PRECISIONS[precision],
+                                           arr.ctypes.data_as(C.c_void_p) if n else None, n))
+        return self.rates()
+
+    def vorticity_field(self, pairs=None) -> np.ndarray:
+        """Smoothed vorticity field at every particle (N x 3) of the listed
+        wake blocks, or of the whole system."""
+        arr, n = self._pairs(pairs)
+        out = np.zeros(3 * self.n)
+        _check(self._lib.pw_vorticity_field(self._h, arr.ctypes.data_as(C.c_void_p) if arr is not None else None,
+                                            n, out, len(out)))
+        return out.reshape(-1, 3)
+
+    def diagnose_wakes(self, a: int, b: int) -> np.ndarray:
+        out = np.zeros(16)
+        _check(self._lib.pw_diagnose_wakes(self._h, int(a), int(b), out, 16))
+        return out
 
     def velocity_at(self, points) -> np.ndarray:
         """Velocity (FP64) at points (m x 3) from every particle of the

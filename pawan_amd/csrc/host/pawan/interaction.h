@@ -38,18 +38,62 @@ class __interaction : public __system {
   double _Zc = 0.0;
   int _precision;                   /*!< PW_FP64 (default), PW_FP32C, ... */
 
-  //! zero the rates, then all self and cross pairs (interaction.cpp:196-210)
+  //! Diagnostics of one wake and of one pair of wakes (interaction.h:
+  //! 41-111, interaction.cpp:489-718).  Computed on the device
+  //! (pw_diagnose_wakes); diagnose() calls them per wake and per pair in the
+  //! reference's order, so a subclass may replace any of them.
+  virtual double calculateKineticEnergy(__wake *W);
+  virtual double calculateKineticEnergyF(__wake *W);
+  virtual double calculateKineticEnergy(__wake *W1, __wake *W2);
+  virtual double calculateKineticEnergyF(__wake *W1, __wake *W2);
+  virtual double calculateEnstrophy(__wake *W);
+  virtual double calculateEnstrophyF(__wake *W);
+  virtual double calculateEnstrophy(__wake *W1, __wake *W2);
+  virtual double calculateEnstrophyF(__wake *W1, __wake *W2);
+  virtual double calculateHelicity(__wake *W);
+  virtual double calculateHelicity(__wake *W1, __wake *W2);
+  virtual void calculateAngularImpulse(__wake *W, vec *A);
+  virtual void calculateLinearImpulse(__wake *W, vec *I);
+  virtual void calculateTotalVorticity(__wake *W, vec *O);
+
+  //! Zero the rates, then interact(W) for every wake and interact(W1, W2)
+  //! for every pair (interaction.cpp:196-210).  The blocks left to the base
+  //! class run on the device, in one launch when no block is overridden.
   virtual void interact();
+  //! Zero the vorticity field, then influence(W) and influence(W1, W2)
+  //! likewise (interaction.cpp:212-226).
+  virtual void influence();
 
  private:
   std::vector<__wake *> _W;
   pw_ctx *_ctx = nullptr;
   bool _deviceStale = true;   // host wake state newer than the device's
+  std::vector<int> _pending;  // wake blocks the base hooks left to the device
+  int _interactHooks = -1;    // -1 not yet seen, 0 all base, 1 some overridden
+  int _influenceHooks = -1;   // the same for the influence blocks
+  // per-block diagnostics of the current diagnose() call
+  std::vector<double> _diagBlocks;
+  bool _diagValid = false;
   void ensureDevice();
   void pushState();           // host wakes -> device
   void pullState();           // device -> host wakes (positions, strengths)
   void pullRates();           // device rates -> host wakes
+  void addRates();            // host wake rates += device rates
+  int wakeIndex(const __wake *W) const;
+  void queueBlock(__wake *W1, __wake *W2);  // a block left to the device
+  const double *blockDiag(__wake *W1, __wake *W2);
   [[noreturn]] void fail(const char *what) const;
+
+  //! The pairs within one wake (interaction.cpp:262-281) and between two
+  //! wakes (interaction.cpp:283-302), added to the wakes' velocity and
+  //! vorticity-rate rows.  Overridable, as __parallel does in the reference
+  //! (parallel.h:48-56); the base versions hand the block to the device.
+  virtual void interact(__wake *W);
+  virtual void interact(__wake *W1, __wake *W2);
+  //! The smoothed vorticity field of one wake / between two wakes, added to
+  //! the wakes' _vorticityfield rows (interaction.cpp:228-260).
+  virtual void influence(__wake *W);
+  virtual void influence(__wake *W1, __wake *W2);
 
  public:
   __interaction();
@@ -60,6 +104,8 @@ class __interaction : public __system {
   virtual void diagnose() override;
   virtual void addWake(__wake *W);
   virtual void solve() override;
+  //! influence() (interaction.cpp:58-61)
+  virtual void resolve();
   virtual void write(FILE *f) override;
   virtual void writediagnosis(FILE *fdiag) override;
   virtual void writenu(FILE *fdiag) override;

@@ -7,13 +7,16 @@
 
 namespace pawan {
 
-//! rebuild(IN, IO) gives the ring held by IN the cell strengths whose
-//! smoothed vorticity field at its first station matches the ring's ideal
-//! core profile (resolve.cpp:12-63), computed on the device
-//! (pw_vring_resolve: influence matrix and pivoted LU solve), and loads the
-//! resolved vorticity back into the host wake.  IN must hold exactly one
-//! wake, a __vring, as in the reference's driver (src/pawan.cpp:57-70).
-//! The .wakeinfluence file the reference writes through IO is not written.
+//! rebuild(IN, IO) (resolve.cpp:12-63): the influence matrix of IN's state
+//! vector, column i the rates after resolve() of the unit state e_i; the
+//! state solving it for the ideal rates (LU with partial pivoting) is set
+//! into IN.  With IO non-null, the initial and every unit state are written
+//! to <file>.wakeinfluence as [t][IN->write] records (t = 0, 1, ..., m), as
+//! the reference does.  For an interaction holding one __vring (the
+//! reference's use, src/pawan.cpp:57-70) whose influence blocks are the
+//! base class's, the matrix and the solve come from the device in one call
+//! (pw_vring_resolve); otherwise every column goes through the virtual
+//! resolve() / getRates(), and the solve runs on the host.
 class __resolve {
  public:
   __resolve() = default;

@@ -80,20 +80,35 @@ class __wake {
 //! The vortex-ring initial condition (src/wake/vring.h, vring.cpp:50-100):
 //! nAzi azimuthal stations of (2 nLayers + 1)^2 cross-section cells, a
 //! centre particle and layers of 8j particles, with zero vorticity until
-//! __resolve::rebuild gives the cells their strengths.  Its state vector is
-//! the generic wake's (positions and vorticity); the reference's __vring
-//! exposes the cell strengths as its state instead (vring.cpp:201-221).
+//! __resolve::rebuild gives the cells their strengths.  As in the
+//! reference, its state vector is the m = (2 nLayers + 1)^2 cell strengths
+//! (_size = _maxsize = m, vring.cpp:10-13, 59-62): setStates sets them and
+//! the particles' vorticity from them, getRates gives the in-plane
+//! magnitude of the smoothed vorticity field at the first station's m
+//! particles (vring.cpp:201-221), getIdealRates the Gaussian core profile
+//! (vring.cpp:223-242).  Copy it into a plain __wake to integrate it
+//! (src/pawan.cpp:76-77).
 class __vring : public __wake {
+ private:
+  //! vorticity of every particle from its cell's strength (vring.cpp:25-48)
+  virtual void set_vorticity();
+
  public:
   double _R;             /*!< ring radius                                  */
   double _sigmaR;        /*!< core radius of the ideal profile             */
   size_t _nLayers;       /*!< layers around the centre particle            */
   size_t _nAzi;          /*!< azimuthal stations                           */
+  double _dPsi;          /*!< azimuthal step                               */
   vec *_strength;        /*!< (2 nLayers + 1)^2 cell strengths             */
 
   __vring(const double &radius, const double &core, const int &nLayers, const int &nAzi,
           const double &sig);
   virtual ~__vring();
+
+  virtual void setStates(const vec *state) override;
+  virtual void getRates(vec *rate) override;
+  virtual void getStates(vec *state) override;
+  virtual void getIdealRates(vec *rate) override;
 };
 
 }  // namespace pawan

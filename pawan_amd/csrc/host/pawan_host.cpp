@@ -1,5 +1,6 @@
 // pawan_host.cpp -- the C++ class API (pawan::__wake, __interaction,
 // __integration, __rk4, __rk3, __io) over the C-ABI of include/pawan_gpu.h.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -7,6 +8,7 @@
 #include <cstdlib>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "pawan_gpu.h"
 #include "pawan/integration.h"
@@ -289,15 +291,100 @@ void __interaction::pullRates() {
       fail("pw_get_wake_rates");
 }
 
-void __interaction::interact() {
-  pushState();
-  if (pw_solve(_ctx, _precision) != PW_OK) fail("pw_solve");
-  pullRates();
+void __interaction::addRates() {
+  for (size_t k = 0; k < _W.size(); ++k) {
+    __wake *w = _W[k];
+    const size_t n3 = 3 * w->_numParticles;
+    std::vector<double> u(n3), r(n3);
+    if (pw_get_wake_rates(_ctx, static_cast<int>(k), u.data(), r.data()) != PW_OK) fail("pw_get_wake_rates");
+    double *hu = w->_velocity->ptr(), *hr = w->_retvorcity->ptr();
+    for (size_t i = 0; i < n3; ++i) {
+      hu[i] += u[i];
+      hr[i] += r[i];
+    }
+  }
 }
+
+int __interaction::wakeIndex(const __wake *W) const {
+  for (size_t k = 0; k < _W.size(); ++k)
+    if (_W[k] == W) return static_cast<int>(k);
+  throw std::invalid_argument("wake not held by this interaction");
+}
+
+// interaction.cpp:196-210.  The base hooks queue their blocks; the queue
+// then runs on the device and its rates are added to the wakes' rows, which
+// an overriding hook may already have filled.
+void __interaction::interact() {
+  for (__wake *w : _W) {
+    w->_velocity->set_zero();
+    w->_retvorcity->set_zero();
+  }
+  _pending.clear();
+  for (__wake *w : _W) interact(w);
+  for (size_t i = 0; i < _nWake; ++i)
+    for (size_t j = i + 1; j < _nWake; ++j) interact(_W[i], _W[j]);
+  const size_t blocks = _nWake * (_nWake + 1) / 2;
+  _interactHooks = _pending.size() / 2 == blocks ? 0 : 1;
+  if (_pending.empty()) return;
+  pushState();
+  const int rc = _interactHooks == 0 ? pw_solve(_ctx, _precision)
+                                     : pw_interact_wakes(_ctx, _precision, _pending.data(),
+                                                         static_cast<int>(_pending.size() / 2));
+  if (rc != PW_OK) fail("pw_interact_wakes");
+  addRates();
+}
+
+void __interaction::queueBlock(__wake *W1, __wake *W2) {
+  const int a = wakeIndex(W1), b = wakeIndex(W2);
+  _pending.push_back(std::min(a, b));
+  _pending.push_back(std::max(a, b));
+}
+
+void __interaction::interact(__wake *W) { queueBlock(W, W); }
+void __interaction::interact(__wake *W1, __wake *W2) { queueBlock(W1, W2); }
+
+// interaction.cpp:212-226, the field blocks likewise
+void __interaction::influence() {
+  for (__wake *w : _W) w->_vorticityfield->set_zero();
+  _pending.clear();
+  for (__wake *w : _W) influence(w);
+  for (size_t i = 0; i < _nWake; ++i)
+    for (size_t j = i + 1; j < _nWake; ++j) influence(_W[i], _W[j]);
+  _influenceHooks = _pending.size() / 2 == _nWake * (_nWake + 1) / 2 ? 0 : 1;
+  if (_pending.empty()) return;
+  pushState();
+  size_t n = 0;
+  for (__wake *w : _W) n += w->_numParticles;
+  std::vector<double> field(3 * n);
+  const size_t blocks = _nWake * (_nWake + 1) / 2;
+  const bool all = _pending.size() / 2 == blocks;
+  if (pw_vorticity_field(_ctx, all ? nullptr : _pending.data(), all ? 0 : static_cast<int>(_pending.size() / 2),
+                         field.data(), static_cast<int64_t>(field.size())) != PW_OK)
+    fail("pw_vorticity_field");
+  size_t off = 0;
+  for (__wake *w : _W) {
+    double *k = w->_vorticityfield->ptr();
+    for (size_t i = 0; i < 3 * w->_numParticles; ++i) k[i] += field[off + i];
+    off += 3 * w->_numParticles;
+  }
+}
+
+void __interaction::influence(__wake *W) { queueBlock(W, W); }
+void __interaction::influence(__wake *W1, __wake *W2) { queueBlock(W1, W2); }
+
+void __interaction::resolve() { influence(); }
 
 void __interaction::solve() { interact(); }
 
+// All stages of a step on the device, when every interaction block is the
+// base class's; a subclass that replaces a block is stepped through the
+// reference's setStates / solve / getRates sequence instead (returns false).
 bool __interaction::stepOnDevice(int scheme, const double &dt, vec *state) {
+  if (_interactHooks < 0) {
+    setStates(state);
+    interact();  // one right-hand side, to learn which blocks are overridden
+  }
+  if (_interactHooks != 0) return false;
   setStates(state);
   pushState();
   if (pw_step(_ctx, scheme, dt, _precision, 1) != PW_OK) fail("pw_step");
@@ -307,21 +394,96 @@ bool __interaction::stepOnDevice(int scheme, const double &dt, vec *state) {
   return true;
 }
 
-void __interaction::diagnose() {
-  pushState();
-  double out[15];
-  if (pw_diagnose(_ctx, out, 15) != PW_OK) fail("pw_diagnose");
-  for (int k = 0; k < 3; ++k) {
-    _totalVorticity->set(k, out[k]);
-    _linearImpulse->set(k, out[3 + k]);
-    _angularImpulse->set(k, out[6 + k]);
+// The diagnostics of the block (W1, W2) (W1 == W2: one wake), computed on
+// the device for every block at the first request of a diagnose() call.
+const double *__interaction::blockDiag(__wake *W1, __wake *W2) {
+  int a = wakeIndex(W1), b = wakeIndex(W2);
+  if (a > b) std::swap(a, b);
+  const size_t nw = _W.size();
+  if (!_diagValid) {
+    pushState();
+    _diagBlocks.assign(16 * nw * nw, 0.0);
+    for (size_t i = 0; i < nw; ++i)
+      for (size_t j = i; j < nw; ++j)
+        if (pw_diagnose_wakes(_ctx, static_cast<int>(i), static_cast<int>(j), &_diagBlocks[16 * (i * nw + j)],
+                              16) != PW_OK)
+          fail("pw_diagnose_wakes");
+    _diagValid = true;
   }
-  _helicity = out[9];
-  _enstrophy = out[10];
-  _enstrophyF = out[11];
-  _kineticEnergy = out[12];
-  _kineticEnergyF = out[13];
-  _Zc = out[14];
+  return &_diagBlocks[16 * (a * nw + b)];
+}
+
+void __interaction::calculateTotalVorticity(__wake *W, vec *O) {
+  const double *d = blockDiag(W, W);
+  for (int k = 0; k < 3; ++k) O->set(k, d[k]);
+}
+void __interaction::calculateLinearImpulse(__wake *W, vec *I) {
+  const double *d = blockDiag(W, W);
+  for (int k = 0; k < 3; ++k) I->set(k, d[3 + k]);
+}
+void __interaction::calculateAngularImpulse(__wake *W, vec *A) {
+  const double *d = blockDiag(W, W);
+  for (int k = 0; k < 3; ++k) A->set(k, d[6 + k]);
+}
+double __interaction::calculateHelicity(__wake *W) { return blockDiag(W, W)[9]; }
+double __interaction::calculateHelicity(__wake *W1, __wake *W2) { return blockDiag(W1, W2)[9]; }
+double __interaction::calculateEnstrophy(__wake *W) { return blockDiag(W, W)[10]; }
+double __interaction::calculateEnstrophy(__wake *W1, __wake *W2) { return blockDiag(W1, W2)[10]; }
+double __interaction::calculateEnstrophyF(__wake *W) { return blockDiag(W, W)[11]; }
+double __interaction::calculateEnstrophyF(__wake *W1, __wake *W2) { return blockDiag(W1, W2)[11]; }
+double __interaction::calculateKineticEnergy(__wake *W) { return blockDiag(W, W)[12]; }
+double __interaction::calculateKineticEnergy(__wake *W1, __wake *W2) { return blockDiag(W1, W2)[12]; }
+double __interaction::calculateKineticEnergyF(__wake *W) { return blockDiag(W, W)[13]; }
+double __interaction::calculateKineticEnergyF(__wake *W1, __wake *W2) { return blockDiag(W1, W2)[13]; }
+
+// interaction.cpp:64-157: the reference's loops over the calculate*
+// virtuals; Zc from the last wake with a non-zero denominator
+void __interaction::diagnose() {
+  _diagValid = false;
+  _totalVorticity->set_zero();
+  _linearImpulse->set_zero();
+  _angularImpulse->set_zero();
+  vec O(3), I(3), A(3);
+  for (__wake *w : _W) {
+    calculateTotalVorticity(w, &O);
+    calculateLinearImpulse(w, &I);
+    calculateAngularImpulse(w, &A);
+    for (int k = 0; k < 3; ++k) {
+      _totalVorticity->data[k] += O.data[k];
+      _linearImpulse->data[k] += I.data[k];
+      _angularImpulse->data[k] += A.data[k];
+    }
+  }
+  _helicity = 0.0;
+  for (size_t i = 0; i < _nWake; ++i) {
+    _helicity += calculateHelicity(_W[i]);
+    for (size_t j = i + 1; j < _nWake; ++j) _helicity += calculateHelicity(_W[i], _W[j]);
+  }
+  _enstrophy = 0.0;
+  for (size_t i = 0; i < _nWake; ++i) {
+    _enstrophy += calculateEnstrophy(_W[i]);
+    for (size_t j = i + 1; j < _nWake; ++j) _enstrophy += calculateEnstrophy(_W[i], _W[j]);
+  }
+  _enstrophyF = 0.0;
+  for (size_t i = 0; i < _nWake; ++i) {
+    _enstrophyF += calculateEnstrophyF(_W[i]);
+    for (size_t j = i + 1; j < _nWake; ++j) _enstrophyF += calculateEnstrophyF(_W[i], _W[j]);
+  }
+  _kineticEnergy = 0.0;
+  for (size_t i = 0; i < _nWake; ++i) {
+    _kineticEnergy += calculateKineticEnergy(_W[i]);
+    for (size_t j = i + 1; j < _nWake; ++j) _kineticEnergy += calculateKineticEnergy(_W[i], _W[j]);
+  }
+  _kineticEnergyF = 0.0;
+  for (size_t i = 0; i < _nWake; ++i) {
+    _kineticEnergyF += calculateKineticEnergyF(_W[i]);
+    for (size_t j = i + 1; j < _nWake; ++j) _kineticEnergyF += calculateKineticEnergyF(_W[i], _W[j]);
+  }
+  for (__wake *w : _W) {
+    const double *d = blockDiag(w, w);
+    if (d[15] != 0.0) _Zc = d[14] / d[15];
+  }
+  _diagValid = false;
 }
 
 void __interaction::write(FILE *f) {
@@ -387,7 +549,16 @@ void __interaction::getStates(vec *state) {
   }
 }
 
-void __interaction::getIdealRates(vec *rate) { rate->set_zero(); }
+// interaction.cpp:480-487: each wake's ideal rates in its block
+void __interaction::getIdealRates(vec *rate) {
+  size_t offset = 0;
+  for (__wake *w : _W) {
+    vec rt = subvector(rate, offset, w->_maxsize);
+    w->getIdealRates(&rt);
+    for (size_t i = 0; i < w->_maxsize; ++i) rate->data[offset + i] = rt.data[i];
+    offset += w->_maxsize;
+  }
+}
 
 void __interaction::getVi(const vec *r, vec *vi, const size_t & /*n*/) {
   pushState();
@@ -544,27 +715,161 @@ __vring::__vring(const double &radius, const double &core, const int &nLayers, c
   _sigmaR = core;
   _nLayers = static_cast<size_t>(nLayers);
   _nAzi = static_cast<size_t>(nAzi);
-  _strength = new vec((2 * _nLayers + 1) * (2 * _nLayers + 1));
+  _dPsi = 2.0 * M_PI / static_cast<double>(nAzi);
+  const size_t m = (2 * _nLayers + 1) * (2 * _nLayers + 1);
+  _size = m;  // the state is the cell strengths (vring.cpp:62, 13)
+  _maxsize = m;
+  _strength = new vec(m);
   if (pw_vring_particles(radius, core, nLayers, nAzi, sig, _position->ptr(), _radius->ptr(),
                          _volume->ptr()) != PW_OK)
     throw std::runtime_error(std::string("__vring: ") + pw_last_error());
-  for (size_t i = 0; i < _numParticles; ++i) {
-    for (size_t j = 0; j < 3; ++j) _vorticity->set(i, j, 0.0);
-    _birthstrength->set(i, 1.0);  // vring.cpp:81, 98
-  }
+  for (size_t i = 0; i < _numParticles; ++i) _birthstrength->set(i, 1.0);  // vring.cpp:81, 98
+  set_vorticity();
 }
 
 __vring::~__vring() { delete _strength; }
 
+void __vring::set_vorticity() {
+  const size_t m = _strength->size;
+  size_t p = 0;
+  for (size_t a = 0; a < _nAzi; ++a) {
+    const double psi = static_cast<double>(a) * _dPsi, sp = std::sin(psi), cp = std::cos(psi);
+    for (size_t i = 0; i < m; ++i, ++p) {
+      _vorticity->set(p, 0, -sp * _strength->get(i));
+      _vorticity->set(p, 1, cp * _strength->get(i));
+      _vorticity->set(p, 2, 0.0);
+    }
+  }
+}
+
+void __vring::setStates(const vec *state) {
+  for (size_t i = 0; i < _strength->size; ++i) _strength->set(i, state->get(i));
+  set_vorticity();
+}
+
+void __vring::getRates(vec *rate) {
+  for (size_t i = 0; i < _size; ++i) {
+    const double ax = _vorticityfield->get(i, 0), ay = _vorticityfield->get(i, 1);
+    rate->set(i, std::sqrt(ax * ax + ay * ay));
+  }
+}
+
+void __vring::getStates(vec *state) {
+  for (size_t i = 0; i < _strength->size; ++i) state->set(i, _strength->get(i));
+}
+
+void __vring::getIdealRates(vec *rate) {
+  const double mag0 = 0.5 / (M_PI * _sigmaR * _sigmaR);
+  const double r0 = 0.35 / (2 * static_cast<double>(_nLayers) + 1);  // vring.cpp:227
+  size_t idx = 0;
+  rate->set(idx++, mag0);
+  for (size_t j = 1; j <= _nLayers; ++j) {
+    const double jj = static_cast<double>(j);
+    const double r = r0 * (12. * jj * jj + 1.0) / (6. * jj);
+    const size_t ncore = 8 * j;
+    const double dth = 2. * M_PI / static_cast<double>(ncore);
+    const double decay = std::exp(-0.5 * r * r / (_sigmaR * _sigmaR));
+    for (size_t k = 0; k < ncore; ++k)
+      rate->set(idx++, mag0 * (1.0 + r * std::cos(static_cast<double>(k) * dth) / _R) * decay);
+  }
+}
+
+// A x = b by LU with partial pivoting, the row of largest |a_ij| at or below
+// the diagonal (first on ties), as gsl_linalg_LU_decomp / LU_solve; A is
+// overwritten.  Returns false if a pivot is zero or not finite.
+static bool lu_solve(size_t n, std::vector<double> &A, const std::vector<double> &b, std::vector<double> &x) {
+  std::vector<size_t> perm(n);
+  for (size_t i = 0; i < n; ++i) perm[i] = i;
+  bool ok = true;
+  for (size_t j = 0; j < n; ++j) {
+    size_t ip = j;
+    double mx = std::fabs(A[j * n + j]);
+    for (size_t i = j + 1; i < n; ++i)
+      if (std::fabs(A[i * n + j]) > mx) {
+        mx = std::fabs(A[i * n + j]);
+        ip = i;
+      }
+    if (ip != j) {
+      for (size_t c = 0; c < n; ++c) std::swap(A[j * n + c], A[ip * n + c]);
+      std::swap(perm[j], perm[ip]);
+    }
+    const double piv = A[j * n + j];
+    if (!(std::fabs(piv) > 0.0) || !std::isfinite(piv)) {
+      ok = false;
+      continue;
+    }
+    for (size_t i = j + 1; i < n; ++i) {
+      const double l = A[i * n + j] / piv;
+      A[i * n + j] = l;
+      for (size_t c = j + 1; c < n; ++c) A[i * n + c] -= l * A[j * n + c];
+    }
+  }
+  x.resize(n);
+  for (size_t i = 0; i < n; ++i) x[i] = b[perm[i]];
+  for (size_t i = 0; i < n; ++i)
+    for (size_t c = 0; c < i; ++c) x[i] -= A[i * n + c] * x[c];
+  for (size_t i = n; i-- > 0;) {
+    for (size_t c = i + 1; c < n; ++c) x[i] -= A[i * n + c] * x[c];
+    x[i] /= A[i * n + i];
+  }
+  return ok;
+}
+
+// resolve.cpp:12-63: column i of the influence matrix is the rate vector of
+// the unit state e_i after resolve(), every unit state is written to the
+// .wakeinfluence file after the initial one, and the ideal rates are solved
+// for.  A single __vring whose influence blocks are the base class's takes
+// the matrix (and the solve) from the device in one call
+// (pw_vring_resolve); the file records are written from the host state.
 void __resolve::rebuild(__interaction *IN, __io *IO) {
-  (void)IO;
+  const size_t m = IN->_size;  // resolve.cpp:13-14
+  vec states(IN->_totalmaxsize);
+  std::vector<double> M(m * m);
+  FILE *f = IO ? IO->create_binary_file(".wakeinfluence") : nullptr;
+  double t = 0.0;
+  if (f) {
+    std::fwrite(&t, sizeof(double), 1, f);
+    IN->write(f);
+  }
+  IN->getStates(&states);
   __vring *ring = IN->_W.size() == 1 ? dynamic_cast<__vring *>(IN->_W[0]) : nullptr;
-  if (!ring) throw std::runtime_error("__resolve::rebuild: the interaction must hold one __vring wake");
-  pw_ctx *ctx = IN->context();
-  if (pw_vring_resolve(ctx, 0, static_cast<int>(ring->_nLayers), static_cast<int>(ring->_nAzi), ring->_R,
-                       ring->_sigmaR, ring->_strength->ptr(), nullptr) != PW_OK)
-    IN->fail("pw_vring_resolve");
-  IN->pullState();
+  bool device = false;
+  vec col(IN->_totalmaxsize);
+  for (size_t i = 0; i < m; ++i) {
+    states.set(i, 1.0);
+    IN->setStates(&states);
+    if (!device) {
+      IN->resolve();
+      IN->getRates(&col);
+      for (size_t p = 0; p < m; ++p) M[p * m + i] = col.get(p);
+      // the first column shows whether the influence blocks are the base
+      // class's; then the device builds the whole matrix
+      if (i == 0 && ring && IN->_influenceHooks == 0) device = true;
+    }
+    states.set(i, 0.0);
+    t += 1.0;
+    if (f) {
+      std::fwrite(&t, sizeof(double), 1, f);
+      IN->write(f);
+    }
+  }
+  if (f) std::fclose(f);
+  vec ideal(IN->_totalmaxsize);
+  IN->getIdealRates(&ideal);
+  std::vector<double> s(m);
+  if (device) {
+    pw_ctx *ctx = IN->context();
+    if (pw_vring_resolve(ctx, 0, static_cast<int>(ring->_nLayers), static_cast<int>(ring->_nAzi), ring->_R,
+                         ring->_sigmaR, s.data(), M.data()) != PW_OK)
+      IN->fail("pw_vring_resolve");
+  } else {
+    std::vector<double> b(ideal.data.begin(), ideal.data.begin() + static_cast<std::ptrdiff_t>(m)), x;
+    if (!lu_solve(m, M, b, x)) throw std::runtime_error("__resolve::rebuild: singular influence matrix");
+    s = x;
+  }
+  vec result(IN->_totalmaxsize);
+  for (size_t i = 0; i < m; ++i) result.set(i, s[i]);
+  IN->setStates(&result);  // resolve.cpp:58
 }
 
 }  // namespace pawan

@@ -258,6 +258,34 @@ int ensure_kind(pw_ctx* c, int kind) {
   return PW_OK;
 }
 
+// Blocks of wake pairs for the per-wake hooks: pairs (a, b) with a <= b,
+// a == b the self pairs of wake a, a < b the cross pairs of a and b.
+int check_blocks(const pw_ctx* c, const int* pairs, int npairs) {
+  if (npairs < 0 || (npairs > 0 && !pairs)) return fail(PW_ERR_ARG, "null or negative wake-pair list");
+  const int nw = static_cast<int>(c->wn.size());
+  for (int k = 0; k < npairs; ++k) {
+    const int a = pairs[2 * k], b = pairs[2 * k + 1];
+    if (a < 0 || b < a || b >= nw) return fail(PW_ERR_ARG, "wake pairs need 0 <= a <= b < wakes");
+    if (a != b && !c->coupled) return fail(PW_ERR_STATE, "cross-wake pairs need a coupled context");
+  }
+  return PW_OK;
+}
+
+// targets of wake a (256 at a time) against the sources of wake b
+void append_items(const pw_ctx* c, int a, int b, int nself, std::vector<WorkItem>& out) {
+  for (int64_t t0 = c->woff[a]; t0 < c->woff[a + 1]; t0 += kTargetsPerItem)
+    out.push_back({static_cast<int32_t>(t0), static_cast<int32_t>(std::min<int64_t>(t0 + kTargetsPerItem, c->woff[a + 1])),
+                   static_cast<int32_t>(c->woff[b]), static_cast<int32_t>(c->woff[b + 1]), -1, nself, 0, 0});
+}
+
+// the pair-kernel kind a precision launches under the context's kernel and
+// summation settings
+int launch_kind_of(const pw_ctx* c, int kind) {
+  if (c->smoothing == PW_KERNEL_HIGHORDER) return kind == pw::kPairFP32C ? pw::kPairFP32CHO : pw::kPairFP64HO;
+  if (kind == pw::kPairFP32C && !c->compensated) return pw::kPairFP32CPlain;
+  return kind;
+}
+
 // One right-hand side f(state) -> out, with optional event timing.
 int rhs(pw_ctx* c, int kind, const Rec64* state, Rate64* out) {
   if (kind == kKindLearned) {
@@ -293,9 +321,7 @@ int rhs(pw_ctx* c, int kind, const Rec64* state, Rate64* out) {
   args.queue = c->d_queue;
   args.nu = c->nu;
   // the high-order kernel replaces the Gaussian kinds (pw_set_kernel)
-  const int launch_kind = c->smoothing != PW_KERNEL_HIGHORDER ? kind
-                          : kind == pw::kPairFP32C            ? pw::kPairFP32CHO
-                                                              : pw::kPairFP64HO;
+  const int launch_kind = launch_kind_of(c, kind);
   const int grid = c->num_cus * pw::pair_kernel_waves(launch_kind, k.cfg);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c->profiling) {
@@ -865,6 +891,169 @@ int pw_step(pw_ctx* c, int scheme, double dt, int precision, int64_t nsteps) {
   }
   if (nsteps > 0) c->rates = last_rates(c, scheme);
   return check_flag(c, "pw_step");
+}
+
+int pw_interact_wakes(pw_ctx* c, int precision, const int* pairs, int npairs) {
+  PW_CTX(c);
+  int kind = 0;
+  PW_TRY(kind_of(precision, &kind));
+  if (kind == kKindLearned) return fail(PW_ERR_UNSUPPORTED, "the tree path evaluates the whole system at once");
+  PW_TRY(check_blocks(c, pairs, npairs));
+  PW_TRY(ensure_built(c));
+  std::vector<WorkItem> items;
+  std::vector<int> first(npairs + 1, 0);
+  for (int k = 0; k < npairs; ++k) {
+    const int a = pairs[2 * k], b = pairs[2 * k + 1];
+    append_items(c, a, b, 0, items);
+    if (a != b) append_items(c, b, a, 0, items);
+    first[k + 1] = static_cast<int>(items.size());
+  }
+  const int n = static_cast<int>(c->n);
+  Rate64* acc = c->k[0];
+  Rate64* blk = c->k[1];
+  PW_HIP(pw::launch_fill_int(c->d_flag, 1, 0, c->stream));
+  PW_HIP(pw::launch_zero_doubles(reinterpret_cast<double*>(acc), 6 * static_cast<int64_t>(n), c->stream));
+  WorkItem* d_items = nullptr;
+  if (!items.empty() && dalloc(&d_items, items.size()) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(PW_ERR_NOMEM, "work items");
+  }
+  hipError_t e = items.empty() ? hipSuccess
+                               : hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(WorkItem),
+                                                hipMemcpyHostToDevice, c->stream);
+  const int lk = launch_kind_of(c, kind);
+  for (int k = 0; k < npairs && e == hipSuccess; ++k) {
+    // one block at a time: its items have distinct targets and write their
+    // rates into blk, which is then added to the targets' totals
+    pw::PairArgs args{c->y, d_items + first[k], first[k + 1] - first[k], n, blk, nullptr, c->d_queue, c->nu};
+    e = pw::launch_pair(lk, pw::PairConfig{}, args, args.nitems, c->stream);
+    const int a = pairs[2 * k], b = pairs[2 * k + 1];
+    if (e == hipSuccess)
+      e = pw::launch_add_rates(acc, blk, static_cast<int>(c->woff[a]), static_cast<int>(c->woff[a + 1]), c->stream);
+    if (e == hipSuccess && a != b)
+      e = pw::launch_add_rates(acc, blk, static_cast<int>(c->woff[b]), static_cast<int>(c->woff[b + 1]), c->stream);
+  }
+  if (e == hipSuccess) e = pw::launch_check_finite(acc, n, c->d_flag, c->stream);
+  (void)hipStreamSynchronize(c->stream);  // the items are freed below
+  (void)hipFree(d_items);
+  if (e != hipSuccess) return fail(PW_ERR_HIP, std::string("pw_interact_wakes: ") + hipGetErrorString(e));
+  c->rates = acc;
+  return check_flag(c, "pw_interact_wakes");
+}
+
+int pw_vorticity_field(pw_ctx* c, const int* pairs, int npairs, double* field, int64_t len) {
+  PW_CTX(c);
+  if (!field) return fail(PW_ERR_ARG, "null output");
+  if (pairs) PW_TRY(check_blocks(c, pairs, npairs));
+  PW_TRY(ensure_built(c));
+  const int n = static_cast<int>(c->n);
+  if (len != 3 * static_cast<int64_t>(n)) return fail(PW_ERR_ARG, "field needs 3 doubles per particle");
+  // launch groups whose items have distinct targets: the whole system at
+  // once, or one block of wake pairs at a time
+  std::vector<WorkItem> items;
+  std::vector<int> first{0};
+  if (!pairs) {
+    items = c->h_pair_items;
+    first.push_back(static_cast<int>(items.size()));
+  } else {
+    for (int k = 0; k < npairs; ++k) {
+      const int a = pairs[2 * k], b = pairs[2 * k + 1];
+      append_items(c, a, b, 0, items);
+      if (a != b) append_items(c, b, a, 0, items);
+      first.push_back(static_cast<int>(items.size()));
+    }
+  }
+  double* d_field = reinterpret_cast<double*>(c->k[1]);  // 6 doubles per particle
+  WorkItem* d_items = nullptr;
+  if (!items.empty() && dalloc(&d_items, items.size()) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(PW_ERR_NOMEM, "work items");
+  }
+  hipError_t e = pw::launch_zero_doubles(d_field, 3 * static_cast<int64_t>(n), c->stream);
+  if (e == hipSuccess && !items.empty())
+    e = hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice, c->stream);
+  const bool ho = c->smoothing == PW_KERNEL_HIGHORDER;
+  for (size_t g = 0; g + 1 < first.size() && e == hipSuccess; ++g)
+    e = pw::launch_vorticity_field(c->y, d_items + first[g], first[g + 1] - first[g], ho, d_field, c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(field, d_field, sizeof(double) * 3 * static_cast<size_t>(n), hipMemcpyDeviceToHost,
+                       c->stream);
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(d_items);
+  if (e != hipSuccess) return fail(PW_ERR_HIP, std::string("pw_vorticity_field: ") + hipGetErrorString(e));
+  return PW_OK;
+}
+
+int pw_diagnose_wakes(pw_ctx* c, int wa, int wb, double* out, int64_t out_len) {
+  PW_CTX(c);
+  if (!out || out_len < 16) return fail(PW_ERR_ARG, "output needs 16 doubles");
+  const int pair[2] = {wa, wb};
+  PW_TRY(check_blocks(c, pair, 1));
+  PW_TRY(ensure_built(c));
+  std::vector<WorkItem> items;
+  append_items(c, wa, wb, wa == wb ? 0 : 1, items);
+  constexpr int Q = pw::kDiagQuad;
+  const int ni = static_cast<int>(items.size());
+  // the wake's linear-sum items are contiguous in the context's list
+  int l0 = 0, nl = 0;
+  for (int b = 0; b < c->nlin; ++b)
+    if (c->lin_wake[b] == wa) {
+      if (nl == 0) l0 = b;
+      ++nl;
+    }
+  WorkItem* d_items = nullptr;
+  double* d_q = nullptr;
+  if (ni > 0 && (dalloc(&d_items, items.size()) != hipSuccess || dalloc(&d_q, Q * items.size()) != hipSuccess)) {
+    (void)hipGetLastError();
+    (void)hipFree(d_items);
+    return fail(PW_ERR_NOMEM, "diagnostic buffers");
+  }
+  std::vector<double> quad(Q * static_cast<size_t>(ni)), lin(11 * static_cast<size_t>(nl));
+  hipError_t e = hipSuccess;
+  if (ni > 0) {
+    e = hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+      e = pw::launch_diag_pairs(c->y, c->d_radius, d_items, ni, d_q, c->smoothing == PW_KERNEL_HIGHORDER,
+                                c->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(quad.data(), d_q, quad.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  }
+  if (e == hipSuccess && wa == wb && nl > 0) {
+    e = pw::launch_diag_linear(c->y, c->lin_items + l0, nl, c->d_lin + 11 * static_cast<size_t>(l0), c->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(lin.data(), c->d_lin + 11 * static_cast<size_t>(l0), lin.size() * sizeof(double),
+                         hipMemcpyDeviceToHost, c->stream);
+  }
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(d_items);
+  (void)hipFree(d_q);
+  if (e != hipSuccess) return fail(PW_ERR_HIP, std::string("pw_diagnose_wakes: ") + hipGetErrorString(e));
+  for (int m = 0; m < 16; ++m) out[m] = 0.0;
+  if (wa == wb) {
+    double v[11] = {0};
+    for (int b = 0; b < nl; ++b)
+      for (int m = 0; m < 11; ++m) v[m] += lin[11 * b + m];
+    for (int m = 0; m < 3; ++m) {
+      out[m] = v[m];                       // calculateTotalVorticity
+      out[3 + m] = v[3 + m] * 0.5;         // calculateLinearImpulse
+      out[6 + m] = v[6 + m] * (1.0 / 3.0); // calculateAngularImpulse
+    }
+    out[14] = v[9];
+    out[15] = v[10];
+  }
+  double q[Q] = {0};
+  for (int b = 0; b < ni; ++b)
+    for (int m = 0; m < Q; ++m) q[m] += quad[Q * b + m];
+  // ordered sum over the block's (target, source) pairs: within a wake it is
+  // 2 sum_{i<j} plus the self terms; across wakes one direction, doubled as
+  // in calculate*(W1, W2) (interaction.cpp:551-718)
+  const double f = wa == wb ? 1.0 : 2.0;
+  out[9] = f * q[0];   // helicity
+  out[10] = f * q[3];  // enstrophy
+  out[11] = f * q[4];  // div-free enstrophy
+  out[12] = f * q[1];  // kinetic energy
+  out[13] = f * q[2];  // div-free kinetic energy
+  return PW_OK;
 }
 
 int pw_diagnose(pw_ctx* c, double* out, int64_t out_len) {

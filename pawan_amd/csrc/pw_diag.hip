@@ -175,8 +175,9 @@ __global__ __launch_bounds__(kTB) void k_diag_pairs(
       }
     }
   }
-  if (active) {
-    // self terms (interaction_utils.h:249-267)
+  if (active && it.nself == 0) {
+    // self terms (interaction_utils.h:249-267); items of a cross-wake block
+    // (nself = 1) have none
     const double sg = radius[t];
     double a2 = 0.0;
     a2 += me.ax * me.ax;

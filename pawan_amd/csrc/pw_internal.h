@@ -29,10 +29,11 @@ struct alignas(16) Rate64 {
 // the item covers all of its targets' sources and writes their rates; part
 // >= 0: it covers one of several source ranges and writes a partial sum to
 // slice `part` of the partial buffer.  (The diagnostics reuse the
-// structure with {first, last, wake, -}.)
+// structure with {first, last, wake, -}; for their pair sums, nself = 1
+// marks an item of a cross-wake block, which adds no self terms.)
 struct WorkItem {
   int32_t t0, t1, s0, s1;
-  int32_t part, pad0, pad1, pad2;
+  int32_t part, nself, pad1, pad2;
 };
 
 // Arguments of a pair-kernel launch (passed by value).
@@ -107,6 +108,18 @@ hipError_t launch_check_finite(const Rate64* k, int n, int* flag,
 // zero-fill of a small int buffer (a kernel, not hipMemsetAsync, so that
 // the step sequence can be captured into a HIP graph)
 hipError_t launch_fill_int(int* p, int n, int value, hipStream_t st);
+// p[0, n) = 0 (a kernel, for the same reason)
+hipError_t launch_zero_doubles(double* p, int64_t n, hipStream_t st);
+// acc[i] += add[i] for i in [first, last)
+hipError_t launch_add_rates(Rate64* acc, const Rate64* add, int first, int last, hipStream_t st);
+
+// Smoothed vorticity field (__interaction::influence, interaction.cpp:
+// 212-260): field[3 t + k] += sum over the item's sources s of
+// zeta_sigma(|x_t - x_s|) a_s, sigma^2 = (s_s^2 + s_t^2)/2, with
+// zeta_{s_t}(0) a_t for s = t (SELFINFLUENCE).  Items of one launch must
+// have distinct targets.
+hipError_t launch_vorticity_field(const Rec64* y, const WorkItem* items, int nitems, bool high_order,
+                                  double* field, hipStream_t st);
 
 // Diagnostics (FP64).  Linear sums: items[b] = {first, last, -, -} particle
 // ranges inside one wake; out[11 * b + m] gets the block's sums of

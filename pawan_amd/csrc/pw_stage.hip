@@ -139,9 +139,37 @@ __global__ void k_fill_int(int* __restrict__ p, int n, int value) {
   if (i < n) p[i] = value;
 }
 
+__global__ void k_zero_doubles(double* __restrict__ p, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+__global__ __launch_bounds__(kEW) void k_add_rates(Rate64* __restrict__ acc, const Rate64* __restrict__ add,
+                                                   int first, int last) {
+  const int i = first + blockIdx.x * kEW + static_cast<int>(threadIdx.x);
+  if (i >= last) return;
+  Rate64 a = acc[i];
+  const Rate64 b = add[i];
+  a.ux += b.ux; a.uy += b.uy; a.uz += b.uz;
+  a.wx += b.wx; a.wy += b.wy; a.wz += b.wz;
+  acc[i] = a;
+}
+
 inline int ew_blocks(int n) { return (n + kEW - 1) / kEW; }
 
 }  // namespace
+
+hipError_t launch_zero_doubles(double* p, int64_t n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero_doubles, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_add_rates(Rate64* acc, const Rate64* add, int first, int last, hipStream_t st) {
+  if (last <= first) return hipSuccess;
+  hipLaunchKernelGGL(k_add_rates, dim3(ew_blocks(last - first)), dim3(kEW), 0, st, acc, add, first, last);
+  return hipGetLastError();
+}
 
 hipError_t launch_stage(const Rec64* y, const Rate64* k, double c, Rec64* out,
                         int n, hipStream_t st) {

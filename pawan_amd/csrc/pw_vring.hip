@@ -37,6 +37,47 @@ __device__ __forceinline__ double zetasig(double rho, double sigma, bool ho) {
   return exp(-0.5 * rb * rb) / sig3 / kP15;
 }
 
+// Smoothed vorticity field over work items (one block of up to 256 targets
+// each, sources through LDS): INFLUENCE / SELFINFLUENCE
+// (interaction_utils.h:626-703) from the target's side, with each pair's
+// sigma = sqrt((s_s^2 + s_t^2)/2) and the target's own radius for itself.
+__global__ __launch_bounds__(kTB) void k_vorticity_field(const Rec64* __restrict__ y,
+                                                         const WorkItem* __restrict__ items, int ho,
+                                                         double* __restrict__ field) {
+  __shared__ Rec64 tile[kTB];
+  const WorkItem it = items[blockIdx.x];
+  const int t = it.t0 + static_cast<int>(threadIdx.x);
+  const bool active = t < it.t1;
+  const Rec64 me = y[active ? t : it.t0];
+  double kx = 0.0, ky = 0.0, kz = 0.0;
+  for (int base = it.s0; base < it.s1; base += kTB) {
+    const int cnt = min(kTB, it.s1 - base);
+    __syncthreads();
+    if (static_cast<int>(threadIdx.x) < cnt) tile[threadIdx.x] = y[base + threadIdx.x];
+    __syncthreads();
+    if (!active) continue;
+    for (int l = 0; l < cnt; ++l) {
+      const Rec64& s = tile[l];
+      double Z;
+      if (base + l == t) {
+        Z = zetasig(0.0, sqrt(me.s2), ho != 0);
+      } else {
+        const double dx = me.x - s.x, dy = me.y - s.y, dz = me.z - s.z;
+        const double rho = sqrt(dx * dx + dy * dy + dz * dz);
+        Z = zetasig(rho, sqrt(0.5 * (s.s2 + me.s2)), ho != 0);
+      }
+      kx += Z * s.ax;
+      ky += Z * s.ay;
+      kz += Z * s.az;
+    }
+  }
+  if (active) {
+    field[3 * static_cast<size_t>(t)] += kx;
+    field[3 * static_cast<size_t>(t) + 1] += ky;
+    field[3 * static_cast<size_t>(t) + 2] += kz;
+  }
+}
+
 // M[p * m + i] for p, i < m; ring particles start at `off` in the state.
 __global__ __launch_bounds__(kTB) void k_vring_influence(const Rec64* __restrict__ y, int off, int m,
                                                          int nazi, double dpsi, int ho,
@@ -173,6 +214,13 @@ __global__ __launch_bounds__(kTB) void k_vring_set_alpha(Rec64* __restrict__ y, 
 }
 
 }  // namespace
+
+hipError_t launch_vorticity_field(const Rec64* y, const WorkItem* items, int nitems, bool high_order,
+                                  double* field, hipStream_t st) {
+  if (nitems <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_vorticity_field, dim3(nitems), dim3(kTB), 0, st, y, items, high_order ? 1 : 0, field);
+  return hipGetLastError();
+}
 
 hipError_t launch_vring_resolve(Rec64* y, int off, int m, int nazi, bool high_order, double* M,
                                 double* lu, const double* ideal, double* s, int* flag, hipStream_t st) {

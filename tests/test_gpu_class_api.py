@@ -130,3 +130,19 @@ def test_driver_resolves_a_vring(tmp_path, oracle):
     print(f"  e_pos={normwise(p1, ref.pos):.2e} e_alpha={normwise(v1, ref.alpha):.2e}", flush=True)
     assert normwise(p1, ref.pos) < 1e-12
     assert normwise(v1, ref.alpha) < 1e-11
+    # the .wakeinfluence file of the resolve (resolve.cpp:16-35): the initial
+    # state, then one record per unit cell strength, t = 0, 1, ..., m
+    m = (2 * nl + 1) ** 2
+    infl = read_wake(tmp_path / "vr_vringWake.wakeinfluence")
+    assert [t for t, _ in infl] == [float(i) for i in range(m + 1)]
+    psi = np.repeat(np.arange(nazi) * 2 * np.pi / nazi, m)
+    cell = np.tile(np.arange(m), nazi)
+    for i in (1, 2, m):
+        n_i, p_i, v_i = infl[i][1][0]
+        assert n_i == len(pos)
+        np.testing.assert_allclose(p_i, pos, rtol=0, atol=4e-16)
+        on = (cell == i - 1).astype(float)
+        np.testing.assert_allclose(v_i[:, 0], -np.sin(psi) * on, rtol=0, atol=1e-15)
+        np.testing.assert_allclose(v_i[:, 1], np.cos(psi) * on, rtol=0, atol=1e-15)
+        assert np.all(v_i[cell != i - 1] == 0.0)
+    assert np.all(infl[0][1][0][2] == 0.0)  # the ring starts without vorticity
