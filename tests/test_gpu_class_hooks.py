@@ -108,7 +108,7 @@ def test_subclass_overriding_the_hooks(tmp_path, oracle):
     _write_input(inp, s)
     dt = 0.002
     env = dict(os.environ, PAWAN_DATA_DIR=str(tmp_path))
-    name = str(tmp_path / "hk")
+    name = "hk"  # the driver's files: hk.rates, hk.diag (cwd) and hkWake.wake (__io)
     r = subprocess.run([HOOKS, str(inp), name, repr(dt), repr(s.nu)], env=env, capture_output=True, text=True,
                        timeout=120, cwd=str(tmp_path))
     assert r.returncode == 0, r.stderr
@@ -117,7 +117,7 @@ def test_subclass_overriding_the_hooks(tmp_path, oracle):
     # solve() once, then four stages of the step on the host path, two
     # wakes each; diagnose() once per wake
     assert calls == {"self_calls": 10, "ke_calls": 2}
-    rates = np.fromfile(name + ".rates", dtype=np.float64)
+    rates = np.fromfile(str(tmp_path / "hk.rates"), dtype=np.float64)
     u, w = [], []
     o = 0
     for k in range(s.nw):
@@ -128,7 +128,7 @@ def test_subclass_overriding_the_hooks(tmp_path, oracle):
     eu, ew = normwise(np.concatenate(u), ru), normwise(np.concatenate(w), rw)
     _say(f"host self pairs + device cross pairs vs oracle: e_u={eu:.2e} e_w={ew:.2e}")
     assert eu < 1e-12 and ew < 1e-12
-    d = np.fromfile(name + ".diag", dtype=np.float64)
+    d = np.fromfile(str(tmp_path / "hk.diag"), dtype=np.float64)
     ref = oracle.diagnose(s)
     ke_self = sum(oracle.diagnose(s.wake(k))[12] for k in range(s.nw))
     assert d[12] == pytest.approx(ref[12] + ke_self, rel=1e-12)   # the override doubled the self parts
