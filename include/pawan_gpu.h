@@ -181,6 +181,19 @@ int pw_get_rates(pw_ctx *ctx, double *rate, int64_t len);
  * Blocking; allocates its point buffers per call. */
 int pw_velocity_at(pw_ctx *ctx, const double *points, int64_t m, double *u);
 
+/* pw_velocity_at for callers that cannot allocate (code under HIP graph
+ * capture, tight serving or coupling loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Same arguments and results
+ * as pw_velocity_at (bit-identical: the same kernel), but the points and
+ * velocities are staged through a static device array of the library, so
+ * the call makes no hipMalloc / hipFree.  Limits: m <= 65,536 points
+ * (PW_ERR_ARG beyond); the context must already hold its particles on the
+ * device (its first pw_solve, pw_step or pw_velocity_at puts them there),
+ * else PW_ERR_STATE.  Each GPU has one staging array, shared by every
+ * context on it: calls that overlap on one GPU (from several host threads)
+ * race.  Blocking. */
+int pw_velocity_at_noalloc(pw_ctx *ctx, const double *points, int64_t m, double *u);
+
 /* One right-hand-side evaluation: zero the rates, then all self and (when
  * coupled) cross-wake pair interactions, leaving velocity and vorticity
  * rate of every particle on the device (__interaction::solve ->

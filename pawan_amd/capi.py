@@ -38,7 +38,7 @@ EXPORTS = [
     "pw_ff_mlp_forward", "pw_ff_mlp_grad", "pw_ff_set_conserve", "pw_ff_get_conserve",
     "pw_velocity_at", "pw_set_kernel", "pw_get_kernel", "pw_vring_particles", "pw_vring_resolve",
     "pw_set_compensation", "pw_ff_get_timing", "pw_interact_wakes", "pw_vorticity_field",
-    "pw_diagnose_wakes",
+    "pw_diagnose_wakes", "pw_velocity_at_noalloc",
 ]
 
 # pw_ff_export selectors
@@ -99,6 +99,7 @@ def load(path: str | None = None) -> C.CDLL:
         "pw_get_rates": ([vp, _dp, i64], i),
         "pw_solve": ([vp, i], i),
         "pw_velocity_at": ([vp, _dp, i64, _dp], i),
+        "pw_velocity_at_noalloc": ([vp, _dp, i64, _dp], i),
         "pw_step": ([vp, i, d, i, i64], i),
         "pw_diagnose": ([vp, _dp, i64], i),
         "pw_synchronize": ([vp], i),
@@ -341,6 +342,14 @@ class Solver:
         pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
         u = np.empty_like(pts)
         _check(self._lib.pw_velocity_at(self._h, pts, len(pts), u))
+        return u
+
+    def velocity_at_noalloc(self, points) -> np.ndarray:
+        """velocity_at through pw_velocity_at_noalloc: no device allocation,
+        at most 65,536 points, the particles already on the device."""
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        u = np.empty_like(pts)
+        _check(self._lib.pw_velocity_at_noalloc(self._h, pts, len(pts), u))
         return u
 
     def step(self, scheme: str, dt: float, precision: str = "fp64", nsteps: int = 1) -> None:

@@ -38,7 +38,38 @@ int fail(int code, const std::string& msg) {
 
 }  // namespace pwi
 
+// Staging of pw_velocity_at_noalloc: the points, then their velocities, for
+// at most kNoAllocPoints points (3 MiB).  Every GPU the library runs on holds
+// its own copy until the process ends; host code reaches it only through the
+// per-device address that hipGetSymbolAddress gives (noalloc_staging).
+constexpr int64_t kNoAllocPoints = 65536;
+static __device__ __align__(256) double scratch[2 * 3 * kNoAllocPoints];
+
 namespace {
+
+// Device address of scratch on each device, resolved by pw_create with the
+// device current, once per device.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+int resolve_noalloc_staging(int device) {
+  if (device < 0 || device >= kMaxDevices) return fail(PW_ERR_UNSUPPORTED, "device index beyond 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return PW_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(PW_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_ptr[device] = static_cast<double*>(p);
+  return PW_OK;
+}
+
+double* noalloc_staging(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
 
 // Device buffers of the context.  Captured step graphs hold kernel nodes
 // whose arguments point into these buffers, so they are destroyed first,
@@ -454,6 +485,12 @@ int pw_create(pw_ctx** out, int device) {
     delete c;
     return fail(PW_ERR_HIP, "stream creation failed");
   }
+  const int rc = resolve_noalloc_staging(device);
+  if (rc != PW_OK) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return rc;
+  }
   *out = c;
   return PW_OK;
 }
@@ -680,6 +717,34 @@ int pw_velocity_at(pw_ctx* c, const double* points, int64_t m, double* u) {
   (void)hipFree(d_pts);
   (void)hipFree(d_u);
   if (e != hipSuccess) return fail(PW_ERR_HIP, std::string("pw_velocity_at: ") + hipGetErrorString(e));
+  return PW_OK;
+}
+
+// pw_velocity_at without its per-call buffers: the points and velocities
+// pass through the device's static staging array.  Calls on one GPU share
+// that array, so calls that overlap on one GPU (several host threads) race.
+int pw_velocity_at_noalloc(pw_ctx* c, const double* points, int64_t m, double* u) {
+  PW_CTX(c);
+  if (m < 0 || m > kNoAllocPoints)
+    return fail(PW_ERR_ARG, "m must be in [0, " + std::to_string(kNoAllocPoints) +
+                                "], the capacity of the static staging buffer");
+  if (m == 0) return PW_OK;
+  if (!points || !u) return fail(PW_ERR_ARG, "null points or output");
+  if (!c->built)
+    return fail(PW_ERR_STATE, "the context's particles are not on the device yet: its first pw_solve, pw_step "
+                              "or pw_velocity_at puts them there (and allocates)");
+  double* st = noalloc_staging(c->device);
+  if (!st) return fail(PW_ERR_STATE, "staging buffer of this device not resolved");
+  double *d_pts = st, *d_u = st + 3 * kNoAllocPoints;
+  const size_t bytes = 3 * static_cast<size_t>(m) * sizeof(double);
+  hipError_t e = hipMemcpyAsync(d_pts, points, bytes, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess)
+    e = pw::launch_point_velocity(c->y, static_cast<int>(c->n), d_pts, static_cast<int>(m), d_u,
+                                  c->smoothing == PW_KERNEL_HIGHORDER, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(u, d_u, bytes, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  else (void)hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return fail(PW_ERR_HIP, std::string("pw_velocity_at_noalloc: ") + hipGetErrorString(e));
   return PW_OK;
 }
 
