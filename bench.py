@@ -469,23 +469,44 @@ def trajectory(steps: int, every: int, n: int, tune: bool) -> dict:
     return out
 
 
+def device_copy(nbytes: int) -> None:
+    """One device-to-device copy of nbytes through the HIP runtime (its blit
+    kernel, __amd_rocclr_copyBuffer, reads and writes nbytes), after a fill
+    of the source: the known-size copy that counters in KiB per dispatch are
+    scaled by."""
+    import ctypes
+
+    hip = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so.7")
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    n = ctypes.c_size_t(nbytes)
+
+    def ok(rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed: {rc}")
+
+    ok(hip.hipMalloc(ctypes.byref(src), n), "hipMalloc")
+    try:
+        ok(hip.hipMalloc(ctypes.byref(dst), n), "hipMalloc")
+        try:
+            ok(hip.hipMemset(src, 1, n), "hipMemset")
+            ok(hip.hipMemcpy(dst, src, n, 3), "hipMemcpy")  # 3: hipMemcpyDeviceToDevice
+            ok(hip.hipDeviceSynchronize(), "hipDeviceSynchronize")
+        finally:
+            hip.hipFree(dst)
+    finally:
+        hip.hipFree(src)
+
+
 def profile_rhs(args) -> dict:
     """Right-hand sides of one workload (for rocprofv3 counter and trace
-    runs), after a plain device copy of known size (torch's elementwise copy
-    of --copy-mb MiB of float32): the copy reads and writes a known number
-    of bytes, so counters in KiB per dispatch scale to bytes as factor =
-    counter(copy) / (copy bytes / 1024)."""
-    import torch
-
+    runs), after a plain device copy of known size (--copy-mb MiB): the copy
+    reads and writes a known number of bytes, so counters in KiB per
+    dispatch scale to bytes as factor = counter(copy) / (copy bytes / 1024)."""
     from pawan_amd import capi, workloads
 
     nbytes = args.copy_mb << 20
-    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
-    dst = torch.empty_like(src)
-    dst.copy_(src)
-    torch.cuda.synchronize()
+    device_copy(nbytes)
     log(f"copy of {nbytes} bytes done")
-    del src, dst
     system = workloads.helix_wake(args.n) if args.workload == "helix" else workloads.WORKLOADS[args.workload]["make"]()
     rec = {"mode": "profile-rhs", "workload": system.name, "particles": system.n, "precision": args.precision,
            "copy_bytes": nbytes, "rhs_s": []}

@@ -112,8 +112,10 @@ def test_subclass_overriding_the_hooks(tmp_path, oracle):
     r = subprocess.run([HOOKS, str(inp), name, repr(dt), repr(s.nu)], env=env, capture_output=True, text=True,
                        timeout=120, cwd=str(tmp_path))
     assert r.returncode == 0, r.stderr
-    _say(r.stdout.strip())
-    calls = dict(zip(r.stdout.split()[0::2], map(int, r.stdout.split()[1::2])))
+    # integrate() prints its progress; the driver's own last line counts the hook calls
+    last = [l for l in r.stdout.splitlines() if l.startswith("self_calls")][-1]
+    _say(last)
+    calls = dict(zip(last.split()[0::2], map(int, last.split()[1::2])))
     # solve() once, then four stages of the step on the host path, two
     # wakes each; diagnose() once per wake
     assert calls == {"self_calls": 10, "ke_calls": 2}
