@@ -520,8 +520,8 @@ def profile_rhs(args) -> dict:
                 g.ff_set_err_coef(float(w["coef_md"]), float(w["coef_mlp"]))
             g.ff_set_params(args.theta, args.tol, use_mlp=bool(args.weights))
         elif args.config:
-            parts, pers, w8 = (int(v) for v in args.config.split(","))
-            g.set_pair_config(args.precision, parts, bool(pers), bool(w8))
+            v = [int(x) for x in args.config.split(",")] + [0]
+            g.set_pair_config(args.precision, v[0], bool(v[1]), bool(v[2]), bool(v[3]))
         g.set_profiling(True)
         for k in range(args.reps):
             t = time.perf_counter()
@@ -596,7 +596,8 @@ def main() -> int:
                     help="profile-rhs, tree path: absolute far-field tolerance (0: opening angle only)")
     ap.add_argument("--weights", default="", help="profile-rhs, tree path: network (npz from --save-weights)")
     ap.add_argument("--config", default="",
-                    help="profile-rhs: pair-kernel launch shape 'parts,persistent,waves8' (e.g. 8,1,1)")
+                    help="profile-rhs: pair-kernel launch shape 'parts,persistent,waves8[,scalar_feed]' "
+                         "(e.g. 8,1,1 or 4,1,1,1)")
     ap.add_argument("--kernel", default="gaussian", choices=["gaussian", "highorder"])
     args = ap.parse_args()
     if args.gpus != 1:

@@ -271,14 +271,24 @@ int pw_get_pair_timing(pw_ctx *ctx, double *total_ms, int64_t *launches);
 /* How the pair kernel of `precision` is launched.  parts: source ranges
  * each 256-target block is split into (partial sums reduced per target in
  * FP64, in a fixed order); persistent: resident blocks pull work items from
- * an atomic queue (largest first) instead of one block per item; waves8:
- * FP32C kernel built for 8 waves per SIMD instead of 7.  No counterpart in
- * the reference (its OpenMP loops have no launch shape); defaults are
- * parts 1, grid launch, 7 waves. */
+ * an atomic queue (largest first) instead of one block per item; variant
+ * (FP32C only, ignored for the other precisions), a combination of
+ *   PW_VARIANT_WAVES8       the kernel built for 8 waves per SIMD instead
+ *                           of 7 (64 vector registers);
+ *   PW_VARIANT_SCALAR_FEED  each wave of 64 targets streams the sources
+ *                           through scalar loads, chunk by chunk, instead
+ *                           of the block's four waves sharing LDS tiles.
+ * No counterpart in the reference (its OpenMP loops have no launch shape);
+ * defaults are parts 1, grid launch, variant 0.  A variant outside those
+ * bits is PW_ERR_ARG. */
+enum {
+  PW_VARIANT_WAVES8 = 1,
+  PW_VARIANT_SCALAR_FEED = 2
+};
 int pw_set_pair_config(pw_ctx *ctx, int precision, int parts, int persistent,
-                       int waves8);
+                       int variant);
 int pw_get_pair_config(pw_ctx *ctx, int precision, int *parts,
-                       int *persistent, int *waves8);
+                       int *persistent, int *variant);
 
 /* Time the candidate launch configurations of `precision` on the
  * context's current state (one right-hand side each) and keep the fastest;

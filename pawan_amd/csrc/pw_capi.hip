@@ -90,6 +90,10 @@ void free_buffers(pw_ctx* c) {
   for (auto& k : c->ks) (void)hipFree(k.d_items);
   (void)hipFree(c->partial);
   (void)hipFree(c->d_queue);
+  (void)hipFree(c->d_chunks);
+  (void)hipFree(c->d_src32);
+  c->d_chunks = nullptr;
+  c->d_src32 = nullptr;
   c->y = c->ys = nullptr;
   for (auto& p : c->k) p = nullptr;
   c->rates = nullptr;
@@ -119,6 +123,12 @@ void release(pw_ctx* c) {
     (void)hipEventDestroy(e.second);
   }
   if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+// index of the source chunk holding particle s (chunk_first is ascending)
+int32_t chunk_of(const pw_ctx* c, int64_t s) {
+  const auto it = std::upper_bound(c->chunk_first.begin(), c->chunk_first.end(), static_cast<int32_t>(s));
+  return static_cast<int32_t>(it - c->chunk_first.begin()) - 1;
 }
 
 // Work items: blocks of up to 256 consecutive targets of one system, each
@@ -161,6 +171,17 @@ void build_items(pw_ctx* c) {
     }
   c->npair = static_cast<int>(c->h_pair_items.size());
   c->nlin = static_cast<int>(c->h_lin_items.size());
+  // source chunks of the scalar-fed FP32C kernel: runs of up to 64
+  // particles of one system, in particle order
+  c->chunk_first.clear();
+  if (c->coupled) {
+    for (int64_t i = 0; i < c->n; i += 64) c->chunk_first.push_back(static_cast<int32_t>(i));
+  } else {
+    for (int w = 0; w < nw; ++w)
+      for (int64_t i = c->woff[w]; i < c->woff[w + 1]; i += 64) c->chunk_first.push_back(static_cast<int32_t>(i));
+  }
+  c->nchunks = static_cast<int>(c->chunk_first.size());
+  for (auto& it : c->h_pair_items) it.chunk0 = chunk_of(c, it.s0);
 }
 
 int ensure_built(pw_ctx* c) {
@@ -178,7 +199,9 @@ int ensure_built(pw_ctx* c) {
       dalloc(&c->lin_items, c->h_lin_items.size()) != hipSuccess ||
       dalloc(&c->d_lin, 11 * c->h_lin_items.size()) != hipSuccess ||
       dalloc(&c->d_quad, pw::kDiagQuad * c->h_pair_items.size()) != hipSuccess ||
-      dalloc(&c->d_flag, 1) != hipSuccess || dalloc(&c->d_queue, 1) != hipSuccess) {
+      dalloc(&c->d_flag, 1) != hipSuccess || dalloc(&c->d_queue, 1) != hipSuccess ||
+      dalloc(&c->d_chunks, static_cast<size_t>(c->nchunks)) != hipSuccess ||
+      dalloc(&c->d_src32, n) != hipSuccess) {
     free_buffers(c);  // the stream stays: the context remains usable
     (void)hipGetLastError();
     return fail(PW_ERR_NOMEM, "device allocation failed");
@@ -190,6 +213,17 @@ int ensure_built(pw_ctx* c) {
   PW_HIP(hipMemcpyAsync(c->pair_items, c->h_pair_items.data(),
                         c->h_pair_items.size() * sizeof(WorkItem), hipMemcpyHostToDevice,
                         c->stream));
+  {
+    // chunk extents now; their spheres are rebuilt before every launch
+    std::vector<pw::Chunk32> ch(static_cast<size_t>(c->nchunks));
+    for (int k = 0; k < c->nchunks; ++k) {
+      std::memset(&ch[k], 0, sizeof(pw::Chunk32));
+      ch[k].first = c->chunk_first[k];
+      const int64_t end = k + 1 < c->nchunks ? c->chunk_first[k + 1] : c->n;
+      ch[k].count = static_cast<int32_t>(std::min<int64_t>(64, end - c->chunk_first[k]));
+    }
+    PW_HIP(hipMemcpy(c->d_chunks, ch.data(), ch.size() * sizeof(pw::Chunk32), hipMemcpyHostToDevice));
+  }
   PW_HIP(hipMemcpyAsync(c->lin_items, c->h_lin_items.data(),
                         c->h_lin_items.size() * sizeof(WorkItem), hipMemcpyHostToDevice,
                         c->stream));
@@ -246,7 +280,7 @@ std::vector<WorkItem> make_items(const pw_ctx* c, int parts) {
         const int64_t a = s.a + kTargetsPerItem * (ntile * p / parts);
         const int64_t b = std::min<int64_t>(s.b, s.a + kTargetsPerItem * (ntile * (p + 1) / parts));
         items.push_back({static_cast<int32_t>(t0), t1, static_cast<int32_t>(a),
-                         static_cast<int32_t>(std::max(a, b)), parts == 1 ? -1 : p, 0, 0, 0});
+                         static_cast<int32_t>(std::max(a, b)), parts == 1 ? -1 : p, 0, chunk_of(c, a), 0});
       }
     }
   }
@@ -306,7 +340,8 @@ int check_blocks(const pw_ctx* c, const int* pairs, int npairs) {
 void append_items(const pw_ctx* c, int a, int b, int nself, std::vector<WorkItem>& out) {
   for (int64_t t0 = c->woff[a]; t0 < c->woff[a + 1]; t0 += kTargetsPerItem)
     out.push_back({static_cast<int32_t>(t0), static_cast<int32_t>(std::min<int64_t>(t0 + kTargetsPerItem, c->woff[a + 1])),
-                   static_cast<int32_t>(c->woff[b]), static_cast<int32_t>(c->woff[b + 1]), -1, nself, 0, 0});
+                   static_cast<int32_t>(c->woff[b]), static_cast<int32_t>(c->woff[b + 1]), -1, nself,
+                   chunk_of(c, c->woff[b]), 0});
 }
 
 // the pair-kernel kind a precision launches under the context's kernel and
@@ -335,7 +370,8 @@ int rhs(pw_ctx* c, int kind, const Rec64* state, Rate64* out) {
   }
   if (kind == pw::kPairFP32C && !c->compensated && c->smoothing == PW_KERNEL_GAUSSIAN) {
     // plain FP32 sums: one item per 256-target block over all its sources
-    pw::PairArgs args{state, c->pair_items, c->npair, static_cast<int>(c->n), out, nullptr, c->d_queue, c->nu};
+    pw::PairArgs args{state, c->pair_items, c->npair, static_cast<int>(c->n), out, nullptr, c->d_queue, c->nu,
+                      c->d_chunks, c->d_src32, c->nchunks};
     PW_HIP(pw::launch_pair(pw::kPairFP32CPlain, pw::PairConfig{}, args, c->npair, c->stream));
     PW_HIP(pw::launch_check_finite(out, static_cast<int>(c->n), c->d_flag, c->stream));
     return PW_OK;
@@ -351,6 +387,9 @@ int rhs(pw_ctx* c, int kind, const Rec64* state, Rate64* out) {
   args.partial = c->partial;
   args.queue = c->d_queue;
   args.nu = c->nu;
+  args.chunks = c->d_chunks;
+  args.src = c->d_src32;
+  args.nchunks = c->nchunks;
   // the high-order kernel replaces the Gaussian kinds (pw_set_kernel)
   const int launch_kind = launch_kind_of(c, kind);
   const int grid = c->num_cus * pw::pair_kernel_waves(launch_kind, k.cfg);
@@ -990,7 +1029,8 @@ int pw_interact_wakes(pw_ctx* c, int precision, const int* pairs, int npairs) {
   for (int k = 0; k < npairs && e == hipSuccess; ++k) {
     // one block at a time: its items have distinct targets and write their
     // rates into blk, which is then added to the targets' totals
-    pw::PairArgs args{c->y, d_items + first[k], first[k + 1] - first[k], n, blk, nullptr, c->d_queue, c->nu};
+    pw::PairArgs args{c->y,     d_items + first[k], first[k + 1] - first[k], n, blk, nullptr, c->d_queue, c->nu,
+                      c->d_chunks, c->d_src32,        c->nchunks};
     e = pw::launch_pair(lk, pw::PairConfig{}, args, args.nitems, c->stream);
     const int a = pairs[2 * k], b = pairs[2 * k + 1];
     if (e == hipSuccess)
@@ -1198,22 +1238,25 @@ int pw_get_pair_timing(pw_ctx* c, double* total_ms, int64_t* launches) {
   return PW_OK;
 }
 
-int pw_set_pair_config(pw_ctx* c, int precision, int parts, int persistent, int waves8) {
+int pw_set_pair_config(pw_ctx* c, int precision, int parts, int persistent, int variant) {
   PW_CTX(c);
   int kind = 0;
   PW_TRY(kind_of(precision, &kind));
   if (kind == kKindLearned) return fail(PW_ERR_UNSUPPORTED, "launch configuration applies to the direct-sum kinds");
   if (parts < 1 || parts > 256) return fail(PW_ERR_ARG, "parts must be in [1, 256]");
   auto& k = c->ks[kind];
+  if (variant < 0 || variant > (PW_VARIANT_WAVES8 | PW_VARIANT_SCALAR_FEED))
+    return fail(PW_ERR_ARG, "variant must be a combination of PW_VARIANT_WAVES8 and PW_VARIANT_SCALAR_FEED");
   k.cfg.parts = parts;
   k.cfg.persistent = persistent != 0;
-  k.cfg.waves8 = waves8 != 0 && kind == pw::kPairFP32C;
+  k.cfg.waves8 = (variant & PW_VARIANT_WAVES8) != 0 && kind == pw::kPairFP32C;
+  k.cfg.scalar_feed = (variant & PW_VARIANT_SCALAR_FEED) != 0 && kind == pw::kPairFP32C;
   k.tuned = true;
   drop_graphs(c);
   return PW_OK;
 }
 
-int pw_get_pair_config(pw_ctx* c, int precision, int* parts, int* persistent, int* waves8) {
+int pw_get_pair_config(pw_ctx* c, int precision, int* parts, int* persistent, int* variant) {
   PW_CTX(c);
   int kind = 0;
   PW_TRY(kind_of(precision, &kind));
@@ -1221,7 +1264,7 @@ int pw_get_pair_config(pw_ctx* c, int precision, int* parts, int* persistent, in
   const auto& k = c->ks[kind];
   if (parts) *parts = k.cfg.parts;
   if (persistent) *persistent = k.cfg.persistent ? 1 : 0;
-  if (waves8) *waves8 = k.cfg.waves8 ? 1 : 0;
+  if (variant) *variant = (k.cfg.waves8 ? PW_VARIANT_WAVES8 : 0) | (k.cfg.scalar_feed ? PW_VARIANT_SCALAR_FEED : 0);
   return PW_OK;
 }
 
@@ -1238,7 +1281,8 @@ namespace {
 
 // Tuning results shared by every context of the process, keyed by the
 // problem's shape; optionally also appended to / read from the text file
-// named by $PAWAN_TUNE_CACHE ("kind n nsys largest parts persistent waves8").
+// named by $PAWAN_TUNE_CACHE ("kind n nsys largest parts persistent variant",
+// variant the PW_VARIANT_* bits).
 using TuneKey = std::tuple<int, int64_t, int, int64_t>;
 std::map<TuneKey, pw::PairConfig>& tune_cache() {
   static std::map<TuneKey, pw::PairConfig> m;
@@ -1266,7 +1310,9 @@ void load_tune_file() {
     pw::PairConfig cfg;
     cfg.parts = parts;
     cfg.persistent = pers != 0;
-    cfg.waves8 = w8 != 0 && kind == pw::kPairFP32C;
+    if (w8 < 0 || w8 > (PW_VARIANT_WAVES8 | PW_VARIANT_SCALAR_FEED)) continue;
+    cfg.waves8 = (w8 & PW_VARIANT_WAVES8) != 0 && kind == pw::kPairFP32C;
+    cfg.scalar_feed = (w8 & PW_VARIANT_SCALAR_FEED) != 0 && kind == pw::kPairFP32C;
     tune_cache()[TuneKey(kind, n, nsys, largest)] = cfg;
   }
   std::fclose(f);
@@ -1279,7 +1325,8 @@ void save_tune_file(const TuneKey& key, const pw::PairConfig& cfg) {
   if (!f) return;
   std::fprintf(f, "%d %lld %d %lld %d %d %d\n", std::get<0>(key), static_cast<long long>(std::get<1>(key)),
                std::get<2>(key), static_cast<long long>(std::get<3>(key)), cfg.parts,
-               cfg.persistent ? 1 : 0, cfg.waves8 ? 1 : 0);
+               cfg.persistent ? 1 : 0,
+               (cfg.waves8 ? PW_VARIANT_WAVES8 : 0) | (cfg.scalar_feed ? PW_VARIANT_SCALAR_FEED : 0));
   std::fclose(f);
 }
 
@@ -1362,12 +1409,13 @@ int pw_autotune(pw_ctx* c, int precision, double* best_ms) {
     max_tiles = std::max(max_tiles, nt);
   }
   std::vector<pw::PairConfig> cands;
-  auto add = [&](int parts, bool pers, bool w8) {
+  auto add = [&](int parts, bool pers, bool w8, bool sf = false) {
     if (parts > max_tiles) return;
     pw::PairConfig p;
     p.parts = parts;
     p.persistent = pers;
     p.waves8 = w8;
+    p.scalar_feed = sf;
     cands.push_back(p);
   };
   const bool f32 = kind == pw::kPairFP32C;
@@ -1380,6 +1428,12 @@ int pw_autotune(pw_ctx* c, int precision, double* best_ms) {
     add(1, false, true);
     add(4, true, true);
     add(8, true, true);
+    // sources through scalar loads, one wave per 64 targets
+    add(1, false, true, true);
+    add(1, true, true, true);
+    add(4, true, true, true);
+    add(8, true, true, true);
+    add(4, true, false, true);
   }
   const int64_t resident = static_cast<int64_t>(c->num_cus) * 8;
   for (int parts = 16; parts <= 64; parts *= 2)

@@ -118,6 +118,12 @@ struct pw_ctx {
   pw::Rate64* partial = nullptr;  // source-split partial rates, parts x n
   int partial_parts = 0;
   int* d_queue = nullptr;     // work counter of the persistent kernels
+  // source chunks of the scalar-fed FP32C kernel: runs of up to 64
+  // particles of one system (chunk_first: their first particles, ascending)
+  std::vector<int32_t> chunk_first;
+  pw::Chunk32* d_chunks = nullptr;
+  pw::Src32* d_src32 = nullptr;
+  int nchunks = 0;
   int num_cus = 256;
   struct SysRange {
     int64_t a, b;

@@ -33,7 +33,27 @@ struct alignas(16) Rate64 {
 // marks an item of a cross-wake block, which adds no self terms.)
 struct WorkItem {
   int32_t t0, t1, s0, s1;
-  int32_t part, nself, pad1, pad2;
+  int32_t part, nself;
+  int32_t chunk0;  // source chunk (Chunk32) holding s0, for the scalar-fed FP32C kernel
+  int32_t pad2;
+};
+
+// Sources as the scalar-fed FP32C kernel reads them, rebuilt from the state
+// before each of its launches.  The sources of each system are cut into
+// chunks of up to 64 consecutive particles; a chunk carries its bounding
+// sphere, a source its position relative to its chunk's centre.  Both are
+// read by whole waves at once (scalar loads), so they need no LDS staging.
+struct alignas(16) Chunk32 {
+  double cx, cy, cz;      // centre of the chunk's bounding box
+  float r;                // radius of its bounding sphere (rounded up)
+  float h2;               // largest sigma^2 / 2 of its sources
+  int32_t first, count;   // its particles (set once, when the context is built)
+};
+struct alignas(16) Src32 {
+  float rx, ry, rz;  // position relative to the chunk's centre (rounded once)
+  float ax, ay, az;  // strength
+  float mx, my, mz;  // a x r, in FP32 from the rounded a and r
+  float hs2, vol, pad;
 };
 
 // Arguments of a pair-kernel launch (passed by value).
@@ -46,6 +66,9 @@ struct PairArgs {
   Rate64* partial;    // parts x n, used by items with part >= 0
   int* queue;         // work counter of the persistent kernels
   double nu;
+  Chunk32* chunks;    // scalar-fed FP32C: source chunks (geometry rebuilt per launch)
+  Src32* src;         //   and source records, one per particle
+  int nchunks;
 };
 
 // How a right-hand side is launched (chosen by the autotuner).
@@ -53,6 +76,9 @@ struct PairConfig {
   int parts = 1;           // source ranges per target block
   bool persistent = false; // resident blocks pulling items from a queue
   bool waves8 = false;     // FP32C: register budget of 8 waves/SIMD
+  bool scalar_feed = false;// FP32C: each wave streams the sources through
+                           // scalar loads (Chunk32/Src32) instead of LDS
+                           // tiles shared by the block's four waves
 };
 
 // Pair-kernel variants.
@@ -75,8 +101,9 @@ enum PairKind : int {
 // persistent form.
 hipError_t launch_pair(int kind, const PairConfig& cfg, const PairArgs& args, int grid,
                        hipStream_t st);
-// resident blocks per CU of the persistent pair kernel of `kind` (equal to
-// its waves per SIMD: 256-thread blocks), from the occupancy calculator
+// resident blocks per CU of the persistent pair kernel of `kind` (256-thread
+// blocks: its waves per SIMD; the scalar-fed FP32C kernel's one-wave blocks:
+// its waves per CU), from the occupancy calculator
 int pair_kernel_waves(int kind, const PairConfig& cfg);
 // resident one-wave blocks per CU of the tree path's near-field kernel
 int near_kernel_blocks_per_cu();

@@ -622,15 +622,18 @@ struct alignas(16) SrcRel {
   float ay, az, hs2, vol;
 };
 
-// Direct-sum source record: what the separated loop reads first (relative
-// FP32 position and strength: one 16-byte and one 8-byte broadcast read, 24
-// bytes, where the LDS read bandwidth bounds that loop), then the radius and
-// volume, then the FP64 position for the other loops.
+// Direct-sum source record (64 B): what the separated loop reads first
+// (position relative to the block's targets' centre, strength and m = a x r:
+// 36 bytes in three broadcast reads), the radius, then the FP64 position for
+// the other loops.  The volume, read by those loops only, sits in an array
+// of its own.
 struct alignas(16) SrcT2 {
   float rx, ry, rz, ax;
-  float ay, az, hs2, vol;
-  double x, y, z, pad;
+  float ay, az, mx, my;
+  float mz, hs2;
+  double x, y, z;
 };
+static_assert(sizeof(SrcT2) == 64, "one 64-byte record per source");
 
 template <typename S>
 __device__ __forceinline__ void pair32_rel(float tx, float ty, float tz, float tax, float tay, float taz,
@@ -681,6 +684,100 @@ __device__ __forceinline__ void fold(const P32& p, float tax, float tay, float t
   two_sum(hi[4], lo[4], fmaf(taz, p.ax, fmaf(-tax, p.az, -3.0f * p.by)));
   two_sum(hi[5], lo[5], fmaf(tax, p.ay, fmaf(-tay, p.ax, -3.0f * p.bz)));
 }
+
+// Sums of a separated source range, arranged so that a pair costs 26 vector
+// instructions instead of 29.  Positions are relative to a common centre c
+// (t' for the target, r_s for the source); with m_s = a_s x r_s (per source)
+// and c_t = t' x a_t (per target),
+//   a_t . (a_s x d) = a_s . c_t - a_t . m_s,
+//   sum q' (a_s x d) = A x t' - M,   A = sum q' a_s,  M = sum q' m_s,
+// so a pair forms no cross product of its own.  The cancellation this adds
+// is bounded by the separation tests that admit a range (the distances to c
+// are within a few pair distances).
+struct P32S {
+  float ax = 0, ay = 0, az = 0;  // A = sum q' a_s
+  float mx = 0, my = 0, mz = 0;  // M = sum q' m_s
+  float bx = 0, by = 0, bz = 0;  // B = sum f' (a_t . v) d, v = a_s x d
+};
+
+// one pair of a separated chunk: (tx, ty, tz) = t', (cx, cy, cz) = t' x a_t
+template <bool HighOrder, typename S>
+__device__ __forceinline__ void pair32s(float tx, float ty, float tz, float th2, float tax, float tay, float taz,
+                                        float cx, float cy, float cz, const S& s, P32S& p) {
+  const float dx = tx - s.rx;
+  const float dy = ty - s.ry;
+  const float dz = tz - s.rz;
+  const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+  float q, f;
+  if (HighOrder) {
+    const float sig2 = s.hs2 + th2;
+    const float ib = __builtin_amdgcn_rsqf(r2 + sig2);
+    const float ib2 = ib * ib;
+    const float ib5 = ib2 * ib2 * ib;
+    q = fmaf(2.5f, sig2, r2) * ib5;
+    f = fmaf(3.5f, sig2, r2) * (ib5 * ib2);
+  } else {
+    const float ir = __builtin_amdgcn_rsqf(r2);
+    const float ir2 = ir * ir;
+    q = ir * ir2;
+    f = q * ir2;
+  }
+  p.ax = fmaf(q, s.ax, p.ax);
+  p.ay = fmaf(q, s.ay, p.ay);
+  p.az = fmaf(q, s.az, p.az);
+  p.mx = fmaf(q, s.mx, p.mx);
+  p.my = fmaf(q, s.my, p.my);
+  p.mz = fmaf(q, s.mz, p.mz);
+  const float h = fmaf(s.az, cz, fmaf(s.ay, cy, fmaf(s.ax, cx, -fmaf(taz, s.mz, fmaf(tay, s.my, tax * s.mx)))));
+  const float g = f * h;
+  p.bx = fmaf(g, dx, p.bx);
+  p.by = fmaf(g, dy, p.by);
+  p.bz = fmaf(g, dz, p.bz);
+}
+
+// a separated chunk's sums into the compensated totals: u = A x t' - M,
+// w = a_t x A - 3 B
+__device__ __forceinline__ void fold_s(const P32S& p, float tx, float ty, float tz, float tax, float tay, float taz,
+                                       float* hi, float* lo) {
+  two_sum(hi[0], lo[0], fmaf(p.ay, tz, fmaf(-p.az, ty, -p.mx)));
+  two_sum(hi[1], lo[1], fmaf(p.az, tx, fmaf(-p.ax, tz, -p.my)));
+  two_sum(hi[2], lo[2], fmaf(p.ax, ty, fmaf(-p.ay, tx, -p.mz)));
+  two_sum(hi[3], lo[3], fmaf(tay, p.az, fmaf(-taz, p.ay, -3.0f * p.bx)));
+  two_sum(hi[4], lo[4], fmaf(taz, p.ax, fmaf(-tax, p.az, -3.0f * p.by)));
+  two_sum(hi[5], lo[5], fmaf(tax, p.ay, fmaf(-tay, p.ax, -3.0f * p.bz)));
+}
+
+// Read-only data of a launch seen through the constant address space: loads
+// at wave-uniform addresses then become scalar loads.
+template <typename T>
+using ConstPtr = const __attribute__((address_space(4))) T*;
+template <typename T>
+__device__ __forceinline__ ConstPtr<T> as_const(const T* p) {
+  return (ConstPtr<T>)(p);
+}
+// field by field (a struct copy out of that address space has no constructor)
+__device__ __forceinline__ Chunk32 load(ConstPtr<Chunk32> p, int i) {
+  Chunk32 h;
+  h.cx = p[i].cx; h.cy = p[i].cy; h.cz = p[i].cz;
+  h.r = p[i].r; h.h2 = p[i].h2;
+  h.first = p[i].first; h.count = p[i].count;
+  return h;
+}
+__device__ __forceinline__ Src32 load(ConstPtr<Src32> p, int i) {
+  Src32 s;
+  s.rx = p[i].rx; s.ry = p[i].ry; s.rz = p[i].rz;
+  s.ax = p[i].ax; s.ay = p[i].ay; s.az = p[i].az;
+  s.mx = p[i].mx; s.my = p[i].my; s.mz = p[i].mz;
+  s.hs2 = p[i].hs2; s.vol = p[i].vol; s.pad = 0.0f;
+  return s;
+}
+
+// a source of a range that is not separated, as the core-capable pair reads
+// it: FP64 position, the rest in FP32
+struct SrcCore {
+  double x, y, z;
+  float hs2, ax, ay, az, vol;
+};
 
 // Bounding boxes of each wave's 64 targets and of each 64-source chunk of a
 // staged tile decide, per (wave, chunk), whether every pair is far enough
@@ -734,6 +831,7 @@ template <bool HighOrder, bool Comp = true>
 __device__ __forceinline__ void item_fp32c(const PairArgs& pa, const WorkItem& it, SrcT2* tile) {
   __shared__ double s_tb[4][6];
   __shared__ float s_b[4][8];
+  __shared__ float s_vol[kTB];
   const Rec64* __restrict__ state = pa.state;
   const float nu = static_cast<float>(pa.nu);
   const int t = it.t0 + static_cast<int>(threadIdx.x);
@@ -781,11 +879,14 @@ __device__ __forceinline__ void item_fp32c(const PairArgs& pa, const WorkItem& i
     rW2 += h * h;
   }
   const float rW = uniform(sqrtf(rW2) * 1.00001f);
+  // c_t = t' x a_t for the separated ranges
+  const float ctx = rty * me.az - rtz * me.ay, cty = rtz * me.ax - rtx * me.az, ctz = rtx * me.ay - rty * me.ax;
   float hi[kAcc], lo[kAcc];
 #pragma unroll
   for (int k = 0; k < kAcc; ++k) hi[k] = lo[k] = 0.0f;
   PSE32 pse;
-  P32 whole;  // Comp == false: the item's plain FP32 sums
+  P32 whole;     // Comp == false: the item's plain FP32 sums (one frame: the
+  P32S whole_s;  // separated ones fold once, at the end)
   for (int base = it.s0; base < it.s1; base += kTB) {
     const int cnt = min(kTB, it.s1 - base);
     __syncthreads();
@@ -800,10 +901,12 @@ __device__ __forceinline__ void item_fp32c(const PairArgs& pa, const WorkItem& i
       f.ax = static_cast<float>(r.ax);
       f.ay = static_cast<float>(r.ay);
       f.az = static_cast<float>(r.az);
-      f.vol = static_cast<float>(r.vol);
+      f.mx = f.ay * f.rz - f.az * f.ry;
+      f.my = f.az * f.rx - f.ax * f.rz;
+      f.mz = f.ax * f.ry - f.ay * f.rx;
       f.x = r.x; f.y = r.y; f.z = r.z;
-      f.pad = 0.0;
       tile[threadIdx.x] = f;
+      s_vol[threadIdx.x] = static_cast<float>(r.vol);
       bl[0] = bh[0] = f.rx;
       bl[1] = bh[1] = f.ry;
       bl[2] = bh[2] = f.rz;
@@ -838,33 +941,35 @@ __device__ __forceinline__ void item_fp32c(const PairArgs& pa, const WorkItem& i
       const float gap = uniform(sqrtf(d2) * 0.99999f - rW - rS);
       const bool sep = gap > 0.0f && gap * gap > kFar32 * (h2T + h2S) * 1.0001f &&
                        rT + uniform(sqrtf(cs2) * 1.00001f) + rS <= kSepFP32 * gap;
+      if (sep) {
+        P32S chunk_s;
+        P32S& q = Comp ? chunk_s : whole_s;
+#pragma unroll 2
+        for (int l = c0; l < c1; ++l)
+          pair32s<HighOrder>(rtx, rty, rtz, me.hs2, me.ax, me.ay, me.az, ctx, cty, ctz, tile[l], q);
+        if (Comp) fold_s(q, rtx, rty, rtz, me.ax, me.ay, me.az, hi, lo);
+        continue;
+      }
       P32 chunk;
       P32& p = Comp ? chunk : whole;
-      if (sep) {
-        if (HighOrder) {
-#pragma unroll 2
-          for (int l = c0; l < c1; ++l) pair32_rel_ho(rtx, rty, rtz, me.hs2, me.ax, me.ay, me.az, tile[l], p);
-        } else {
-#pragma unroll 2
-          for (int l = c0; l < c1; ++l) pair32_rel(rtx, rty, rtz, me.ax, me.ay, me.az, tile[l], p);
-        }
-      } else if (diag && cw == wave) {  // the chunk holding this wave's own targets
-        for (int l = c0; l < c1; ++l)
-          if (l != self) {
-            if (HighOrder) pair32_ho(me, tile[l], p, pse);
-            else pair32(me, tile[l], p, pse);
-          }
-      } else {
+      const bool own = diag && cw == wave;  // the chunk holding this wave's own targets
 #pragma unroll 1
-        for (int l = c0; l < c1; ++l) {
-          if (HighOrder) pair32_ho(me, tile[l], p, pse);
-          else pair32(me, tile[l], p, pse);
-        }
+      for (int l = c0; l < c1; ++l) {
+        if (own && l == self) continue;  // the self pair is never evaluated
+        const SrcT2& f = tile[l];
+        SrcCore sc;
+        sc.x = f.x; sc.y = f.y; sc.z = f.z;
+        sc.hs2 = f.hs2; sc.ax = f.ax; sc.ay = f.ay; sc.az = f.az; sc.vol = s_vol[l];
+        if (HighOrder) pair32_ho(me, sc, p, pse);
+        else pair32(me, sc, p, pse);
       }
       if (Comp) fold(p, me.ax, me.ay, me.az, hi, lo);
     }
   }
-  if (!Comp) fold(whole, me.ax, me.ay, me.az, hi, lo);
+  if (!Comp) {
+    fold(whole, me.ax, me.ay, me.az, hi, lo);
+    fold_s(whole_s, rtx, rty, rtz, me.ax, me.ay, me.az, hi, lo);
+  }
   if (active) {
     double acc[kAcc];
 #pragma unroll
@@ -877,6 +982,181 @@ __device__ __forceinline__ void item_fp32c(const PairArgs& pa, const WorkItem& i
     r.wy = kQ * acc[4] + static_cast<double>(pse.y);
     r.wz = kQ * acc[5] + static_cast<double>(pse.z);
     out_rates(pa, it)[t] = r;
+  }
+}
+
+// ---------------------------------------------------------------------
+// FP32C direct sum, scalar-fed (PairConfig::scalar_feed).  One wave owns 64
+// targets of a work item and streams the item's sources chunk by chunk
+// straight from memory with scalar loads: a source record is the same for
+// every lane, so it sits in scalar registers and enters the vector
+// instructions as an operand, with no LDS staging, no per-lane copies and
+// no barriers.  k_src_chunks rebuilds the chunks' spheres and the relative
+// source records from the state before each launch.
+//
+// A chunk whose bounding sphere is separated from the wave's targets' (gap
+// beyond sqrt(44) sigma, and at least the chunk's radius) runs on positions
+// relative to the chunk's centre c: t' = x_t - c (FP64, rounded once per
+// chunk) against r_s = x_s - c (rounded once per launch).  The displacement
+// t' - r_s then carries a rounding error of at most 2^-24 (2|d| + 2 r_S),
+// i.e. 4 * 2^-24 of |d|, the bound of the LDS-fed kernel's separated loop;
+// its sums are the 26-instruction ones (pair32s), c_t formed per chunk.  The
+// other chunks take the LDS-fed kernel's pair (FP64 displacement, Gaussian
+// core, stretching formed per pair for sources inside some lane's core).
+// ---------------------------------------------------------------------
+
+// Chunk spheres and relative source records from the state (one wave per
+// chunk; first and count were set when the context was built).
+__global__ __launch_bounds__(64) void k_src_chunks(const Rec64* __restrict__ state, Chunk32* __restrict__ chunks,
+                                                   Src32* __restrict__ src) {
+  Chunk32& h = chunks[blockIdx.x];
+  const int first = h.first, count = h.count;
+  const int lane = static_cast<int>(threadIdx.x);
+  const bool on = lane < count;
+  const Rec64 r = state[first + (on ? lane : 0)];
+  const double lo[3] = {wave_mind(r.x), wave_mind(r.y), wave_mind(r.z)};
+  const double hi[3] = {wave_maxd(r.x), wave_maxd(r.y), wave_maxd(r.z)};
+  double c[3];
+  float e2 = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    c[k] = 0.5 * (lo[k] + hi[k]);
+    const float e = static_cast<float>(0.5 * (hi[k] - lo[k]));
+    e2 += e * e;
+  }
+  const float hs2 = static_cast<float>(0.5 * r.s2);
+  const float h2 = wave_max(hs2);
+  if (lane == 0) {
+    h.cx = c[0];
+    h.cy = c[1];
+    h.cz = c[2];
+    h.r = sqrtf(e2) * 1.00001f + 1e-30f;
+    h.h2 = h2;
+  }
+  if (on) {
+    Src32 o;
+    o.rx = static_cast<float>(r.x - c[0]);
+    o.ry = static_cast<float>(r.y - c[1]);
+    o.rz = static_cast<float>(r.z - c[2]);
+    o.ax = static_cast<float>(r.ax);
+    o.ay = static_cast<float>(r.ay);
+    o.az = static_cast<float>(r.az);
+    o.mx = o.ay * o.rz - o.az * o.ry;
+    o.my = o.az * o.rx - o.ax * o.rz;
+    o.mz = o.ax * o.ry - o.ay * o.rx;
+    o.hs2 = hs2;
+    o.vol = static_cast<float>(r.vol);
+    o.pad = 0.0f;
+    src[first + lane] = o;
+  }
+}
+
+// Targets [tb, tb + 64) of item it (a 64-lane wave of its own).
+template <bool HighOrder>
+__device__ __forceinline__ void wave_fp32s(const WorkItem& it, int tb, double nu, const Rec64* state_g,
+                                           const Chunk32* chunks_g, int nchunks, const Src32* src_g,
+                                           Rate64* __restrict__ out) {
+  const ConstPtr<Rec64> state = as_const(state_g);
+  const ConstPtr<Chunk32> chunks = as_const(chunks_g);
+  const ConstPtr<Src32> src = as_const(src_g);
+  const int lane = static_cast<int>(threadIdx.x) & 63;
+  const int t = tb + lane;
+  const bool active = t < it.t1;
+  const Rec64 m = state_g[active ? t : tb];
+  T32 me;
+  me.x = m.x; me.y = m.y; me.z = m.z;
+  me.hs2 = static_cast<float>(0.5 * m.s2);
+  me.ax = static_cast<float>(m.ax);
+  me.ay = static_cast<float>(m.ay);
+  me.az = static_cast<float>(m.az);
+  me.vol = static_cast<float>(m.vol);
+  me.nu2 = 2.0f * static_cast<float>(nu);
+  // the wave's targets' sphere (inactive lanes repeat the first target)
+  double cW[3];
+  float rW2 = 0.0f;
+  {
+    const double lo[3] = {wave_mind(m.x), wave_mind(m.y), wave_mind(m.z)};
+    const double hi[3] = {wave_maxd(m.x), wave_maxd(m.y), wave_maxd(m.z)};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      cW[k] = uniform(0.5 * (lo[k] + hi[k]));
+      const float e = static_cast<float>(0.5 * (hi[k] - lo[k]));
+      rW2 += e * e;
+    }
+  }
+  const float rW = uniform(sqrtf(rW2) * 1.00001f);
+  const float h2W = uniform(wave_max(me.hs2));
+  float hi[kAcc], lo[kAcc];
+#pragma unroll
+  for (int k = 0; k < kAcc; ++k) hi[k] = lo[k] = 0.0f;
+  PSE32 pse;
+  for (int ci = it.chunk0; ci < nchunks; ++ci) {
+    const Chunk32 h = load(chunks, ci);
+    if (h.first >= it.s1) break;
+    const int l0 = max(h.first, it.s0), l1 = min(h.first + h.count, it.s1);
+    const double ddx = h.cx - cW[0], ddy = h.cy - cW[1], ddz = h.cz - cW[2];
+    const float dist = static_cast<float>(sqrt(fma(ddz, ddz, fma(ddy, ddy, ddx * ddx))));
+    const float gap = dist * 0.99999f - rW - h.r;
+    const bool sep = gap > 0.0f && gap * gap > kFar32 * (h2W + h.h2) * 1.0001f && h.r <= gap;
+    if (sep) {
+      const float tx = static_cast<float>(me.x - h.cx), ty = static_cast<float>(me.y - h.cy),
+                  tz = static_cast<float>(me.z - h.cz);
+      const float cx = ty * me.az - tz * me.ay, cy = tz * me.ax - tx * me.az, cz = tx * me.ay - ty * me.ax;
+      P32S p;
+#pragma unroll 4
+      for (int l = l0; l < l1; ++l)
+        pair32s<HighOrder>(tx, ty, tz, me.hs2, me.ax, me.ay, me.az, cx, cy, cz, load(src, l), p);
+      fold_s(p, tx, ty, tz, me.ax, me.ay, me.az, hi, lo);
+    } else {
+      P32 p;
+#pragma unroll 1
+      for (int l = l0; l < l1; ++l) {
+        const Src32 sr = load(src, l);
+        SrcCore sc;
+        sc.x = state[l].x; sc.y = state[l].y; sc.z = state[l].z;
+        sc.hs2 = sr.hs2; sc.ax = sr.ax; sc.ay = sr.ay; sc.az = sr.az; sc.vol = sr.vol;
+        if (l != t) {  // the self pair is never evaluated
+          if (HighOrder) pair32_ho(me, sc, p, pse);
+          else pair32(me, sc, p, pse);
+        }
+      }
+      fold(p, me.ax, me.ay, me.az, hi, lo);
+    }
+  }
+  if (active) {
+    double acc[kAcc];
+#pragma unroll
+    for (int k = 0; k < kAcc; ++k) acc[k] = static_cast<double>(hi[k]) + static_cast<double>(lo[k]);
+    Rate64 r;
+    r.ux = kQ * acc[0];
+    r.uy = kQ * acc[1];
+    r.uz = kQ * acc[2];
+    r.wx = kQ * acc[3] + static_cast<double>(pse.x);
+    r.wy = kQ * acc[4] + static_cast<double>(pse.y);
+    r.wz = kQ * acc[5] + static_cast<double>(pse.z);
+    out[t] = r;
+  }
+}
+
+// Entry: one-wave blocks; wave index w = 4 item + quarter (grid form: w =
+// blockIdx.x; persistent form: pulled from the queue).
+template <bool HighOrder, bool Persistent>
+__device__ __forceinline__ void fp32s_entry(const PairArgs& a) {
+  const int waves = 4 * a.nitems;
+  for (int w = Persistent ? -1 : static_cast<int>(blockIdx.x);;) {
+    if (Persistent) {
+      int idx = 0;
+      if (threadIdx.x == 0) idx = atomicAdd(a.queue, 1);
+      w = __builtin_amdgcn_readfirstlane(idx);  // lane 0's: every lane is active here
+    }
+    if (w >= waves) break;
+    const WorkItem it = a.items[w >> 2];
+    const int tb = it.t0 + 64 * (w & 3);
+    if (tb < it.t1) {
+      Rate64* out = it.part < 0 ? a.rates : a.partial + static_cast<size_t>(it.part) * a.n;
+      wave_fp32s<HighOrder>(it, tb, a.nu, a.state, a.chunks, a.nchunks, a.src, out);
+    }
+    if (!Persistent) break;
   }
 }
 
@@ -1143,6 +1423,21 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
   pair_entry<kPairFP32CHO, P>(a);
 }
 
+// FP32C fed through scalar loads: one-wave blocks, at the register budgets
+// of 7 and 8 waves/SIMD, Gaussian and high-order kernels
+template <bool P>
+__global__ __launch_bounds__(64) void k_pair_fp32s(PairArgs a) { fp32s_entry<false, P>(a); }
+template <bool P>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_pair_fp32s_w8(PairArgs a) {
+  fp32s_entry<false, P>(a);
+}
+template <bool P>
+__global__ __launch_bounds__(64) void k_pair_fp32s_ho(PairArgs a) { fp32s_entry<true, P>(a); }
+template <bool P>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_pair_fp32s_ho_w8(PairArgs a) {
+  fp32s_entry<true, P>(a);
+}
+
 // FP32 pairs summed without compensation (grid launch only): a measuring
 // aid, not a production path
 __global__ __launch_bounds__(kTB) void k_pair_fp32c_plain(PairArgs a) { pair_entry<kPairFP32CPlain, false>(a); }
@@ -1163,9 +1458,49 @@ __global__ __launch_bounds__(256) void k_reduce_parts(const Rate64* __restrict__
 
 }  // namespace
 
+namespace {
+// the scalar-fed FP32C launch: chunk records, then one wave per quarter item
+hipError_t launch_fp32s(int kind, const PairConfig& cfg, const PairArgs& args, int grid, hipStream_t st) {
+  if (!args.chunks || !args.src || args.nchunks <= 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_src_chunks, dim3(args.nchunks), dim3(64), 0, st, args.state, args.chunks, args.src);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int waves = 4 * args.nitems;
+  const bool P = cfg.persistent;
+  if (P) {
+    e = launch_fill_int(args.queue, 1, 0, st);
+    if (e != hipSuccess) return e;
+  }
+  const dim3 g(P ? static_cast<unsigned>(std::min(grid, waves)) : static_cast<unsigned>(waves));
+  const dim3 block(64);
+  const bool ho = kind == kPairFP32CHO;
+  if (ho && cfg.waves8) {
+    if (P) hipLaunchKernelGGL(k_pair_fp32s_ho_w8<true>, g, block, 0, st, args);
+    else hipLaunchKernelGGL(k_pair_fp32s_ho_w8<false>, g, block, 0, st, args);
+  } else if (ho) {
+    if (P) hipLaunchKernelGGL(k_pair_fp32s_ho<true>, g, block, 0, st, args);
+    else hipLaunchKernelGGL(k_pair_fp32s_ho<false>, g, block, 0, st, args);
+  } else if (cfg.waves8) {
+    if (P) hipLaunchKernelGGL(k_pair_fp32s_w8<true>, g, block, 0, st, args);
+    else hipLaunchKernelGGL(k_pair_fp32s_w8<false>, g, block, 0, st, args);
+  } else {
+    if (P) hipLaunchKernelGGL(k_pair_fp32s<true>, g, block, 0, st, args);
+    else hipLaunchKernelGGL(k_pair_fp32s<false>, g, block, 0, st, args);
+  }
+  return hipGetLastError();
+}
+}  // namespace
+
 hipError_t launch_pair(int kind, const PairConfig& cfg, const PairArgs& args, int grid,
                        hipStream_t st) {
   if (args.nitems <= 0) return hipSuccess;
+  if (cfg.scalar_feed && (kind == kPairFP32C || kind == kPairFP32CHO)) {
+    hipError_t e = launch_fp32s(kind, cfg, args, grid, st);
+    if (e != hipSuccess || cfg.parts <= 1) return e;
+    hipLaunchKernelGGL(k_reduce_parts, dim3((args.n + 255) / 256), dim3(256), 0, st, args.partial,
+                       cfg.parts, args.n, args.rates);
+    return hipGetLastError();
+  }
   const dim3 block(kTB);
   const dim3 g(cfg.persistent ? static_cast<unsigned>(std::min(grid, args.nitems))
                               : static_cast<unsigned>(args.nitems));
@@ -1263,6 +1598,16 @@ int blocks_per_cu(const void* kernel, int threads = kTB) {
 
 int pair_kernel_waves(int kind, const PairConfig& cfg) {
   const void* k = nullptr;
+  if (cfg.scalar_feed && (kind == kPairFP32C || kind == kPairFP32CHO)) {
+    // one-wave blocks: resident blocks per CU = resident waves per CU
+    if (kind == kPairFP32C)
+      k = cfg.waves8 ? reinterpret_cast<const void*>(k_pair_fp32s_w8<true>)
+                     : reinterpret_cast<const void*>(k_pair_fp32s<true>);
+    else
+      k = cfg.waves8 ? reinterpret_cast<const void*>(k_pair_fp32s_ho_w8<true>)
+                     : reinterpret_cast<const void*>(k_pair_fp32s_ho<true>);
+    return blocks_per_cu(k, 64);
+  }
   if (kind == kPairFP32C)
     k = cfg.waves8 ? reinterpret_cast<const void*>(k_pair_fp32c_w8<true>)
                    : reinterpret_cast<const void*>(k_pair_fp32c<true>);

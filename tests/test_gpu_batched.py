@@ -9,8 +9,10 @@ from pawan_amd import capi, workloads as W
 
 pytestmark = pytest.mark.gpu
 
-CONFIGS = [(1, False, False), (3, False, False), (1, True, False), (8, True, False),
-           (4, True, True), (2, False, True)]
+# (parts, persistent, waves8, scalar_feed); the last two apply to FP32C only
+CONFIGS = [(1, False, False, False), (3, False, False, False), (1, True, False, False), (8, True, False, False),
+           (4, True, True, False), (2, False, True, False), (1, False, False, True), (3, True, True, True),
+           (8, True, False, True)]
 
 
 def _say(msg):
@@ -33,28 +35,28 @@ def _check_wakes(oracle, s, u, w, tol_u, tol_w, sample=64):
             assert normwise(w[a:b][idx], rw) < tol_w, (k, normwise(w[a:b][idx], rw))
 
 
-@pytest.mark.parametrize("parts,persistent,waves8", CONFIGS)
-def test_launch_shapes_coupled(oracle, parts, persistent, waves8):
+@pytest.mark.parametrize("parts,persistent,waves8,feed", CONFIGS)
+def test_launch_shapes_coupled(oracle, parts, persistent, waves8, feed):
     s = W.leapfrog_64k(nst=1024)  # 16,384 particles, two coupled wakes
     idx = np.unique(np.linspace(0, s.n - 1, 300).astype(np.int64))
     ru, rw = oracle.rates_targets(s, idx)
-    _say(f"leapfrog N={s.n} parts={parts} persistent={persistent} waves8={waves8}")
+    _say(f"leapfrog N={s.n} parts={parts} persistent={persistent} waves8={waves8} scalar_feed={feed}")
     with capi.Solver(s) as g:
         for prec, tu, tw in (("fp64", 1e-12, 1e-12), ("fp32c", 1e-6, 1e-5)):
-            g.set_pair_config(prec, parts, persistent, waves8)
+            g.set_pair_config(prec, parts, persistent, waves8, feed)
             u, w = g.solve(prec)
             assert normwise(u[idx], ru) < tu
             assert normwise(w[idx], rw) < tw
 
 
-@pytest.mark.parametrize("parts,persistent,waves8", CONFIGS)
-def test_launch_shapes_independent_wakes(oracle, batched, parts, persistent, waves8):
-    _say(f"batched {batched.nw} wakes N={batched.n} parts={parts} persistent={persistent}")
+@pytest.mark.parametrize("parts,persistent,waves8,feed", CONFIGS)
+def test_launch_shapes_independent_wakes(oracle, batched, parts, persistent, waves8, feed):
+    _say(f"batched {batched.nw} wakes N={batched.n} parts={parts} persistent={persistent} scalar_feed={feed}")
     with capi.Solver(batched) as g:
-        g.set_pair_config("fp64", parts, persistent, waves8)
+        g.set_pair_config("fp64", parts, persistent, waves8, feed)
         u, _ = g.solve("fp64")
         _check_wakes(oracle, batched, u, None, 1e-12, None)
-        g.set_pair_config("fp32c", parts, persistent, waves8)
+        g.set_pair_config("fp32c", parts, persistent, waves8, feed)
         u, _ = g.solve("fp32c")
         _check_wakes(oracle, batched, u, None, 1e-6, None)
 
@@ -70,14 +72,14 @@ def test_autotune_keeps_results(oracle, batched):
             _check_wakes(oracle, batched, u, None, tol, None, sample=16)
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp32c"])
-def test_graph_replay_equals_eager_steps(batched, prec):
+@pytest.mark.parametrize("prec,feed", [("fp64", False), ("fp32c", False), ("fp32c", True)])
+def test_graph_replay_equals_eager_steps(batched, prec, feed):
     """Steps replayed from a captured HIP graph give bit-identical states to
     eagerly launched steps, on every replay, with eager work (diagnostics,
     a state read) between the replays."""
     with capi.Solver(batched) as eager, capi.Solver(batched) as graph:
-        eager.set_pair_config(prec, 4, True, prec == "fp32c")
-        graph.set_pair_config(prec, 4, True, prec == "fp32c")
+        eager.set_pair_config(prec, 4, True, prec == "fp32c", feed)
+        graph.set_pair_config(prec, 4, True, prec == "fp32c", feed)
         graph.set_graph(True)
         for rep in range(4):
             eager.step("rk4", 2e-3, prec, 1)
@@ -162,7 +164,7 @@ def test_config3_batched_256_wakes(oracle):
                 assert max_w < tw and rms_w < 1e-4
             # graph replay against eager steps of the same launch shape
             with capi.Solver(s) as e:
-                e.set_pair_config(prec, cfg["parts"], cfg["persistent"], cfg["waves8"])
+                e.set_pair_config(prec, cfg["parts"], cfg["persistent"], cfg["waves8"], cfg["scalar_feed"])
                 g.set_graph(True)
                 g.step("rk4", dt, prec, 2)
                 e.step("rk4", dt, prec, 2)

@@ -119,13 +119,16 @@ def test_point_velocity(ho):
     assert normwise(u, ref) < FP64_TOL
 
 
-def test_fp32c_rhs(ho):
-    """The compensated FP32 form of the kernel, at the Gaussian FP32C bars:
-    velocity 1e-6 and vorticity rate 1e-5 normwise against the FP64 oracle."""
+@pytest.mark.parametrize("feed", [False, True])
+def test_fp32c_rhs(ho, feed):
+    """The compensated FP32 form of the kernel (LDS-tiled and scalar-fed), at
+    the Gaussian FP32C bars: velocity 1e-6 and vorticity rate 1e-5 normwise
+    against the FP64 oracle."""
     for s in (W.ring_1k(), W.leapfrog_64k()):
         idx = np.arange(0, s.n, max(1, s.n // 700))
         ref_u, ref_w = ho.rates_targets(s, idx)
         with _solver(s) as g:
+            g.set_pair_config("fp32c", 1, False, False, feed)
             u, w = g.solve("fp32c")
             u64, w64 = g.solve("fp64")
         eu, ew = normwise(u[idx], ref_u), normwise(w[idx], ref_w)
@@ -136,14 +139,16 @@ def test_fp32c_rhs(ho):
         assert normwise(u, u64) < 1e-6 and normwise(w, w64) < 1e-5
 
 
-def test_fp32c_separated_tiles_on_the_helix(ho):
+@pytest.mark.parametrize("feed", [False, True])
+def test_fp32c_separated_tiles_on_the_helix(ho, feed):
     """On the 1M helix most tile pairs take the separated loop (FP32
-    positions relative to the targets' centre, the kernel evaluated exactly):
-    sampled targets against the oracle."""
+    positions relative to the targets' or the source chunk's centre, the
+    kernel evaluated exactly): sampled targets against the oracle."""
     s = W.helix_wake(1 << 20)
     idx = np.random.default_rng(3).choice(s.n, size=48, replace=False)
     ref_u, ref_w = ho.rates_targets(s, idx)
     with _solver(s) as g:
+        g.set_pair_config("fp32c", 4, True, True, feed)
         u, w = g.solve("fp32c")
     eu, ew = normwise(u[idx], ref_u), normwise(w[idx], ref_w)
     _say(f"helix 1M high-order fp32c, 48 targets: e_u={eu:.3e} e_w={ew:.3e}")
