@@ -159,6 +159,9 @@ int pw_get_size(pw_ctx *ctx, int64_t *num_particles, int *num_wakes);
  * _velocity/_retvorcity rows, src/wake/wake.h:49-52). */
 int pw_set_wake_state(pw_ctx *ctx, int wake, const double *pos,
                       const double *alpha);
+/* The volumes of wake `wake` (n doubles), as __wake::_volume after a merge
+ * zeroes the absorbed particles' (wake.cpp:147). */
+int pw_set_wake_volume(pw_ctx *ctx, int wake, const double *volume);
 int pw_get_wake_state(pw_ctx *ctx, int wake, double *pos, double *alpha);
 int pw_get_wake_rates(pw_ctx *ctx, int wake, double *vel, double *retvor);
 
@@ -193,6 +196,34 @@ int pw_velocity_at(pw_ctx *ctx, const double *points, int64_t m, double *u);
  * context on it: calls that overlap on one GPU (from several host threads)
  * race.  Blocking. */
 int pw_velocity_at_noalloc(pw_ctx *ctx, const double *points, int64_t m, double *u);
+
+/* ---- particle management of one wake ----------------------------------
+ * __wake::split and __wake::merge (src/wake/wake.cpp:61-110, 112-159; relax
+ * is split then merge, :161-165), on the device, for the arrays of one wake
+ * as __wake holds them (host pointers, updated in place; rows of 3 for pos
+ * and alpha).  Both visit particles from the first while they are older than
+ * 45 steps (stepnum - active > 45, the first younger one ends the loop).
+ * Results are bit-for-bit the reference's sequential ones.
+ *
+ * pw_wake_split: a particle with |a| >= 2 birth and birth > 1e-4 becomes two
+ * with a/2 and birth |a|/2 each, 0.25 sigma apart along a; the new ones are
+ * appended after the last particle, in the order of their parents, and
+ * *num_particles and *npidx advance.  Arrays have maxn rows.  The reference
+ * writes new particles at _npidx and wraps at MAXNUMPARTICLES, overwriting
+ * its first particles: this call requires *npidx == *num_particles and
+ * returns PW_ERR_UNSUPPORTED, changing nothing, if *npidx differs or the new
+ * particles would not fit in maxn rows.  *nsplit (may be NULL) gets the
+ * count.
+ *
+ * pw_wake_merge: particle i absorbs every later j within (sigma_i +
+ * sigma_j)/8 whose strength is not aligned with a_i (1 - cos >= 1e-4):
+ * x_i moves to the |a|-weighted mean, a_i += a_j, and j keeps its row with
+ * zero strength and volume.  *nmerged (may be NULL) gets the count. */
+int pw_wake_split(int device, double stepnum, int64_t maxn, int64_t *num_particles, int64_t *npidx,
+                  double *pos, double *alpha, double *radius, double *volume, double *birthstrength,
+                  double *active, int64_t *nsplit);
+int pw_wake_merge(int device, double stepnum, int64_t num_particles, double *pos, double *alpha,
+                  const double *radius, double *volume, const double *active, int64_t *nmerged);
 
 /* One right-hand-side evaluation: zero the rates, then all self and (when
  * coupled) cross-wake pair interactions, leaving velocity and vorticity

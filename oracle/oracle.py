@@ -63,6 +63,11 @@ def _load() -> C.CDLL:
     lib.or_enst_self.restype = d
     lib.or_enstf_self.argtypes = [d]
     lib.or_enstf_self.restype = d
+    pi64 = C.POINTER(C.c_int64)
+    lib.or_wake_split.argtypes = [d, i64, pi64, pi64, dp, dp, dp, dp, dp, dp]
+    lib.or_wake_split.restype = i64
+    lib.or_wake_merge.argtypes = [d, i64, dp, dp, dp, dp, dp]
+    lib.or_wake_merge.restype = i64
     return lib
 
 
@@ -309,3 +314,48 @@ def vring_resolve(R: float, core: float, nlayers: int, nazi: int, sig: float):
     strength = np.tile(s, nazi)
     alpha = np.stack([-np.sin(psi) * strength, np.cos(psi) * strength, np.zeros_like(psi)], axis=1)
     return s, M, (pos, alpha, radius, volume)
+
+
+class WakeParticles:
+    """The particle arrays of one wake as __wake holds them: maxn rows, the
+    first `n` in use (positions, strengths: rows of 3)."""
+
+    def __init__(self, pos, alpha, radius, volume, birth=None, active=None, maxn=None):
+        n = len(pos)
+        maxn = n if maxn is None else maxn
+        self.maxn = maxn
+        self.n = n
+        self.npidx = n
+        self.pos = np.zeros((maxn, 3))
+        self.alpha = np.zeros((maxn, 3))
+        self.radius = np.zeros(maxn)
+        self.volume = np.zeros(maxn)
+        self.birth = np.zeros(maxn)
+        self.active = np.zeros(maxn)
+        self.pos[:n], self.alpha[:n] = pos, alpha
+        self.radius[:n], self.volume[:n] = radius, volume
+        if birth is not None:
+            self.birth[:n] = birth
+        if active is not None:
+            self.active[:n] = active
+
+    def copy(self):
+        w = WakeParticles(self.pos[:self.n], self.alpha[:self.n], self.radius[:self.n], self.volume[:self.n],
+                          self.birth[:self.n], self.active[:self.n], self.maxn)
+        w.npidx = self.npidx
+        return w
+
+
+def wake_split(w: WakeParticles, stepnum: float) -> int:
+    """__wake::split (wake.cpp:61-110) in place; the number split, or -1
+    when nothing was done (npidx != n, or no room in maxn rows)."""
+    n, k = C.c_int64(w.n), C.c_int64(w.npidx)
+    r = lib().or_wake_split(float(stepnum), w.maxn, C.byref(n), C.byref(k), w.pos, w.alpha, w.radius,
+                            w.volume, w.birth, w.active)
+    w.n, w.npidx = n.value, k.value
+    return int(r)
+
+
+def wake_merge(w: WakeParticles, stepnum: float) -> int:
+    """__wake::merge (wake.cpp:112-159) in place; the number of merges."""
+    return int(lib().or_wake_merge(float(stepnum), w.n, w.pos, w.alpha, w.radius, w.volume, w.active))

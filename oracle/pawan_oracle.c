@@ -974,3 +974,110 @@ int or_lu_solve(int n, double *A, const double *b, double *x) {
   free(perm);
   return singular;
 }
+
+/* ---------------------------------------------------------------------- */
+/* Particle management: __wake::split and __wake::merge                    */
+/* ---------------------------------------------------------------------- */
+
+/* src/wake/wake.cpp:61-110.  The loop visits particles from the first while
+ * they are older than 45 steps (stepnum - active > 45; the first younger one
+ * ends it) and re-reads the particle count, which grows as particles are
+ * appended.  A particle splits when |a| >= 2 birth and birth > 1e-4: both
+ * halves carry a/2 and birth |a|/2, the new one at x + dx and the old one at
+ * x - dx, dx = a (0.25 sigma / |a|), with the old one's radius, volume and
+ * age.  The new particle goes to row *npidx; the reference then advances
+ * *npidx, wraps it to 0 at MAXNUMPARTICLES (wake.cpp:96-99) and sets the
+ * count to it.  Restated here for *npidx == *num_particles and no wrap. */
+int64_t or_wake_split(double stepnum, int64_t maxn, int64_t *num_particles, int64_t *npidx,
+                      double *pos, double *alpha, double *radius, double *volume,
+                      double *birth, double *active) {
+  if (*npidx != *num_particles) return -1;
+  /* the new particles must fit: count the splits first (a split does not
+     change any other particle's test) */
+  int64_t want = 0;
+  for (int64_t i = 0; i < *num_particles && stepnum - active[i] > 45; ++i) {
+    const double m = or_dnrm2(&alpha[3 * i]);
+    if (m >= 2 * birth[i] && birth[i] > 0.0001) ++want;
+  }
+  if (*npidx + want > maxn) return -1;
+  int64_t nsplit = 0;
+  for (int64_t i = 0; i < *num_particles && stepnum - active[i] > 45; ++i) {
+    const double ib = birth[i];
+    double *iv = &alpha[3 * i];
+    const double im = or_dnrm2(iv);
+    if (im >= 2 * ib && ib > 0.0001) {
+      double *ip = &pos[3 * i];
+      const double ir = radius[i], vol = volume[i];
+      double dp[3] = {iv[0], iv[1], iv[2]};
+      const double sc = 0.25 * ir / im; /* gsl_blas_dscal(0.25*iradius/ivormag, deltapos) */
+      for (int c = 0; c < 3; ++c) dp[c] *= sc;
+      const int64_t k = *npidx;
+      for (int c = 0; c < 3; ++c) {
+        pos[3 * k + c] = ip[c];
+        pos[3 * k + c] += dp[c];
+        alpha[3 * k + c] = iv[c];
+        alpha[3 * k + c] *= 0.5;
+      }
+      radius[k] = ir;
+      volume[k] = vol;
+      birth[k] = 0.5 * im;
+      active[k] = active[i];
+      for (int c = 0; c < 3; ++c) {
+        ip[c] -= dp[c];
+        iv[c] *= 0.5;
+      }
+      birth[i] = 0.5 * im;
+      ++*npidx;
+      *num_particles = *npidx;
+      ++nsplit;
+    }
+  }
+  return nsplit;
+}
+
+/* src/wake/wake.cpp:112-159.  Outer particles i from the first while older
+ * than 45 steps; inner j over all later particles.  |a_i| is taken once per
+ * i (the merges below grow a_i but not that norm); a pair merges when
+ * |x_i - x_j| < (sigma_i + sigma_j)/8, both strengths are non-zero and
+ * 1 - a_i.a_j / |a_i| / |a_j| >= 1e-4: x_i moves to (|a_i| x_i + |a_j| x_j)
+ * scaled by 1/(|a_i| + |a_j|), a_i += a_j, and particle j keeps its place
+ * with zero strength and volume. */
+int64_t or_wake_merge(double stepnum, int64_t n, double *pos, double *alpha,
+                      const double *radius, double *volume, const double *active) {
+  int64_t nmerge = 0;
+  for (int64_t i = 0; i < n && stepnum - active[i] > 45; ++i) {
+    double *ip = &pos[3 * i], *iv = &alpha[3 * i];
+    const double im = or_dnrm2(iv);
+    const double ir = radius[i];
+    for (int64_t j = i + 1; j < n; ++j) {
+      double *jp = &pos[3 * j], *jv = &alpha[3 * j];
+      const double jm = or_dnrm2(jv);
+      const double jr = radius[j];
+      double d[3] = {ip[0], ip[1], ip[2]};
+      for (int c = 0; c < 3; ++c) d[c] -= jp[c];
+      const double rho = or_dnrm2(d);
+      if (rho < 0.125 * (ir + jr) && jm != 0.0 && im != 0.0) {
+        double al = ddot3(iv, jv);
+        al = 1 - al / im / jm;
+        if (al >= 0.0001) {
+          double m[3] = {ip[0], ip[1], ip[2]}, t[3] = {jp[0], jp[1], jp[2]};
+          for (int c = 0; c < 3; ++c) {
+            m[c] *= im;
+            t[c] *= jm;
+          }
+          for (int c = 0; c < 3; ++c) m[c] += t[c];
+          const double sc = 1 / (im + jm);
+          for (int c = 0; c < 3; ++c) m[c] *= sc;
+          for (int c = 0; c < 3; ++c) {
+            ip[c] = m[c];
+            iv[c] += jv[c];
+            jv[c] = 0.0;
+          }
+          volume[j] = 0.0;
+          ++nmerge;
+        }
+      }
+    }
+  }
+  return nmerge;
+}

@@ -143,6 +143,19 @@ void or_vring_influence(int nLayers, int nAzi, const double *pos, const double *
                         double *M);
 int or_lu_solve(int n, double *A, const double *b, double *x);
 
+/* Particle management of one wake (__wake::split, merge, relax,
+ * src/wake/wake.cpp:61-165), on arrays of maxn rows (pos, alpha: x 3).
+ * or_wake_split appends each new particle at *npidx, which must equal
+ * *num_particles (the coupling's bookkeeping; the capacity must not wrap):
+ * returns the number of particles split, or -1 (nothing changed) if
+ * *npidx != *num_particles or the new particles would not fit in maxn rows.
+ * or_wake_merge returns the number of merges. */
+int64_t or_wake_split(double stepnum, int64_t maxn, int64_t *num_particles, int64_t *npidx,
+                      double *pos, double *alpha, double *radius, double *volume,
+                      double *birth, double *active);
+int64_t or_wake_merge(double stepnum, int64_t n, double *pos, double *alpha,
+                      const double *radius, double *volume, const double *active);
+
 #ifdef __cplusplus
 }
 #endif

@@ -39,7 +39,7 @@ EXPORTS = [
     "pw_ff_mlp_forward", "pw_ff_mlp_grad", "pw_ff_set_conserve", "pw_ff_get_conserve",
     "pw_velocity_at", "pw_set_kernel", "pw_get_kernel", "pw_vring_particles", "pw_vring_resolve",
     "pw_set_compensation", "pw_ff_get_timing", "pw_interact_wakes", "pw_vorticity_field",
-    "pw_diagnose_wakes", "pw_velocity_at_noalloc",
+    "pw_diagnose_wakes", "pw_velocity_at_noalloc", "pw_wake_split", "pw_wake_merge", "pw_set_wake_volume",
 ]
 
 # pw_ff_export selectors
@@ -76,6 +76,7 @@ def load(path: str | None = None) -> C.CDLL:
     lib = C.CDLL(path)
     vp, i64, d, i = C.c_void_p, C.c_int64, C.c_double, C.c_int
     pvp = C.POINTER(C.c_void_p)
+    pi64 = C.POINTER(C.c_int64)
     sig = {
         "pw_version": ([], C.c_char_p),
         "pw_last_error": ([], C.c_char_p),
@@ -93,6 +94,7 @@ def load(path: str | None = None) -> C.CDLL:
         "pw_get_kernel": ([vp, C.POINTER(C.c_int)], i),
         "pw_get_size": ([vp, C.POINTER(C.c_int64), C.POINTER(C.c_int)], i),
         "pw_set_wake_state": ([vp, i, _dp, _dp], i),
+        "pw_set_wake_volume": ([vp, i, _dp], i),
         "pw_get_wake_state": ([vp, i, _dp, _dp], i),
         "pw_get_wake_rates": ([vp, i, _dp, _dp], i),
         "pw_set_states": ([vp, _dp, i64], i),
@@ -101,6 +103,8 @@ def load(path: str | None = None) -> C.CDLL:
         "pw_solve": ([vp, i], i),
         "pw_velocity_at": ([vp, _dp, i64, _dp], i),
         "pw_velocity_at_noalloc": ([vp, _dp, i64, _dp], i),
+        "pw_wake_split": ([i, d, i64, pi64, pi64, _dp, _dp, _dp, _dp, _dp, _dp, pi64], i),
+        "pw_wake_merge": ([i, d, i64, _dp, _dp, _dp, _dp, _dp, pi64], i),
         "pw_step": ([vp, i, d, i, i64], i),
         "pw_diagnose": ([vp, _dp, i64], i),
         "pw_synchronize": ([vp], i),
@@ -490,3 +494,33 @@ class Solver:
         loss = C.c_double()
         _check(self._lib.pw_ff_mlp_grad(self._h, X, Y, len(X), g, C.byref(loss)))
         return g, loss.value
+
+
+def wake_split(pos, alpha, radius, volume, birth, active, n: int, stepnum: float, maxn: int | None = None,
+               device: int = 0):
+    """__wake::split on the device (pw_wake_split), in place on arrays of
+    maxn rows whose first n rows are particles; returns (n after, splits)."""
+    maxn = len(radius) if maxn is None else maxn
+    arrs = [pos, alpha, radius, volume, birth, active]
+    for a in arrs:
+        if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]):
+            raise TypeError("split arrays must be C-contiguous float64 numpy arrays (updated in place)")
+    if len(pos) < maxn or len(alpha) < maxn or min(len(radius), len(volume), len(birth), len(active)) < maxn:
+        raise ValueError("arrays need maxn rows")
+    nn, k, ns = C.c_int64(n), C.c_int64(n), C.c_int64(0)
+    _check(load().pw_wake_split(device, float(stepnum), int(maxn), C.byref(nn), C.byref(k), pos, alpha, radius,
+                                volume, birth, active, C.byref(ns)))
+    return nn.value, ns.value
+
+
+def wake_merge(pos, alpha, radius, volume, active, n: int, stepnum: float, device: int = 0) -> int:
+    """__wake::merge on the device (pw_wake_merge), in place; returns the
+    number of merges."""
+    for a in (pos, alpha, radius, volume, active):
+        if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]):
+            raise TypeError("merge arrays must be C-contiguous float64 numpy arrays (updated in place)")
+    if len(pos) < n or len(alpha) < n or min(len(radius), len(volume), len(active)) < n:
+        raise ValueError("arrays need n rows")
+    nm = C.c_int64(0)
+    _check(load().pw_wake_merge(device, float(stepnum), int(n), pos, alpha, radius, volume, active, C.byref(nm)))
+    return nm.value

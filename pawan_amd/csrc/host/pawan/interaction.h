@@ -102,6 +102,8 @@ class __interaction : public __system {
   virtual ~__interaction();
 
   virtual void diagnose() override;
+  //! relax every wake (interaction.cpp:367-373); the device copy follows
+  virtual void relax(size_t &stepnum) override;
   virtual void addWake(__wake *W);
   virtual void solve() override;
   //! influence() (interaction.cpp:58-61)

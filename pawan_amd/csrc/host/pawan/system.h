@@ -25,6 +25,8 @@ class __system {
   virtual void getRates(vec *rate) {}
   virtual void getStates(vec *state) {}
   virtual void diagnose() {}
+  //! particle management of the wakes (system.h:75); nothing by default
+  virtual void relax(size_t &stepnum) {}
 
   //! Advance the system by one step of `scheme` (1 Euler, 3 RK3, 4 RK4)
   //! without leaving the device; `state` is the integrator's state vector,

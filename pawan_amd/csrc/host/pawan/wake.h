@@ -4,8 +4,9 @@
 //
 // Same public members, state-vector layout and constructors as the
 // reference, with the gsl containers replaced by pawan::vec / pawan::mat.
-// Dymore coupling members (addParticles(PawanRecvData), split/merge/relax,
-// bound-vortex effects) are not part of this library.
+// split / merge / relax run on the device (pw_wake_split, pw_wake_merge).
+// Dymore coupling members (addParticles(PawanRecvData), bound-vortex
+// effects) are not part of this library.
 #pragma once
 
 #include <cstdio>
@@ -28,6 +29,12 @@ class __wake {
   size_t _maxsize;          /*!< state-vector block of the wake, 6 maxN  */
   size_t _numParticles;     /*!< particles N                            */
   size_t _maxnumParticles;  /*!< rows allocated, maxN                   */
+  //! Row of the next particle split off (wake.h:44).  The reference leaves
+  //! it 0 (initialise_memory) or unset outside the Dymore coupling, whose
+  //! particle emission keeps it at _numParticles; here it follows
+  //! _numParticles, so split appends after the last particle.
+  size_t _npidx = 0;
+  bool _maxmem = false;     /*!< rows exhausted (wake.h:45; not reached here) */
   mat *_position;           /*!< N x 3                                  */
   mat *_velocity;
   mat *_vorticity;
@@ -52,6 +59,16 @@ class __wake {
   //! Copy the particles of W into this wake (wake.cpp:542-556)
   virtual void addParticles(__wake *W);
   virtual ~__wake();
+
+  //! Split particles older than 45 steps whose strength has doubled since
+  //! birth (wake.cpp:61-110), on the device; throws std::runtime_error when
+  //! the new particles do not fit in _maxnumParticles rows (the reference
+  //! wraps and overwrites its first particles)
+  virtual void split(size_t &stepnum);
+  //! Merge close, misaligned particles (wake.cpp:112-159), on the device
+  virtual void merge(size_t &stepnum);
+  //! split, then merge (wake.cpp:161-165)
+  virtual void relax(size_t &stepnum);
 
   virtual void print();
   //! [N][pos][alpha][radius][volume][birthstrength] (wake.cpp:821-830)

@@ -24,6 +24,8 @@ namespace pawan {
 
 void __wake::create_particles(const int &n) {
   _numParticles = n;
+  _npidx = _numParticles;
+  _maxmem = false;
   _size = 2 * _numParticles * 3;
   _maxnumParticles = _numParticles;
   _maxsize = 2 * _maxnumParticles * 3;
@@ -40,6 +42,8 @@ void __wake::create_particles(const int &n) {
 
 void __wake::initialise_memory() {
   _numParticles = 0;
+  _npidx = 0;
+  _maxmem = false;
   _size = 0;
   _maxnumParticles = MAXNUMPARTICLES;
   _maxsize = 2 * _maxnumParticles * 3;
@@ -91,7 +95,34 @@ void __wake::addParticles(__wake *W) {
     _birthstrength->set(i, W->_birthstrength->get(i));
   }
   _numParticles = W->_numParticles;
+  _npidx = _numParticles;
   _size = 2 * _numParticles * 3;
+}
+
+// wake.cpp:61-110, on the device
+void __wake::split(size_t &stepnum) {
+  int64_t n = static_cast<int64_t>(_numParticles), k = static_cast<int64_t>(_npidx), ns = 0;
+  if (pw_wake_split(0, static_cast<double>(stepnum), static_cast<int64_t>(_maxnumParticles), &n, &k,
+                    _position->ptr(), _vorticity->ptr(), _radius->ptr(), _volume->ptr(), _birthstrength->ptr(),
+                    _active->ptr(), &ns) != PW_OK)
+    throw std::runtime_error(std::string("__wake::split: ") + pw_last_error());
+  _numParticles = static_cast<size_t>(n);
+  _npidx = static_cast<size_t>(k);
+  _size = 2 * _numParticles * 3;
+}
+
+// wake.cpp:112-159, on the device
+void __wake::merge(size_t &stepnum) {
+  int64_t nm = 0;
+  if (pw_wake_merge(0, static_cast<double>(stepnum), static_cast<int64_t>(_numParticles), _position->ptr(),
+                    _vorticity->ptr(), _radius->ptr(), _volume->ptr(), _active->ptr(), &nm) != PW_OK)
+    throw std::runtime_error(std::string("__wake::merge: ") + pw_last_error());
+}
+
+// wake.cpp:161-165
+void __wake::relax(size_t &stepnum) {
+  split(stepnum);
+  merge(stepnum);
 }
 
 __wake::__wake(__wake *W) {
@@ -243,6 +274,30 @@ void __interaction::addWake(__wake *W) {
     pw_destroy(_ctx);
     _ctx = nullptr;
   }
+}
+
+// interaction.cpp:367-373.  The wakes change on the host; a wake that grew
+// needs a new device copy (as after addWake), otherwise positions,
+// strengths and volumes are sent to the existing one.
+void __interaction::relax(size_t &stepnum) {
+  std::vector<size_t> n0;
+  for (__wake *w : _W) n0.push_back(w->_numParticles);
+  for (__wake *w : _W) w->relax(stepnum);
+  bool grew = false;
+  for (size_t k = 0; k < _W.size(); ++k) grew = grew || _W[k]->_numParticles != n0[k];
+  if (!_ctx) return;
+  if (grew) {
+    pw_destroy(_ctx);
+    _ctx = nullptr;
+    _diagValid = false;
+    return;
+  }
+  for (size_t k = 0; k < _W.size(); ++k)
+    if (pw_set_wake_state(_ctx, static_cast<int>(k), _W[k]->_position->ptr(), _W[k]->_vorticity->ptr()) != PW_OK ||
+        pw_set_wake_volume(_ctx, static_cast<int>(k), _W[k]->_volume->ptr()) != PW_OK)
+      fail("__interaction::relax");
+  _deviceStale = false;
+  _diagValid = false;
 }
 
 void __interaction::ensureDevice() {

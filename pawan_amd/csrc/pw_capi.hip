@@ -645,6 +645,19 @@ int pw_set_wake_state(pw_ctx* c, int w, const double* pos, const double* alpha) 
   return upload_state(c, h);
 }
 
+int pw_set_wake_volume(pw_ctx* c, int w, const double* volume) {
+  PW_CTX(c);
+  if (w < 0 || w >= static_cast<int>(c->wn.size())) return fail(PW_ERR_ARG, "wake index out of range");
+  if (c->wn[w] > 0 && !volume) return fail(PW_ERR_ARG, "null array");
+  std::vector<Rec64> h;
+  PW_TRY(download_state(c, h));
+  for (int64_t i = 0; i < c->wn[w]; ++i) {
+    h[c->woff[w] + i].vol = volume[i];
+    c->volume[c->woff[w] + i] = volume[i];
+  }
+  return upload_state(c, h);
+}
+
 int pw_get_wake_state(pw_ctx* c, int w, double* pos, double* alpha) {
   PW_CTX(c);
   if (w < 0 || w >= static_cast<int>(c->wn.size())) return fail(PW_ERR_ARG, "wake index out of range");
