@@ -322,23 +322,33 @@ def drift_leg(steps: int, every: int, learned_cfg: dict | None, target_err: floa
     return out
 
 
-def measured_traffic(workload: str, n: int):
-    """HBM bytes per launch of the FP32C pair kernel from the committed
-    rocprofv3 counter runs (FETCH_SIZE and WRITE_SIZE, scaled by the factor
-    the same counters read for a copy kernel of known size), for the
-    workload they were measured on; None otherwise."""
-    path = os.path.join(ROOT, "profiles", "pair_fp32c_helix1m_traffic.txt")
-    if workload != "helix" or n != 1 << 20 or not os.path.exists(path):
+def pair_kernel_name(precision: str, cfg: dict | None) -> str:
+    """The pair kernel a tuned launch runs (the persistent or grid form of it)."""
+    if precision == "fp64":
+        return "k_pair_fp64"
+    cfg = cfg or {}
+    return ("k_pair_fp32s" if cfg.get("scalar_feed") else "k_pair_fp32c") + ("_w8" if cfg.get("waves8") else "")
+
+
+def measured_traffic(workload: str, n: int, precision: str, cfg: dict | None):
+    """HBM bytes per launch of the pair kernel from the committed rocprofv3
+    counter runs (FETCH_SIZE and WRITE_SIZE in KiB per dispatch, scaled by
+    the factor the same counters read for a copy of known size), for the
+    workload and launch shape they were measured on; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "pair_kernels_helix1m_traffic.txt")
+    if workload != "helix" or n != 1 << 20 or not cfg or not os.path.exists(path):
         return None
     with open(path) as f:
         t = json.load(f)
-    return {
-        "bytes": (t["fetch_kib"] * t["fetch_factor"] + t["write_kib"] * t["write_factor"]) * 1024.0,
-        "fetch_bytes": t["fetch_kib"] * t["fetch_factor"] * 1024.0,
-        "write_bytes": t["write_kib"] * t["write_factor"] * 1024.0,
-        "launch_config": t["launch_config"],
-        "summary": t["summary"],
-    }
+    want = {k: bool(cfg.get(k)) if k != "parts" else int(cfg.get(k, 1))
+            for k in ("parts", "persistent", "waves8", "scalar_feed")}
+    for e in t["entries"]:
+        if e["precision"] == precision and e["launch_config"] == want:
+            fetch = e["fetch_kib"] * t["fetch_factor"] * 1024.0
+            write = e["write_kib"] * t["write_factor"] * 1024.0
+            return {"bytes": fetch + write, "fetch_bytes": fetch, "write_bytes": write, "kernel": e["kernel"],
+                    "launch_config": want, "summary": t["summary"]}
+    return None
 
 
 def fp32c_error(system):
@@ -654,7 +664,9 @@ def main() -> int:
             "fp64_frac_of_peak": ach64 / PEAK_FP64_TFLOPS,
             "roofline_fp64": {
                 "bound": "valu", "kernel": "k_pair_fp64", "achieved": ach64, "peak": PEAK_FP64_TFLOPS,
-                "unit": "TFLOP/s", "frac": ach64 / PEAK_FP64_TFLOPS, "traffic": None,
+                "unit": "TFLOP/s", "frac": ach64 / PEAK_FP64_TFLOPS,
+                "traffic": (lambda t: t["bytes"] if t else None)(
+                    measured_traffic(args.workload, system.n, "fp64", tune64)),
                 "avg_launch_ms": avg64, "flop_per_launch": FLOP_PER_PAIR * pairs, "launch_config": tune64},
         }
         log(f"fp64: {out64['fp64_steps_per_s']:.4f} steps/s, pair kernel avg {avg64:.1f} ms")
@@ -662,7 +674,7 @@ def main() -> int:
         log(f"fp32c error vs fp64: {out64['fp32c_error_vs_fp64']}")
 
     achieved = FLOP_PER_PAIR * pairs / (avg32 * 1e-3) / 1e12
-    traffic = measured_traffic(args.workload, system.n)
+    traffic = measured_traffic(args.workload, system.n, "fp32c", tune32)
     result = {
         "metric": METRIC,
         "value": sps32,
@@ -693,7 +705,7 @@ def main() -> int:
             "traffic": traffic["bytes"] if traffic else None,
             "traffic_detail": traffic,
             "algorithmic_bytes_per_launch": 112 * system.n,
-            "kernel": "k_pair_fp32c" + ("_w8" if tune32 and tune32.get("waves8") else ""),
+            "kernel": pair_kernel_name("fp32c", tune32),
             "launch_config": tune32,
             "avg_launch_ms": avg32,
             "flop_per_launch": FLOP_PER_PAIR * pairs,

@@ -613,13 +613,14 @@ __device__ __forceinline__ void pair32_ho(const T32& t, const S& s, P32& p, PSE3
     accumulate32(dx, dy, dz, q, f, t.ax, t.ay, t.az, s, p);
 }
 
-// Far-only pair on positions relative to the target leaf's centre, rounded
-// to FP32 once per particle: for sources beyond 44 sigma^2 the rounding
-// (about 6e-8 of the distance to the centre) is a relative displacement
-// error below 1e-6, and the pair needs no FP64 at all.
+// Near-field source record of the tree path: position relative to the
+// target leaf's centre, rounded to FP32 once per particle (for sources beyond
+// 44 sigma^2 the rounding, about 6e-8 of the distance to the centre, is a
+// relative displacement error below 1e-6, and the pair needs no FP64).
 struct alignas(16) SrcRel {
-  float rx, ry, rz, ax;  // what the far-only loop reads: 24 bytes
-  float ay, az, hs2, vol;
+  float rx, ry, rz, ax;  // what the far-only loop reads first: 36 bytes
+  float ay, az, mx, my;  // (m = a x r, for the 26-instruction sums)
+  float mz, hs2, vol, pad;
 };
 
 // Direct-sum source record (64 B): what the separated loop reads first
@@ -634,37 +635,6 @@ struct alignas(16) SrcT2 {
   double x, y, z;
 };
 static_assert(sizeof(SrcT2) == 64, "one 64-byte record per source");
-
-template <typename S>
-__device__ __forceinline__ void pair32_rel(float tx, float ty, float tz, float tax, float tay, float taz,
-                                           const S& s, P32& p) {
-  const float dx = tx - s.rx;
-  const float dy = ty - s.ry;
-  const float dz = tz - s.rz;
-  const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-  const float ir = __builtin_amdgcn_rsqf(r2);
-  const float ir2 = ir * ir;
-  const float q = ir * ir2;
-  accumulate32(dx, dy, dz, q, q * ir2, tax, tay, taz, s, p);
-}
-
-// The same with the high-order algebraic kernel, exact at any distance:
-// q' = (rho^2 + 5/2 sigma^2) b^-5/2, f' = (rho^2 + 7/2 sigma^2) b^-7/2 with
-// b = rho^2 + sigma^2, sigma^2 = the source's s^2/2 + th2 (the target's).
-template <typename S>
-__device__ __forceinline__ void pair32_rel_ho(float tx, float ty, float tz, float th2, float tax, float tay,
-                                              float taz, const S& s, P32& p) {
-  const float dx = tx - s.rx;
-  const float dy = ty - s.ry;
-  const float dz = tz - s.rz;
-  const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-  const float sig2 = s.hs2 + th2;
-  const float ib = __builtin_amdgcn_rsqf(r2 + sig2);
-  const float ib2 = ib * ib;
-  const float ib5 = ib2 * ib2 * ib;
-  accumulate32(dx, dy, dz, fmaf(2.5f, sig2, r2) * ib5, fmaf(3.5f, sig2, r2) * (ib5 * ib2), tax, tay, taz, s,
-               p);
-}
 
 // Knuth's two-sum: (hi, lo) += p with the rounding error of hi + p kept.
 __device__ __forceinline__ void two_sum(float& hi, float& lo, float p) {
@@ -1243,6 +1213,8 @@ __device__ __forceinline__ void near_slice(const NearArgs& a, const NearItem& it
   me.nu2 = 2.0f * a.nu;
   const float rx = static_cast<float>(m.x - tcx), ry = static_cast<float>(m.y - tcy),
               rz = static_cast<float>(m.z - tcz);
+  // c_t = t' x a_t for the far-only leaves (positions relative to T's centre)
+  const float ctx = ry * me.az - rz * me.ay, cty = rz * me.ax - rx * me.az, ctz = rx * me.ay - ry * me.ax;
   float hi[kAcc], lo[kAcc];
 #pragma unroll
   for (int k = 0; k < kAcc; ++k) hi[k] = lo[k] = 0.0f;
@@ -1270,16 +1242,24 @@ __device__ __forceinline__ void near_slice(const NearArgs& a, const NearItem& it
         f.ax = static_cast<float>(r.ax);
         f.ay = static_cast<float>(r.ay);
         f.az = static_cast<float>(r.az);
+        f.mx = f.ay * f.rz - f.az * f.ry;
+        f.my = f.az * f.rx - f.ax * f.rz;
+        f.mz = f.ax * f.ry - f.ay * f.rx;
         f.vol = static_cast<float>(r.vol);
+        f.pad = 0.0f;
         tile[lane] = f;
       }
       __syncthreads();
       if (!active) continue;
-      P32 p;
       if (all_far) {
+        P32S q;
 #pragma unroll 2
-        for (int l = 0; l < nc; ++l) pair32_rel(rx, ry, rz, me.ax, me.ay, me.az, tile[l], p);
-      } else if (diag) {
+        for (int l = 0; l < nc; ++l) pair32s<false>(rx, ry, rz, me.hs2, me.ax, me.ay, me.az, ctx, cty, ctz, tile[l], q);
+        fold_s(q, rx, ry, rz, me.ax, me.ay, me.az, hi, lo);
+        continue;
+      }
+      P32 p;
+      if (diag) {
         const int own = self - c0;  // this target's slot in the tile, if there
         for (int l = 0; l < nc; ++l)
           if (l != own) pair32_rel_core(rx, ry, rz, me, tile[l], p, pse);
