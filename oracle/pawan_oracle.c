@@ -178,9 +178,10 @@ double or_qsig(double rho, double sigma) {
 }
 
 /* ENST(rho, sigma, F1, F2), highorder.h:77-87; zero for the Gaussian
- * (gaussian.h:69-75).  Pinned in part: F1 at rho -> 0 equals ENST(sigma)
- * and the pair terms enter the enstrophy sums; the rho-dependence of F1
- * and F2 is restated from highorder.h:83-86 with no independent check. */
+ * (gaussian.h:69-75).  Pinned: the pair term F1 a1.a2 + F2 (a1.x)(a2.x) is
+ * a1.(Z I - grad grad G) a2 with G the Newton potential of the core, so
+ * F1 = Z + M/rho^3 and F2 = (Z - 3 M/rho^3)/rho^2 with M the enclosed core,
+ * checked by quadrature of Z alone; F1 at rho -> 0 equals ENST(sigma). */
 void or_enst_pair(double rho, double sigma, double *F1, double *F2) {
   if (smoothing != OR_HIGHORDER) {
     *F1 = 0.0;

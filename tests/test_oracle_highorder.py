@@ -141,6 +141,43 @@ def test_enstrophy_factors(ho):
     assert ho.enstf_pair(0.0, SIG) == pytest.approx(ho.enstf_self(SIG), rel=1e-14)
 
 
+def test_enstrophy_pair_is_the_hessian_form_of_the_core_potential(ho):
+    """ENST's pair term F1 a1.a2 + F2 (a1.x)(a2.x) is a1.(Z I - grad grad G) a2
+    with G the Newton potential of the core (-lap G = Z):
+        G(r) = (1/r) int_0^r Z s^2 ds + int_r^inf Z s ds,
+        G' = -M/r^2 with M = int_0^r Z s^2 ds,  G'' = -Z - 2 G'/r,
+    so F1 = Z - G'/r = Z + M/r^3 and F2 = -(G'' - G'/r)/r^2 = (Z - 3 M/r^3)/r^2.
+    M and the outer integral come from quadrature of the core alone; the
+    Laplacian of G is checked by finite differences of the quadrature, so a
+    wrong coefficient in either factor's rho-dependence fails."""
+    Z = lambda r: ho.kernel(r, SIG)[2]
+
+    def potential(r):
+        inner, _ = integrate.quad(lambda s: Z(s) * s * s, 0, r, epsabs=0, epsrel=1e-13)
+        outer, _ = integrate.quad(lambda s: Z(s) * s, r, np.inf, epsabs=0, epsrel=1e-13, limit=200)
+        return inner / r + outer, inner
+
+    for rb in (0.15, 0.6, 1.0, 1.7, 3.0, 8.0):
+        r = rb * SIG
+        G, M = potential(r)
+        h = 1e-2 * r
+        g = [potential(r + k * h)[0] if k else G for k in (-2, -1, 0, 1, 2)]
+        d1 = (g[0] - 8 * g[1] + 8 * g[3] - g[4]) / (12 * h)  # five-point stencils
+        d2 = (-g[0] + 16 * g[1] - 30 * g[2] + 16 * g[3] - g[4]) / (12 * h * h)
+        assert d1 == pytest.approx(-M / r ** 2, rel=1e-7)
+        if rb <= 3.0:  # beyond, -lap G is a small difference of large terms
+            assert -(d2 + 2 * d1 / r) == pytest.approx(Z(r), rel=1e-5)  # -lap G = Z
+        F1, F2 = ho.enst_pair(r, SIG)
+        assert F1 == pytest.approx(Z(r) + M / r ** 3, rel=1e-11)
+        assert F2 == pytest.approx((Z(r) - 3 * M / r ** 3) / r ** 2, rel=1e-10)
+    # far from the core: F1 -> 1/(4 pi r^3) and F2 r^2 -> -3/(4 pi r^3), the
+    # singular Hessian of 1/(4 pi r)
+    r = 400 * SIG
+    F1, F2 = ho.enst_pair(r, SIG)
+    assert F1 * 4 * math.pi * r ** 3 == pytest.approx(1.0, rel=1e-4)
+    assert F2 * 4 * math.pi * r ** 5 == pytest.approx(-3.0, rel=1e-4)
+
+
 def test_diagnostics(ho, oracle):
     """Helicity = -sum a.u with the oracle's own high-order velocity; the
     enstrophy sums are the self terms plus twice the pair terms of i < j."""
