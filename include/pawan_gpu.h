@@ -114,11 +114,11 @@ int pw_set_coupled(pw_ctx *ctx, int coupled);
  * and ring resolve: PW_KERNEL_GAUSSIAN (default) or PW_KERNEL_HIGHORDER
  * (KERNEL, QSIG, ENST, ENSTF of highorder.h:20-106 in place of gaussian.h's;
  * the reference makes this choice at compile time,
- * interaction_utils.h:22-26).  The high-order kernel is built for the
- * direct sums: PW_FP64 and PW_FP64_REF both run its FP64 form, PW_FP32C its
- * compensated FP32 form; pw_solve/pw_step return PW_ERR_UNSUPPORTED for
- * PW_LEARNED, and pw_autotune for any precision, while it is selected.
- * May be changed between steps. */
+ * interaction_utils.h:22-26).  PW_FP64 and PW_FP64_REF both run its FP64
+ * form, PW_FP32C its compensated FP32 form; PW_LEARNED runs it in the near
+ * field and, in the far field, keeps each node's deviation of the kernel
+ * from the singular law, (15/8)(sigma/r)^4 of it at most, within the
+ * tolerance (pw_ff_set_params).  May be changed between steps. */
 int pw_set_kernel(pw_ctx *ctx, int kernel);
 int pw_get_kernel(pw_ctx *ctx, int *kernel);
 
@@ -324,9 +324,9 @@ int pw_get_pair_config(pw_ctx *ctx, int precision, int *parts,
 /* Time the candidate launch configurations of `precision` on the
  * context's current state (one right-hand side each) and keep the fastest;
  * *best_ms (may be NULL) gets its time.  Results are cached per problem
- * shape for the process, and in the text file named by $PAWAN_TUNE_CACHE
- * when set.  Leaves the rates of the current state on the device, as
- * pw_solve does. */
+ * shape and smoothing kernel for the process, and in the text file named
+ * by $PAWAN_TUNE_CACHE when set.  Leaves the rates of the current state on
+ * the device, as pw_solve does. */
 int pw_autotune(pw_ctx *ctx, int precision, double *best_ms);
 
 /* enable = 1: pw_step captures one step as a HIP graph (per scheme,
@@ -344,7 +344,12 @@ int pw_set_graph(pw_ctx *ctx, int enable);
  * calibrated coefficient of the evaluation in use), is at most tol.  Far
  * nodes are evaluated by the analytic monopole + dipole expansion plus,
  * when use_mlp, the network's residual; every other pair is summed
- * directly (FP32C).  tol <= 0 disables the error test. */
+ * directly (FP32C).  tol <= 0 disables the error test.  With the
+ * high-order kernel the error a node's far evaluation is held to also
+ * counts (15/8) sigma_max^4 s0 / (4 pi gap^6), the largest deviation of the
+ * kernel from the singular law over the node (gap = d - r_node - r_leaf,
+ * s0 = sum |alpha|); with tol <= 0 a far node needs
+ * gap^2 > 5600 sigma_max^2 instead, where that deviation is below 2^-24. */
 int pw_ff_set_params(pw_ctx *ctx, double theta, double tol, int use_mlp);
 int pw_ff_get_params(pw_ctx *ctx, double *theta, double *tol, int *use_mlp);
 /* enable = 1 (default): remove the mean vorticity rate after every

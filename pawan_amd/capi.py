@@ -243,7 +243,7 @@ class Solver:
         return s, M
 
     def set_kernel(self, kernel: int) -> None:
-        """PW_KERNEL_GAUSSIAN or PW_KERNEL_HIGHORDER (direct sums only)."""
+        """PW_KERNEL_GAUSSIAN or PW_KERNEL_HIGHORDER."""
         _check(self._lib.pw_set_kernel(self._h, int(kernel)))
 
     def set_compensation(self, on: bool) -> None:

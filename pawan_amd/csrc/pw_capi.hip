@@ -257,14 +257,6 @@ int kind_of(int precision, int* kind) {
   }
 }
 
-// The high-order algebraic kernel is built for the direct sums only.
-int check_smoothing(const pw_ctx* c, int kind) {
-  if (c->smoothing == PW_KERNEL_HIGHORDER && kind == kKindLearned)
-    return fail(PW_ERR_UNSUPPORTED, "the high-order algebraic kernel is computed by the direct sums only "
-                                    "(PW_FP64, PW_FP64_REF, PW_FP32C)");
-  return PW_OK;
-}
-
 // Work items for a launch split into `parts` source ranges per target
 // block (parts == 1: one item per block writing its rates directly).
 // Split points fall on tile boundaries of the block's system, so the
@@ -875,7 +867,7 @@ int pw_vring_resolve(pw_ctx* c, int wake, int nlayers, int nazi, double R, doubl
   int* d_flag = nullptr;
   const size_t mm = static_cast<size_t>(m) * m;
   if (dalloc(&d_M, mm) != hipSuccess || dalloc(&d_lu, mm) != hipSuccess || dalloc(&d_b, m) != hipSuccess ||
-      dalloc(&d_s, m) != hipSuccess || dalloc(&d_flag, 1) != hipSuccess) {
+      dalloc(&d_s, m) != hipSuccess || dalloc(&d_flag, 1 + static_cast<size_t>(m)) != hipSuccess) {
     (void)hipGetLastError();
     (void)hipFree(d_M); (void)hipFree(d_lu); (void)hipFree(d_b); (void)hipFree(d_s); (void)hipFree(d_flag);
     return fail(PW_ERR_NOMEM, "resolve buffers");
@@ -902,7 +894,6 @@ int pw_solve(pw_ctx* c, int precision) {
   PW_CTX(c);
   int kind = 0;
   PW_TRY(kind_of(precision, &kind));
-  PW_TRY(check_smoothing(c, kind));
   PW_TRY(ensure_built(c));
   PW_HIP(pw::launch_fill_int(c->d_flag, 1, 0, c->stream));
   PW_TRY(rhs(c, kind, c->y, c->k[0]));
@@ -989,7 +980,6 @@ int pw_step(pw_ctx* c, int scheme, double dt, int precision, int64_t nsteps) {
   PW_CTX(c);
   int kind = 0;
   PW_TRY(kind_of(precision, &kind));
-  PW_TRY(check_smoothing(c, kind));
   if (scheme != PW_EULER && scheme != PW_RK3 && scheme != PW_RK4)
     return fail(PW_ERR_ARG, "scheme must be PW_EULER, PW_RK3 or PW_RK4");
   if (!std::isfinite(dt)) return fail(PW_ERR_ARG, "dt must be finite");
@@ -1295,7 +1285,8 @@ namespace {
 // Tuning results shared by every context of the process, keyed by the
 // problem's shape; optionally also appended to / read from the text file
 // named by $PAWAN_TUNE_CACHE ("kind n nsys largest parts persistent variant",
-// variant the PW_VARIANT_* bits).
+// kind the pair-kernel family: 0-2 for the Gaussian precisions, 4 and 5
+// for the high-order FP64 and FP32C kernels; variant the PW_VARIANT_* bits).
 using TuneKey = std::tuple<int, int64_t, int, int64_t>;
 std::map<TuneKey, pw::PairConfig>& tune_cache() {
   static std::map<TuneKey, pw::PairConfig> m;
@@ -1319,13 +1310,14 @@ void load_tune_file() {
   long long n, largest;
   while (std::fscanf(f, "%d %lld %d %lld %d %d %d", &kind, &n, &nsys, &largest, &parts, &pers, &w8) == 7) {
     // the ranges pw_set_pair_config accepts; other lines are skipped
-    if (kind < 0 || kind > 2 || parts < 1 || parts > 256) continue;
+    if (kind < 0 || kind > pw::kPairFP32CHO || kind == kKindLearned || parts < 1 || parts > 256) continue;
     pw::PairConfig cfg;
     cfg.parts = parts;
     cfg.persistent = pers != 0;
     if (w8 < 0 || w8 > (PW_VARIANT_WAVES8 | PW_VARIANT_SCALAR_FEED)) continue;
-    cfg.waves8 = (w8 & PW_VARIANT_WAVES8) != 0 && kind == pw::kPairFP32C;
-    cfg.scalar_feed = (w8 & PW_VARIANT_SCALAR_FEED) != 0 && kind == pw::kPairFP32C;
+    const bool f32 = kind == pw::kPairFP32C || kind == pw::kPairFP32CHO;
+    cfg.waves8 = (w8 & PW_VARIANT_WAVES8) != 0 && f32;
+    cfg.scalar_feed = (w8 & PW_VARIANT_SCALAR_FEED) != 0 && f32;
     tune_cache()[TuneKey(kind, n, nsys, largest)] = cfg;
   }
   std::fclose(f);
@@ -1378,11 +1370,12 @@ int pw_autotune(pw_ctx* c, int precision, double* best_ms) {
   int kind = 0;
   PW_TRY(kind_of(precision, &kind));
   if (kind == kKindLearned) return fail(PW_ERR_UNSUPPORTED, "launch configuration applies to the direct-sum kinds");
-  if (c->smoothing != PW_KERNEL_GAUSSIAN)
-    return fail(PW_ERR_UNSUPPORTED, "the autotuner times the Gaussian kernels");
   PW_TRY(ensure_built(c));
   const int64_t largest = c->systems.empty() ? 0 : c->systems.front().b - c->systems.front().a;
-  const TuneKey key(kind, c->n, c->nsys(), largest);
+  // keyed by the kernel family the launches run (the high-order kernels
+  // have their own best shapes)
+  const int family = c->smoothing == PW_KERNEL_HIGHORDER ? launch_kind_of(c, kind) : kind;
+  const TuneKey key(family, c->n, c->nsys(), largest);
   auto& k = c->ks[kind];
   const bool was_profiling = c->profiling;
   c->profiling = false;

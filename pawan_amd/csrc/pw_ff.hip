@@ -141,12 +141,26 @@ int ensure_ff(pw_ctx* c) {
   return PW_OK;
 }
 
-pw::ListParams list_params(const pwi::FarField& f, bool mac_only) {
+// mac_only: the lists training samples are drawn from (opening angle and
+// the Gaussian core only; the samples' exact sums are those of the singular
+// law there, whatever the smoothing kernel)
+pw::ListParams list_params(const pw_ctx* c, bool mac_only) {
+  const auto& f = c->ff;
   pw::ListParams lp;
   lp.theta = f.theta;
   lp.tol = mac_only ? 0.0f : f.tol;
   lp.err_coef = static_cast<float>(f.use_mlp ? f.coef_mlp : f.coef_md);
   lp.sigma_max2 = f.sigma_max2;
+  lp.ho_coef = 0.0f;
+  lp.ho_gap2 = 0.0f;
+  if (c->smoothing == PW_KERNEL_HIGHORDER && !mac_only) {
+    // 1 - 4 pi r^3 q = (15/8) x^2 - (35/8) x^3 + ..., x = (sigma/r)^2: at
+    // most (15/8) x^2 for r >= sigma, so a node's velocity differs from the
+    // singular law by at most (15/8) sigma^4 s0 / (4 pi gap^6)
+    const double s4 = static_cast<double>(f.sigma_max2) * f.sigma_max2;
+    if (f.tol > 0.0f) lp.ho_coef = static_cast<float>(1.875 * s4 / (4.0 * M_PI));
+    else lp.ho_gap2 = 5600.0f;  // (15/8) / gap_bar^4 <= 2^-24
+  }
   return lp;
 }
 
@@ -167,7 +181,7 @@ int build(pw_ctx* c, const Rec64* state, bool mac_only) {
   }
   f.shape = pw::tree_shape(n, nleaf);
   PW_HIP(pw::build_moments(n, f.tb, f.shape, c->stream));
-  const pw::ListParams lp = list_params(f, mac_only);
+  const pw::ListParams lp = list_params(c, mac_only);
   PW_HIP(pw::build_lists(f.tb, f.shape, lp, c->stream));
   // lists longer than their capacity: grow them and build again (the
   // lengths are those of this tree, so the second build fits)
@@ -305,8 +319,9 @@ int ff_rhs_body(pw_ctx* c, const Rec64* state, Rate64* out, const StageEvents* e
   const auto& tb = f.tb;
   const int n = static_cast<int>(c->n);
   PW_HIP(pw::build_near_items(tb, f.shape.nleaf, c->stream));
-  PW_HIP(pw::launch_near_fp32c(tb.sorted, tb.nodes, tb, f.shape.nleaf, c->nu, f.sigma_max2, c->num_cus * pw::near_kernel_blocks_per_cu(),
-                               f.rates_s, c->stream));
+  const bool ho = c->smoothing == PW_KERNEL_HIGHORDER;
+  PW_HIP(pw::launch_near_fp32c(tb.sorted, tb.nodes, tb, f.shape.nleaf, c->nu, f.sigma_max2, ho,
+                               c->num_cus * pw::near_kernel_blocks_per_cu(ho), f.rates_s, c->stream));
   PW_HIP(mark(2));
   PW_HIP(pw::launch_far_md(tb.sorted, tb.nodes, tb.far_list, tb.far_cnt, tb.cap_far, f.shape.nleaf, f.rates_s,
                            c->stream));

@@ -106,7 +106,7 @@ hipError_t launch_pair(int kind, const PairConfig& cfg, const PairArgs& args, in
 // its waves per CU), from the occupancy calculator
 int pair_kernel_waves(int kind, const PairConfig& cfg);
 // resident one-wave blocks per CU of the tree path's near-field kernel
-int near_kernel_blocks_per_cu();
+int near_kernel_blocks_per_cu(bool highorder);
 
 // Stage arithmetic of the steppers (FP64, no contraction, the reference's
 // operation order).  n = number of particles.
@@ -257,6 +257,14 @@ struct ListParams {
   float tol;
   float err_coef;
   float sigma_max2;
+  // high-order algebraic kernel: its velocity differs from the singular law
+  // the far field expands by at most (15/8)(sigma/r)^4 of it, so a far node
+  // also needs ho_coef s0 / gap^6 (ho_coef = (15/8) sigma_max^4 / 4 pi) added
+  // to its self-estimated error to stay within tol, or, without a tolerance,
+  // gap^2 > ho_gap2 sigma_max^2 (the deviation below FP32 rounding); both 0
+  // for the Gaussian
+  float ho_coef;
+  float ho_gap2;
 };
 
 hipError_t tree_sort_temp_bytes(int n, size_t* bytes);
@@ -301,19 +309,22 @@ inline hipError_t ensure_lds_limit(const void* kernel, size_t bytes, std::atomic
   return e;
 }
 
-// velocity at m points (AoS doubles, device) from the n particles of state
 // Vortex-ring strength resolution (pw_vring.hip): influence matrix M (m x m)
 // of the ring at `off` (nazi stations of m cells), its LU solve against
 // `ideal` on a copy `lu`, and the strengths s written into the ring's
-// vorticity.  flag[0] = 1 if the matrix is singular.  m <= kVringMaxCells.
+// vorticity.  flag holds 1 + m ints: flag[0] = 1 if the matrix is
+// singular, the rest the row permutation.  m <= kVringMaxCells.
 constexpr int kVringMaxCells = 1024;
 hipError_t launch_vring_resolve(Rec64* y, int off, int m, int nazi, bool high_order, double* M,
                                 double* lu, const double* ideal, double* s, int* flag, hipStream_t st);
 
+// velocity at m points (AoS doubles, device) from the n particles of state
 hipError_t launch_point_velocity(const Rec64* state, int n, const double* pts, int m, double* u,
                                  bool high_order, hipStream_t st);
+// near field of the tree path (highorder: the algebraic kernel's pairs)
 hipError_t launch_near_fp32c(const Rec64* sorted, const NodeRec* nodes, const TreeBuffers& tb, int nleaf,
-                             double nu, float sigma_max2, int grid, Rate64* rates_s, hipStream_t st);
+                             double nu, float sigma_max2, bool highorder, int grid, Rate64* rates_s,
+                             hipStream_t st);
 
 // Far field over the lists, analytic monopole + dipole (FP32), added to
 // rates_s; with mlp != nullptr the learned residual is added as well.

@@ -1166,7 +1166,10 @@ constexpr int kNearSlice = 64;                       // targets per wave
 static_assert(kTB == kLeafSize, "partial-sum rows hold one leaf");
 constexpr int kNearSlices = kLeafSize / kNearSlice;  // slices per leaf at most
 
-// pair on relative FP32 positions, Gaussian core included
+// pair on relative FP32 positions, Gaussian core included (HighOrder: the
+// algebraic kernel, exact for every pair, PSE within 44 sigma^2 as in
+// pair32_ho)
+template <bool HighOrder>
 __device__ __forceinline__ void pair32_rel_core(float tx, float ty, float tz, const T32& t, const SrcRel& s,
                                                 P32& p, PSE32& e) {
   const float dx = tx - s.rx;
@@ -1174,14 +1177,26 @@ __device__ __forceinline__ void pair32_rel_core(float tx, float ty, float tz, co
   const float dz = tz - s.rz;
   const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
   const float sig2 = s.hs2 + t.hs2;
-  const float ir = __builtin_amdgcn_rsqf(r2);
-  const float ir2 = ir * ir;
-  float q = ir * ir2;
-  float f = q * ir2;
+  float q, f;
+  float ib7 = 0.0f, ib2 = 0.0f;
+  if (HighOrder) {
+    const float ib = __builtin_amdgcn_rsqf(r2 + sig2);
+    ib2 = ib * ib;
+    const float ib5 = ib2 * ib2 * ib;
+    ib7 = ib5 * ib2;
+    q = fmaf(2.5f, sig2, r2) * ib5;
+    f = fmaf(3.5f, sig2, r2) * ib7;
+  } else {
+    const float ir = __builtin_amdgcn_rsqf(r2);
+    const float ir2 = ir * ir;
+    q = ir * ir2;
+    f = q * ir2;
+  }
   const bool core = r2 <= kFar32 * sig2;
   if (core) {
     float c;
-    near32(r2, sig2, t.nu2, q, f, c);
+    if (HighOrder) c = t.nu2 * kHoEta32 * (sig2 * sig2) * (ib7 * ib2);
+    else near32(r2, sig2, t.nu2, q, f, c);
     e.x = fmaf(c, t.vol * s.ax - s.vol * t.ax, e.x);
     e.y = fmaf(c, t.vol * s.ay - s.vol * t.ay, e.y);
     e.z = fmaf(c, t.vol * s.az - s.vol * t.az, e.z);
@@ -1193,6 +1208,7 @@ __device__ __forceinline__ void pair32_rel_core(float tx, float ty, float tz, co
     accumulate32(dx, dy, dz, q, f, t.ax, t.ay, t.az, s, p);
 }
 
+template <bool HighOrder>
 __device__ __forceinline__ void near_slice(const NearArgs& a, const NearItem& it, int slice, SrcRel* tile) {
   const int T = it.T;
   const int lane = static_cast<int>(threadIdx.x);
@@ -1254,7 +1270,8 @@ __device__ __forceinline__ void near_slice(const NearArgs& a, const NearItem& it
       if (all_far) {
         P32S q;
 #pragma unroll 2
-        for (int l = 0; l < nc; ++l) pair32s<false>(rx, ry, rz, me.hs2, me.ax, me.ay, me.az, ctx, cty, ctz, tile[l], q);
+        for (int l = 0; l < nc; ++l)
+          pair32s<HighOrder>(rx, ry, rz, me.hs2, me.ax, me.ay, me.az, ctx, cty, ctz, tile[l], q);
         fold_s(q, rx, ry, rz, me.ax, me.ay, me.az, hi, lo);
         continue;
       }
@@ -1262,10 +1279,10 @@ __device__ __forceinline__ void near_slice(const NearArgs& a, const NearItem& it
       if (diag) {
         const int own = self - c0;  // this target's slot in the tile, if there
         for (int l = 0; l < nc; ++l)
-          if (l != own) pair32_rel_core(rx, ry, rz, me, tile[l], p, pse);
+          if (l != own) pair32_rel_core<HighOrder>(rx, ry, rz, me, tile[l], p, pse);
       } else {
 #pragma unroll 1
-        for (int l = 0; l < nc; ++l) pair32_rel_core(rx, ry, rz, me, tile[l], p, pse);
+        for (int l = 0; l < nc; ++l) pair32_rel_core<HighOrder>(rx, ry, rz, me, tile[l], p, pse);
       }
       fold(p, me.ax, me.ay, me.az, hi, lo);
     }
@@ -1286,8 +1303,8 @@ __device__ __forceinline__ void near_slice(const NearArgs& a, const NearItem& it
   }
 }
 
-__global__ __launch_bounds__(kNearSlice) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_near_fp32c(NearArgs a) {
-  __shared__ SrcRel tile[kNearSlice];
+template <bool HighOrder>
+__device__ __forceinline__ void near_entry(const NearArgs& a, SrcRel* tile) {
   const int slices = kNearSlices * *a.total;
   for (;;) {
     int idx = 0;
@@ -1297,8 +1314,17 @@ __global__ __launch_bounds__(kNearSlice) __attribute__((amdgpu_waves_per_eu(6, 8
     const NearItem it = a.items[idx / kNearSlices];
     const int slice = idx % kNearSlices;
     if (a.nodes[it.T].first + kNearSlice * slice >= a.nodes[it.T].last) continue;
-    near_slice(a, it, slice, tile);
+    near_slice<HighOrder>(a, it, slice, tile);
   }
+}
+
+__global__ __launch_bounds__(kNearSlice) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_near_fp32c(NearArgs a) {
+  __shared__ SrcRel tile[kNearSlice];
+  near_entry<false>(a, tile);
+}
+__global__ __launch_bounds__(kNearSlice) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_near_fp32c_ho(NearArgs a) {
+  __shared__ SrcRel tile[kNearSlice];
+  near_entry<true>(a, tile);
 }
 
 // rates of a split leaf: sum of its items' partial sums, in item order
@@ -1545,13 +1571,15 @@ hipError_t launch_point_velocity(const Rec64* state, int n, const double* pts, i
 }
 
 hipError_t launch_near_fp32c(const Rec64* sorted, const NodeRec* nodes, const TreeBuffers& tb, int nleaf,
-                             double nu, float sigma_max2, int grid, Rate64* rates_s, hipStream_t st) {
+                             double nu, float sigma_max2, bool highorder, int grid, Rate64* rates_s,
+                             hipStream_t st) {
   if (nleaf <= 0) return hipSuccess;
   hipError_t e = launch_fill_int(tb.near_queue, 1, 0, st);
   if (e != hipSuccess) return e;
   NearArgs a{sorted, nodes, tb.near_list, tb.cap_near, tb.near_items, tb.near_total, tb.near_queue,
              static_cast<float>(nu), kFar32 * sigma_max2, rates_s, tb.near_partial};
-  hipLaunchKernelGGL(k_near_fp32c, dim3(grid), dim3(kNearSlice), 0, st, a);
+  if (highorder) hipLaunchKernelGGL(k_near_fp32c_ho, dim3(grid), dim3(kNearSlice), 0, st, a);
+  else hipLaunchKernelGGL(k_near_fp32c, dim3(grid), dim3(kNearSlice), 0, st, a);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_near_reduce, dim3(nleaf), dim3(kTB), 0, st, nodes, tb.near_parts, tb.near_offs,
@@ -1603,6 +1631,10 @@ int pair_kernel_waves(int kind, const PairConfig& cfg) {
   return blocks_per_cu(k);
 }
 
-int near_kernel_blocks_per_cu() { return blocks_per_cu(reinterpret_cast<const void*>(k_near_fp32c), kNearSlice); }
+int near_kernel_blocks_per_cu(bool highorder) {
+  return blocks_per_cu(highorder ? reinterpret_cast<const void*>(k_near_fp32c_ho)
+                                 : reinterpret_cast<const void*>(k_near_fp32c),
+                       kNearSlice);
+}
 
 }  // namespace pw

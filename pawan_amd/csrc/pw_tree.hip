@@ -312,9 +312,15 @@ __device__ __forceinline__ bool is_far(const NodeRec& q, double tcx, double tcy,
   const float d = static_cast<float>(sqrt(ddx * ddx + ddy * ddy + ddz * ddz));
   const float gap = d - q.r - tr;
   bool far = (q.r + tr) < lp.theta * d && gap > 0.0f && gap * gap > kFar32List * lp.sigma_max2;
+  if (far && lp.ho_gap2 > 0.0f) far = gap * gap > lp.ho_gap2 * lp.sigma_max2;
   if (far && lp.tol > 0.0f) {
     const float dm = d - tr;
-    far = lp.err_coef * q.s2 <= lp.tol * (dm * dm) * (dm * dm);
+    if (lp.ho_coef > 0.0f) {
+      const float g2 = gap * gap;
+      far = lp.err_coef * q.s2 / ((dm * dm) * (dm * dm)) + lp.ho_coef * q.s0 / (g2 * g2 * g2) <= lp.tol;
+    } else {
+      far = lp.err_coef * q.s2 <= lp.tol * (dm * dm) * (dm * dm);
+    }
   }
   return far;
 }

@@ -11,10 +11,13 @@
 // Here one lane computes one entry M[p][i] directly: the field at particle p
 // from cell i's nAzi particles in ascending station order (the terms of the
 // other cells are exact zeros in the reference), with ZETASIG(0, s_p) for
-// p itself.  The m x m system M s = ideal profile is then solved by one
-// block with partial pivoting (GSL's LU_decomp/LU_solve order), and the
-// strengths are written into the ring's particles.
+// p itself.  The m x m system M s = ideal profile is then solved with
+// partial pivoting (GSL's LU_decomp/LU_solve order) by one block for small
+// rings and by every CU, one launch pair per column, for large ones, and
+// the strengths are written into the ring's particles.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "pw_internal.h"
 
@@ -22,6 +25,9 @@ namespace pw {
 namespace {
 
 constexpr int kTB = 256;
+// matrices of this order and up are factored by all CUs, column by column
+// (k_lu_pivot, k_lu_update), smaller ones by one block (k_lu_solve)
+constexpr int kLuSpreadFrom = 256;
 constexpr double k1Pi = 0.31830988618379067154;  // M_1_PI
 constexpr double kP15 = 0x1.f7fccdff344acp+3;    // pow(2 pi, 1.5)
 
@@ -197,6 +203,102 @@ __global__ __launch_bounds__(kTB) void k_lu_solve(double* __restrict__ A, int n,
   for (int r = tid; r < n; r += kTB) x[r] = xs[r];
 }
 
+// The same factorization spread over the chip for larger matrices: per
+// column j, k_lu_pivot (one block: pivot search, row swap, multipliers)
+// and k_lu_update (every block: the trailing submatrix, one element per
+// lane); then k_lu_substitute (one block: the two triangular solves).  Each
+// element sees the same operations in the same order as in k_lu_solve, so
+// the factors are bit for bit the same.  perm (n entries) lives in device
+// memory between the launches.
+__global__ __launch_bounds__(kTB) void k_lu_pivot(double* __restrict__ A, int n, int j, int* __restrict__ perm,
+                                                  int* __restrict__ flag) {
+#pragma clang fp contract(off)
+  __shared__ double red_v[kTB];
+  __shared__ int red_i[kTB];
+  const int tid = static_cast<int>(threadIdx.x);
+  if (j == 0) {
+    for (int r = tid; r < n; r += kTB) perm[r] = r;
+    if (tid == 0) flag[0] = 0;
+    __syncthreads();
+  }
+  double best = -1.0;
+  int bi = j;
+  for (int r = j + tid; r < n; r += kTB) {
+    const double v = fabs(A[static_cast<size_t>(r) * n + j]);
+    if (v > best) { best = v; bi = r; }
+  }
+  red_v[tid] = best;
+  red_i[tid] = bi;
+  __syncthreads();
+  for (int st = kTB / 2; st > 0; st >>= 1) {
+    if (tid < st) {
+      const double v2 = red_v[tid + st];
+      const int i2 = red_i[tid + st];
+      if (v2 > red_v[tid] || (v2 == red_v[tid] && i2 < red_i[tid])) {
+        red_v[tid] = v2;
+        red_i[tid] = i2;
+      }
+    }
+    __syncthreads();
+  }
+  const int ip = red_i[0];
+  if (ip != j) {
+    for (int c = tid; c < n; c += kTB) {
+      const double t = A[static_cast<size_t>(j) * n + c];
+      A[static_cast<size_t>(j) * n + c] = A[static_cast<size_t>(ip) * n + c];
+      A[static_cast<size_t>(ip) * n + c] = t;
+    }
+    if (tid == 0) {
+      const int t = perm[j];
+      perm[j] = perm[ip];
+      perm[ip] = t;
+    }
+  }
+  __syncthreads();
+  const double piv = A[static_cast<size_t>(j) * n + j];
+  if (!(fabs(piv) > 0.0) || !isfinite(piv)) {
+    if (tid == 0) flag[0] = 1;
+    return;
+  }
+  for (int r = j + 1 + tid; r < n; r += kTB) A[static_cast<size_t>(r) * n + j] /= piv;
+}
+
+__global__ __launch_bounds__(kTB) void k_lu_update(double* __restrict__ A, int n, int j) {
+#pragma clang fp contract(off)
+  const double piv = A[static_cast<size_t>(j) * n + j];
+  if (!(fabs(piv) > 0.0) || !isfinite(piv)) return;  // k_lu_pivot flagged it and left the column
+  const int w = n - j - 1;
+  const double* prow = A + static_cast<size_t>(j) * n;
+  for (int e = blockIdx.x * kTB + static_cast<int>(threadIdx.x); e < w * w; e += gridDim.x * kTB) {
+    const int r = j + 1 + e / w, c = j + 1 + e % w;
+    double* row = A + static_cast<size_t>(r) * n;
+    row[c] -= row[j] * prow[c];
+  }
+}
+
+__global__ __launch_bounds__(kTB) void k_lu_substitute(const double* __restrict__ A, int n,
+                                                       const int* __restrict__ perm,
+                                                       const double* __restrict__ b, double* __restrict__ x) {
+#pragma clang fp contract(off)
+  __shared__ double xs[kVringMaxCells];
+  const int tid = static_cast<int>(threadIdx.x);
+  for (int r = tid; r < n; r += kTB) xs[r] = b[perm[r]];
+  __syncthreads();
+  for (int c = 0; c + 1 < n; ++c) {
+    const double xc = xs[c];
+    for (int r = c + 1 + tid; r < n; r += kTB) xs[r] -= A[static_cast<size_t>(r) * n + c] * xc;
+    __syncthreads();
+  }
+  for (int c = n - 1; c >= 0; --c) {
+    if (tid == 0) xs[c] /= A[static_cast<size_t>(c) * n + c];
+    __syncthreads();
+    const double xc = xs[c];
+    for (int r = tid; r < c; r += kTB) xs[r] -= A[static_cast<size_t>(r) * n + c] * xc;
+    __syncthreads();
+  }
+  for (int r = tid; r < n; r += kTB) x[r] = xs[r];
+}
+
 // a = s_i (-sin psi_a, cos psi_a, 0) for the ring's nazi * m particles
 // (__vring::set_vorticity, vring.cpp:25-48); left unchanged if the solve
 // found the matrix singular.
@@ -232,9 +334,24 @@ hipError_t launch_vring_resolve(Rec64* y, int off, int m, int nazi, bool high_or
   if (e != hipSuccess) return e;
   e = hipMemcpyAsync(lu, M, sizeof(double) * m * m, hipMemcpyDeviceToDevice, st);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_lu_solve, dim3(1), dim3(kTB), 0, st, lu, m, ideal, s, flag);
-  e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  if (m < kLuSpreadFrom) {
+    hipLaunchKernelGGL(k_lu_solve, dim3(1), dim3(kTB), 0, st, lu, m, ideal, s, flag);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  } else {
+    int* perm = flag + 1;
+    for (int j = 0; j < m; ++j) {
+      hipLaunchKernelGGL(k_lu_pivot, dim3(1), dim3(kTB), 0, st, lu, m, j, perm, flag);
+      const int w = m - j - 1;
+      if (w > 0) {
+        const int blocks = std::min((w * w + kTB - 1) / kTB, 1024);
+        hipLaunchKernelGGL(k_lu_update, dim3(blocks), dim3(kTB), 0, st, lu, m, j);
+      }
+    }
+    hipLaunchKernelGGL(k_lu_substitute, dim3(1), dim3(kTB), 0, st, lu, m, perm, ideal, s);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   hipLaunchKernelGGL(k_vring_set_alpha, dim3((m * nazi + kTB - 1) / kTB), dim3(kTB), 0, st, y, off, m, nazi,
                      dpsi, s, flag);
   return hipGetLastError();

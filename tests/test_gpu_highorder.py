@@ -156,16 +156,34 @@ def test_fp32c_separated_tiles_on_the_helix(ho, feed):
     assert ew < 1e-5
 
 
-def test_unsupported_combinations_are_errors():
+def test_autotune_under_the_high_order_kernel(ho, tmp_path, monkeypatch):
+    """The autotuner times the high-order kernels and keys its result by
+    them (family 4 = FP64, 5 = FP32C in the tune file, apart from the
+    Gaussian's 1 and 2); the tuned launches keep the oracle's bars."""
+    path = tmp_path / "tune.txt"
+    monkeypatch.setenv("PAWAN_TUNE_CACHE", str(path))
+    s = W.batched_rings(nwakes=5, nmin=300, nmax=2500, seed=23)
+    ref = [ho.solve(s.wake(k)) for k in range(s.nw)]
+    with _solver(s) as g:
+        for prec, fam, bar in (("fp64", 4, (FP64_TOL, 2e-11)), ("fp32c", 5, (1e-6, 1e-5))):
+            ms = g.autotune(prec)
+            u, w = g.solve(prec)
+            _say(f"high-order {prec} tuned: {g.pair_config(prec)} {ms:.3f} ms")
+            for k in range(s.nw):
+                a, b = s.off[k], s.off[k + 1]
+                assert normwise(u[a:b], ref[k][0]) < bar[0]
+                assert normwise(w[a:b], ref[k][1]) < bar[1]
+            lines = [l.split() for l in path.read_text().splitlines()]
+            assert any(int(l[0]) == fam and int(l[1]) == s.n for l in lines), lines
+        g.set_kernel(capi.PW_KERNEL_GAUSSIAN)
+        g.autotune("fp32c")  # same shape, other kernel: a search of its own
+    lines = [l.split() for l in path.read_text().splitlines()]
+    assert sorted(int(l[0]) for l in lines if int(l[1]) == s.n) == [2, 4, 5]
+
+
+def test_bad_kernel_is_an_error():
     s = W.ring_1k()
     with _solver(s) as g:
-        for prec in ("learned",):
-            with pytest.raises(capi.PawanError) as e:
-                g.solve(prec)
-            assert e.value.code == capi.PW_ERR_UNSUPPORTED
-        with pytest.raises(capi.PawanError) as e:
-            g.autotune("fp64")
-        assert e.value.code == capi.PW_ERR_UNSUPPORTED
         with pytest.raises(capi.PawanError) as e:
             g.set_kernel(7)
         assert e.value.code == capi.PW_ERR_ARG

@@ -91,9 +91,10 @@ def test_bad_rings_are_errors():
 
 @pytest.mark.parametrize("nl,nazi,sig", [(8, 16, 0.05), (15, 8, 0.02)])
 def test_resolve_rings_beyond_one_block(oracle, nl, nazi, sig):
-    """m = 289 and 961 cells, more rows than the LU block's 256 lanes, so
-    every lane handles several rows in the pivot search, the trailing update
-    and both triangular solves (condition numbers 5e5 and 2e3)."""
+    """m = 289 and 961 cells: factored column by column over every CU
+    (k_lu_pivot, k_lu_update), each lane of the pivot search and of the
+    triangular solves handling several rows (condition numbers 5e5 and
+    2e3)."""
     pos, radius, volume = capi.vring_particles(R, CORE, nl, nazi, sig)
     _say(f"vring L={nl} nAzi={nazi} ({(2 * nl + 1) ** 2} cells): {len(pos)} particles")
     with capi.Solver() as g:
