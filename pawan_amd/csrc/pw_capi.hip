@@ -261,16 +261,19 @@ int kind_of(int precision, int* kind) {
 // block (parts == 1: one item per block writing its rates directly).
 // Split points fall on tile boundaries of the block's system, so the
 // diagonal tile (and its self pairs) stays inside one part.  Items are
-// ordered by cost, largest first.
+// ordered by cost, largest first, and among equal costs part by part: the
+// items in flight at any time then read one source range, which the L2 of
+// every XCD holds (a part of a 1M-particle wake split 16 ways is 3 MB of
+// FP32 source records), instead of every range at once.
 std::vector<WorkItem> make_items(const pw_ctx* c, int parts) {
   std::vector<WorkItem> items;
   for (const auto& s : c->systems) {
     const int64_t ntile = (s.b - s.a + kTargetsPerItem - 1) / kTargetsPerItem;
-    for (int64_t t0 = s.a; t0 < s.b; t0 += kTargetsPerItem) {
-      const int32_t t1 = static_cast<int32_t>(std::min<int64_t>(t0 + kTargetsPerItem, s.b));
-      for (int p = 0; p < parts; ++p) {
-        const int64_t a = s.a + kTargetsPerItem * (ntile * p / parts);
-        const int64_t b = std::min<int64_t>(s.b, s.a + kTargetsPerItem * (ntile * (p + 1) / parts));
+    for (int p = 0; p < parts; ++p) {
+      const int64_t a = s.a + kTargetsPerItem * (ntile * p / parts);
+      const int64_t b = std::min<int64_t>(s.b, s.a + kTargetsPerItem * (ntile * (p + 1) / parts));
+      for (int64_t t0 = s.a; t0 < s.b; t0 += kTargetsPerItem) {
+        const int32_t t1 = static_cast<int32_t>(std::min<int64_t>(t0 + kTargetsPerItem, s.b));
         items.push_back({static_cast<int32_t>(t0), t1, static_cast<int32_t>(a),
                          static_cast<int32_t>(std::max(a, b)), parts == 1 ? -1 : p, 0, chunk_of(c, a), 0});
       }
@@ -1444,6 +1447,13 @@ int pw_autotune(pw_ctx* c, int precision, double* best_ms) {
   const int64_t resident = static_cast<int64_t>(c->num_cus) * 8;
   for (int parts = 16; parts <= 64; parts *= 2)
     if (blocks * parts <= 8 * resident) add(parts, true, f32);
+  // finer source ranges for the scalar-fed kernel at any size: with the
+  // items ordered part by part, a part of 1/16 or 1/32 of a large system
+  // stays in every XCD's L2 while its items run
+  if (f32) {
+    add(16, true, true, true);
+    add(32, true, true, true);
+  }
   pw::PairConfig best_cfg = cands.front();
   double best = 1e300;
   for (const auto& cand : cands) {

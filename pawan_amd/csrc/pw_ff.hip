@@ -36,7 +36,7 @@ void free_leaf_buffers(pwi::FarField& f) {
                   static_cast<void*>(tb.near_cnt), static_cast<void*>(tb.far_list), static_cast<void*>(tb.far_cnt),
                   static_cast<void*>(tb.near_parts), static_cast<void*>(tb.near_offs),
                   static_cast<void*>(tb.near_items), static_cast<void*>(tb.near_partial),
-                  static_cast<void*>(tb.scan_temp)})
+                  static_cast<void*>(tb.leaf_chunks), static_cast<void*>(tb.scan_temp)})
     (void)hipFree(p);
   tb.leaf_first = nullptr;
   tb.nodes = nullptr;
@@ -44,6 +44,7 @@ void free_leaf_buffers(pwi::FarField& f) {
   tb.near_parts = tb.near_offs = nullptr;
   tb.near_items = nullptr;
   tb.near_partial = nullptr;
+  tb.leaf_chunks = nullptr;
   tb.scan_temp = nullptr;
   tb.leaf_cap = tb.node_cap = tb.cap_near = tb.cap_far = 0;
 }
@@ -72,6 +73,7 @@ int ensure_leaf_capacity(pw_ctx* c, int nleaf) {
       dalloc(&tb.near_parts, nl) != hipSuccess || dalloc(&tb.near_offs, nl) != hipSuccess ||
       dalloc(&tb.near_items, nl * pw::kNearMaxParts) != hipSuccess ||
       dalloc(&tb.near_partial, nl * pw::kNearMaxParts * pw::kLeafSize) != hipSuccess ||
+      dalloc(&tb.leaf_chunks, nl * pw::kLeafChunks) != hipSuccess ||
       pw::near_scan_temp_bytes(cap, &tb.scan_temp_bytes) != hipSuccess ||
       dalloc(reinterpret_cast<unsigned char**>(&tb.scan_temp), tb.scan_temp_bytes) != hipSuccess) {
     (void)hipGetLastError();
@@ -276,6 +278,7 @@ void ff_release(pw_ctx* c) {
   (void)hipFree(tb.near_total);
   (void)hipFree(tb.near_queue);
   (void)hipFree(tb.near_partial);
+  (void)hipFree(tb.leaf_chunks);
   (void)hipFree(tb.scan_temp);
   (void)hipFree(tb.leaf_first);
   (void)hipFree(tb.cell_start);

@@ -206,6 +206,16 @@ constexpr int kHistNeed = 13;     // hist slots of the longest near and far list
 constexpr size_t kListLdsMax = 128 * 1024;  // LDS of the list kernel's level buffers at most
 constexpr int kNearChunk = 128;   // near leaves per work item
 constexpr int kNearMaxParts = 16; // work items per leaf at most
+constexpr int kLeafChunks = kLeafSize / 64;  // 64-particle runs of a leaf
+
+// Bounding sphere of a leaf's 64-particle run (runs from the leaf's first
+// particle; r < 0 for a run past the leaf's end): the near kernel tests its
+// 64-target slices against the sources' runs with these.
+struct alignas(16) LeafChunk {
+  double cx, cy, cz;
+  float r;
+  float pad;
+};
 
 struct TreeBuffers {
   double* bbox_part = nullptr;  // 6 x 1024 block extremes
@@ -243,6 +253,7 @@ struct TreeBuffers {
   int* near_total = nullptr;
   int* near_queue = nullptr;
   Rate64* near_partial = nullptr;  // item x 256 partial rates
+  LeafChunk* leaf_chunks = nullptr;  // leaf x kLeafChunks run spheres
   void* scan_temp = nullptr;
   size_t scan_temp_bytes = 0;
 };

@@ -507,6 +507,74 @@ __device__ __forceinline__ void near32(float r2, float sig2, float nu2, float& q
   c = nu2 * kKf * e * isg3 * isg2;  // 2 nu Z / sigma^2
 }
 
+// The Gaussian core without branches, for the tree path's near field:
+//   G(s) = exp(-s/2) B(s),  q' = sqrt(2/pi) (e/3 + s G) / sigma^3,
+//   f' = sqrt(2/pi) G / sigma^5,
+// the same series as near32 with the exponential factored, so one function
+// serves every lane whatever its s (near32 runs both of its branches when a
+// wave's pairs straddle s = 4).  G comes from a table of 32 pieces of degree
+// 5 in t = s - s_c on 0 <= s <= 44 (rows (c0 .. c5, s_c, 0); a Chebyshev
+// fit, fit_core_table.py beside this file; largest relative error of this
+// FP32 evaluation 1.3e-7), staged in LDS and read per lane.
+constexpr int kCorePieces = 32;
+constexpr float kCoreScale = 0x1.74539p-1f;  // kCorePieces / (44 * 1.0001) in FP32
+__constant__ float kCoreTable[32][8] = {
+    {0x1.ac6372p-5f, -0x1.2b22b2p-6f, 0x1.caa97p-9f, -0x1.ef67e8p-12f, 0x1.a2051ap-15f, -0x1.200b42p-18f, 0x1.600902p-1f, 0.0f},
+    {0x1.0c2748p-5f, -0x1.640e14p-7f, 0x1.085a3ap-9f, -0x1.174eap-12f, 0x1.cfd25p-16f, -0x1.3bc1ecp-19f, 0x1.0806c2p+1f, 0.0f},
+    {0x1.5812fcp-6f, -0x1.af4d8ap-8f, 0x1.34a898p-10f, -0x1.3e034ap-13f, 0x1.03443cp-16f, -0x1.5c391cp-20f, 0x1.b80b44p+1f, 0.0f},
+    {0x1.c54018p-7f, -0x1.0a513ep-8f, 0x1.6d920ap-11f, -0x1.6e037ap-14f, 0x1.2448f6p-17f, -0x1.82966cp-21f, 0x1.3407e2p+2f, 0.0f},
+    {0x1.32d212p-7f, -0x1.4fd76ep-9f, 0x1.b7d7d8p-12f, -0x1.aa555ep-15f, 0x1.4c9486p-18f, -0x1.b0583cp-22f, 0x1.8c0a24p+2f, 0.0f},
+    {0x1.ab16d2p-8f, -0x1.b1106ap-10f, 0x1.0d2f6ap-12f, -0x1.f730d6p-16f, 0x1.7e550ap-19f, -0x1.e7756p-23f, 0x1.e40c64p+2f, 0.0f},
+    {0x1.31829ep-8f, -0x1.1dcbd8p-10f, 0x1.4fa2aap-13f, -0x1.2d44bep-16f, 0x1.bc844ap-20f, -0x1.1544aap-23f, 0x1.1e0752p+3f, 0.0f},
+    {0x1.c0c2d8p-9f, -0x1.82508ep-11f, 0x1.aac9c4p-14f, -0x1.6e6efp-17f, 0x1.059ea4p-20f, -0x1.3e8972p-24f, 0x1.4a0872p+3f, 0.0f},
+    {0x1.51dcb8p-9f, -0x1.0b663cp-11f, 0x1.14f86cp-14f, -0x1.c53874p-18f, 0x1.381cbep-21f, -0x1.71e6cp-25f, 0x1.760994p+3f, 0.0f},
+    {0x1.043f02p-9f, -0x1.7af134p-12f, 0x1.6f2284p-15f, -0x1.1d4a2ap-18f, 0x1.79c52cp-22f, -0x1.b298e6p-26f, 0x1.a20ab4p+3f, 0.0f},
+    {0x1.995076p-10f, -0x1.129e06p-12f, 0x1.f11666p-16f, -0x1.6ddbaap-19f, 0x1.d056a6p-23f, -0x1.028bdep-26f, 0x1.ce0bd4p+3f, 0.0f},
+    {0x1.47eb32p-10f, -0x1.9694a6p-13f, 0x1.57aa7ap-16f, -0x1.de29ep-20f, 0x1.220ad2p-23f, -0x1.37cf54p-27f, 0x1.fa0cf4p+3f, 0.0f},
+    {0x1.0b1612p-10f, -0x1.32ff0ep-13f, 0x1.e504a6p-17f, -0x1.3e85f4p-20f, 0x1.70838cp-24f, -0x1.7d7a74p-28f, 0x1.13070ap+4f, 0.0f},
+    {0x1.b97e02p-11f, -0x1.d82138p-14f, 0x1.5d09f6p-17f, -0x1.b08bf2p-21f, 0x1.dc688ep-25f, -0x1.d9cb62p-29f, 0x1.29079ap+4f, 0.0f},
+    {0x1.71a59cp-11f, -0x1.7120cap-14f, 0x1.ffc8c6p-18f, -0x1.2b40b6p-21f, 0x1.3967b4p-25f, -0x1.2adc86p-29f, 0x1.3f082ap+4f, 0.0f},
+    {0x1.390f86p-11f, -0x1.24fafp-14f, 0x1.7dc96ap-18f, -0x1.a5a7f4p-22f, 0x1.a3a6d2p-26f, -0x1.7f210cp-30f, 0x1.5508bap+4f, 0.0f},
+    {0x1.0bd8dp-11f, -0x1.d775fcp-15f, 0x1.217112p-18f, -0x1.2e4498p-22f, 0x1.1dde18p-26f, -0x1.f334aap-31f, 0x1.6b094ap+4f, 0.0f},
+    {0x1.ce7eeap-12f, -0x1.80072ep-15f, 0x1.bd710ap-19f, -0x1.b88dd6p-23f, 0x1.8c191p-27f, -0x1.4a8fa6p-31f, 0x1.8109dcp+4f, 0.0f},
+    {0x1.928bc2p-12f, -0x1.3c4d5p-15f, 0x1.5b790ap-19f, -0x1.460dp-23f, 0x1.16eb5ep-27f, -0x1.bce2ecp-32f, 0x1.970a6cp+4f, 0.0f},
+    {0x1.60ec86p-12f, -0x1.072606p-15f, 0x1.12763cp-19f, -0x1.e99f5ep-24f, 0x1.8ef722p-28f, -0x1.30221ep-32f, 0x1.ad0afcp+4f, 0.0f},
+    {0x1.37747ep-12f, -0x1.b9db6p-16f, 0x1.b692d8p-20f, -0x1.749514p-24f, 0x1.2192a2p-28f, -0x1.a63a1p-33f, 0x1.c30b8cp+4f, 0.0f},
+    {0x1.147f42p-12f, -0x1.760b9cp-16f, 0x1.621898p-20f, -0x1.1f0e8ep-24f, 0x1.aa382ap-29f, -0x1.296c72p-33f, 0x1.d90c1cp+4f, 0.0f},
+    {0x1.ed966p-13f, -0x1.3f09a2p-16f, 0x1.20a5a6p-20f, -0x1.bf6d4ep-25f, 0x1.3dc722p-29f, -0x1.a8ee76p-34f, 0x1.ef0cacp+4f, 0.0f},
+    {0x1.babe74p-13f, -0x1.1200ccp-16f, 0x1.dac0fep-21f, -0x1.6068bep-25f, 0x1.dfa46ep-30f, -0x1.339c5ap-34f, 0x1.02869ep+5f, 0.0f},
+    {0x1.8ef0c6p-13f, -0x1.d9a3e6p-17f, 0x1.899a7ap-21f, -0x1.184d2cp-25f, 0x1.6e1928p-30f, -0x1.c2fb7ep-35f, 0x1.0d86e6p+5f, 0.0f},
+    {0x1.68f8d2p-13f, -0x1.9bc24cp-17f, 0x1.48c584p-21f, -0x1.c1f48cp-26f, 0x1.1a6512p-30f, -0x1.4e82d4p-35f, 0x1.18872ep+5f, 0.0f},
+    {0x1.47e03cp-13f, -0x1.67e502p-17f, 0x1.148598p-21f, -0x1.6c3146p-26f, 0x1.b7f4d8p-31f, -0x1.f5c3a6p-36f, 0x1.238776p+5f, 0.0f},
+    {0x1.2ae08ap-13f, -0x1.3c229p-17f, 0x1.d423ap-22f, -0x1.291578p-26f, 0x1.59dd22p-31f, -0x1.7c3f62p-36f, 0x1.2e87bep+5f, 0.0f},
+    {0x1.115882p-13f, -0x1.16fc2p-17f, 0x1.8ea22cp-22f, -0x1.e835f2p-27f, 0x1.12394cp-31f, -0x1.22f9b6p-36f, 0x1.398806p+5f, 0.0f},
+    {0x1.f5887p-14f, -0x1.ee873p-18f, 0x1.55555p-22f, -0x1.93decap-27f, 0x1.b652bap-32f, -0x1.c1667ep-37f, 0x1.44884ep+5f, 0.0f},
+    {0x1.cd6dd6p-14f, -0x1.b81126p-18f, 0x1.25c8f4p-22f, -0x1.50377p-27f, 0x1.60ee16p-32f, -0x1.5e04acp-37f, 0x1.4f8896p+5f, 0.0f},
+    {0x1.a9accep-14f, -0x1.891494p-18f, 0x1.fc2c6ap-23f, -0x1.198e14p-27f, 0x1.1e29a8p-32f, -0x1.12ceeap-37f, 0x1.5a88dep+5f, 0.0f},
+};
+
+__device__ __forceinline__ void near32t(float r2, float sig2, float nu2, const float4* __restrict__ tab,
+                                        float& qp, float& fp, float& c) {
+  const float isg = __builtin_amdgcn_rsqf(sig2);
+  const float isg2 = isg * isg;
+  const float isg3 = isg2 * isg;
+  const float sb = r2 * isg2;
+  const float e = __builtin_amdgcn_exp2f(kMHalfLog2e * sb);
+  // fminf keeps the index in the table for any sb (NaN included)
+  const int k = static_cast<int>(fminf(sb * kCoreScale, static_cast<float>(kCorePieces - 1)));
+  const float4 lo = tab[2 * k], hi = tab[2 * k + 1];  // (c0, c1, c2, c3), (c4, c5, s_c, 0)
+  const float t = sb - hi.z;
+  float g = fmaf(hi.y, t, hi.x);
+  g = fmaf(g, t, lo.w);
+  g = fmaf(g, t, lo.z);
+  g = fmaf(g, t, lo.y);
+  g = fmaf(g, t, lo.x);
+  const float ke = kFourPiK * isg3;
+  qp = ke * fmaf(sb, g, e * 0x1.555556p-2f);
+  fp = ke * g * isg2;
+  c = nu2 * kKf * e * isg3 * isg2;  // 2 nu Z / sigma^2
+}
+
 // One pair's velocity and stretching into the partial sums (v = a_s x d),
 // the stretching factored (A and B).  Used where the source is beyond the
 // Gaussian core of every lane's target: there q'|a_s| is small against the
@@ -1157,21 +1225,23 @@ struct NearArgs {
   const int* total;  // items in use (device)
   int* queue;        // slices handed out
   float nu;
-  float gap2;  // 44 sigma_max^2: leaves further apart take the far-only loop
+  float gap2;  // 44 sigma_max^2: leaves (or runs) further apart take the far-only loop
   Rate64* out;
   Rate64* partial;  // item x 256
+  const LeafChunk* chunks;  // leaf x kLeafChunks run spheres
 };
 
 constexpr int kNearSlice = 64;                       // targets per wave
 static_assert(kTB == kLeafSize, "partial-sum rows hold one leaf");
 constexpr int kNearSlices = kLeafSize / kNearSlice;  // slices per leaf at most
+static_assert(kNearSlices == kLeafChunks && kNearSlice == 64, "a slice is one 64-particle run of its leaf");
 
 // pair on relative FP32 positions, Gaussian core included (HighOrder: the
 // algebraic kernel, exact for every pair, PSE within 44 sigma^2 as in
 // pair32_ho)
 template <bool HighOrder>
 __device__ __forceinline__ void pair32_rel_core(float tx, float ty, float tz, const T32& t, const SrcRel& s,
-                                                P32& p, PSE32& e) {
+                                                const float4* __restrict__ tab, P32& p, PSE32& e) {
   const float dx = tx - s.rx;
   const float dy = ty - s.ry;
   const float dz = tz - s.rz;
@@ -1196,7 +1266,7 @@ __device__ __forceinline__ void pair32_rel_core(float tx, float ty, float tz, co
   if (core) {
     float c;
     if (HighOrder) c = t.nu2 * kHoEta32 * (sig2 * sig2) * (ib7 * ib2);
-    else near32(r2, sig2, t.nu2, q, f, c);
+    else near32t(r2, sig2, t.nu2, tab, q, f, c);
     e.x = fmaf(c, t.vol * s.ax - s.vol * t.ax, e.x);
     e.y = fmaf(c, t.vol * s.ay - s.vol * t.ay, e.y);
     e.z = fmaf(c, t.vol * s.az - s.vol * t.az, e.z);
@@ -1209,12 +1279,14 @@ __device__ __forceinline__ void pair32_rel_core(float tx, float ty, float tz, co
 }
 
 template <bool HighOrder>
-__device__ __forceinline__ void near_slice(const NearArgs& a, const NearItem& it, int slice, SrcRel* tile) {
+__device__ __forceinline__ void near_slice(const NearArgs& a, const NearItem& it, int slice, SrcRel* tile,
+                                           const float4* __restrict__ tab) {
   const int T = it.T;
   const int lane = static_cast<int>(threadIdx.x);
   const int t0 = a.nodes[T].first, t1 = a.nodes[T].last;
   const double tcx = a.nodes[T].cx, tcy = a.nodes[T].cy, tcz = a.nodes[T].cz;
   const float tr = a.nodes[T].r;
+  const LeafChunk ts = a.chunks[kLeafChunks * T + slice];  // this slice's targets
   const int self = kNearSlice * slice + lane;  // index of this lane's target in T
   const int t = t0 + self;
   const bool active = t < t1;
@@ -1267,7 +1339,15 @@ __device__ __forceinline__ void near_slice(const NearArgs& a, const NearItem& it
       }
       __syncthreads();
       if (!active) continue;
-      if (all_far) {
+      // a run of the leaf beyond the core of every target of the slice
+      bool sep = all_far;
+      if (!sep) {
+        const LeafChunk& cs = a.chunks[kLeafChunks * id + (c0 >> 6)];
+        const double ex = cs.cx - ts.cx, ey = cs.cy - ts.cy, ez = cs.cz - ts.cz;
+        const float g = static_cast<float>(sqrt(ex * ex + ey * ey + ez * ez)) - ts.r - cs.r;
+        sep = g > 0.0f && g * g > a.gap2 * 1.0001f;
+      }
+      if (sep) {
         P32S q;
 #pragma unroll 2
         for (int l = 0; l < nc; ++l)
@@ -1279,10 +1359,10 @@ __device__ __forceinline__ void near_slice(const NearArgs& a, const NearItem& it
       if (diag) {
         const int own = self - c0;  // this target's slot in the tile, if there
         for (int l = 0; l < nc; ++l)
-          if (l != own) pair32_rel_core<HighOrder>(rx, ry, rz, me, tile[l], p, pse);
+          if (l != own) pair32_rel_core<HighOrder>(rx, ry, rz, me, tile[l], tab, p, pse);
       } else {
 #pragma unroll 1
-        for (int l = 0; l < nc; ++l) pair32_rel_core<HighOrder>(rx, ry, rz, me, tile[l], p, pse);
+        for (int l = 0; l < nc; ++l) pair32_rel_core<HighOrder>(rx, ry, rz, me, tile[l], tab, p, pse);
       }
       fold(p, me.ax, me.ay, me.az, hi, lo);
     }
@@ -1304,7 +1384,14 @@ __device__ __forceinline__ void near_slice(const NearArgs& a, const NearItem& it
 }
 
 template <bool HighOrder>
-__device__ __forceinline__ void near_entry(const NearArgs& a, SrcRel* tile) {
+__device__ __forceinline__ void near_entry(const NearArgs& a, SrcRel* tile, float4* tab) {
+  if (!HighOrder) {  // the core table into LDS (one 64-lane block: a float4 per lane)
+    static_assert(2 * kCorePieces == kNearSlice, "one table row half per lane");
+    const int l = static_cast<int>(threadIdx.x);
+    const float* row = &kCoreTable[l >> 1][4 * (l & 1)];
+    tab[l] = make_float4(row[0], row[1], row[2], row[3]);
+    __syncthreads();
+  }
   const int slices = kNearSlices * *a.total;
   for (;;) {
     int idx = 0;
@@ -1314,17 +1401,18 @@ __device__ __forceinline__ void near_entry(const NearArgs& a, SrcRel* tile) {
     const NearItem it = a.items[idx / kNearSlices];
     const int slice = idx % kNearSlices;
     if (a.nodes[it.T].first + kNearSlice * slice >= a.nodes[it.T].last) continue;
-    near_slice<HighOrder>(a, it, slice, tile);
+    near_slice<HighOrder>(a, it, slice, tile, tab);
   }
 }
 
 __global__ __launch_bounds__(kNearSlice) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_near_fp32c(NearArgs a) {
   __shared__ SrcRel tile[kNearSlice];
-  near_entry<false>(a, tile);
+  __shared__ float4 tab[2 * kCorePieces];
+  near_entry<false>(a, tile, tab);
 }
 __global__ __launch_bounds__(kNearSlice) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_near_fp32c_ho(NearArgs a) {
   __shared__ SrcRel tile[kNearSlice];
-  near_entry<true>(a, tile);
+  near_entry<true>(a, tile, nullptr);
 }
 
 // rates of a split leaf: sum of its items' partial sums, in item order
@@ -1577,7 +1665,7 @@ hipError_t launch_near_fp32c(const Rec64* sorted, const NodeRec* nodes, const Tr
   hipError_t e = launch_fill_int(tb.near_queue, 1, 0, st);
   if (e != hipSuccess) return e;
   NearArgs a{sorted, nodes, tb.near_list, tb.cap_near, tb.near_items, tb.near_total, tb.near_queue,
-             static_cast<float>(nu), kFar32 * sigma_max2, rates_s, tb.near_partial};
+             static_cast<float>(nu), kFar32 * sigma_max2, rates_s, tb.near_partial, tb.leaf_chunks};
   if (highorder) hipLaunchKernelGGL(k_near_fp32c_ho, dim3(grid), dim3(kNearSlice), 0, st, a);
   else hipLaunchKernelGGL(k_near_fp32c, dim3(grid), dim3(kNearSlice), 0, st, a);
   e = hipGetLastError();

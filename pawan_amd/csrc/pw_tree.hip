@@ -438,6 +438,38 @@ __global__ __launch_bounds__(256) void k_near_emit(const int* __restrict__ near_
   if (T == nleaf - 1) *total = o + P;
 }
 
+// Bounding sphere of every 64-particle run of every leaf, one wave per run:
+// the centre of the run's bounding box and the largest distance from it
+// (FP64, rounded up to FP32).
+__device__ __forceinline__ double wave_reduce_min(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_reduce_max(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+__global__ __launch_bounds__(64) void k_leaf_chunks(const Rec64* __restrict__ sorted,
+                                                    const NodeRec* __restrict__ nodes,
+                                                    LeafChunk* __restrict__ out) {
+  const int L = blockIdx.x / kLeafChunks, c = blockIdx.x % kLeafChunks;
+  const int first = nodes[L].first + 64 * c, last = nodes[L].last;
+  const int lane = static_cast<int>(threadIdx.x);
+  if (first >= last) {  // the same for every lane
+    if (lane == 0) out[blockIdx.x] = LeafChunk{0.0, 0.0, 0.0, -1.0f, 0.0f};
+    return;
+  }
+  const int i = first + lane < last ? first + lane : first;
+  const double x = sorted[i].x, y = sorted[i].y, z = sorted[i].z;
+  const double cx = 0.5 * (wave_reduce_min(x) + wave_reduce_max(x));
+  const double cy = 0.5 * (wave_reduce_min(y) + wave_reduce_max(y));
+  const double cz = 0.5 * (wave_reduce_min(z) + wave_reduce_max(z));
+  const double d2 = (x - cx) * (x - cx) + (y - cy) * (y - cy) + (z - cz) * (z - cz);
+  const double r = sqrt(wave_reduce_max(d2));
+  if (lane == 0) out[blockIdx.x] = LeafChunk{cx, cy, cz, static_cast<float>(r * (1.0 + 1e-6)), 0.0f};
+}
+
 // Per target leaf: near pairs (targets x near sources, self pairs
 // excluded), far evaluations (targets x far nodes), and the near pairs of
 // listed leaves within the Gaussian core range (bounding spheres at most
@@ -488,6 +520,10 @@ hipError_t build_near_items(const TreeBuffers& tb, int nleaf, hipStream_t st) {
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_near_emit, dim3(nb), dim3(256), 0, st, tb.near_cnt, tb.near_parts, tb.near_offs, nleaf,
                      tb.near_items, tb.near_total);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_leaf_chunks, dim3(nleaf * kLeafChunks), dim3(64), 0, st, tb.sorted, tb.nodes,
+                     tb.leaf_chunks);
   return hipGetLastError();
 }
 

@@ -159,20 +159,29 @@ def test_fp32c_separated_tiles_on_the_helix(ho, feed):
 def test_autotune_under_the_high_order_kernel(ho, tmp_path, monkeypatch):
     """The autotuner times the high-order kernels and keys its result by
     them (family 4 = FP64, 5 = FP32C in the tune file, apart from the
-    Gaussian's 1 and 2); the tuned launches keep the oracle's bars."""
+    Gaussian's 1 and 2); the tuned launches keep the oracle's bars (for the
+    FP32C vorticity rate of these thin rings, the conditioning-scaled bar
+    of the batched Gaussian test: 8 x 2^29 times the wake's FP64 error,
+    1e-5 at least)."""
     path = tmp_path / "tune.txt"
     monkeypatch.setenv("PAWAN_TUNE_CACHE", str(path))
     s = W.batched_rings(nwakes=5, nmin=300, nmax=2500, seed=23)
     ref = [ho.solve(s.wake(k)) for k in range(s.nw)]
+    ew64 = np.zeros(s.nw)
     with _solver(s) as g:
-        for prec, fam, bar in (("fp64", 4, (FP64_TOL, 2e-11)), ("fp32c", 5, (1e-6, 1e-5))):
+        for prec, fam in (("fp64", 4), ("fp32c", 5)):
             ms = g.autotune(prec)
             u, w = g.solve(prec)
             _say(f"high-order {prec} tuned: {g.pair_config(prec)} {ms:.3f} ms")
             for k in range(s.nw):
                 a, b = s.off[k], s.off[k + 1]
-                assert normwise(u[a:b], ref[k][0]) < bar[0]
-                assert normwise(w[a:b], ref[k][1]) < bar[1]
+                eu, ew = normwise(u[a:b], ref[k][0]), normwise(w[a:b], ref[k][1])
+                _say(f"  wake {k} n={b - a}: e_u={eu:.2e} e_w={ew:.2e}")
+                if prec == "fp64":
+                    assert eu < FP64_TOL and ew < 2e-11
+                    ew64[k] = ew
+                else:
+                    assert eu < 1e-6 and ew < max(1e-5, 8.0 * 2.0 ** 29 * ew64[k])
             lines = [l.split() for l in path.read_text().splitlines()]
             assert any(int(l[0]) == fam and int(l[1]) == s.n for l in lines), lines
         g.set_kernel(capi.PW_KERNEL_GAUSSIAN)
