@@ -1447,12 +1447,12 @@ int pw_autotune(pw_ctx* c, int precision, double* best_ms) {
   const int64_t resident = static_cast<int64_t>(c->num_cus) * 8;
   for (int parts = 16; parts <= 64; parts *= 2)
     if (blocks * parts <= 8 * resident) add(parts, true, f32);
-  // finer source ranges for the scalar-fed kernel at any size: with the
-  // items ordered part by part, a part of 1/16 or 1/32 of a large system
-  // stays in every XCD's L2 while its items run
-  if (f32) {
-    add(16, true, true, true);
-    add(32, true, true, true);
+  // finer source ranges at any size: with the items ordered part by part,
+  // a part of 1/16 or 1/32 of a large system stays in every XCD's L2 while
+  // its items run
+  for (int parts = 16; parts <= 32; parts *= 2) {
+    if (blocks * parts > 8 * resident) add(parts, true, f32);
+    if (f32) add(parts, true, true, true);
   }
   pw::PairConfig best_cfg = cands.front();
   double best = 1e300;
