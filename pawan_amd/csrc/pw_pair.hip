@@ -208,8 +208,86 @@ __device__ __forceinline__ void pair64(const T64& t, const Rec64& s, AccF64& a) 
   }
 }
 
-__device__ __forceinline__ void item_fp64(const PairArgs& pa, const WorkItem& it, Rec64* tile) {
+// Wave reductions and block-uniform values.
+__device__ __forceinline__ float wave_max(float v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ float wave_min(float v) {
+  for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_maxd(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_mind(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+  return v;
+}
+// a block-uniform value held in scalar registers
+__device__ __forceinline__ float uniform(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ double uniform(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane(static_cast<int>(b & 0xffffffff));
+  const int hi = __builtin_amdgcn_readfirstlane(static_cast<int>(b >> 32));
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
+// Separated source ranges of the FP64 kernel.  Positions relative to the
+// block's targets' centre c_T (t' = x_t - c_T, r_s = x_s - c_T, FP64), with
+// m_s = a_s x r_s staged with each source and c_t = t' x a_t per target:
+//   sum q' (a_s x d) = A x t' - M,   a_t.(a_s x d) = a_s.c_t - a_t.m_s,
+// A = sum q' a_s, M = sum q' m_s, B = sum f' (a_t.(a_s x d)) d, with
+// q' = 1/rho^3 and f' = q'/rho^2 (the singular law; the range is beyond
+// 90 sigma^2 of every pair, where the Gaussian terms are below FP64
+// rounding), and u = (A x t' - M)/4pi, w = (a_t x A - 3 B)/4pi.  The
+// subtractions cancel at most the distances to c_T over the pair
+// distance, which the separation test bounds by kSepFP64: 2^-53 x 16 of
+// each term.
+constexpr double kSepFP64 = 16.0;
+
+struct TileF64 {
+  Rec64 rec;             // the state record (the core-capable pairs)
+  double rx, ry, rz;     // x - c_T
+  double mx, my, mz;     // a x (x - c_T)
+};
+
+struct P64S {
+  double ax = 0, ay = 0, az = 0;  // A
+  double mx = 0, my = 0, mz = 0;  // M
+  double bx = 0, by = 0, bz = 0;  // B
+};
+
+__device__ __forceinline__ void pair64s(double tx, double ty, double tz, double tax, double tay, double taz,
+                                        double cx, double cy, double cz, const TileF64& s, P64S& p) {
+  const double dx = tx - s.rx, dy = ty - s.ry, dz = tz - s.rz;
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  const double ir = rsqrt_pos(r2);
+  const double ir2 = ir * ir;
+  const double q = ir * ir2;
+  const double f = q * ir2;
+  const double sax = s.rec.ax, say = s.rec.ay, saz = s.rec.az;
+  p.ax += q * sax;
+  p.ay += q * say;
+  p.az += q * saz;
+  p.mx += q * s.mx;
+  p.my += q * s.my;
+  p.mz += q * s.mz;
+  const double h = (sax * cx + say * cy + saz * cz) - (tax * s.mx + tay * s.my + taz * s.mz);
+  const double g = f * h;
+  p.bx += g * dx;
+  p.by += g * dy;
+  p.bz += g * dz;
+}
+
+__device__ __forceinline__ void item_fp64(const PairArgs& pa, const WorkItem& it, TileF64* tile) {
+  __shared__ double s_tb[kTB / 64][6];  // each wave's targets' box
+  __shared__ double s_b[kTB / 64][7];   // each staged chunk's box (relative to c_T) and largest sigma^2/2
   const Rec64* __restrict__ state = pa.state;
+  const int lane = static_cast<int>(threadIdx.x) & 63, wave = static_cast<int>(threadIdx.x) >> 6;
   const int t = it.t0 + static_cast<int>(threadIdx.x);
   const bool active = t < it.t1;
   const Rec64 m = state[active ? t : it.t0];
@@ -217,28 +295,103 @@ __device__ __forceinline__ void item_fp64(const PairArgs& pa, const WorkItem& it
   me.x = m.x; me.y = m.y; me.z = m.z; me.hs2 = 0.5 * m.s2;
   me.ax = m.ax; me.ay = m.ay; me.az = m.az; me.vol = m.vol;
   me.nu2 = 2.0 * pa.nu;
+  // the block's and this wave's targets (inactive lanes repeat the first)
+  {
+    const double v[6] = {wave_mind(m.x), wave_mind(m.y), wave_mind(m.z),
+                         wave_maxd(m.x), wave_maxd(m.y), wave_maxd(m.z)};
+    __syncthreads();
+    if (lane == 0)
+      for (int k = 0; k < 6; ++k) s_tb[wave][k] = v[k];
+  }
+  __syncthreads();
+  double cT[3], rT2 = 0.0, cW[3], rW2 = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double lo = fmin(fmin(s_tb[0][k], s_tb[1][k]), fmin(s_tb[2][k], s_tb[3][k]));
+    const double hi = fmax(fmax(s_tb[0][3 + k], s_tb[1][3 + k]), fmax(s_tb[2][3 + k], s_tb[3][3 + k]));
+    cT[k] = uniform(0.5 * (lo + hi));
+    rT2 += 0.25 * (hi - lo) * (hi - lo);
+    const double wl = s_tb[wave][k], wh = s_tb[wave][3 + k];
+    cW[k] = uniform(0.5 * (wl + wh) - cT[k]);
+    rW2 += 0.25 * (wh - wl) * (wh - wl);
+  }
+  const double rT = uniform(sqrt(rT2) * 1.000001), rW = uniform(sqrt(rW2) * 1.000001);
+  const double h2T = uniform(wave_maxd(me.hs2));
+  const double tx = m.x - cT[0], ty = m.y - cT[1], tz = m.z - cT[2];
+  const double ctx = ty * me.az - tz * me.ay, cty = tz * me.ax - tx * me.az, ctz = tx * me.ay - ty * me.ax;
   AccF64 a;
+  P64S sp;
   for (int base = it.s0; base < it.s1; base += kTB) {
     const int cnt = min(kTB, it.s1 - base);
     __syncthreads();
-    if (static_cast<int>(threadIdx.x) < cnt) tile[threadIdx.x] = state[base + threadIdx.x];
+    double bl[3] = {INFINITY, INFINITY, INFINITY}, bh[3] = {-INFINITY, -INFINITY, -INFINITY}, h2 = 0.0;
+    if (static_cast<int>(threadIdx.x) < cnt) {
+      const Rec64 r = state[base + threadIdx.x];
+      TileF64 f;
+      f.rec = r;
+      f.rx = r.x - cT[0];
+      f.ry = r.y - cT[1];
+      f.rz = r.z - cT[2];
+      f.mx = r.ay * f.rz - r.az * f.ry;
+      f.my = r.az * f.rx - r.ax * f.rz;
+      f.mz = r.ax * f.ry - r.ay * f.rx;
+      tile[threadIdx.x] = f;
+      bl[0] = bh[0] = f.rx;
+      bl[1] = bh[1] = f.ry;
+      bl[2] = bh[2] = f.rz;
+      h2 = 0.5 * r.s2;
+    }
+    {
+      const double v[7] = {wave_mind(bl[0]), wave_mind(bl[1]), wave_mind(bl[2]), wave_maxd(bh[0]),
+                           wave_maxd(bh[1]), wave_maxd(bh[2]), wave_maxd(h2)};
+      if (lane == 0)
+        for (int k = 0; k < 7; ++k) s_b[wave][k] = v[k];
+    }
     __syncthreads();
     if (!active) continue;
-    if (base == it.t0) {
-      // the tile holding this block's own targets: skip the self pair
-      const int self = static_cast<int>(threadIdx.x);
-      for (int l = 0; l < cnt; ++l)
-        if (l != self) pair64(me, tile[l], a);
-    } else {
-      for (int l = 0; l < cnt; ++l) pair64(me, tile[l], a);
+    const bool diag = base == it.t0;
+    const int self = static_cast<int>(threadIdx.x);
+    for (int c0 = 0; c0 < cnt; c0 += 64) {
+      const int c1 = min(cnt, c0 + 64);
+      const int cw = c0 / 64;
+      // the chunk's sphere (relative to c_T) against this wave's
+      double cs2 = 0.0, rs2 = 0.0, d2 = 0.0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const double l = s_b[cw][k], h = s_b[cw][3 + k];
+        const double c = 0.5 * (l + h), e = 0.5 * (h - l);
+        cs2 += c * c;
+        rs2 += e * e;
+        d2 += (c - cW[k]) * (c - cW[k]);
+      }
+      const double rS = uniform(sqrt(rs2) * 1.000001);
+      const double gap = uniform(sqrt(d2) * 0.999999 - rW - rS);
+      const bool sep = gap > 0.0 && gap * gap > kFar64 * (h2T + uniform(s_b[cw][6])) * 1.0001 &&
+                       rT + uniform(sqrt(cs2)) + rS <= kSepFP64 * gap;
+      if (sep) {
+        for (int l = c0; l < c1; ++l) pair64s(tx, ty, tz, me.ax, me.ay, me.az, ctx, cty, ctz, tile[l], sp);
+        continue;
+      }
+      const bool own = diag && cw == wave;  // the chunk holding this wave's own targets
+      for (int l = c0; l < c1; ++l)
+        if (!(own && l == self)) pair64(me, tile[l].rec, a);  // the self pair is never evaluated
     }
   }
   if (active) {
+    // separated ranges: u = (A x t' - M)/4pi, w = (a_t x A - 3 B)/4pi
+    const double ux = (sp.ay * tz - sp.az * ty) - sp.mx;
+    const double uy = (sp.az * tx - sp.ax * tz) - sp.my;
+    const double uz = (sp.ax * ty - sp.ay * tx) - sp.mz;
+    const double wx = (me.ay * sp.az - me.az * sp.ay) - 3.0 * sp.bx;
+    const double wy = (me.az * sp.ax - me.ax * sp.az) - 3.0 * sp.by;
+    const double wz = (me.ax * sp.ay - me.ay * sp.ax) - 3.0 * sp.bz;
     Rate64 r;
-    r.ux = a.ux; r.uy = a.uy; r.uz = a.uz;
-    r.wx = (me.ay * a.az - me.az * a.ay) + a.bx;
-    r.wy = (me.az * a.ax - me.ax * a.az) + a.by;
-    r.wz = (me.ax * a.ay - me.ay * a.ax) + a.bz;
+    r.ux = a.ux + kQ * ux;
+    r.uy = a.uy + kQ * uy;
+    r.uz = a.uz + kQ * uz;
+    r.wx = ((me.ay * a.az - me.az * a.ay) + a.bx) + kQ * wx;
+    r.wy = ((me.az * a.ax - me.ax * a.az) + a.by) + kQ * wy;
+    r.wz = ((me.ax * a.ay - me.ay * a.ax) + a.bz) + kQ * wz;
     out_rates(pa, it)[t] = r;
   }
 }
@@ -831,33 +984,6 @@ struct SrcCore {
 // under the same bound.
 constexpr float kSepFP32 = 4.0f;
 
-__device__ __forceinline__ float wave_max(float v) {
-  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
-  return v;
-}
-__device__ __forceinline__ float wave_min(float v) {
-  for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off, 64));
-  return v;
-}
-__device__ __forceinline__ double wave_maxd(double v) {
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-  return v;
-}
-__device__ __forceinline__ double wave_mind(double v) {
-  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
-  return v;
-}
-// a block-uniform value held in scalar registers
-__device__ __forceinline__ float uniform(float v) {
-  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
-}
-__device__ __forceinline__ double uniform(double v) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readfirstlane(static_cast<int>(b & 0xffffffff));
-  const int hi = __builtin_amdgcn_readfirstlane(static_cast<int>(b >> 32));
-  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
-}
-
 // HighOrder: the algebraic kernel (pair32_ho, and pair32_rel_ho on the
 // separated tiles, which need the same gap as the Gaussian's: it keeps the
 // relative positions' rounding small and the PSE terms beyond it are
@@ -1444,6 +1570,10 @@ struct TileOf {
   using type = Rec64;
 };
 template <>
+struct TileOf<kPairFP64> {
+  using type = TileF64;
+};
+template <>
 struct TileOf<kPairFP32C> {
   using type = SrcT2;
 };
@@ -1494,7 +1624,7 @@ __device__ __forceinline__ void pair_entry(const PairArgs& a) {
 template <bool P>
 __global__ __launch_bounds__(kTB) void k_pair_fp64_ref(PairArgs a) { pair_entry<kPairFP64Exact, P>(a); }
 template <bool P>
-__global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_pair_fp64(PairArgs a) {
+__global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_pair_fp64(PairArgs a) {
   pair_entry<kPairFP64, P>(a);
 }
 template <bool P>
