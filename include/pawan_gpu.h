@@ -302,13 +302,15 @@ int pw_get_pair_timing(pw_ctx *ctx, double *total_ms, int64_t *launches);
 /* How the pair kernel of `precision` is launched.  parts: source ranges
  * each 256-target block is split into (partial sums reduced per target in
  * FP64, in a fixed order); persistent: resident blocks pull work items from
- * an atomic queue (largest first) instead of one block per item; variant
- * (FP32C only, ignored for the other precisions), a combination of
- *   PW_VARIANT_WAVES8       the kernel built for 8 waves per SIMD instead
- *                           of 7 (64 vector registers);
- *   PW_VARIANT_SCALAR_FEED  each wave of 64 targets streams the sources
+ * an atomic queue (largest first) instead of one block per item; variant,
+ * a combination of
+ *   PW_VARIANT_WAVES8       (FP32C) the kernel built for 8 waves per SIMD
+ *                           instead of 7 (64 vector registers);
+ *   PW_VARIANT_SCALAR_FEED  (FP32C and PW_FP64 with the Gaussian kernel)
+ *                           each wave of 64 targets streams the sources
  *                           through scalar loads, chunk by chunk, instead
- *                           of the block's four waves sharing LDS tiles.
+ *                           of the block's four waves sharing LDS tiles;
+ * bits that do not apply to the precision are ignored.
  * No counterpart in the reference (its OpenMP loops have no launch shape);
  * defaults are parts 1, grid launch, variant 0.  A variant outside those
  * bits is PW_ERR_ARG. */

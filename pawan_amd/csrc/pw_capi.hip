@@ -92,8 +92,10 @@ void free_buffers(pw_ctx* c) {
   (void)hipFree(c->d_queue);
   (void)hipFree(c->d_chunks);
   (void)hipFree(c->d_src32);
+  (void)hipFree(c->d_src64);
   c->d_chunks = nullptr;
   c->d_src32 = nullptr;
+  c->d_src64 = nullptr;
   c->y = c->ys = nullptr;
   for (auto& p : c->k) p = nullptr;
   c->rates = nullptr;
@@ -297,6 +299,12 @@ void drop_graphs(pw_ctx* c) {
 int ensure_kind(pw_ctx* c, int kind) {
   if (kind == kKindLearned) return PW_OK;  // the tree path sets itself up
   auto& k = c->ks[kind];
+  // the scalar-fed FP64 kernel's source records, allocated before any
+  // step is captured (no allocation may happen inside a capture)
+  if (k.cfg.scalar_feed && kind == pw::kPairFP64 && !c->d_src64 && dalloc(&c->d_src64, c->n) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(PW_ERR_NOMEM, "scalar-fed FP64 source records");
+  }
   if (k.built_parts == k.cfg.parts && k.d_items) return PW_OK;
   std::vector<WorkItem> items = make_items(c, k.cfg.parts);
   PW_HIP(hipStreamSynchronize(c->stream));
@@ -366,7 +374,7 @@ int rhs(pw_ctx* c, int kind, const Rec64* state, Rate64* out) {
   if (kind == pw::kPairFP32C && !c->compensated && c->smoothing == PW_KERNEL_GAUSSIAN) {
     // plain FP32 sums: one item per 256-target block over all its sources
     pw::PairArgs args{state, c->pair_items, c->npair, static_cast<int>(c->n), out, nullptr, c->d_queue, c->nu,
-                      c->d_chunks, c->d_src32, c->nchunks};
+                      c->d_chunks, c->d_src32, nullptr, c->nchunks};
     PW_HIP(pw::launch_pair(pw::kPairFP32CPlain, pw::PairConfig{}, args, c->npair, c->stream));
     PW_HIP(pw::launch_check_finite(out, static_cast<int>(c->n), c->d_flag, c->stream));
     return PW_OK;
@@ -384,7 +392,9 @@ int rhs(pw_ctx* c, int kind, const Rec64* state, Rate64* out) {
   args.nu = c->nu;
   args.chunks = c->d_chunks;
   args.src = c->d_src32;
+  args.src64 = nullptr;
   args.nchunks = c->nchunks;
+  if (k.cfg.scalar_feed && launch_kind_of(c, kind) == pw::kPairFP64) args.src64 = c->d_src64;  // ensure_kind made it
   // the high-order kernel replaces the Gaussian kinds (pw_set_kernel)
   const int launch_kind = launch_kind_of(c, kind);
   const int grid = c->num_cus * pw::pair_kernel_waves(launch_kind, k.cfg);
@@ -960,11 +970,14 @@ int step_graph(pw_ctx* c, int scheme, int kind, double dt, hipGraphExec_t* out) 
   const int rc = enqueue_step(c, scheme, kind, dt);
   hipGraph_t captured = nullptr;
   const hipError_t ec = hipStreamEndCapture(c->stream, &captured);
-  if (rc != PW_OK) {
+  if (rc != PW_OK || ec != hipSuccess) {
+    // a failed capture leaves its error as the thread's last error: clear
+    // it, so that the next launch's check does not report it again
+    (void)hipGetLastError();
     if (captured) (void)hipGraphDestroy(captured);
-    return rc;
+    if (rc != PW_OK) return rc;
+    return fail(PW_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
   }
-  if (ec != hipSuccess) return fail(PW_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
   g = captured;
   hipGraphExec_t exec = nullptr;
   const hipError_t ei = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
@@ -1036,7 +1049,7 @@ int pw_interact_wakes(pw_ctx* c, int precision, const int* pairs, int npairs) {
     // one block at a time: its items have distinct targets and write their
     // rates into blk, which is then added to the targets' totals
     pw::PairArgs args{c->y,     d_items + first[k], first[k + 1] - first[k], n, blk, nullptr, c->d_queue, c->nu,
-                      c->d_chunks, c->d_src32,        c->nchunks};
+                      c->d_chunks, c->d_src32,        nullptr, c->nchunks};
     e = pw::launch_pair(lk, pw::PairConfig{}, args, args.nitems, c->stream);
     const int a = pairs[2 * k], b = pairs[2 * k + 1];
     if (e == hipSuccess)
@@ -1256,7 +1269,7 @@ int pw_set_pair_config(pw_ctx* c, int precision, int parts, int persistent, int 
   k.cfg.parts = parts;
   k.cfg.persistent = persistent != 0;
   k.cfg.waves8 = (variant & PW_VARIANT_WAVES8) != 0 && kind == pw::kPairFP32C;
-  k.cfg.scalar_feed = (variant & PW_VARIANT_SCALAR_FEED) != 0 && kind == pw::kPairFP32C;
+  k.cfg.scalar_feed = (variant & PW_VARIANT_SCALAR_FEED) != 0 && (kind == pw::kPairFP32C || kind == pw::kPairFP64);
   k.tuned = true;
   drop_graphs(c);
   return PW_OK;
@@ -1320,7 +1333,7 @@ void load_tune_file() {
     if (w8 < 0 || w8 > (PW_VARIANT_WAVES8 | PW_VARIANT_SCALAR_FEED)) continue;
     const bool f32 = kind == pw::kPairFP32C || kind == pw::kPairFP32CHO;
     cfg.waves8 = (w8 & PW_VARIANT_WAVES8) != 0 && f32;
-    cfg.scalar_feed = (w8 & PW_VARIANT_SCALAR_FEED) != 0 && f32;
+    cfg.scalar_feed = (w8 & PW_VARIANT_SCALAR_FEED) != 0 && (f32 || kind == pw::kPairFP64);
     tune_cache()[TuneKey(kind, n, nsys, largest)] = cfg;
   }
   std::fclose(f);
@@ -1453,6 +1466,13 @@ int pw_autotune(pw_ctx* c, int precision, double* best_ms) {
   for (int parts = 16; parts <= 32; parts *= 2) {
     if (blocks * parts > 8 * resident) add(parts, true, f32);
     if (f32) add(parts, true, true, true);
+  }
+  // the scalar-fed FP64 kernel (Gaussian FP64 only)
+  if (kind == pw::kPairFP64 && c->smoothing == PW_KERNEL_GAUSSIAN) {
+    add(1, false, false, true);
+    add(8, true, false, true);
+    add(16, true, false, true);
+    add(32, true, false, true);
   }
   pw::PairConfig best_cfg = cands.front();
   double best = 1e300;

@@ -123,6 +123,7 @@ struct pw_ctx {
   std::vector<int32_t> chunk_first;
   pw::Chunk32* d_chunks = nullptr;
   pw::Src32* d_src32 = nullptr;
+  pw::Src64* d_src64 = nullptr;  // scalar-fed FP64 records (allocated on first use)
   int nchunks = 0;
   int num_cus = 256;
   struct SysRange {

@@ -55,6 +55,11 @@ struct alignas(16) Src32 {
   float mx, my, mz;  // a x r, in FP32 from the rounded a and r
   float hs2, vol, pad;
 };
+// Source record of the scalar-fed FP64 kernel: position relative to the
+// chunk's centre, strength and m = a x r, in FP64 (80 bytes with padding).
+struct alignas(16) Src64 {
+  double rx, ry, rz, ax, ay, az, mx, my, mz, pad;
+};
 
 // Arguments of a pair-kernel launch (passed by value).
 struct PairArgs {
@@ -68,6 +73,7 @@ struct PairArgs {
   double nu;
   Chunk32* chunks;    // scalar-fed FP32C: source chunks (geometry rebuilt per launch)
   Src32* src;         //   and source records, one per particle
+  Src64* src64;       // scalar-fed FP64: source records relative to their chunk's centre
   int nchunks;
 };
 

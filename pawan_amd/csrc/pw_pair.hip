@@ -261,22 +261,25 @@ struct P64S {
   double bx = 0, by = 0, bz = 0;  // B
 };
 
+// one separated pair: (tx, ty, tz) = t', (cx, cy, cz) = t' x a_t; the
+// source's r, a and m = a x r
 __device__ __forceinline__ void pair64s(double tx, double ty, double tz, double tax, double tay, double taz,
-                                        double cx, double cy, double cz, const TileF64& s, P64S& p) {
-  const double dx = tx - s.rx, dy = ty - s.ry, dz = tz - s.rz;
+                                        double cx, double cy, double cz, double srx, double sry, double srz,
+                                        double sax, double say, double saz, double smx, double smy, double smz,
+                                        P64S& p) {
+  const double dx = tx - srx, dy = ty - sry, dz = tz - srz;
   const double r2 = dx * dx + dy * dy + dz * dz;
   const double ir = rsqrt_pos(r2);
   const double ir2 = ir * ir;
   const double q = ir * ir2;
   const double f = q * ir2;
-  const double sax = s.rec.ax, say = s.rec.ay, saz = s.rec.az;
   p.ax += q * sax;
   p.ay += q * say;
   p.az += q * saz;
-  p.mx += q * s.mx;
-  p.my += q * s.my;
-  p.mz += q * s.mz;
-  const double h = (sax * cx + say * cy + saz * cz) - (tax * s.mx + tay * s.my + taz * s.mz);
+  p.mx += q * smx;
+  p.my += q * smy;
+  p.mz += q * smz;
+  const double h = (sax * cx + say * cy + saz * cz) - (tax * smx + tay * smy + taz * smz);
   const double g = f * h;
   p.bx += g * dx;
   p.by += g * dy;
@@ -369,7 +372,11 @@ __device__ __forceinline__ void item_fp64(const PairArgs& pa, const WorkItem& it
       const bool sep = gap > 0.0 && gap * gap > kFar64 * (h2T + uniform(s_b[cw][6])) * 1.0001 &&
                        rT + uniform(sqrt(cs2)) + rS <= kSepFP64 * gap;
       if (sep) {
-        for (int l = c0; l < c1; ++l) pair64s(tx, ty, tz, me.ax, me.ay, me.az, ctx, cty, ctz, tile[l], sp);
+        for (int l = c0; l < c1; ++l) {
+          const TileF64& f = tile[l];
+          pair64s(tx, ty, tz, me.ax, me.ay, me.az, ctx, cty, ctz, f.rx, f.ry, f.rz, f.rec.ax, f.rec.ay, f.rec.az,
+                  f.mx, f.my, f.mz, sp);
+        }
         continue;
       }
       const bool own = diag && cw == wave;  // the chunk holding this wave's own targets
@@ -1325,6 +1332,155 @@ __device__ __forceinline__ void fp32s_entry(const PairArgs& a) {
 }
 
 // ---------------------------------------------------------------------
+// FP64 direct sum, scalar-fed (PairConfig::scalar_feed for PW_FP64): the
+// FP32C scalar-fed scheme with FP64 records.  One wave owns 64 targets and
+// streams its item's sources chunk by chunk through scalar loads; a chunk
+// separated from the wave's targets (gap beyond sqrt(90) sigma, where the
+// Gaussian terms are below FP64 rounding, and at least the chunk's radius)
+// runs the separated sums of pair64s on positions relative to the chunk's
+// centre (t' rounded once per chunk, r_s once per launch, both FP64: the
+// displacement's rounding is at most 2^-53 (2|d| + 2 r_S) <= 4 x 2^-53 of
+// |d|), the others the LDS kernel's pair64 on FP64 positions.  The 72 bytes
+// a separated pair reads would be 4.6 KB of LDS traffic per wave and pair;
+// from scalar registers they cost no vector-memory or LDS bandwidth.
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_src_chunks64(const Rec64* __restrict__ state, Chunk32* __restrict__ chunks,
+                                                     Src64* __restrict__ src) {
+  Chunk32& h = chunks[blockIdx.x];
+  const int first = h.first, count = h.count;
+  const int lane = static_cast<int>(threadIdx.x);
+  const bool on = lane < count;
+  const Rec64 r = state[first + (on ? lane : 0)];
+  const double lo[3] = {wave_mind(r.x), wave_mind(r.y), wave_mind(r.z)};
+  const double hi[3] = {wave_maxd(r.x), wave_maxd(r.y), wave_maxd(r.z)};
+  double c[3], e2 = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    c[k] = 0.5 * (lo[k] + hi[k]);
+    e2 += 0.25 * (hi[k] - lo[k]) * (hi[k] - lo[k]);
+  }
+  const float h2 = static_cast<float>(wave_maxd(0.5 * r.s2) * 1.000001);
+  if (lane == 0) {
+    h.cx = c[0];
+    h.cy = c[1];
+    h.cz = c[2];
+    h.r = static_cast<float>(sqrt(e2) * 1.000001) + 1e-30f;
+    h.h2 = h2;
+  }
+  if (on) {
+    Src64 o;
+    o.rx = r.x - c[0];
+    o.ry = r.y - c[1];
+    o.rz = r.z - c[2];
+    o.ax = r.ax;
+    o.ay = r.ay;
+    o.az = r.az;
+    o.mx = r.ay * o.rz - r.az * o.ry;
+    o.my = r.az * o.rx - r.ax * o.rz;
+    o.mz = r.ax * o.ry - r.ay * o.rx;
+    o.pad = 0.0;
+    src[first + lane] = o;
+  }
+}
+
+__device__ __forceinline__ Rec64 load(ConstPtr<Rec64> p, int i) {
+  Rec64 r;
+  r.x = p[i].x; r.y = p[i].y; r.z = p[i].z; r.s2 = p[i].s2;
+  r.ax = p[i].ax; r.ay = p[i].ay; r.az = p[i].az; r.vol = p[i].vol;
+  return r;
+}
+
+// Targets [tb, tb + 64) of item it (a 64-lane wave of its own).
+__device__ __forceinline__ void wave_fp64s(const WorkItem& it, int tb, double nu, const Rec64* state_g,
+                                           const Chunk32* chunks_g, int nchunks, const Src64* src_g,
+                                           Rate64* __restrict__ out) {
+  const ConstPtr<Rec64> state = as_const(state_g);
+  const ConstPtr<Chunk32> chunks = as_const(chunks_g);
+  const ConstPtr<Src64> src = as_const(src_g);
+  const int lane = static_cast<int>(threadIdx.x) & 63;
+  const int t = tb + lane;
+  const bool active = t < it.t1;
+  const Rec64 m = state_g[active ? t : tb];
+  T64 me;
+  me.x = m.x; me.y = m.y; me.z = m.z; me.hs2 = 0.5 * m.s2;
+  me.ax = m.ax; me.ay = m.ay; me.az = m.az; me.vol = m.vol;
+  me.nu2 = 2.0 * nu;
+  // the wave's targets' sphere (inactive lanes repeat the first target)
+  double cW[3], rW2 = 0.0;
+  {
+    const double lo[3] = {wave_mind(m.x), wave_mind(m.y), wave_mind(m.z)};
+    const double hi[3] = {wave_maxd(m.x), wave_maxd(m.y), wave_maxd(m.z)};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      cW[k] = uniform(0.5 * (lo[k] + hi[k]));
+      rW2 += 0.25 * (hi[k] - lo[k]) * (hi[k] - lo[k]);
+    }
+  }
+  const double rW = uniform(sqrt(rW2) * 1.000001);
+  const double h2W = uniform(wave_maxd(me.hs2));
+  AccF64 a;
+  for (int ci = it.chunk0; ci < nchunks; ++ci) {
+    const Chunk32 h = load(chunks, ci);
+    if (h.first >= it.s1) break;
+    const int l0 = max(h.first, it.s0), l1 = min(h.first + h.count, it.s1);
+    const double ddx = h.cx - cW[0], ddy = h.cy - cW[1], ddz = h.cz - cW[2];
+    const double gap = sqrt(ddx * ddx + ddy * ddy + ddz * ddz) * 0.999999 - rW - h.r;
+    const bool sep = gap > 0.0 && gap * gap > kFar64 * (h2W + h.h2) * 1.0001 && h.r <= gap;
+    if (sep) {
+      const double tx = me.x - h.cx, ty = me.y - h.cy, tz = me.z - h.cz;
+      const double cx = ty * me.az - tz * me.ay, cy = tz * me.ax - tx * me.az, cz = tx * me.ay - ty * me.ax;
+      P64S p;
+#pragma unroll 2
+      for (int l = l0; l < l1; ++l)
+        pair64s(tx, ty, tz, me.ax, me.ay, me.az, cx, cy, cz, src[l].rx, src[l].ry, src[l].rz, src[l].ax,
+                src[l].ay, src[l].az, src[l].mx, src[l].my, src[l].mz, p);
+      // u += (A x t' - M)/4pi; the stretching's A and B join the factored sums
+      a.ux += kQ * ((p.ay * tz - p.az * ty) - p.mx);
+      a.uy += kQ * ((p.az * tx - p.ax * tz) - p.my);
+      a.uz += kQ * ((p.ax * ty - p.ay * tx) - p.mz);
+      a.ax += kQ * p.ax;
+      a.ay += kQ * p.ay;
+      a.az += kQ * p.az;
+      a.bx += (-3.0 * kQ) * p.bx;
+      a.by += (-3.0 * kQ) * p.by;
+      a.bz += (-3.0 * kQ) * p.bz;
+    } else {
+#pragma unroll 1
+      for (int l = l0; l < l1; ++l)
+        if (l != t) pair64(me, load(state, l), a);  // the self pair is never evaluated
+    }
+  }
+  if (active) {
+    Rate64 r;
+    r.ux = a.ux; r.uy = a.uy; r.uz = a.uz;
+    r.wx = (me.ay * a.az - me.az * a.ay) + a.bx;
+    r.wy = (me.az * a.ax - me.ax * a.az) + a.by;
+    r.wz = (me.ax * a.ay - me.ay * a.ax) + a.bz;
+    out[t] = r;
+  }
+}
+
+template <bool Persistent>
+__device__ __forceinline__ void fp64s_entry(const PairArgs& a) {
+  const int waves = 4 * a.nitems;
+  for (int w = Persistent ? -1 : static_cast<int>(blockIdx.x);;) {
+    if (Persistent) {
+      int idx = 0;
+      if (threadIdx.x == 0) idx = atomicAdd(a.queue, 1);
+      w = __builtin_amdgcn_readfirstlane(idx);  // lane 0's: every lane is active here
+    }
+    if (w >= waves) break;
+    const WorkItem it = a.items[w >> 2];
+    const int tb = it.t0 + 64 * (w & 3);
+    if (tb < it.t1) {
+      Rate64* out = it.part < 0 ? a.rates : a.partial + static_cast<size_t>(it.part) * a.n;
+      wave_fp64s(it, tb, a.nu, a.state, a.chunks, a.nchunks, a.src64, out);
+    }
+    if (!Persistent) break;
+  }
+}
+
+// ---------------------------------------------------------------------
 // Near field of the tree path.  A work item is (target leaf T, a range of
 // T's near list); its targets are cut into 64-lane slices, and one slice of
 // one item is what a wave pulls from the atomic queue (queue index =
@@ -1631,6 +1787,8 @@ template <bool P>
 __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_pair_fp64_ho(PairArgs a) {
   pair_entry<kPairFP64HO, P>(a);
 }
+template <bool P>
+__global__ __launch_bounds__(64) void k_pair_fp64s(PairArgs a) { fp64s_entry<P>(a); }
 // FP32C at the register budget of 7 waves/SIMD (no spills) and of 8
 // waves/SIMD (one spill slot, touched once per tile)
 template <bool P>
@@ -1718,6 +1876,26 @@ hipError_t launch_fp32s(int kind, const PairConfig& cfg, const PairArgs& args, i
 hipError_t launch_pair(int kind, const PairConfig& cfg, const PairArgs& args, int grid,
                        hipStream_t st) {
   if (args.nitems <= 0) return hipSuccess;
+  if (cfg.scalar_feed && kind == kPairFP64) {
+    if (!args.chunks || !args.src64 || args.nchunks <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_src_chunks64, dim3(args.nchunks), dim3(64), 0, st, args.state, args.chunks, args.src64);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int waves = 4 * args.nitems;
+    if (cfg.persistent) {
+      e = launch_fill_int(args.queue, 1, 0, st);
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL(k_pair_fp64s<true>, dim3(static_cast<unsigned>(std::min(grid, waves))), dim3(64), 0, st,
+                         args);
+    } else {
+      hipLaunchKernelGGL(k_pair_fp64s<false>, dim3(static_cast<unsigned>(waves)), dim3(64), 0, st, args);
+    }
+    e = hipGetLastError();
+    if (e != hipSuccess || cfg.parts <= 1) return e;
+    hipLaunchKernelGGL(k_reduce_parts, dim3((args.n + 255) / 256), dim3(256), 0, st, args.partial,
+                       cfg.parts, args.n, args.rates);
+    return hipGetLastError();
+  }
   if (cfg.scalar_feed && (kind == kPairFP32C || kind == kPairFP32CHO)) {
     hipError_t e = launch_fp32s(kind, cfg, args, grid, st);
     if (e != hipSuccess || cfg.parts <= 1) return e;
@@ -1824,6 +2002,7 @@ int blocks_per_cu(const void* kernel, int threads = kTB) {
 
 int pair_kernel_waves(int kind, const PairConfig& cfg) {
   const void* k = nullptr;
+  if (cfg.scalar_feed && kind == kPairFP64) return blocks_per_cu(reinterpret_cast<const void*>(k_pair_fp64s<true>), 64);
   if (cfg.scalar_feed && (kind == kPairFP32C || kind == kPairFP32CHO)) {
     // one-wave blocks: resident blocks per CU = resident waves per CU
     if (kind == kPairFP32C)

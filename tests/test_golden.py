@@ -65,7 +65,8 @@ def test_oracle_reproduces_rk3_steps(oracle):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", ["ring_1k", "leapfrog_64k", "helix_1m"])
-@pytest.mark.parametrize("prec,tol_u,feed", [("fp64", 1e-12, False), ("fp32c", 1e-6, False), ("fp32c", 1e-6, True)])
+@pytest.mark.parametrize("prec,tol_u,feed", [("fp64", 1e-12, False), ("fp64", 1e-12, True), ("fp32c", 1e-6, False),
+                                             ("fp32c", 1e-6, True)])
 def test_gpu_matches_fixture(name, prec, tol_u, feed):
     from pawan_amd import capi
 
@@ -74,7 +75,7 @@ def test_gpu_matches_fixture(name, prec, tol_u, feed):
     print(f"{name} N={s.n} {prec}{' scalar-fed' if feed else ''} vs fixture", flush=True)
     with capi.Solver(s) as solver:
         if feed:
-            solver.set_pair_config(prec, 4, True, True, True)
+            solver.set_pair_config(prec, 4, True, prec == "fp32c", True)
         u, w = solver.solve(prec)
     idx = g["targets"] if "targets" in g else np.arange(s.n)
     eu, ew = normwise(u[idx], g["vel"]), normwise(w[idx], g["retvor"])

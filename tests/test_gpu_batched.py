@@ -9,7 +9,8 @@ from pawan_amd import capi, workloads as W
 
 pytestmark = pytest.mark.gpu
 
-# (parts, persistent, waves8, scalar_feed); the last two apply to FP32C only
+# (parts, persistent, waves8, scalar_feed); waves8 applies to FP32C only,
+# scalar_feed to FP32C and FP64
 CONFIGS = [(1, False, False, False), (3, False, False, False), (1, True, False, False), (8, True, False, False),
            (4, True, True, False), (2, False, True, False), (1, False, False, True), (3, True, True, True),
            (8, True, False, True)]
@@ -72,7 +73,7 @@ def test_autotune_keeps_results(oracle, batched):
             _check_wakes(oracle, batched, u, None, tol, None, sample=16)
 
 
-@pytest.mark.parametrize("prec,feed", [("fp64", False), ("fp32c", False), ("fp32c", True)])
+@pytest.mark.parametrize("prec,feed", [("fp64", False), ("fp64", True), ("fp32c", False), ("fp32c", True)])
 def test_graph_replay_equals_eager_steps(batched, prec, feed):
     """Steps replayed from a captured HIP graph give bit-identical states to
     eagerly launched steps, on every replay, with eager work (diagnostics,

@@ -32,14 +32,16 @@ def test_ring_rhs_fp64(oracle, prec):
     assert normwise(w, ref_w) < FP64_TOL
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp64_ref"])
-def test_leapfrog_rhs_fp64(oracle, prec):
+@pytest.mark.parametrize("prec,feed", [("fp64", False), ("fp64_ref", False), ("fp64", True)])
+def test_leapfrog_rhs_fp64(oracle, prec, feed):
     s = W.leapfrog_64k()
-    _say(f"leapfrog_64k rhs {prec} (two coupled wakes, nu=1e-4)")
+    _say(f"leapfrog_64k rhs {prec}{' scalar-fed' if feed else ''} (two coupled wakes, nu=1e-4)")
     idx = np.concatenate([np.arange(0, 256), np.arange(32768, 33024),
                           np.arange(1000, s.n, 997)])
     ref_u, ref_w = oracle.rates_targets(s, idx)
     with capi.Solver(s) as g:
+        if feed:
+            g.set_pair_config(prec, 8, True, False, True)
         u, w = g.solve(prec)
     assert normwise(u[idx], ref_u) < FP64_TOL
     assert normwise(w[idx], ref_w) < FP64_TOL
@@ -70,8 +72,8 @@ def test_ring_steps_fp64(oracle, scheme):
     assert normwise(alpha, ref.alpha) < FP64_TOL
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp64_ref"])
-def test_independent_wakes(oracle, prec):
+@pytest.mark.parametrize("prec,feed", [("fp64", False), ("fp64_ref", False), ("fp64", True)])
+def test_independent_wakes(oracle, prec, feed):
     """Thin rings far from the origin: each wake against the oracle on that
     wake alone.  On these rings the vorticity rate is a sum of nearly
     cancelling stretching terms (a_t x a_s of almost parallel strengths), and
@@ -80,8 +82,10 @@ def test_independent_wakes(oracle, prec):
     both the oracle and the __float128 truth (and never looser than 1e-12).
     Velocities carry no such cancellation and are held to 1e-12."""
     s = W.batched_rings(nwakes=6, nmin=64, nmax=700, seed=11)
-    _say(f"batched 6 wakes N={s.n} {prec} + fp32c")
+    _say(f"batched 6 wakes N={s.n} {prec}{' scalar-fed' if feed else ''} + fp32c")
     with capi.Solver(s) as g:
+        if feed:
+            g.set_pair_config(prec, 2, False, False, True)
         u, w = g.solve(prec)
         u32, w32 = g.solve("fp32c")
     for k in range(s.nw):
