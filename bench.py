@@ -325,7 +325,7 @@ def drift_leg(steps: int, every: int, learned_cfg: dict | None, target_err: floa
 def pair_kernel_name(precision: str, cfg: dict | None) -> str:
     """The pair kernel a tuned launch runs (the persistent or grid form of it)."""
     if precision == "fp64":
-        return "k_pair_fp64"
+        return "k_pair_fp64s" if (cfg or {}).get("scalar_feed") else "k_pair_fp64"
     cfg = cfg or {}
     return ("k_pair_fp32s" if cfg.get("scalar_feed") else "k_pair_fp32c") + ("_w8" if cfg.get("waves8") else "")
 
@@ -672,7 +672,7 @@ def main() -> int:
             "fp64_tflops": ach64,
             "fp64_frac_of_peak": ach64 / PEAK_FP64_TFLOPS,
             "roofline_fp64": {
-                "bound": "valu", "kernel": "k_pair_fp64", "achieved": ach64, "peak": PEAK_FP64_TFLOPS,
+                "bound": "valu", "kernel": pair_kernel_name("fp64", tune64), "achieved": ach64, "peak": PEAK_FP64_TFLOPS,
                 "unit": "TFLOP/s", "frac": ach64 / PEAK_FP64_TFLOPS,
                 "traffic": (lambda t: t["bytes"] if t else None)(
                     measured_traffic(args.workload, system.n, "fp64", tune64)),
