@@ -41,6 +41,7 @@ struct FarField {
   pw::Rate64* rates_s = nullptr;  // rates in Morton order
   int* d_overflow = nullptr;
   float sigma_max2 = 0.0f;
+  bool uniform_sigma = false;     // every particle of one radius (the near field's constant core factors)
   // traversal parameters
   float theta = 0.5f;
   float tol = 0.0f;                  // absolute velocity tolerance (<= 0: opening angle only)

@@ -139,6 +139,8 @@ int ensure_ff(pw_ctx* c) {
   double smax = 0.0;
   for (double r : c->radius) smax = std::max(smax, r * r);
   f.sigma_max2 = static_cast<float>(smax);
+  f.uniform_sigma = !c->radius.empty() &&
+                    std::all_of(c->radius.begin(), c->radius.end(), [&](double r) { return r == c->radius[0]; });
   f.ready = true;
   return PW_OK;
 }
@@ -323,8 +325,9 @@ int ff_rhs_body(pw_ctx* c, const Rec64* state, Rate64* out, const StageEvents* e
   const int n = static_cast<int>(c->n);
   PW_HIP(pw::build_near_items(tb, f.shape.nleaf, c->stream));
   const bool ho = c->smoothing == PW_KERNEL_HIGHORDER;
-  PW_HIP(pw::launch_near_fp32c(tb.sorted, tb.nodes, tb, f.shape.nleaf, c->nu, f.sigma_max2, ho,
-                               c->num_cus * pw::near_kernel_blocks_per_cu(ho), f.rates_s, c->stream));
+  PW_HIP(pw::launch_near_fp32c(tb.sorted, tb.nodes, tb, f.shape.nleaf, c->nu, f.sigma_max2, f.uniform_sigma, ho,
+                               c->num_cus * pw::near_kernel_blocks_per_cu(ho, f.uniform_sigma), f.rates_s,
+                               c->stream));
   PW_HIP(mark(2));
   PW_HIP(pw::launch_far_md(tb.sorted, tb.nodes, tb.far_list, tb.far_cnt, tb.cap_far, f.shape.nleaf, f.rates_s,
                            c->stream));

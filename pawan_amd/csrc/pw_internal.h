@@ -112,7 +112,7 @@ hipError_t launch_pair(int kind, const PairConfig& cfg, const PairArgs& args, in
 // its waves per CU), from the occupancy calculator
 int pair_kernel_waves(int kind, const PairConfig& cfg);
 // resident one-wave blocks per CU of the tree path's near-field kernel
-int near_kernel_blocks_per_cu(bool highorder);
+int near_kernel_blocks_per_cu(bool highorder, bool uniform_sigma);
 
 // Stage arithmetic of the steppers (FP64, no contraction, the reference's
 // operation order).  n = number of particles.
@@ -340,8 +340,8 @@ hipError_t launch_point_velocity(const Rec64* state, int n, const double* pts, i
                                  bool high_order, hipStream_t st);
 // near field of the tree path (highorder: the algebraic kernel's pairs)
 hipError_t launch_near_fp32c(const Rec64* sorted, const NodeRec* nodes, const TreeBuffers& tb, int nleaf,
-                             double nu, float sigma_max2, bool highorder, int grid, Rate64* rates_s,
-                             hipStream_t st);
+                             double nu, float sigma_max2, bool uniform_sigma, bool highorder, int grid,
+                             Rate64* rates_s, hipStream_t st);
 
 // Far field over the lists, analytic monopole + dipole (FP32), added to
 // rates_s; with mlp != nullptr the learned residual is added as well.

@@ -1511,6 +1511,9 @@ struct NearArgs {
   Rate64* out;
   Rate64* partial;  // item x 256
   const LeafChunk* chunks;  // leaf x kLeafChunks run spheres
+  // every particle of the same radius (uniform sigma): the core's sigma
+  // factors, once per launch (u_cut2 = 44 sigma^2 > 0 marks the case)
+  float u_cut2, u_isg2, u_ke, u_fk, u_ck;
 };
 
 constexpr int kNearSlice = 64;                       // targets per wave
@@ -1521,14 +1524,35 @@ static_assert(kNearSlices == kLeafChunks && kNearSlice == 64, "a slice is one 64
 // pair on relative FP32 positions, Gaussian core included (HighOrder: the
 // algebraic kernel, exact for every pair, PSE within 44 sigma^2 as in
 // pair32_ho)
-template <bool HighOrder>
+// near32t for a pair radius shared by every pair: the sigma factors come
+// from the launch (isg2 = 1/sigma^2, ke = sqrt(2/pi)/sigma^3, fk = ke/sigma^2,
+// ck = 2 nu (2 pi)^-3/2 / sigma^5)
+__device__ __forceinline__ void near32u(float r2, const NearArgs& a, const float4* __restrict__ tab, float& qp,
+                                        float& fp, float& c) {
+  const float sb = r2 * a.u_isg2;
+  const float e = __builtin_amdgcn_exp2f(kMHalfLog2e * sb);
+  const int k = static_cast<int>(fminf(sb * kCoreScale, static_cast<float>(kCorePieces - 1)));
+  const float4 lo = tab[2 * k], hi = tab[2 * k + 1];
+  const float t = sb - hi.z;
+  float g = fmaf(hi.y, t, hi.x);
+  g = fmaf(g, t, lo.w);
+  g = fmaf(g, t, lo.z);
+  g = fmaf(g, t, lo.y);
+  g = fmaf(g, t, lo.x);
+  qp = a.u_ke * fmaf(sb, g, e * 0x1.555556p-2f);
+  fp = a.u_fk * g;
+  c = a.u_ck * e;
+}
+
+template <bool HighOrder, bool Uniform = false>
 __device__ __forceinline__ void pair32_rel_core(float tx, float ty, float tz, const T32& t, const SrcRel& s,
-                                                const float4* __restrict__ tab, P32& p, PSE32& e) {
+                                                const float4* __restrict__ tab, const NearArgs& a, P32& p,
+                                                PSE32& e) {
   const float dx = tx - s.rx;
   const float dy = ty - s.ry;
   const float dz = tz - s.rz;
   const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-  const float sig2 = s.hs2 + t.hs2;
+  const float sig2 = Uniform ? 0.0f : s.hs2 + t.hs2;
   float q, f;
   float ib7 = 0.0f, ib2 = 0.0f;
   if (HighOrder) {
@@ -1544,10 +1568,11 @@ __device__ __forceinline__ void pair32_rel_core(float tx, float ty, float tz, co
     q = ir * ir2;
     f = q * ir2;
   }
-  const bool core = r2 <= kFar32 * sig2;
+  const bool core = Uniform ? r2 <= a.u_cut2 : r2 <= kFar32 * sig2;
   if (core) {
     float c;
     if (HighOrder) c = t.nu2 * kHoEta32 * (sig2 * sig2) * (ib7 * ib2);
+    else if (Uniform) near32u(r2, a, tab, q, f, c);
     else near32t(r2, sig2, t.nu2, tab, q, f, c);
     e.x = fmaf(c, t.vol * s.ax - s.vol * t.ax, e.x);
     e.y = fmaf(c, t.vol * s.ay - s.vol * t.ay, e.y);
@@ -1560,7 +1585,7 @@ __device__ __forceinline__ void pair32_rel_core(float tx, float ty, float tz, co
     accumulate32(dx, dy, dz, q, f, t.ax, t.ay, t.az, s, p);
 }
 
-template <bool HighOrder>
+template <bool HighOrder, bool Uniform>
 __device__ __forceinline__ void near_slice(const NearArgs& a, const NearItem& it, int slice, SrcRel* tile,
                                            const float4* __restrict__ tab) {
   const int T = it.T;
@@ -1641,10 +1666,10 @@ __device__ __forceinline__ void near_slice(const NearArgs& a, const NearItem& it
       if (diag) {
         const int own = self - c0;  // this target's slot in the tile, if there
         for (int l = 0; l < nc; ++l)
-          if (l != own) pair32_rel_core<HighOrder>(rx, ry, rz, me, tile[l], tab, p, pse);
+          if (l != own) pair32_rel_core<HighOrder, Uniform>(rx, ry, rz, me, tile[l], tab, a, p, pse);
       } else {
 #pragma unroll 1
-        for (int l = 0; l < nc; ++l) pair32_rel_core<HighOrder>(rx, ry, rz, me, tile[l], tab, p, pse);
+        for (int l = 0; l < nc; ++l) pair32_rel_core<HighOrder, Uniform>(rx, ry, rz, me, tile[l], tab, a, p, pse);
       }
       fold(p, me.ax, me.ay, me.az, hi, lo);
     }
@@ -1665,7 +1690,7 @@ __device__ __forceinline__ void near_slice(const NearArgs& a, const NearItem& it
   }
 }
 
-template <bool HighOrder>
+template <bool HighOrder, bool Uniform = false>
 __device__ __forceinline__ void near_entry(const NearArgs& a, SrcRel* tile, float4* tab) {
   if (!HighOrder) {  // the core table into LDS (one 64-lane block: a float4 per lane)
     static_assert(2 * kCorePieces == kNearSlice, "one table row half per lane");
@@ -1683,7 +1708,7 @@ __device__ __forceinline__ void near_entry(const NearArgs& a, SrcRel* tile, floa
     const NearItem it = a.items[idx / kNearSlices];
     const int slice = idx % kNearSlices;
     if (a.nodes[it.T].first + kNearSlice * slice >= a.nodes[it.T].last) continue;
-    near_slice<HighOrder>(a, it, slice, tile, tab);
+    near_slice<HighOrder, Uniform>(a, it, slice, tile, tab);
   }
 }
 
@@ -1691,6 +1716,12 @@ __global__ __launch_bounds__(kNearSlice) __attribute__((amdgpu_waves_per_eu(6, 8
   __shared__ SrcRel tile[kNearSlice];
   __shared__ float4 tab[2 * kCorePieces];
   near_entry<false>(a, tile, tab);
+}
+// every particle of one radius
+__global__ __launch_bounds__(kNearSlice) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_near_fp32c_u(NearArgs a) {
+  __shared__ SrcRel tile[kNearSlice];
+  __shared__ float4 tab[2 * kCorePieces];
+  near_entry<false, true>(a, tile, tab);
 }
 __global__ __launch_bounds__(kNearSlice) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_near_fp32c_ho(NearArgs a) {
   __shared__ SrcRel tile[kNearSlice];
@@ -1967,15 +1998,29 @@ hipError_t launch_point_velocity(const Rec64* state, int n, const double* pts, i
 }
 
 hipError_t launch_near_fp32c(const Rec64* sorted, const NodeRec* nodes, const TreeBuffers& tb, int nleaf,
-                             double nu, float sigma_max2, bool highorder, int grid, Rate64* rates_s,
-                             hipStream_t st) {
+                             double nu, float sigma_max2, bool uniform_sigma, bool highorder, int grid,
+                             Rate64* rates_s, hipStream_t st) {
   if (nleaf <= 0) return hipSuccess;
   hipError_t e = launch_fill_int(tb.near_queue, 1, 0, st);
   if (e != hipSuccess) return e;
   NearArgs a{sorted, nodes, tb.near_list, tb.cap_near, tb.near_items, tb.near_total, tb.near_queue,
-             static_cast<float>(nu), kFar32 * sigma_max2, rates_s, tb.near_partial, tb.leaf_chunks};
-  if (highorder) hipLaunchKernelGGL(k_near_fp32c_ho, dim3(grid), dim3(kNearSlice), 0, st, a);
-  else hipLaunchKernelGGL(k_near_fp32c, dim3(grid), dim3(kNearSlice), 0, st, a);
+             static_cast<float>(nu), kFar32 * sigma_max2, rates_s, tb.near_partial, tb.leaf_chunks,
+             0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  if (uniform_sigma && !highorder) {
+    // the pair radius is sigma itself: sigma^2 = (s_s^2 + s_t^2) / 2; the
+    // factors rounded from FP64, as near32t rounds them per pair from FP32
+    const double s2 = sigma_max2, is2 = 1.0 / s2, is3 = is2 / std::sqrt(s2);
+    a.u_cut2 = kFar32 * sigma_max2;
+    a.u_isg2 = static_cast<float>(is2);
+    a.u_ke = static_cast<float>(0.79788456080286535588 * is3);  // sqrt(2/pi)
+    a.u_fk = static_cast<float>(0.79788456080286535588 * is3 * is2);
+    a.u_ck = static_cast<float>(2.0 * nu * 0.063493635934240969785 * is3 * is2);  // (2 pi)^-3/2
+    hipLaunchKernelGGL(k_near_fp32c_u, dim3(grid), dim3(kNearSlice), 0, st, a);
+  } else if (highorder) {
+    hipLaunchKernelGGL(k_near_fp32c_ho, dim3(grid), dim3(kNearSlice), 0, st, a);
+  } else {
+    hipLaunchKernelGGL(k_near_fp32c, dim3(grid), dim3(kNearSlice), 0, st, a);
+  }
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_near_reduce, dim3(nleaf), dim3(kTB), 0, st, nodes, tb.near_parts, tb.near_offs,
@@ -2028,10 +2073,11 @@ int pair_kernel_waves(int kind, const PairConfig& cfg) {
   return blocks_per_cu(k);
 }
 
-int near_kernel_blocks_per_cu(bool highorder) {
-  return blocks_per_cu(highorder ? reinterpret_cast<const void*>(k_near_fp32c_ho)
-                                 : reinterpret_cast<const void*>(k_near_fp32c),
-                       kNearSlice);
+int near_kernel_blocks_per_cu(bool highorder, bool uniform_sigma) {
+  const void* k = highorder        ? reinterpret_cast<const void*>(k_near_fp32c_ho)
+                  : uniform_sigma ? reinterpret_cast<const void*>(k_near_fp32c_u)
+                                  : reinterpret_cast<const void*>(k_near_fp32c);
+  return blocks_per_cu(k, kNearSlice);
 }
 
 }  // namespace pw
