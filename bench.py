@@ -38,6 +38,9 @@ Other measurements (not part of the default run):
   --profile-rhs         right-hand sides of one workload and a plain copy of
                         known size, for rocprofv3 counter runs
   --profile-vring       ring strength resolutions, for rocprofv3 traces
+  --probe-valu          vector-issue ceilings of the device per instruction
+                        form (the default run records them next to the
+                        pair kernels' issue rates)
 """
 from __future__ import annotations
 
@@ -330,6 +333,37 @@ def pair_kernel_name(precision: str, cfg: dict | None) -> str:
     return ("k_pair_fp32s" if cfg.get("scalar_feed") else "k_pair_fp32c") + ("_w8" if cfg.get("waves8") else "")
 
 
+def valu_ceilings() -> dict:
+    """Vector-issue rates this device reaches for the instruction forms the
+    pair kernels are made of (pw_valu_ceiling: long chains of one form on
+    every CU, the best of 16 and 32 resident waves per CU), in wave
+    instructions per CU per nanosecond."""
+    from pawan_amd import capi
+
+    out = {"unit": "wave instructions per CU per ns", "probe": "pw_valu_ceiling"}
+    for form in capi.VALU_FORMS:
+        rates = []
+        for w in (16, 32):
+            r, cus = capi.valu_ceiling(form, w)
+            rates.append(r)
+        out[form] = max(rates)
+    out["cus"] = cus
+    return out
+
+
+def measured_issue(traffic: dict | None, pairs: int, avg_ms: float, ceilings: dict | None, form: str):
+    """The pair kernel's vector-issue rate in the timed launches (its
+    instructions per pair from the committed counter runs of the same launch
+    shape, times the pairs, over the launch time and the CUs), against the
+    measured ceiling of the instruction form it is mostly made of."""
+    if not traffic or "issue" not in traffic or not ceilings:
+        return None
+    vpp = traffic["issue"]["valu_per_pair"]
+    rate = vpp * pairs / 64.0 / (avg_ms * 1e6) / ceilings["cus"]
+    return {"valu_per_pair": vpp, "per_cu_ns": rate, "ceiling_form": form, "ceiling_per_cu_ns": ceilings[form],
+            "frac_of_ceiling": rate / ceilings[form]}
+
+
 def measured_traffic(workload: str, n: int, precision: str, cfg: dict | None):
     """HBM bytes per launch of the pair kernel from the committed rocprofv3
     counter runs (FETCH_SIZE and WRITE_SIZE in KiB per dispatch, scaled by
@@ -608,6 +642,7 @@ def main() -> int:
     ap.add_argument("--trajectory-steps", type=int, default=50)
     ap.add_argument("--profile-rhs", action="store_true", help="right-hand sides + a copy of known size")
     ap.add_argument("--profile-vring", action="store_true", help="ring strength resolutions")
+    ap.add_argument("--probe-valu", action="store_true", help="vector-issue ceilings of the device (pw_valu_ceiling)")
     ap.add_argument("--precision", default="fp32c", choices=["fp64", "fp64_ref", "fp32c", "learned"])
     ap.add_argument("--reps", type=int, default=1)
     ap.add_argument("--copy-mb", type=int, default=256)
@@ -636,6 +671,9 @@ def main() -> int:
     if args.profile_vring:
         print(json.dumps(profile_vring()), flush=True)
         return 0
+    if args.probe_valu:
+        print(json.dumps({"mode": "probe-valu", **valu_ceilings()}), flush=True)
+        return 0
     if args.trajectory:
         print(json.dumps({"mode": "trajectory", **trajectory(args.trajectory_steps, 10, args.n, not args.no_tune)}),
               flush=True)
@@ -659,6 +697,8 @@ def main() -> int:
     avg32 = ms32 / max(1, l32)
     log(f"fp32c: {sps32:.4f} steps/s, pair kernel avg {avg32:.1f} ms over {l32} launches")
 
+    ceil = valu_ceilings()
+    log(f"valu ceilings: {ceil}")
     out64 = {}
     if not args.no_fp64 and args.fp64_steps > 0:
         wall64, ms64, l64, tune64 = time_path(system, "fp64", dt, args.fp64_steps, 0,
@@ -677,6 +717,8 @@ def main() -> int:
                 "traffic": (lambda t: t["bytes"] if t else None)(
                     measured_traffic(args.workload, system.n, "fp64", tune64)),
                 "traffic_detail": measured_traffic(args.workload, system.n, "fp64", tune64),
+                "issue_measured": measured_issue(measured_traffic(args.workload, system.n, "fp64", tune64), pairs,
+                                                 avg64, ceil, "fp64_fma"),
                 "avg_launch_ms": avg64, "flop_per_launch": FLOP_PER_PAIR * pairs, "launch_config": tune64},
         }
         log(f"fp64: {out64['fp64_steps_per_s']:.4f} steps/s, pair kernel avg {avg64:.1f} ms")
@@ -719,6 +761,10 @@ def main() -> int:
             # traffic_detail.issue has the executed instructions and their
             # issue rate
             "flop_convention": "88 FLOP per ordered pair",
+            # the same launches' vector issue against the rate this device
+            # reaches for FP32 FMAs on vector registers
+            "issue_measured": measured_issue(traffic, pairs, avg32, ceil, "fp32_fma"),
+            "valu_ceilings": ceil,
             "algorithmic_bytes_per_launch": 112 * system.n,
             "kernel": pair_kernel_name("fp32c", tune32),
             "launch_config": tune32,

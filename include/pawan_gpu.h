@@ -426,6 +426,16 @@ int pw_ff_mlp_forward(pw_ctx *ctx, const float *X, int64_t n, float *Y);
 int pw_ff_mlp_grad(pw_ctx *ctx, const float *X, const float *Y, int64_t n,
                    float *grad, double *loss);
 
+/* Measurement only (no counterpart in the reference): the vector-issue rate
+ * the current device reaches for one instruction form, in wave instructions
+ * per CU per nanosecond, from long chains of that form on every CU with
+ * waves_per_cu (4..32, a multiple of 4) waves resident.  form 0: FP32 FMA,
+ * vector-register operands; 1: FP32 FMA with a scalar-register operand;
+ * 2: FP64 FMA, vector-register operands; 3: FP32 multiply.  The ceilings
+ * the pair kernels' issue rates are quoted against.  cus (may be NULL) gets
+ * the device's CU count. */
+int pw_valu_ceiling(int form, int waves_per_cu, double *rate, int *cus);
+
 #ifdef __cplusplus
 }
 #endif

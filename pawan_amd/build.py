@@ -32,7 +32,8 @@ HIPFLAGS = [
 ]
 
 HIP_SOURCES = ["pw_pair.hip", "pw_stage.hip", "pw_diag.hip", "pw_capi.hip", "pw_tree.hip",
-               "pw_far.hip", "pw_mlp_train.hip", "pw_ff.hip", "pw_vring.hip", "pw_relax.hip"]
+               "pw_far.hip", "pw_mlp_train.hip", "pw_ff.hip", "pw_vring.hip", "pw_relax.hip",
+               "pw_probe.hip"]
 HOST_SOURCES = ["pawan_host.cpp"]
 
 

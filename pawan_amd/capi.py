@@ -40,6 +40,7 @@ EXPORTS = [
     "pw_velocity_at", "pw_set_kernel", "pw_get_kernel", "pw_vring_particles", "pw_vring_resolve",
     "pw_set_compensation", "pw_ff_get_timing", "pw_interact_wakes", "pw_vorticity_field",
     "pw_diagnose_wakes", "pw_velocity_at_noalloc", "pw_wake_split", "pw_wake_merge", "pw_set_wake_volume",
+    "pw_valu_ceiling",
 ]
 
 # pw_ff_export selectors
@@ -135,6 +136,7 @@ def load(path: str | None = None) -> C.CDLL:
         "pw_ff_train": ([vp, i64, i, d, C.c_uint64, _dp, i64], i),
         "pw_ff_mlp_forward": ([vp, _fp, i64, _fp], i),
         "pw_ff_mlp_grad": ([vp, _fp, _fp, i64, _fp, C.POINTER(C.c_double)], i),
+        "pw_valu_ceiling": ([i, i, C.POINTER(C.c_double), C.POINTER(C.c_int)], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -153,6 +155,18 @@ def device_count() -> int:
     n = C.c_int(0)
     rc = load().pw_device_count(C.byref(n))
     return n.value if rc == PW_OK else 0
+
+
+VALU_FORMS = {"fp32_fma": 0, "fp32_fma_sgpr": 1, "fp64_fma": 2, "fp32_mul": 3}
+
+
+def valu_ceiling(form: str, waves_per_cu: int = 32) -> tuple[float, int]:
+    """Vector-issue rate the device reaches for one instruction form (wave
+    instructions per CU per nanosecond; forms in VALU_FORMS) and the
+    device's CU count."""
+    r, cus = C.c_double(0.0), C.c_int(0)
+    _check(load().pw_valu_ceiling(VALU_FORMS[form], int(waves_per_cu), C.byref(r), C.byref(cus)))
+    return r.value, cus.value
 
 
 def vring_particles(R: float, core: float, nlayers: int, nazi: int, sig: float):

@@ -592,6 +592,11 @@ struct P32 {
 struct PSE32 {
   float x = 0, y = 0, z = 0;  // sum c (V_t a_s - V_s a_t)
 };
+// the same sums with the target's factors taken out: (x, y, z) = sum c a_s,
+// w = sum c V_s, so that the sum is V_t (x, y, z) - w a_t
+struct PSE32F {
+  float x = 0, y = 0, z = 0, w = 0;
+};
 constexpr int kAcc = 6;
 constexpr int kChunk = 64;  // pairs summed in plain FP32 before compensation
 
@@ -1547,7 +1552,7 @@ __device__ __forceinline__ void near32u(float r2, const NearArgs& a, const float
 template <bool HighOrder, bool Uniform = false>
 __device__ __forceinline__ void pair32_rel_core(float tx, float ty, float tz, const T32& t, const SrcRel& s,
                                                 const float4* __restrict__ tab, const NearArgs& a, P32& p,
-                                                PSE32& e) {
+                                                PSE32F& e) {
   const float dx = tx - s.rx;
   const float dy = ty - s.ry;
   const float dz = tz - s.rz;
@@ -1574,15 +1579,14 @@ __device__ __forceinline__ void pair32_rel_core(float tx, float ty, float tz, co
     if (HighOrder) c = t.nu2 * kHoEta32 * (sig2 * sig2) * (ib7 * ib2);
     else if (Uniform) near32u(r2, a, tab, q, f, c);
     else near32t(r2, sig2, t.nu2, tab, q, f, c);
-    e.x = fmaf(c, t.vol * s.ax - s.vol * t.ax, e.x);
-    e.y = fmaf(c, t.vol * s.ay - s.vol * t.ay, e.y);
-    e.z = fmaf(c, t.vol * s.az - s.vol * t.az, e.z);
+    e.x = fmaf(c, s.ax, e.x);
+    e.y = fmaf(c, s.ay, e.y);
+    e.z = fmaf(c, s.az, e.z);
+    e.w = fmaf(c, s.vol, e.w);
   }
-  // wave-uniform choice: per pair if the source is in any lane's core
-  if (__ballot(core))
-    accumulate32_pair(dx, dy, dz, q, f, t.ax, t.ay, t.az, s, p);
-  else
-    accumulate32(dx, dy, dz, q, f, t.ax, t.ay, t.az, s, p);
+  // a run within the core range of the slice: stretching per pair (a_t x a_s
+  // formed per pair keeps its accuracy where a_s is nearly parallel to a_t)
+  accumulate32_pair(dx, dy, dz, q, f, t.ax, t.ay, t.az, s, p);
 }
 
 template <bool HighOrder, bool Uniform>
@@ -1613,7 +1617,7 @@ __device__ __forceinline__ void near_slice(const NearArgs& a, const NearItem& it
   float hi[kAcc], lo[kAcc];
 #pragma unroll
   for (int k = 0; k < kAcc; ++k) hi[k] = lo[k] = 0.0f;
-  PSE32 pse;
+  PSE32F pse;
   const int* __restrict__ L = a.list + static_cast<size_t>(T) * a.cap;
   for (int k = it.k0; k < it.k1; ++k) {
     const int id = L[k];
@@ -1682,9 +1686,11 @@ __device__ __forceinline__ void near_slice(const NearArgs& a, const NearItem& it
     r.ux = kQ * acc[0];
     r.uy = kQ * acc[1];
     r.uz = kQ * acc[2];
-    r.wx = kQ * acc[3] + static_cast<double>(pse.x);
-    r.wy = kQ * acc[4] + static_cast<double>(pse.y);
-    r.wz = kQ * acc[5] + static_cast<double>(pse.z);
+    // PSE: V_t sum c a_s - a_t sum c V_s, the products exact in FP64
+    const double tv = me.vol;
+    r.wx = kQ * acc[3] + (tv * pse.x - static_cast<double>(me.ax) * pse.w);
+    r.wy = kQ * acc[4] + (tv * pse.y - static_cast<double>(me.ay) * pse.w);
+    r.wz = kQ * acc[5] + (tv * pse.z - static_cast<double>(me.az) * pse.w);
     if (it.slot < 0) a.out[t] = r;
     else a.partial[static_cast<size_t>(it.slot) * kLeafSize + self] = r;
   }

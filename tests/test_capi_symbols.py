@@ -50,3 +50,15 @@ def test_ring_particles_reject_bad_radii_on_the_host():
         assert e.value.code == capi.PW_ERR_ARG
     pos, radius, volume = capi.vring_particles(1.0, 0.1, 3, 49, 0.1924)
     assert pos.shape == (49 * 49, 3) and np.all(radius == 0.1924) and np.all(volume > 0)
+
+
+def test_valu_ceiling_rejects_bad_arguments_on_the_host():
+    """pw_valu_ceiling checks its form and occupancy before touching a
+    device."""
+    lib = capi.load()
+    r = ctypes.c_double(0.0)
+    assert lib.pw_valu_ceiling(4, 32, ctypes.byref(r), None) == capi.PW_ERR_ARG
+    assert lib.pw_valu_ceiling(-1, 32, ctypes.byref(r), None) == capi.PW_ERR_ARG
+    assert lib.pw_valu_ceiling(0, 6, ctypes.byref(r), None) == capi.PW_ERR_ARG
+    assert lib.pw_valu_ceiling(0, 36, ctypes.byref(r), None) == capi.PW_ERR_ARG
+    assert lib.pw_valu_ceiling(0, 32, None, None) == capi.PW_ERR_ARG

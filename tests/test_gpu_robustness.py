@@ -110,3 +110,16 @@ def test_compensation_holds_where_a_plain_fp32_sum_fails():
     assert ec < 1e-6
     assert ep > 1e-5
     np.testing.assert_array_equal(u_c2, u_c)  # switching back restores the same sums
+
+
+def test_valu_ceilings_are_ordered():
+    """The issue-rate probe runs every form and reports a positive rate: FP32
+    FMAs on vector registers issue at least as fast as FP64 FMAs, and the CU
+    count is the device's."""
+    rates = {}
+    for form in capi.VALU_FORMS:
+        rates[form], cus = capi.valu_ceiling(form, 16)
+        assert rates[form] > 0.0
+    assert cus >= 1
+    assert rates["fp32_fma"] >= 0.9 * rates["fp64_fma"]
+    print(f"valu ceilings (wave instructions per CU per ns, 16 waves/CU): {rates}, CUs {cus}")
