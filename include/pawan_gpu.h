@@ -50,8 +50,9 @@ enum {
   PW_FP32C = 1, /* FP32 pair math, compensated per-target accumulation,
                    FP64 state, stage combination and diagnostics           */
   PW_LEARNED = 2, /* tree path: direct FP32C near field over Morton-sorted
-                     256-particle leaves, analytic monopole + dipole far
-                     field plus the learned residual network (pw_ff_*);
+                     256-particle leaves; far field by the analytic
+                     expansion to quadrupole order, or by monopole +
+                     dipole plus the learned residual network (pw_ff_*);
                      coupled systems only                                  */
   PW_FP64_REF = 3 /* FP64 with the reference's kernel formula evaluated for
                      every pair, operation by operation (PW_FP64 replaces
@@ -341,12 +342,14 @@ int pw_set_graph(pw_ctx *ctx, int enable);
  * (interaction.cpp:196-210).  Particles are sorted along a Morton curve
  * into leaves of 256, grouped 8 to a node; for each target leaf a node is
  * far when r_node + r_leaf < theta * d, it lies beyond the Gaussian core,
- * and the self-estimated error of its far evaluation,
- * coef * s2 / (d - r_leaf)^4 (s2 = sum |alpha| |x - centre|^2, coef the
- * calibrated coefficient of the evaluation in use), is at most tol.  Far
- * nodes are evaluated by the analytic monopole + dipole expansion plus,
- * when use_mlp, the network's residual; every other pair is summed
- * directly (FP32C).  tol <= 0 disables the error test.  With the
+ * and the self-estimated error of its far evaluation is at most tol.  With
+ * use_mlp = 0 far nodes are evaluated by the analytic expansion to
+ * quadrupole order, whose error is estimated as coef_md * s3 / (d - r_leaf)^5
+ * (s3 bounds sum |alpha| |x - centre|^3); with use_mlp = 1 by monopole +
+ * dipole plus the network's residual (the network stands in for the
+ * quadrupole and beyond), estimated as coef_mlp * s2 / (d - r_leaf)^4
+ * (s2 = sum |alpha| |x - centre|^2).  Every other pair is summed directly
+ * (FP32C).  tol <= 0 disables the error test.  With the
  * high-order kernel the error a node's far evaluation is held to also
  * counts (15/8) sigma_max^4 s0 / (4 pi gap^6), the largest deviation of the
  * kernel from the singular law over the node (gap = d - r_node - r_leaf,
@@ -360,8 +363,9 @@ int pw_ff_get_params(pw_ctx *ctx, double *theta, double *tol, int *use_mlp);
  * (interaction_utils.h:493-495); one-sided far-field evaluations are not. */
 int pw_ff_set_conserve(pw_ctx *ctx, int enable);
 int pw_ff_get_conserve(pw_ctx *ctx, int *enable);
-/* error coefficients of the analytic expansion and of expansion + network
- * (set by pw_ff_train from held-out samples; may be overridden) */
+/* error coefficients of the analytic expansion (quadrupole order, against
+ * s3 / d^5) and of monopole + dipole + network (against s2 / d^4), set by
+ * pw_ff_train from held-out samples; may be overridden */
 int pw_ff_set_err_coef(pw_ctx *ctx, double coef_md, double coef_mlp);
 int pw_ff_get_err_coef(pw_ctx *ctx, double *coef_md, double *coef_mlp);
 /* build the tree and the lists of the current state (done by every
@@ -386,13 +390,16 @@ int pw_ff_get_timing(pw_ctx *ctx, double *out, int64_t len);
  * sum a_i delta_j), first, last, level, child), PW_FF_NEAR_CNT /
  * PW_FF_FAR_CNT int32[leaves], PW_FF_NEAR / PW_FF_FAR int32[leaves][cap],
  * PW_FF_SAMPLES_X float[ns][16], PW_FF_SAMPLES_Y float[ns][12],
- * PW_FF_SAMPLES_AUX double[ns][10] (u scale, gradient scale, |x - c|, s2,
- * |expansion error of 4 pi u|, |4 pi u|, node radius, node, target
- * (sorted index), target leaf) */
+ * PW_FF_SAMPLES_AUX double[ns][12] (u scale, gradient scale, |x - c|, s2,
+ * |monopole + dipole error of 4 pi u|, |4 pi u|, node radius, node, target
+ * (sorted index), target leaf, |error of 4 pi u to quadrupole order|, s3),
+ * PW_FF_QUAD double[nodes][20] (quadrupole
+ * sum a_p delta_j delta_l for p = x, y, z, each as jl = xx yy zz xy xz yz;
+ * then the bounds s1 of sum |a| |delta| and s3 of sum |a| |delta|^3) */
 enum {
   PW_FF_PERM = 0, PW_FF_NODES = 1, PW_FF_NEAR_CNT = 2, PW_FF_NEAR = 3,
   PW_FF_FAR_CNT = 4, PW_FF_FAR = 5, PW_FF_SAMPLES_X = 6, PW_FF_SAMPLES_Y = 7,
-  PW_FF_SAMPLES_AUX = 8
+  PW_FF_SAMPLES_AUX = 8, PW_FF_QUAD = 9
 };
 int pw_ff_export(pw_ctx *ctx, int what, void *out, int64_t bytes);
 
@@ -412,7 +419,8 @@ int pw_ff_get_weights(pw_ctx *ctx, float *params, int64_t n);
  * minibatches of `batch`, learning rate lr with cosine decay to lr/20.
  * Then the BF16 inference network is packed, evaluated on the held-out
  * samples, and the error coefficients are calibrated (99th percentile of
- * |error| / (s2/d^4)).  report (may be NULL) gets up to 10 doubles: final
+ * |error| / (s3/d^5) for the analytic expansion, |error| / (s2/d^4) for
+ * monopole + dipole + network).  report (may be NULL) gets up to 10 doubles: final
  * train loss, held-out loss, held-out loss of the zero residual (expansion
  * only), rms velocity error ratio network/expansion, the same for the
  * gradient, coef_md, coef_mlp, steps, held-out samples, milliseconds. */

@@ -14,6 +14,12 @@ pinned here by mathematics instead (tests/test_oracle_farfield.py):
     dipole term leaves an O(eps) error); the transposed contraction
     alpha^T G equals the reference's pair stretching summed over the cluster
     for a one-particle cluster.
+  * quad_eval (the quadrupole term, written with the explicit second and
+    third derivative tensors of r/|r|^3): md_eval + quad_eval against the
+    exact cluster sum falls as eps^3 (a dropped or wrong quadrupole leaves
+    eps^2); its gradient equals the finite difference of its velocity; for
+    one particle off the centre it equals half the second directional
+    difference of that particle's singular field.
   * morton_perm: the codes of hand-placed points.
   * leaf_first: hand-made codes (a dense cell cut into runs of 256, separate
     cells never sharing a leaf).
@@ -22,7 +28,7 @@ pinned here by mathematics instead (tests/test_oracle_farfield.py):
   * mlp_*: forward and backward of the network, the gradient checked
     against central finite differences.
 
-parity: md_eval, exact_node, morton_perm, node_moments, mlp_forward and
+parity: md_eval, quad_eval, exact_node, morton_perm, node_moments, mlp_forward and
 mlp_grad are pinned as above; build_lists is checked through properties
 (every source leaf covered exactly once per target leaf; far nodes satisfy
 the opening test), not against a reference list.
@@ -136,13 +142,16 @@ def node_moments(pos: np.ndarray, alpha: np.ndarray, sigma: np.ndarray, lf: np.n
     """Moments of every node straight from its particles (sorted arrays;
     lf = leaf_first of the sorted codes).
 
-    Returns dict of arrays: centre (k,3), r, s0, s2, sg, M (k,3), D (k,3,3)
-    with D[i, j] = sum a_i delta_j, plus first/last/level/child.
+    Returns dict of arrays: centre (k,3), r, s0, s1, s2, s3, sg, M (k,3),
+    D (k,3,3) with D[i, j] = sum a_i delta_j, Q (k,3,3,3) with
+    Q[p, j, l] = sum a_p delta_j delta_l, plus first/last/level/child
+    (s_n = sum |a| |delta|^n).
     """
     first, last, level, child = node_ranges(lf)
     k = len(first)
-    out = {"centre": np.zeros((k, 3)), "r": np.zeros(k), "s0": np.zeros(k), "s2": np.zeros(k),
-           "sg": np.zeros(k), "M": np.zeros((k, 3)), "D": np.zeros((k, 3, 3)),
+    out = {"centre": np.zeros((k, 3)), "r": np.zeros(k), "s0": np.zeros(k), "s1": np.zeros(k),
+           "s2": np.zeros(k), "s3": np.zeros(k), "sg": np.zeros(k), "M": np.zeros((k, 3)),
+           "D": np.zeros((k, 3, 3)), "Q": np.zeros((k, 3, 3, 3)),
            "first": first, "last": last, "level": level, "child": child}
     am = np.linalg.norm(alpha, axis=1)
     for i in range(k):
@@ -152,10 +161,14 @@ def node_moments(pos: np.ndarray, alpha: np.ndarray, sigma: np.ndarray, lf: np.n
         out["centre"][i] = c
         out["r"][i] = np.sqrt((d * d).sum(axis=1)).max()
         out["s0"][i] = am[a:b].sum()
-        out["s2"][i] = (am[a:b] * (d * d).sum(axis=1)).sum()
+        dn = np.sqrt((d * d).sum(axis=1))
+        out["s1"][i] = (am[a:b] * dn).sum()
+        out["s2"][i] = (am[a:b] * dn * dn).sum()
+        out["s3"][i] = (am[a:b] * dn ** 3).sum()
         out["sg"][i] = sigma[a:b].mean()
         out["M"][i] = alpha[a:b].sum(axis=0)
         out["D"][i] = alpha[a:b].T @ d
+        out["Q"][i] = np.einsum("sp,sj,sl->pjl", alpha[a:b], d, d)
     return out
 
 
@@ -208,6 +221,54 @@ def md_eval(r: np.ndarray, M: np.ndarray, D: np.ndarray):
     return u, G
 
 
+def _eps():
+    e = np.zeros((3, 3, 3))
+    e[0, 1, 2] = e[1, 2, 0] = e[2, 0, 1] = 1.0
+    e[0, 2, 1] = e[2, 1, 0] = e[1, 0, 2] = -1.0
+    return e
+
+
+QUAD_SLOTS = ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))
+
+
+def quad_from_slots(q18: np.ndarray) -> np.ndarray:
+    """Q[p, j, l] (3,3,3) from the 18 exported values: for each component
+    p, the pairs (j, l) in the order xx yy zz xy xz yz."""
+    Q = np.zeros((3, 3, 3))
+    for p in range(3):
+        for m, (j, l) in enumerate(QUAD_SLOTS):
+            Q[p, j, l] = Q[p, l, j] = q18[6 * p + m]
+    return Q
+
+
+def quad_eval(r: np.ndarray, Q: np.ndarray):
+    """Quadrupole term of the singular kernel at offsets r (k,3) from the
+    centre, Q[p, j, l] = sum a_p delta_j delta_l: u'_k = (1/2) eps_kpi
+    Q[p,j,l] d_j d_l K_i(r), K = r/|r|^3, and G'_km = du'_k/dx_m with the
+    third derivatives of K.  Returns u' (k,3), G' (k,3,3)."""
+    r = np.atleast_2d(r)
+    eye = np.eye(3)
+    eps = _eps()
+    us, Gs = [], []
+    for x in r:
+        r2 = x @ x
+        i5, i7, i9 = r2 ** -2.5, r2 ** -3.5, r2 ** -4.5
+        # d_j d_l K_i
+        T2 = (-3.0 * (np.einsum("ij,l->ijl", eye, x) + np.einsum("il,j->ijl", eye, x)
+                      + np.einsum("jl,i->ijl", eye, x)) * i5
+              + 15.0 * np.einsum("i,j,l->ijl", x, x, x) * i7)
+        # d_m d_j d_l K_i
+        T3 = (-3.0 * (np.einsum("ij,lm->ijlm", eye, eye) + np.einsum("il,jm->ijlm", eye, eye)
+                      + np.einsum("jl,im->ijlm", eye, eye)) * i5
+              + 15.0 * (np.einsum("ij,l,m->ijlm", eye, x, x) + np.einsum("il,j,m->ijlm", eye, x, x)
+                        + np.einsum("jl,i,m->ijlm", eye, x, x) + np.einsum("im,j,l->ijlm", eye, x, x)
+                        + np.einsum("jm,i,l->ijlm", eye, x, x) + np.einsum("lm,i,j->ijlm", eye, x, x)) * i7
+              - 105.0 * np.einsum("i,j,l,m->ijlm", x, x, x, x) * i9)
+        us.append(0.5 * np.einsum("kpi,pjl,ijl->k", eps, Q, T2))
+        Gs.append(0.5 * np.einsum("kpi,pjl,ijlm->km", eps, Q, T3))
+    return np.array(us), np.array(Gs)
+
+
 def exact_node(x: np.ndarray, sig_t: float, pos: np.ndarray, alpha: np.ndarray, sigma: np.ndarray):
     """u', G' at target x of the particles (pos, alpha, sigma) with the
     reference's Gaussian kernel (gaussian.h:47-60): q = (phi/rho_bar - Z) /
@@ -253,7 +314,9 @@ def frame(r: np.ndarray, M: np.ndarray, s0: float) -> np.ndarray:
 
 def features(r: np.ndarray, ell: float, s0: float, M: np.ndarray, D: np.ndarray, sg: float):
     """Network inputs of offsets r (k,3) from a node, in the frame of each
-    offset, with the frames and output scales: (X, su, sG, R)."""
+    offset, with the frames and output scales: (X, su, sG, R); the scales
+    are those of the first term monopole + dipole leave out (the
+    quadrupole), su = s0 l^2 / r^4, sG = su / r."""
     r = np.atleast_2d(r)
     d2 = (r * r).sum(axis=1)
     X = np.zeros((len(r), N_IN))

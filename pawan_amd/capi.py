@@ -44,7 +44,8 @@ EXPORTS = [
 ]
 
 # pw_ff_export selectors
-FF_PERM, FF_NODES, FF_NEAR_CNT, FF_NEAR, FF_FAR_CNT, FF_FAR, FF_SAMPLES_X, FF_SAMPLES_Y, FF_SAMPLES_AUX = range(9)
+(FF_PERM, FF_NODES, FF_NEAR_CNT, FF_NEAR, FF_FAR_CNT, FF_FAR, FF_SAMPLES_X, FF_SAMPLES_Y, FF_SAMPLES_AUX,
+ FF_QUAD) = range(10)
 MLP_PARAMS = 36748
 FF_INFO_NAMES = ["n", "leaves", "nodes", "levels", "cap_near", "cap_far", "samples", "near_pairs",
                  "far_evals", "overflow", "max_near", "max_far", "near_core_pairs", "mlp_evals"]
@@ -464,7 +465,8 @@ class Solver:
             FF_FAR: (np.int32, (nl, info["cap_far"])),
             FF_SAMPLES_X: (np.float32, (info["samples"], 16)),
             FF_SAMPLES_Y: (np.float32, (info["samples"], 12)),
-            FF_SAMPLES_AUX: (np.float64, (info["samples"], 10)),
+            FF_SAMPLES_AUX: (np.float64, (info["samples"], 12)),
+            FF_QUAD: (np.float64, (info["nodes"], 20)),
         }
         dt, shape = shapes[what]
         out = np.zeros(shape, dtype=dt)

@@ -48,8 +48,8 @@ struct FarField {
   bool use_mlp = true;
   bool conserve = true;              // remove the mean vorticity rate
   double* d_dpart = nullptr;         // block sums for it
-  double coef_md = 0.2387324146;     // 3/(4 pi) until calibrated
-  double coef_mlp = 0.2387324146;
+  double coef_md = 0.3183098862;     // analytic path: 4/(4 pi), the octupole's bound, until calibrated
+  double coef_mlp = 0.2387324146;    // network path: 3/(4 pi), the quadrupole's bound, until calibrated
   // network
   bool have_weights = false;
   std::vector<float> h_params;

@@ -1,6 +1,7 @@
-// pw_far.hip -- far field of the tree path: analytic monopole + dipole
-// evaluation, the learned residual (MLP on MFMA), training-sample
-// generation and the scatter back to the caller's particle order.
+// pw_far.hip -- far field of the tree path: analytic monopole + dipole +
+// quadrupole evaluation, the learned residual (MLP on MFMA),
+// training-sample generation and the scatter back to the caller's particle
+// order.
 //
 // Units: inside this file velocities and gradients carry a factor 4 pi
 // (u' = 4 pi u); the kernels multiply by 1/(4 pi) when they add to the
@@ -12,14 +13,24 @@
 //         + 3 w r^T / r^5 + 3 (D_:m x r + p x e_m)_k / r^5 - 15 (p x r) r^T / r^7
 // with w_k = eps_kij D_ij, p = D r, G_km = du_k/dx_m (the singular kernel
 // 1/(4 pi r^3) expanded about c to first order in the particle offsets).
+// The second-order (quadrupole) term, with Q^(p)_jl = sum a_p delta_j
+// delta_l, E_p = Q^(p) r, S_p = r.E_p, P_p = trace Q^(p),
+// V_m = eps_mpq E_pq:
+//   u'_Q = -3 V / r^5 - 1.5 (P x r) / r^5 + 7.5 (S x r) / r^7,
+// and its stretching of a strength a, grad (a.u'_Q):
+//   (-3 g - 1.5 a x P) / r^5 + 7.5 (2 sum_j c_j E_j + a x S) / r^7
+//   + (15 a.V / r^7 + 7.5 a.(P x r) / r^7 - 52.5 a.(S x r) / r^9) r
+// with c = r x a and g_j = sum_m a_m Wq_jm, Wq_jm = eps_mpq Q^(p)_jq.
 // The vorticity rate of the target is the transposed contraction
 // dalpha_m = sum_k alpha_k G_km, which is the reference's transposed
 // stretching q (a_t x a_s) + F [d.(a_t x a_s)] d summed over the node
 // (src/interaction/interaction_utils.h:456-501).
 //
-// Learned residual: an MLP 16 -> 128 -> 128 -> 128 -> 12 (GELU, tanh form)
-// maps node/target features to the residual of the expansion (truth -
-// monopole - dipole) in scaled units:
+// The analytic path (no network) evaluates the expansion to quadrupole
+// order.  The network path evaluates monopole + dipole, and the network
+// stands in for the rest: an MLP 16 -> 128 -> 128 -> 128 -> 12 (GELU, tanh
+// form) maps node/target features to the residual (truth - monopole -
+// dipole) in scaled units:
 //   inputs  e = r l / r^2 (3), M / s0 (3), D / (s0 l) (9), sigma / l (1)
 //   outputs u' residual / (s0 l^2 / r^4) (3), G' residual / (s0 l^2 / r^5) (9)
 // all expressed in a frame fixed by M and r (make_frame below)
@@ -99,6 +110,61 @@ __device__ __forceinline__ void md_eval(const float r[3], const float M[3], cons
   }
 }
 
+// Quadrupole slot of the pair (j, l): xx yy zz xy xz yz.
+__device__ __forceinline__ constexpr int qslot(int j, int l) {
+  return j == l ? j : (j + l == 1 ? 3 : (j + l == 2 ? 4 : 5));
+}
+// Wq_jm = eps_mpq Q^(p)_jq (row-major, 9) and P_p = trace Q^(p), from Q
+__device__ __forceinline__ void quad_derived(const float Q[kQuad], float Wq[9], float P[3]) {
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    Wq[3 * j + 0] = Q[6 * 1 + qslot(j, 2)] - Q[6 * 2 + qslot(j, 1)];
+    Wq[3 * j + 1] = Q[6 * 2 + qslot(j, 0)] - Q[6 * 0 + qslot(j, 2)];
+    Wq[3 * j + 2] = Q[6 * 0 + qslot(j, 1)] - Q[6 * 1 + qslot(j, 0)];
+  }
+#pragma unroll
+  for (int p = 0; p < 3; ++p) P[p] = Q[6 * p] + Q[6 * p + 1] + Q[6 * p + 2];
+}
+
+// Quadrupole of a node at offset r (FP32, 4 pi factored): u += u'_Q when
+// with_u, w += grad (a . u'_Q), the stretching of strength a.  ir2 = 1/r^2,
+// ir5 = 1/r^5.
+__device__ __forceinline__ void quad_accumulate(const float r[3], const float Q[kQuad], const float Wq[9],
+                                                const float P[3], const float a[3], float ir2, float ir5,
+                                                bool with_u, float u[3], float w[3]) {
+  const float ir7 = ir5 * ir2, ir9 = ir7 * ir2;
+  float E[3][3], S[3];
+#pragma unroll
+  for (int p = 0; p < 3; ++p) {
+    const float* q = Q + 6 * p;
+    E[p][0] = fmaf(q[4], r[2], fmaf(q[3], r[1], q[0] * r[0]));
+    E[p][1] = fmaf(q[5], r[2], fmaf(q[1], r[1], q[3] * r[0]));
+    E[p][2] = fmaf(q[2], r[2], fmaf(q[5], r[1], q[4] * r[0]));
+    S[p] = fmaf(E[p][2], r[2], fmaf(E[p][1], r[1], E[p][0] * r[0]));
+  }
+  const float V[3] = {E[1][2] - E[2][1], E[2][0] - E[0][2], E[0][1] - E[1][0]};
+  const float Pr[3] = {P[1] * r[2] - P[2] * r[1], P[2] * r[0] - P[0] * r[2], P[0] * r[1] - P[1] * r[0]};
+  const float Sr[3] = {S[1] * r[2] - S[2] * r[1], S[2] * r[0] - S[0] * r[2], S[0] * r[1] - S[1] * r[0]};
+  const float c5 = -3.0f * ir5, h5 = -1.5f * ir5, c7 = 7.5f * ir7;
+  if (with_u) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) u[k] = fmaf(c5, V[k], fmaf(h5, Pr[k], fmaf(c7, Sr[k], u[k])));
+  }
+  const float f1 = fmaf(a[2], V[2], fmaf(a[1], V[1], a[0] * V[0]));
+  const float f2 = fmaf(a[2], Pr[2], fmaf(a[1], Pr[1], a[0] * Pr[0]));
+  const float f3 = fmaf(a[2], Sr[2], fmaf(a[1], Sr[1], a[0] * Sr[0]));
+  const float cr[3] = {r[1] * a[2] - r[2] * a[1], r[2] * a[0] - r[0] * a[2], r[0] * a[1] - r[1] * a[0]};  // r x a
+  const float aP[3] = {a[1] * P[2] - a[2] * P[1], a[2] * P[0] - a[0] * P[2], a[0] * P[1] - a[1] * P[0]};
+  const float aS[3] = {a[1] * S[2] - a[2] * S[1], a[2] * S[0] - a[0] * S[2], a[0] * S[1] - a[1] * S[0]};
+  const float rad = fmaf(15.0f * ir7, f1, fmaf(7.5f * ir7, f2, -52.5f * ir9 * f3));
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const float g = fmaf(a[2], Wq[3 * m + 2], fmaf(a[1], Wq[3 * m + 1], a[0] * Wq[3 * m]));
+    const float cE = fmaf(cr[2], E[2][m], fmaf(cr[1], E[1][m], cr[0] * E[0][m]));
+    w[m] += fmaf(rad, r[m], fmaf(c5, g, fmaf(h5, aP[m], c7 * fmaf(2.0f, cE, aS[m]))));
+  }
+}
+
 // Node data staged in LDS for the far kernels.
 // ---------------------------------------------------------------------
 // Analytic far field: one thread per target of a leaf, far nodes staged
@@ -116,9 +182,12 @@ struct FarNodeMD {
   float rc[3];       // centre relative to the target leaf's centre
   float M[3], W[3];  // monopole, dipole's antisymmetric part
   float D[9];
+  float Q[kQuad], Wq[9], P[3];  // quadrupole and the parts derived from it
 };
 
-__device__ __forceinline__ void stage_md(const NodeRec& q, double tcx, double tcy, double tcz, FarNodeMD& f) {
+template <bool Quad>
+__device__ __forceinline__ void stage_md(const NodeRec& q, const NodeQuad* qq, double tcx, double tcy, double tcz,
+                                         FarNodeMD& f) {
   f.rc[0] = static_cast<float>(q.cx - tcx);
   f.rc[1] = static_cast<float>(q.cy - tcy);
   f.rc[2] = static_cast<float>(q.cz - tcz);
@@ -129,10 +198,16 @@ __device__ __forceinline__ void stage_md(const NodeRec& q, double tcx, double tc
   f.W[0] = q.D[5] - q.D[7];
   f.W[1] = q.D[6] - q.D[2];
   f.W[2] = q.D[1] - q.D[3];
+  if (Quad) {
+#pragma unroll
+    for (int k = 0; k < kQuad; ++k) f.Q[k] = qq->Q[k];
+    quad_derived(f.Q, f.Wq, f.P);
+  }
 }
 
 // u += field of node n at offset r, w += its stretching of strength a
-// (4 pi factored; the same terms as md_eval)
+// (4 pi factored; the same terms as md_eval, then with Quad the quadrupole)
+template <bool Quad>
 __device__ __forceinline__ void md_accumulate(const float r[3], const FarNodeMD& n, const float a[3], float u[3],
                                               float w[3]) {
   const float r2 = fmaf(r[2], r[2], fmaf(r[1], r[1], r[0] * r[0]));
@@ -160,17 +235,20 @@ __device__ __forceinline__ void md_accumulate(const float r[3], const FarNodeMD&
   w[0] += fmaf(av, r[0], fmaf(a[1], c[2], fmaf(-a[2], c[1], b5 * fmaf(D[6], sz, fmaf(D[3], sy, D[0] * sx)))));
   w[1] += fmaf(av, r[1], fmaf(a[2], c[0], fmaf(-a[0], c[2], b5 * fmaf(D[7], sz, fmaf(D[4], sy, D[1] * sx)))));
   w[2] += fmaf(av, r[2], fmaf(a[0], c[1], fmaf(-a[1], c[0], b5 * fmaf(D[8], sz, fmaf(D[5], sy, D[2] * sx)))));
+  if (Quad) quad_accumulate(r, n.Q, n.Wq, n.P, a, ir2, a3 * ir2, true, u, w);
 }
 
 struct FarArgs {
   const Rec64* sorted;
   const NodeRec* nodes;
+  const NodeQuad* quad;  // the analytic path's quadrupoles (nullptr on the network path)
   const int* list;
   const int* cnt;
   int cap;
   Rate64* rates;
 };
 
+template <bool Quad>
 __global__ __launch_bounds__(256) void k_far_md(FarArgs a) {
   __shared__ FarNodeMD sn[kNodeChunk];
   const int T = blockIdx.x;
@@ -188,12 +266,15 @@ __global__ __launch_bounds__(256) void k_far_md(FarArgs a) {
   for (int base = 0; base < nf; base += kNodeChunk) {
     const int c = min(kNodeChunk, nf - base);
     __syncthreads();
-    if (static_cast<int>(threadIdx.x) < c) stage_md(a.nodes[L[base + threadIdx.x]], tcx, tcy, tcz, sn[threadIdx.x]);
+    if (static_cast<int>(threadIdx.x) < c) {
+      const int id = L[base + threadIdx.x];
+      stage_md<Quad>(a.nodes[id], Quad ? a.quad + id : nullptr, tcx, tcy, tcz, sn[threadIdx.x]);
+    }
     __syncthreads();
     if (!active) continue;
     for (int k = 0; k < c; ++k) {
       const float r[3] = {rt[0] - sn[k].rc[0], rt[1] - sn[k].rc[1], rt[2] - sn[k].rc[2]};
-      md_accumulate(r, sn[k], at, u, w);
+      md_accumulate<Quad>(r, sn[k], at, u, w);
     }
   }
   if (!active) return;
@@ -365,7 +446,8 @@ __device__ __forceinline__ void make_frame(const float r[3], const float M[3], f
 }
 
 // All 16 inputs of one (point, node) sample at offset r = x - c, the frame,
-// and the output scales: u' scale s0 l^2 / r^4, G' scale s0 l^2 / r^5.
+// and the output scales: u' scale s0 l^2 / r^4, G' scale s0 l^2 / r^5 (those
+// of the quadrupole, the first term monopole + dipole leave out).
 __device__ __forceinline__ void sample_inputs(const NodeRec& q, const float r[3], float feat[16], float R[9],
                                               float& su, float& sG) {
   const float ell = q.r + q.sg;
@@ -557,8 +639,10 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const float* __restrict__ X
 // ---------------------------------------------------------------------
 // Training samples: sample k = (target t of leaf T, the node at position
 // slot of T's far list).  Exact node sum at the target in FP64 with the
-// reference's Gaussian kernel (gaussian.h:47-60), minus the FP32 analytic
-// expansion above, in the scaled units of the network.
+// reference's Gaussian kernel (gaussian.h:47-60), minus the FP32 monopole +
+// dipole, in the scaled units of the network (its target); the error of
+// the expansion to quadrupole order goes to the side data (the analytic
+// path's calibration).
 // ---------------------------------------------------------------------
 __device__ __forceinline__ void exact_pair(double dx, double dy, double dz, double sig2, const Rec64& s,
                                            double u[3], double G[9]) {
@@ -587,6 +671,7 @@ __device__ __forceinline__ void exact_pair(double dx, double dy, double dz, doub
 struct SampleArgs {
   const Rec64* sorted;
   const NodeRec* nodes;
+  const NodeQuad* quad;
   const int* far_list;
   int cap;
   const int* spec;  // per sample: leaf T, list slot, target (sorted index)
@@ -620,16 +705,29 @@ __global__ __launch_bounds__(256) void k_ff_samples(SampleArgs a) {
                       static_cast<float>(me.z - q.cz)};
   float uu[3], gg[9];
   md_eval(r, q.M, q.D, uu, gg);
+  float uq[3] = {uu[0], uu[1], uu[2]};  // + the quadrupole's velocity
+  {
+    const NodeQuad qq = a.quad[id];
+    float Wq[9], P[3], wq[3] = {0.0f, 0.0f, 0.0f};
+    quad_derived(qq.Q, Wq, P);
+    const float r2 = fmaf(r[2], r[2], fmaf(r[1], r[1], r[0] * r[0]));
+    const float ir2 = 1.0f / r2;
+    const float ir5 = ir2 * ir2 * sqrtf(ir2);
+    const float e0[3] = {1.0f, 0.0f, 0.0f};
+    quad_accumulate(r, qq.Q, Wq, P, e0, ir2, ir5, true, uq, wq);
+  }
   // inputs in FP32 (the network rounds them to BF16 itself) and the frame
   float feat[16], R[9], su, sG;
   sample_inputs(q, r, feat, R, su, sG);
-  double res[12], err_md = 0.0, un = 0.0;
+  double res[12], err_md = 0.0, un = 0.0, err_q = 0.0;
   for (int m = 0; m < 12; ++m) {
     const double fourpi_exact = v[m] * 12.566370614359172954;  // 4 pi
     res[m] = fourpi_exact - static_cast<double>(m < 3 ? uu[m] : gg[m - 3]);
     if (m < 3) {
       err_md += res[m] * res[m];
       un += fourpi_exact * fourpi_exact;
+      const double eq = fourpi_exact - static_cast<double>(uq[m]);
+      err_q += eq * eq;
     }
   }
   // targets in the frame: R res_u / su, R res_G R^T / sG
@@ -657,6 +755,8 @@ __global__ __launch_bounds__(256) void k_ff_samples(SampleArgs a) {
   o[7] = id;               // node id
   o[8] = t;                // target (Morton-sorted index)
   o[9] = T;                // target leaf
+  o[10] = sqrt(err_q);     // |u' exact - u' to quadrupole order|
+  o[11] = q.s3;            // node s3 (bound)
 }
 
 // Conservation of total vorticity: the pairwise antisymmetry that makes
@@ -719,25 +819,27 @@ __global__ __launch_bounds__(256) void k_scatter(const Rate64* __restrict__ in, 
 
 size_t mlp_far_lds_bytes() { return sizeof(MlpLds) + (kMlpWaves * 32 * kXch + 16 * kMlpWaves + 16) * sizeof(float); }
 
-hipError_t launch_far(const Rec64* sorted, const NodeRec* nodes, const int* far_list, const int* far_cnt,
-                      int cap_far, int nleaf, const MlpWeights* mlp, Rate64* rates_s, hipStream_t st) {
-  hipError_t e = launch_far_md(sorted, nodes, far_list, far_cnt, cap_far, nleaf, rates_s, st);
+hipError_t launch_far(const Rec64* sorted, const NodeRec* nodes, const NodeQuad* quad, const int* far_list,
+                      const int* far_cnt, int cap_far, int nleaf, const MlpWeights* mlp, Rate64* rates_s,
+                      hipStream_t st) {
+  hipError_t e = launch_far_md(sorted, nodes, quad, far_list, far_cnt, cap_far, nleaf, rates_s, st);
   if (e != hipSuccess || mlp == nullptr) return e;
   return launch_far_mlp(sorted, nodes, far_list, far_cnt, cap_far, nleaf, *mlp, rates_s, st);
 }
 
-hipError_t launch_far_md(const Rec64* sorted, const NodeRec* nodes, const int* far_list, const int* far_cnt,
-                         int cap_far, int nleaf, Rate64* rates_s, hipStream_t st) {
+hipError_t launch_far_md(const Rec64* sorted, const NodeRec* nodes, const NodeQuad* quad, const int* far_list,
+                         const int* far_cnt, int cap_far, int nleaf, Rate64* rates_s, hipStream_t st) {
   if (nleaf <= 0) return hipSuccess;
-  FarArgs a{sorted, nodes, far_list, far_cnt, cap_far, rates_s};
-  hipLaunchKernelGGL(k_far_md, dim3(nleaf), dim3(256), 0, st, a);
+  FarArgs a{sorted, nodes, quad, far_list, far_cnt, cap_far, rates_s};
+  if (quad) hipLaunchKernelGGL(k_far_md<true>, dim3(nleaf), dim3(256), 0, st, a);
+  else hipLaunchKernelGGL(k_far_md<false>, dim3(nleaf), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 
 hipError_t launch_far_mlp(const Rec64* sorted, const NodeRec* nodes, const int* far_list, const int* far_cnt,
                           int cap_far, int nleaf, const MlpWeights& mlp, Rate64* rates_s, hipStream_t st) {
   if (nleaf <= 0) return hipSuccess;
-  FarArgs a{sorted, nodes, far_list, far_cnt, cap_far, rates_s};
+  FarArgs a{sorted, nodes, nullptr, far_list, far_cnt, cap_far, rates_s};
   hipError_t e;
   const size_t lds = mlp_far_lds_bytes();
   static std::atomic<unsigned long long> lds_raised{0};  // LDS beyond the 64 KB default
@@ -774,11 +876,11 @@ hipError_t launch_mlp_forward(const float* X, int n, float* Y, const MlpWeights&
   return hipGetLastError();
 }
 
-hipError_t launch_ff_samples(const Rec64* sorted, const NodeRec* nodes, const int* far_list, int cap,
-                             const int* spec, int nsamples, float* X, float* Y, double* aux,
+hipError_t launch_ff_samples(const Rec64* sorted, const NodeRec* nodes, const NodeQuad* quad, const int* far_list,
+                             int cap, const int* spec, int nsamples, float* X, float* Y, double* aux,
                              hipStream_t st) {
   if (nsamples <= 0) return hipSuccess;
-  SampleArgs a{sorted, nodes, far_list, cap, spec, nsamples, X, Y, aux};
+  SampleArgs a{sorted, nodes, quad, far_list, cap, spec, nsamples, X, Y, aux};
   hipLaunchKernelGGL(k_ff_samples, dim3(nsamples), dim3(256), 0, st, a);
   return hipGetLastError();
 }

@@ -32,7 +32,8 @@ uint64_t splitmix(uint64_t& s) {
 
 void free_leaf_buffers(pwi::FarField& f) {
   auto& tb = f.tb;
-  for (void* p : {static_cast<void*>(tb.leaf_first), static_cast<void*>(tb.nodes), static_cast<void*>(tb.near_list),
+  for (void* p : {static_cast<void*>(tb.leaf_first), static_cast<void*>(tb.nodes), static_cast<void*>(tb.quad),
+                  static_cast<void*>(tb.near_list),
                   static_cast<void*>(tb.near_cnt), static_cast<void*>(tb.far_list), static_cast<void*>(tb.far_cnt),
                   static_cast<void*>(tb.near_parts), static_cast<void*>(tb.near_offs),
                   static_cast<void*>(tb.near_items), static_cast<void*>(tb.near_partial),
@@ -40,6 +41,7 @@ void free_leaf_buffers(pwi::FarField& f) {
     (void)hipFree(p);
   tb.leaf_first = nullptr;
   tb.nodes = nullptr;
+  tb.quad = nullptr;
   tb.near_list = tb.near_cnt = tb.far_list = tb.far_cnt = nullptr;
   tb.near_parts = tb.near_offs = nullptr;
   tb.near_items = nullptr;
@@ -68,6 +70,7 @@ int ensure_leaf_capacity(pw_ctx* c, int nleaf) {
   if (const char* e = std::getenv("PAWAN_FF_LIST_CAP")) cap0 = std::max(1, std::atoi(e));
   const int cap_near = std::min(cap, cap0), cap_far = std::min(sh.nnodes, cap0);
   if (dalloc(&tb.leaf_first, nl + 1) != hipSuccess || dalloc(&tb.nodes, sh.nnodes) != hipSuccess ||
+      dalloc(&tb.quad, sh.nnodes) != hipSuccess ||
       dalloc(&tb.near_list, nl * cap_near) != hipSuccess || dalloc(&tb.near_cnt, nl) != hipSuccess ||
       dalloc(&tb.far_list, nl * cap_far) != hipSuccess || dalloc(&tb.far_cnt, nl) != hipSuccess ||
       dalloc(&tb.near_parts, nl) != hipSuccess || dalloc(&tb.near_offs, nl) != hipSuccess ||
@@ -155,6 +158,7 @@ pw::ListParams list_params(const pw_ctx* c, bool mac_only) {
   lp.tol = mac_only ? 0.0f : f.tol;
   lp.err_coef = static_cast<float>(f.use_mlp ? f.coef_mlp : f.coef_md);
   lp.sigma_max2 = f.sigma_max2;
+  lp.quad = f.use_mlp ? 0 : 1;  // the network rides on monopole + dipole
   lp.ho_coef = 0.0f;
   lp.ho_gap2 = 0.0f;
   if (c->smoothing == PW_KERNEL_HIGHORDER && !mac_only) {
@@ -329,8 +333,8 @@ int ff_rhs_body(pw_ctx* c, const Rec64* state, Rate64* out, const StageEvents* e
                                c->num_cus * pw::near_kernel_blocks_per_cu(ho, f.uniform_sigma), f.rates_s,
                                c->stream));
   PW_HIP(mark(2));
-  PW_HIP(pw::launch_far_md(tb.sorted, tb.nodes, tb.far_list, tb.far_cnt, tb.cap_far, f.shape.nleaf, f.rates_s,
-                           c->stream));
+  PW_HIP(pw::launch_far_md(tb.sorted, tb.nodes, f.use_mlp ? nullptr : tb.quad, tb.far_list, tb.far_cnt,
+                           tb.cap_far, f.shape.nleaf, f.rates_s, c->stream));
   PW_HIP(mark(3));
   if (f.use_mlp) {
     pw::MlpWeights W{f.d_frag, f.d_bias};
@@ -559,6 +563,22 @@ int pw_ff_export(pw_ctx* c, int what, void* out, int64_t bytes) {
       }
       return PW_OK;
     }
+    case PW_FF_QUAD: {
+      const int64_t nn = f.shape.nnodes;
+      if (bytes != 8 * 20 * nn) return fail(PW_ERR_ARG, "export needs exactly " + std::to_string(8 * 20 * nn) + " bytes");
+      std::vector<pw::NodeRec> h(static_cast<size_t>(nn));
+      std::vector<pw::NodeQuad> hq(static_cast<size_t>(nn));
+      PW_HIP(hipMemcpyAsync(h.data(), tb.nodes, h.size() * sizeof(pw::NodeRec), hipMemcpyDeviceToHost, c->stream));
+      PW_HIP(hipMemcpyAsync(hq.data(), tb.quad, hq.size() * sizeof(pw::NodeQuad), hipMemcpyDeviceToHost, c->stream));
+      PW_HIP(hipStreamSynchronize(c->stream));
+      double* o = static_cast<double*>(out);
+      for (int64_t i = 0; i < nn; ++i, o += 20) {
+        for (int k = 0; k < pw::kQuad; ++k) o[k] = hq[static_cast<size_t>(i)].Q[k];
+        o[18] = h[static_cast<size_t>(i)].s1;
+        o[19] = h[static_cast<size_t>(i)].s3;
+      }
+      return PW_OK;
+    }
     default:
       return fail(PW_ERR_ARG, "unknown export");
   }
@@ -624,7 +644,7 @@ int pw_ff_sample(pw_ctx* c, int64_t nsamples, uint64_t seed) {
   if (dalloc(&d_spec, spec.size()) != hipSuccess) return fail(PW_ERR_NOMEM, "sample spec");
   hipError_t e = hipMemcpyAsync(d_spec, spec.data(), sizeof(int) * spec.size(), hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess)
-    e = pw::launch_ff_samples(f.tb.sorted, f.tb.nodes, f.tb.far_list, f.tb.cap_far, d_spec,
+    e = pw::launch_ff_samples(f.tb.sorted, f.tb.nodes, f.tb.quad, f.tb.far_list, f.tb.cap_far, d_spec,
                               static_cast<int>(nsamples), f.d_X + 16 * f.ns, f.d_Y + 12 * f.ns,
                               f.d_aux + pw::kSampleAux * f.ns, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -769,12 +789,13 @@ int pw_ff_train(pw_ctx* c, int64_t steps, int batch, double lr, uint64_t seed, d
     eu_net += un * su * su;
     eg_md += gm * sg * sg;
     eg_net += gn * sg * sg;
+    // the analytic path (quadrupole order) against s3 / d^5, the network
+    // path (monopole + dipole + network) against s2 / d^4
     const double d = ak[2];
-    const double bound = ak[3] / (d * d * d * d);
-    if (bound > 0) {
-      r_md.push_back(std::sqrt(um) * su / bound);
-      r_net.push_back(std::sqrt(un) * su / bound);
-    }
+    const double d4 = d * d * d * d;
+    const double bound_q = ak[11] / (d4 * d), bound_d = ak[3] / d4;
+    if (bound_q > 0) r_md.push_back(ak[10] * kQ / bound_q);
+    if (bound_d > 0) r_net.push_back(std::sqrt(un) * su / bound_d);
   }
   f.coef_md = quantile(r_md, 0.99);
   f.coef_mlp = quantile(r_net, 0.99);

@@ -182,7 +182,9 @@ constexpr int kMaxLevels = 12;  // 256 * 8^11 particles
 // (particle mean), bounding radius r about it, and the vorticity moments
 //   M = sum a, D[3i+j] = sum a_i delta_j, s0 = sum |a|,
 //   S1 = sum |a| delta, s2 = sum |a| |delta|^2, sg = mean sigma
-// with delta = x - centre.  child = index of the first child (level > 0).
+// with delta = x - centre; s1 and s3 bound sum |a| |delta| and
+// sum |a| |delta|^3 (exact on leaves, shifted with the triangle inequality
+// above).  child = index of the first child, nchild their count (level > 0).
 struct alignas(16) NodeRec {
   double cx, cy, cz;
   float r, s0, s2, sg;
@@ -190,9 +192,19 @@ struct alignas(16) NodeRec {
   float D[9];
   float S1[3];
   int first, last, level, child;
-  int pad[3];
+  int nchild;
+  float s1, s3;
 };
 static_assert(sizeof(NodeRec) == 128, "NodeRec is 128 B");
+
+// Quadrupole moment of a node, Q^(p)_jl = sum a_p delta_j delta_l, 6 per
+// component p in the order (jl) = xx, yy, zz, xy, xz, yz.
+constexpr int kQuad = 18;
+struct alignas(16) NodeQuad {
+  float Q[kQuad];
+  float pad[2];
+};
+static_assert(sizeof(NodeQuad) == 80, "NodeQuad is 80 B");
 
 // Level layout of the node array: level L holds level_count[L] nodes from
 // level_off[L]; level 0 are the leaves, in Morton order.
@@ -245,6 +257,7 @@ struct TreeBuffers {
   int leaf_cap = 0;             // leaves the leaf-indexed buffers hold
   int node_cap = 0;
   NodeRec* nodes = nullptr;
+  NodeQuad* quad = nullptr;     // quadrupole of each node
   int* near_list = nullptr;     // nleaf x cap_near leaf ids
   int* near_cnt = nullptr;
   int cap_near = 0;
@@ -266,7 +279,9 @@ struct TreeBuffers {
 
 // Traversal parameters: a node is far from target leaf T when
 //   r_node + r_T < theta d,  (d - r_node - r_T)^2 > 44 sigma_max^2
-//   and err_coef * s2 <= tol * (d - r_T)^4
+//   and err_coef * s3 <= tol * (d - r_T)^5 (quad: the analytic expansion to
+//   quadrupole order) or err_coef * s2 <= tol * (d - r_T)^4 (monopole +
+//   dipole and the network's residual)
 // (the last is the self-estimated truncation error of the far
 // evaluation, err_coef calibrated for the evaluation in use).
 struct ListParams {
@@ -274,6 +289,7 @@ struct ListParams {
   float tol;
   float err_coef;
   float sigma_max2;
+  int quad;
   // high-order algebraic kernel: its velocity differs from the singular law
   // the far field expands by at most (15/8)(sigma/r)^4 of it, so a far node
   // also needs ho_coef s0 / gap^6 (ho_coef = (15/8) sigma_max^4 / 4 pi) added
@@ -343,16 +359,17 @@ hipError_t launch_near_fp32c(const Rec64* sorted, const NodeRec* nodes, const Tr
                              double nu, float sigma_max2, bool uniform_sigma, bool highorder, int grid,
                              Rate64* rates_s, hipStream_t st);
 
-// Far field over the lists, analytic monopole + dipole (FP32), added to
-// rates_s; with mlp != nullptr the learned residual is added as well.
+// Far field over the lists added to rates_s: the analytic expansion to
+// quadrupole order (quad != nullptr), or monopole + dipole and, with
+// mlp != nullptr, the learned residual.
 struct MlpWeights;  // packed BF16 fragments, pw_mlp.hip
-hipError_t launch_far(const Rec64* sorted, const NodeRec* nodes, const int* far_list,
+hipError_t launch_far(const Rec64* sorted, const NodeRec* nodes, const NodeQuad* quad, const int* far_list,
                       const int* far_cnt, int cap_far, int nleaf, const MlpWeights* mlp,
                       Rate64* rates_s, hipStream_t st);
 // the two passes of launch_far: the analytic expansion, and the network's
 // residual added to it
-hipError_t launch_far_md(const Rec64* sorted, const NodeRec* nodes, const int* far_list, const int* far_cnt,
-                         int cap_far, int nleaf, Rate64* rates_s, hipStream_t st);
+hipError_t launch_far_md(const Rec64* sorted, const NodeRec* nodes, const NodeQuad* quad, const int* far_list,
+                         const int* far_cnt, int cap_far, int nleaf, Rate64* rates_s, hipStream_t st);
 hipError_t launch_far_mlp(const Rec64* sorted, const NodeRec* nodes, const int* far_list, const int* far_cnt,
                           int cap_far, int nleaf, const MlpWeights& mlp, Rate64* rates_s, hipStream_t st);
 

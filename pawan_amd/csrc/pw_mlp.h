@@ -47,13 +47,13 @@ void pack_mlp(const float* params, std::vector<uint16_t>& frag, std::vector<floa
 void init_mlp(uint64_t seed, float* params);
 
 // Per-sample side data written with the training samples (doubles).
-constexpr int kSampleAux = 10;
+constexpr int kSampleAux = 12;
 
 // Inference over given scaled inputs (n x 16) -> n x 12.
 hipError_t launch_mlp_forward(const float* X, int n, float* Y, const MlpWeights& W, hipStream_t st);
 // Training samples from the far lists (spec: n x {leaf, slot, target}).
-hipError_t launch_ff_samples(const Rec64* sorted, const NodeRec* nodes, const int* far_list, int cap,
-                             const int* spec, int nsamples, float* X, float* Y, double* aux,
+hipError_t launch_ff_samples(const Rec64* sorted, const NodeRec* nodes, const NodeQuad* quad, const int* far_list,
+                             int cap, const int* spec, int nsamples, float* X, float* Y, double* aux,
                              hipStream_t st);
 
 // Training (FP32).  One call = loss and gradient of

@@ -9,14 +9,18 @@
 // near to every other leaf.)  8 consecutive nodes of one level form a node
 // of the next.  Every node stores its centre (particle mean), a bounding radius,
 // and the vorticity moments the far field needs:
-//   M  = sum a,   D_ij = sum a_i delta_j,   (delta = x - centre)
-//   s0 = sum |a|, S1 = sum |a| delta,  s2 = sum |a| |delta|^2
-// (D, S1, s2 of a parent from its children by the parallel-axis shift).
+//   M  = sum a,   D_ij = sum a_i delta_j,   Q^(p)_jl = sum a_p delta_j delta_l
+//   (delta = x - centre), s0 = sum |a|, S1 = sum |a| delta,
+//   s_n = sum |a| |delta|^n (n = 1, 2, 3)
+// (D, Q, S1, s2 of a parent from its children by the parallel-axis shift;
+// s1 and s3 of a parent are bounds: |delta + h| <= |delta| + |h|).
 // For each target leaf the tree is walked once per right-hand side:
 // nodes that pass the opening-angle test (r_node + r_leaf < theta d), lie
 // beyond the Gaussian core, and whose estimated far-evaluation error
-// c s2 / (d - r_leaf)^4 (the first neglected, quadrupole, order; c is
-// calibrated for the evaluation in use) is within the velocity tolerance
+// c s3 / (d - r_leaf)^5 for the analytic expansion to quadrupole order, or
+// c s2 / (d - r_leaf)^4 for monopole + dipole and the network's residual
+// (the first order the expansion leaves out; c is calibrated for the
+// evaluation in use) is within the velocity tolerance
 // go to the far list; leaves that fail go to the near list (evaluated by
 // the direct sum), other nodes that fail are opened.
 #include <hip/hip_runtime.h>
@@ -195,8 +199,8 @@ __device__ __forceinline__ void block_sum_d(double (&v)[M], double* red) {
 
 __global__ __launch_bounds__(kLeaf) void k_leaf_moments(const Rec64* __restrict__ ys,
                                                         const int* __restrict__ leaf_first,
-                                                        NodeRec* __restrict__ nodes) {
-  __shared__ double red[4 * 18];
+                                                        NodeRec* __restrict__ nodes, NodeQuad* __restrict__ quad) {
+  __shared__ double red[4 * 20];
   const int leaf = blockIdx.x;
   const int first = leaf_first[leaf], last = leaf_first[leaf + 1];
   const int i = first + static_cast<int>(threadIdx.x);
@@ -221,6 +225,22 @@ __global__ __launch_bounds__(kLeaf) void k_leaf_moments(const Rec64* __restrict_
     for (int m = 0; m < 18; ++m) v[m] = 0.0;
   }
   block_sum_d<18>(v, red);
+  // quadrupole (a_p delta_j delta_l, (jl) = xx yy zz xy xz yz), s1, s3
+  double q[20];
+  if (on) {
+    const double dd[6] = {dx * dx, dy * dy, dz * dz, dx * dy, dx * dz, dy * dz};
+    const double ap[3] = {p.ax, p.ay, p.az};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int m = 0; m < 6; ++m) q[6 * a + m] = ap[a] * dd[m];
+    const double dn = sqrt(dx * dx + dy * dy + dz * dz);
+    q[18] = am * dn;
+    q[19] = am * dn * dn * dn;
+  } else {
+    for (int m = 0; m < 20; ++m) q[m] = 0.0;
+  }
+  block_sum_d<20>(q, red);
   // bounding radius about the centre
   double d = on ? sqrt(dx * dx + dy * dy + dz * dz) : 0.0;
   for (int off = 32; off > 0; off >>= 1) d = fmax(d, __shfl_xor(d, off, 64));
@@ -240,8 +260,14 @@ __global__ __launch_bounds__(kLeaf) void k_leaf_moments(const Rec64* __restrict_
     nd.sg = static_cast<float>(v[17] / cnt);
     nd.first = first; nd.last = last;
     nd.level = 0; nd.child = -1;
-    nd.pad[0] = nd.pad[1] = nd.pad[2] = 0;
+    nd.nchild = 0;
+    nd.s1 = static_cast<float>(q[18]);
+    nd.s3 = static_cast<float>(q[19]);
     nodes[leaf] = nd;
+    NodeQuad nq;
+    for (int m = 0; m < kQuad; ++m) nq.Q[m] = static_cast<float>(q[m]);
+    nq.pad[0] = nq.pad[1] = 0.0f;
+    quad[leaf] = nq;
   }
 }
 
@@ -249,9 +275,11 @@ __global__ __launch_bounds__(kLeaf) void k_leaf_moments(const Rec64* __restrict_
 // Moments about the parent centre from the children's by the parallel-axis
 // shift h = c_child - c_parent:
 //   D += M_c h^T,  S1 += S1_c + s0_c h,  s2 += s2_c + 2 S1_c.h + s0_c |h|^2,
-//   r = max(|h| + r_c).
-__global__ __launch_bounds__(64) void k_parent_moments(NodeRec* __restrict__ nodes, int child_off,
-                                                       int nchild, int off, int count, int level) {
+//   Q^(p)_jl += Q_c^(p)_jl + D_c[p][j] h_l + D_c[p][l] h_j + M_c[p] h_j h_l,
+//   s1 += s1_c + s0_c |h|,  s3 += s3_c + 3 s2_c |h| + 3 s1_c |h|^2 + s0_c |h|^3
+//   (bounds, then s1 <= r s0 and s3 <= r s2), r = max(|h| + r_c).
+__global__ __launch_bounds__(64) void k_parent_moments(NodeRec* __restrict__ nodes, NodeQuad* __restrict__ quad,
+                                                       int child_off, int nchild, int off, int count, int level) {
   const int j = blockIdx.x * 64 + threadIdx.x;
   if (j >= count) return;
   const int c0 = child_off + kFan * j, c1 = child_off + min(kFan * (j + 1), nchild);
@@ -264,22 +292,40 @@ __global__ __launch_bounds__(64) void k_parent_moments(NodeRec* __restrict__ nod
   NodeRec p;
   p.cx = sx / cnt; p.cy = sy / cnt; p.cz = sz / cnt;
   double M[3] = {0, 0, 0}, D[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, s0 = 0, S1[3] = {0, 0, 0}, s2 = 0, r = 0;
+  double Q[kQuad], s1 = 0, s3 = 0;
+  for (int m = 0; m < kQuad; ++m) Q[m] = 0.0;
+  constexpr int kJ[6] = {0, 1, 2, 0, 0, 1}, kL[6] = {0, 1, 2, 1, 2, 2};  // (jl) of each Q slot
   for (int c = c0; c < c1; ++c) {
     const NodeRec& q = nodes[c];
+    const NodeQuad& qq = quad[c];
     const double h[3] = {q.cx - p.cx, q.cy - p.cy, q.cz - p.cz};
     double h2 = 0, s1h = 0;
     for (int a = 0; a < 3; ++a) {
       M[a] += q.M[a];
       for (int b = 0; b < 3; ++b) D[3 * a + b] += q.D[3 * a + b] + static_cast<double>(q.M[a]) * h[b];
+      for (int m = 0; m < 6; ++m) {
+        const int j = kJ[m], l = kL[m];
+        Q[6 * a + m] += qq.Q[6 * a + m] + static_cast<double>(q.D[3 * a + j]) * h[l] +
+                        static_cast<double>(q.D[3 * a + l]) * h[j] + static_cast<double>(q.M[a]) * h[j] * h[l];
+      }
       S1[a] += q.S1[a] + static_cast<double>(q.s0) * h[a];
       h2 += h[a] * h[a];
       s1h += q.S1[a] * h[a];
     }
+    const double hn = sqrt(h2);
     s0 += q.s0;
     s2 += q.s2 + 2.0 * s1h + q.s0 * h2;
-    r = fmax(r, sqrt(h2) + q.r);
+    s1 += q.s1 + q.s0 * hn;
+    s3 += q.s3 + 3.0 * q.s2 * hn + 3.0 * q.s1 * h2 + q.s0 * h2 * hn;
+    r = fmax(r, hn + q.r);
   }
   p.r = static_cast<float>(r * (1.0 + 1e-6));
+  p.s1 = static_cast<float>(fmin(s1, r * s0));
+  p.s3 = static_cast<float>(fmin(s3, r * fmax(s2, 0.0)));
+  NodeQuad pq;
+  for (int m = 0; m < kQuad; ++m) pq.Q[m] = static_cast<float>(Q[m]);
+  pq.pad[0] = pq.pad[1] = 0.0f;
+  quad[off + j] = pq;
   p.s0 = static_cast<float>(s0);
   p.s2 = static_cast<float>(s2);
   p.sg = static_cast<float>(sg / cnt);
@@ -290,8 +336,7 @@ __global__ __launch_bounds__(64) void k_parent_moments(NodeRec* __restrict__ nod
   p.last = nodes[c1 - 1].last;
   p.level = level;
   p.child = c0;
-  p.pad[0] = c1 - c0;
-  p.pad[1] = p.pad[2] = 0;
+  p.nchild = c1 - c0;
   nodes[off + j] = p;
 }
 
@@ -315,11 +360,14 @@ __device__ __forceinline__ bool is_far(const NodeRec& q, double tcx, double tcy,
   if (far && lp.ho_gap2 > 0.0f) far = gap * gap > lp.ho_gap2 * lp.sigma_max2;
   if (far && lp.tol > 0.0f) {
     const float dm = d - tr;
+    // the first order the far evaluation leaves out
+    const float dmn = lp.quad ? (dm * dm) * (dm * dm) * dm : (dm * dm) * (dm * dm);
+    const float sn = lp.quad ? q.s3 : q.s2;
     if (lp.ho_coef > 0.0f) {
       const float g2 = gap * gap;
-      far = lp.err_coef * q.s2 / ((dm * dm) * (dm * dm)) + lp.ho_coef * q.s0 / (g2 * g2 * g2) <= lp.tol;
+      far = lp.err_coef * sn / dmn + lp.ho_coef * q.s0 / (g2 * g2 * g2) <= lp.tol;
     } else {
-      far = lp.err_coef * q.s2 <= lp.tol * (dm * dm) * (dm * dm);
+      far = lp.err_coef * sn <= lp.tol * dmn;
     }
   }
   return far;
@@ -365,7 +413,7 @@ __global__ __launch_bounds__(64) void k_lists(const NodeRec* __restrict__ nodes,
       int id = -1;
       if (j < np) {
         const NodeRec& P = nodes[open[cur][j]];
-        if (k < P.pad[0]) id = P.child + k;
+        if (k < P.nchild) id = P.child + k;
       }
       bool far = false, near = false, opn = false;
       if (id >= 0) {
@@ -601,10 +649,11 @@ hipError_t write_leaf_first(int n, TreeBuffers& tb, hipStream_t st) {
 
 hipError_t build_moments(int n, TreeBuffers& tb, const TreeShape& shape, hipStream_t st) {
   (void)n;
-  hipLaunchKernelGGL(k_leaf_moments, dim3(shape.nleaf), dim3(kLeaf), 0, st, tb.sorted, tb.leaf_first, tb.nodes);
+  hipLaunchKernelGGL(k_leaf_moments, dim3(shape.nleaf), dim3(kLeaf), 0, st, tb.sorted, tb.leaf_first, tb.nodes,
+                     tb.quad);
   for (int L = 1; L < shape.nlevel; ++L)
     hipLaunchKernelGGL(k_parent_moments, dim3((shape.level_count[L] + 63) / 64), dim3(64), 0, st,
-                       tb.nodes, shape.level_off[L - 1], shape.level_count[L - 1], shape.level_off[L],
+                       tb.nodes, tb.quad, shape.level_off[L - 1], shape.level_count[L - 1], shape.level_off[L],
                        shape.level_count[L], L);
   return hipGetLastError();
 }

@@ -31,7 +31,8 @@ def built(helix):
         g.ff_build()
         out = {k: g.ff_export(w) for k, w in (("perm", capi.FF_PERM), ("nodes", capi.FF_NODES),
                                               ("near_cnt", capi.FF_NEAR_CNT), ("near", capi.FF_NEAR),
-                                              ("far_cnt", capi.FF_FAR_CNT), ("far", capi.FF_FAR))}
+                                              ("far_cnt", capi.FF_FAR_CNT), ("far", capi.FF_FAR),
+                                              ("quad", capi.FF_QUAD))}
         out["info"] = g.ff_info()
         u, w = g.solve("learned")
         out["u"], out["w"] = u, w
@@ -72,6 +73,17 @@ def test_node_moments_match_oracle(helix, built):
     assert np.all(np.abs(nd[:, 7:10] - mo["M"]) <= 2e-6 * scale_m)
     scale_d = (mo["s0"] * mo["r"])[:, None]
     assert np.all(np.abs(nd[:, 10:19] - mo["D"].reshape(-1, 9)) <= 2e-5 * scale_d)
+    # quadrupole by the parallel-axis shift against the particles' own sums
+    qd = built["quad"]
+    Qg = np.array([FF.quad_from_slots(row) for row in qd[:, :18]])
+    scale_q = (mo["s0"] * mo["r"] ** 2)[:, None, None, None]
+    assert np.all(np.abs(Qg - mo["Q"]) <= 5e-5 * scale_q)
+    # s1, s3: exact on the leaves, bounds above
+    np.testing.assert_allclose(qd[leaf, 18], mo["s1"][leaf], rtol=2e-6)
+    np.testing.assert_allclose(qd[leaf, 19], mo["s3"][leaf], rtol=2e-5)
+    assert np.all(qd[:, 18] >= mo["s1"] * (1 - 2e-6))
+    assert np.all(qd[:, 19] >= mo["s3"] * (1 - 2e-5))
+    assert np.all(qd[:, 19] <= nd[:, 3] * nd[:, 5] * (1 + 1e-5))  # s3 <= r s2
 
 
 def test_lists_cover_every_pair_once(built):
@@ -119,9 +131,10 @@ def test_far_nodes_pass_the_opening_test(built):
             assert nd[q, 3] + nd[T, 3] < 0.5 * d * (1 + 1e-5)
 
 
-def _oracle_rates(oracle, helix, built, targets_sorted):
+def _oracle_rates(oracle, helix, built, targets_sorted, quad=True):
     """FP64 rates of sorted targets with the GPU's own lists: near leaves
-    by the oracle's pair sums, far nodes by the expansion."""
+    by the oracle's pair sums, far nodes by the expansion (to quadrupole
+    order, or monopole + dipole for the network path)."""
     perm, nd = built["perm"], built["nodes"]
     sp, sa = helix.pos[perm], helix.alpha[perm]
     sr, sv = helix.radius[perm], helix.volume[perm]
@@ -139,9 +152,13 @@ def _oracle_rates(oracle, helix, built, targets_sorted):
         u, w = oracle.rates_targets(sub, [0])
         u, w = u[0].copy(), w[0].copy()
         for q in built["far"][T, :built["far_cnt"][T]]:
-            uq, Gq = FF.md_eval((sp[t] - nd[q, 0:3])[None], nd[q, 7:10], nd[q, 10:19].reshape(3, 3))
-            u += uq[0] / FF.FOUR_PI
-            w += sa[t] @ Gq[0] / FF.FOUR_PI
+            r = (sp[t] - nd[q, 0:3])[None]
+            um, Gm = FF.md_eval(r, nd[q, 7:10], nd[q, 10:19].reshape(3, 3))
+            if quad:
+                uq, Gq = FF.quad_eval(r, FF.quad_from_slots(built["quad"][q, :18]))
+                um, Gm = um + uq, Gm + Gq
+            u += um[0] / FF.FOUR_PI
+            w += sa[t] @ Gm[0] / FF.FOUR_PI
         us.append(u)
         ws.append(w)
     return np.array(us), np.array(ws)
@@ -172,7 +189,8 @@ def test_near_field_with_particles_of_many_radii(oracle, helix):
         g.ff_build()
         b = {k: g.ff_export(w) for k, w in (("perm", capi.FF_PERM), ("nodes", capi.FF_NODES),
                                             ("near_cnt", capi.FF_NEAR_CNT), ("near", capi.FF_NEAR),
-                                            ("far_cnt", capi.FF_FAR_CNT), ("far", capi.FF_FAR))}
+                                            ("far_cnt", capi.FF_FAR_CNT), ("far", capi.FF_FAR),
+                                            ("quad", capi.FF_QUAD))}
         b["info"] = g.ff_info()
         u, w = g.solve("learned")
     targets = np.sort(rng.choice(s.n, size=40, replace=False))
@@ -216,7 +234,7 @@ def test_learned_far_field_matches_oracle_on_gpu_lists(oracle, helix, built):
         u, w = g.solve("learned")
     rng = np.random.default_rng(1)
     targets = np.sort(rng.choice(helix.n, size=32, replace=False))
-    ru, rw = _oracle_rates(oracle, helix, built, targets)
+    ru, rw = _oracle_rates(oracle, helix, built, targets, quad=False)
     sp, sa = helix.pos[perm], helix.alpha[perm]
     du = []
     nl = built["info"]["leaves"]
@@ -298,6 +316,11 @@ def test_samples_match_oracle(helix, built):
         ue, Ge = FF.exact_node(sp[t], sr[t], sp[a:b], sa[a:b], sr[a:b])
         r = sp[t] - nd[q, 0:3]
         um, Gm = FF.md_eval(r[None], nd[q, 7:10], nd[q, 10:19].reshape(3, 3))
+        uq, _ = FF.quad_eval(r[None], FF.quad_from_slots(built["quad"][q, :18]))
+        # side data: the error of the expansion to quadrupole order, and s3
+        eq = np.linalg.norm(ue - um[0] - uq[0])
+        assert abs(A[k, 10] - eq) <= 1e-5 * np.linalg.norm(um[0]) + 1e-3 * eq, k
+        np.testing.assert_allclose(A[k, 11], built["quad"][q, 19], rtol=1e-6)
         xo, su, sG, R = FF.features(r[None], nd[q, 3] + nd[q, 6], nd[q, 4], nd[q, 7:10],
                                     nd[q, 10:19].reshape(3, 3), nd[q, 6])
         np.testing.assert_allclose(X[k], xo[0], rtol=1e-5, atol=5e-6)  # FP32 rotations: a few ulp of O(1)

@@ -47,7 +47,8 @@ def _build(helix, kernel, tol):
         g.ff_build()
         out = {k: g.ff_export(w) for k, w in (("perm", capi.FF_PERM), ("nodes", capi.FF_NODES),
                                               ("near_cnt", capi.FF_NEAR_CNT), ("near", capi.FF_NEAR),
-                                              ("far_cnt", capi.FF_FAR_CNT), ("far", capi.FF_FAR))}
+                                              ("far_cnt", capi.FF_FAR_CNT), ("far", capi.FF_FAR),
+                                              ("quad", capi.FF_QUAD))}
         out["info"] = g.ff_info()
         out["coef"] = g.ff_err_coef()[0]
         out["u"], out["w"] = g.solve("learned")
@@ -71,7 +72,7 @@ def test_far_nodes_keep_the_kernel_deviation_within_the_tolerance(helix, direct)
         for q in b["far"][T, :b["far_cnt"][T]]:
             d = float(np.linalg.norm(nd[q, 0:3] - nd[T, 0:3]))
             gap, dm = d - nd[q, 3] - nd[T, 3], d - nd[T, 3]
-            e = b["coef"] * nd[q, 5] / dm ** 4 + 1.875 * s4 * nd[q, 4] / (4 * math.pi * gap ** 6)
+            e = b["coef"] * b["quad"][q, 19] / dm ** 5 + 1.875 * s4 * nd[q, 4] / (4 * math.pi * gap ** 6)
             worst = max(worst, e / tol)
             nfar += 1
     _say(f"high-order lists at tol {tol:.3e}: {nfar} far nodes, worst budget use {worst:.4f}, "
@@ -103,9 +104,11 @@ def test_near_field_matches_the_oracle_on_gpu_lists(oracle, helix, direct):
             u, w = oracle.rates_targets(sub, [0])
             u, w = u[0].copy(), w[0].copy()
             for q in b["far"][T, :b["far_cnt"][T]]:
-                uq, Gq = FF.md_eval((sp[t] - nd[q, 0:3])[None], nd[q, 7:10], nd[q, 10:19].reshape(3, 3))
-                u += uq[0] / FF.FOUR_PI
-                w += sa[t] @ Gq[0] / FF.FOUR_PI
+                r = (sp[t] - nd[q, 0:3])[None]
+                um, Gm = FF.md_eval(r, nd[q, 7:10], nd[q, 10:19].reshape(3, 3))
+                uq, Gq = FF.quad_eval(r, FF.quad_from_slots(b["quad"][q, :18]))
+                u += (um[0] + uq[0]) / FF.FOUR_PI
+                w += sa[t] @ (Gm[0] + Gq[0]) / FF.FOUR_PI
             ru.append(u)
             rw.append(w)
     gu, gw = b["u"][perm[targets]], b["w"][perm[targets]]

@@ -88,6 +88,69 @@ def test_md_without_dipole_is_first_order():
     assert 1.6 < errs[0] / errs[1] < 2.4
 
 
+def _q_of(pos, alpha):
+    c = pos.mean(axis=0)
+    d = pos - c
+    return np.einsum("sp,sj,sl->pjl", alpha, d, d)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_quadrupole_error_is_third_order(seed):
+    """With the quadrupole added, shrinking the cluster by 2 shrinks the
+    error of velocity and gradient by ~8 (the octupole is the first term
+    left out); the monopole + dipole alone fall by ~4 on the same clusters,
+    so a dropped or wrong quadrupole term fails the bar."""
+    rng = np.random.default_rng(seed)
+    pos0, alpha, sigma = _cluster(rng, 40)
+    x = np.array([7.0, -5.0, 6.0])
+    errs = []
+    for eps in (0.2, 0.1, 0.05):
+        pos = pos0 * eps
+        c, M, D = _md_of(pos, alpha)
+        u, G = FF.md_eval((x - c)[None], M, D)
+        uq, Gq = FF.quad_eval((x - c)[None], _q_of(pos, alpha))
+        ue, Ge = FF.exact_node(x, 1e-3, pos, alpha, sigma)
+        errs.append((np.linalg.norm(u[0] + uq[0] - ue), np.linalg.norm(G[0] + Gq[0] - Ge)))
+    for k in range(2):
+        for j in range(2):
+            ratio = errs[k][j] / errs[k + 1][j]
+            assert 6.5 < ratio < 9.5, (k, j, ratio)
+
+
+def test_quadrupole_gradient_is_derivative_of_velocity():
+    rng = np.random.default_rng(4)
+    Q = rng.normal(size=(3, 3, 3))
+    Q = 0.5 * (Q + Q.transpose(0, 2, 1))  # symmetric in the two offsets
+    x = np.array([2.0, 1.5, -2.5])
+    _, G = FF.quad_eval(x[None], Q)
+    h = 1e-5
+    fd = np.zeros((3, 3))
+    for m in range(3):
+        e = np.zeros(3)
+        e[m] = h
+        up, _ = FF.quad_eval((x + e)[None], Q)
+        um, _ = FF.quad_eval((x - e)[None], Q)
+        fd[:, m] = (up[0] - um[0]) / (2 * h)
+    np.testing.assert_allclose(G[0], fd, rtol=1e-7, atol=1e-10)
+
+
+def test_quadrupole_of_one_particle_is_half_its_second_difference():
+    """One particle at offset delta from the centre: the quadrupole term is
+    (1/2) (delta . grad)^2 of its field a x r/|r|^3, here by a central second
+    difference along delta."""
+    a = np.array([0.3, -0.7, 1.1])
+    delta = np.array([0.02, -0.01, 0.015])
+    x = np.array([3.0, -1.0, 2.0])
+    uq, _ = FF.quad_eval(x[None], np.einsum("p,j,l->pjl", a, delta, delta))
+
+    def field(y):
+        return np.cross(a, y) / np.linalg.norm(y) ** 3
+
+    h = 1.0
+    sd = (field(x + h * delta) - 2.0 * field(x) + field(x - h * delta)) / (h * h)
+    np.testing.assert_allclose(uq[0], 0.5 * sd, rtol=5e-4)
+
+
 def test_transposed_contraction_matches_pair_stretching(oracle):
     """alpha_t^T G' / (4 pi) of a one-particle node equals the oracle's pair
     stretching (interaction_utils.h:456-501) for a far pair."""
