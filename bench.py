@@ -203,7 +203,9 @@ def learned_leg(system, dt: float, steps: int, target_err: float, theta: float, 
             nrhs = max(1, launches)
             rhs_ms = ms / nrhs
             st = {k: v / max(1.0, stages["rhs"]) for k, v in stages.items() if k != "rhs"}
-            out[key] = {"steps_per_s": steps / wall, "rhs_ms": rhs_ms,
+            out[key] = {"far_field": ("monopole + dipole + network residual" if use_mlp else
+                                      "monopole + dipole + quadrupole (analytic)"),
+                        "steps_per_s": steps / wall, "rhs_ms": rhs_ms,
                         "velocity_error": err["velocity_maxnorm"],
                         "velocity_error_maxnorm": err["velocity_maxnorm"], "errors_vs_fp64": err,
                         "near_pairs_per_s": info["near_pairs"] / (st["near_ms"] * 1e-3),
