@@ -6,10 +6,10 @@ from which that kernel forms both pair factors with no branch,
     -(4 pi/3) F sigma^5 = sqrt(2/pi) G
 (the incomplete-gamma series of erf, A&S 6.5.29, with exp(-s/2) factored:
 every term positive).  On 0 <= s <= 44 (the Gaussian core, widened by 1e-4)
-G is cut into 32 equal pieces, each a degree-5 polynomial in t = s - s_c
+G is cut into 64 equal pieces, each a degree-4 polynomial in t = s - s_c
 (s_c the piece's centre), a Chebyshev least-squares fit in float64 converted
 to the monomial basis and rounded to float32.  A row of the table is
-(c0, .., c5, s_c, 0): two 16-byte reads per pair.
+(c0, .., c4, 0, s_c, 0): two 16-byte reads per pair.
 
 G in float64: the series for s < 2 (positive terms, ratio s/(2k+7)); above,
 G = (H - e/3)/s with H = exp(-s/2) (1/3 + s B) = s^-3/2 erf(x) / sqrt(2/pi)
@@ -21,8 +21,8 @@ digits.
 import numpy as np
 from scipy.special import erf
 
-PIECES = 32
-DEG = 5
+PIECES = 64
+DEG = 4
 S_HI = 44.0 * 1.0001
 K = np.sqrt(2.0 / np.pi)
 

@@ -677,45 +677,77 @@ __device__ __forceinline__ void near32(float r2, float sig2, float nu2, float& q
 //   f' = sqrt(2/pi) G / sigma^5,
 // the same series as near32 with the exponential factored, so one function
 // serves every lane whatever its s (near32 runs both of its branches when a
-// wave's pairs straddle s = 4).  G comes from a table of 32 pieces of degree
-// 5 in t = s - s_c on 0 <= s <= 44 (rows (c0 .. c5, s_c, 0); a Chebyshev
+// wave's pairs straddle s = 4).  G comes from a table of 64 pieces of degree
+// 4 in t = s - s_c on 0 <= s <= 44 (rows (c0 .. c4, 0, s_c, 0); a Chebyshev
 // fit, fit_core_table.py beside this file; largest relative error of this
-// FP32 evaluation 1.3e-7), staged in LDS and read per lane.
-constexpr int kCorePieces = 32;
-constexpr float kCoreScale = 0x1.74539p-1f;  // kCorePieces / (44 * 1.0001) in FP32
-__constant__ float kCoreTable[32][8] = {
-    {0x1.ac6372p-5f, -0x1.2b22b2p-6f, 0x1.caa97p-9f, -0x1.ef67e8p-12f, 0x1.a2051ap-15f, -0x1.200b42p-18f, 0x1.600902p-1f, 0.0f},
-    {0x1.0c2748p-5f, -0x1.640e14p-7f, 0x1.085a3ap-9f, -0x1.174eap-12f, 0x1.cfd25p-16f, -0x1.3bc1ecp-19f, 0x1.0806c2p+1f, 0.0f},
-    {0x1.5812fcp-6f, -0x1.af4d8ap-8f, 0x1.34a898p-10f, -0x1.3e034ap-13f, 0x1.03443cp-16f, -0x1.5c391cp-20f, 0x1.b80b44p+1f, 0.0f},
-    {0x1.c54018p-7f, -0x1.0a513ep-8f, 0x1.6d920ap-11f, -0x1.6e037ap-14f, 0x1.2448f6p-17f, -0x1.82966cp-21f, 0x1.3407e2p+2f, 0.0f},
-    {0x1.32d212p-7f, -0x1.4fd76ep-9f, 0x1.b7d7d8p-12f, -0x1.aa555ep-15f, 0x1.4c9486p-18f, -0x1.b0583cp-22f, 0x1.8c0a24p+2f, 0.0f},
-    {0x1.ab16d2p-8f, -0x1.b1106ap-10f, 0x1.0d2f6ap-12f, -0x1.f730d6p-16f, 0x1.7e550ap-19f, -0x1.e7756p-23f, 0x1.e40c64p+2f, 0.0f},
-    {0x1.31829ep-8f, -0x1.1dcbd8p-10f, 0x1.4fa2aap-13f, -0x1.2d44bep-16f, 0x1.bc844ap-20f, -0x1.1544aap-23f, 0x1.1e0752p+3f, 0.0f},
-    {0x1.c0c2d8p-9f, -0x1.82508ep-11f, 0x1.aac9c4p-14f, -0x1.6e6efp-17f, 0x1.059ea4p-20f, -0x1.3e8972p-24f, 0x1.4a0872p+3f, 0.0f},
-    {0x1.51dcb8p-9f, -0x1.0b663cp-11f, 0x1.14f86cp-14f, -0x1.c53874p-18f, 0x1.381cbep-21f, -0x1.71e6cp-25f, 0x1.760994p+3f, 0.0f},
-    {0x1.043f02p-9f, -0x1.7af134p-12f, 0x1.6f2284p-15f, -0x1.1d4a2ap-18f, 0x1.79c52cp-22f, -0x1.b298e6p-26f, 0x1.a20ab4p+3f, 0.0f},
-    {0x1.995076p-10f, -0x1.129e06p-12f, 0x1.f11666p-16f, -0x1.6ddbaap-19f, 0x1.d056a6p-23f, -0x1.028bdep-26f, 0x1.ce0bd4p+3f, 0.0f},
-    {0x1.47eb32p-10f, -0x1.9694a6p-13f, 0x1.57aa7ap-16f, -0x1.de29ep-20f, 0x1.220ad2p-23f, -0x1.37cf54p-27f, 0x1.fa0cf4p+3f, 0.0f},
-    {0x1.0b1612p-10f, -0x1.32ff0ep-13f, 0x1.e504a6p-17f, -0x1.3e85f4p-20f, 0x1.70838cp-24f, -0x1.7d7a74p-28f, 0x1.13070ap+4f, 0.0f},
-    {0x1.b97e02p-11f, -0x1.d82138p-14f, 0x1.5d09f6p-17f, -0x1.b08bf2p-21f, 0x1.dc688ep-25f, -0x1.d9cb62p-29f, 0x1.29079ap+4f, 0.0f},
-    {0x1.71a59cp-11f, -0x1.7120cap-14f, 0x1.ffc8c6p-18f, -0x1.2b40b6p-21f, 0x1.3967b4p-25f, -0x1.2adc86p-29f, 0x1.3f082ap+4f, 0.0f},
-    {0x1.390f86p-11f, -0x1.24fafp-14f, 0x1.7dc96ap-18f, -0x1.a5a7f4p-22f, 0x1.a3a6d2p-26f, -0x1.7f210cp-30f, 0x1.5508bap+4f, 0.0f},
-    {0x1.0bd8dp-11f, -0x1.d775fcp-15f, 0x1.217112p-18f, -0x1.2e4498p-22f, 0x1.1dde18p-26f, -0x1.f334aap-31f, 0x1.6b094ap+4f, 0.0f},
-    {0x1.ce7eeap-12f, -0x1.80072ep-15f, 0x1.bd710ap-19f, -0x1.b88dd6p-23f, 0x1.8c191p-27f, -0x1.4a8fa6p-31f, 0x1.8109dcp+4f, 0.0f},
-    {0x1.928bc2p-12f, -0x1.3c4d5p-15f, 0x1.5b790ap-19f, -0x1.460dp-23f, 0x1.16eb5ep-27f, -0x1.bce2ecp-32f, 0x1.970a6cp+4f, 0.0f},
-    {0x1.60ec86p-12f, -0x1.072606p-15f, 0x1.12763cp-19f, -0x1.e99f5ep-24f, 0x1.8ef722p-28f, -0x1.30221ep-32f, 0x1.ad0afcp+4f, 0.0f},
-    {0x1.37747ep-12f, -0x1.b9db6p-16f, 0x1.b692d8p-20f, -0x1.749514p-24f, 0x1.2192a2p-28f, -0x1.a63a1p-33f, 0x1.c30b8cp+4f, 0.0f},
-    {0x1.147f42p-12f, -0x1.760b9cp-16f, 0x1.621898p-20f, -0x1.1f0e8ep-24f, 0x1.aa382ap-29f, -0x1.296c72p-33f, 0x1.d90c1cp+4f, 0.0f},
-    {0x1.ed966p-13f, -0x1.3f09a2p-16f, 0x1.20a5a6p-20f, -0x1.bf6d4ep-25f, 0x1.3dc722p-29f, -0x1.a8ee76p-34f, 0x1.ef0cacp+4f, 0.0f},
-    {0x1.babe74p-13f, -0x1.1200ccp-16f, 0x1.dac0fep-21f, -0x1.6068bep-25f, 0x1.dfa46ep-30f, -0x1.339c5ap-34f, 0x1.02869ep+5f, 0.0f},
-    {0x1.8ef0c6p-13f, -0x1.d9a3e6p-17f, 0x1.899a7ap-21f, -0x1.184d2cp-25f, 0x1.6e1928p-30f, -0x1.c2fb7ep-35f, 0x1.0d86e6p+5f, 0.0f},
-    {0x1.68f8d2p-13f, -0x1.9bc24cp-17f, 0x1.48c584p-21f, -0x1.c1f48cp-26f, 0x1.1a6512p-30f, -0x1.4e82d4p-35f, 0x1.18872ep+5f, 0.0f},
-    {0x1.47e03cp-13f, -0x1.67e502p-17f, 0x1.148598p-21f, -0x1.6c3146p-26f, 0x1.b7f4d8p-31f, -0x1.f5c3a6p-36f, 0x1.238776p+5f, 0.0f},
-    {0x1.2ae08ap-13f, -0x1.3c229p-17f, 0x1.d423ap-22f, -0x1.291578p-26f, 0x1.59dd22p-31f, -0x1.7c3f62p-36f, 0x1.2e87bep+5f, 0.0f},
-    {0x1.115882p-13f, -0x1.16fc2p-17f, 0x1.8ea22cp-22f, -0x1.e835f2p-27f, 0x1.12394cp-31f, -0x1.22f9b6p-36f, 0x1.398806p+5f, 0.0f},
-    {0x1.f5887p-14f, -0x1.ee873p-18f, 0x1.55555p-22f, -0x1.93decap-27f, 0x1.b652bap-32f, -0x1.c1667ep-37f, 0x1.44884ep+5f, 0.0f},
-    {0x1.cd6dd6p-14f, -0x1.b81126p-18f, 0x1.25c8f4p-22f, -0x1.50377p-27f, 0x1.60ee16p-32f, -0x1.5e04acp-37f, 0x1.4f8896p+5f, 0.0f},
-    {0x1.a9accep-14f, -0x1.891494p-18f, 0x1.fc2c6ap-23f, -0x1.198e14p-27f, 0x1.1e29a8p-32f, -0x1.12ceeap-37f, 0x1.5a88dep+5f, 0.0f},
+// FP32 evaluation 1.2e-7), staged in LDS and read per lane.
+constexpr int kCorePieces = 64;
+constexpr float kCoreScale = 0x1.74539p+0f;  // kCorePieces / (44 * 1.0001) in FP32
+__constant__ float kCoreTable[64][8] = {
+    {0x1.e35bd2p-5f, -0x1.557002p-6f, 0x1.07b0e8p-8f, -0x1.1ea542p-11f, 0x1.e3402ap-15f, 0.0f, 0x1.600902p-2f, 0.0f},
+    {0x1.7c34aap-5f, -0x1.0654f4p-6f, 0x1.8f30cap-9f, -0x1.ad7d32p-12f, 0x1.675396p-15f, 0.0f, 0x1.0806c2p+0f, 0.0f},
+    {0x1.2cccb2p-5f, -0x1.94bdf8p-7f, 0x1.2f0d26p-9f, -0x1.4279ep-12f, 0x1.0ba584p-15f, 0.0f, 0x1.b80b44p+0f, 0.0f},
+    {0x1.ded68ap-6f, -0x1.399146p-7f, 0x1.cd91dcp-10f, -0x1.e56572p-13f, 0x1.8f72d4p-16f, 0.0f, 0x1.3407e2p+1f, 0.0f},
+    {0x1.7f870cp-6f, -0x1.e80d76p-8f, 0x1.60a9bap-10f, -0x1.6e393cp-13f, 0x1.2aa7b8p-16f, 0.0f, 0x1.8c0a24p+1f, 0.0f},
+    {0x1.353016p-6f, -0x1.7d9c8cp-8f, 0x1.0e6414p-10f, -0x1.15095ap-13f, 0x1.bf807p-17f, 0.0f, 0x1.e40c64p+1f, 0.0f},
+    {0x1.f5d95p-7f, -0x1.2bdc3ap-8f, 0x1.a024cep-11f, -0x1.a44d18p-14f, 0x1.4ffcap-17f, 0.0f, 0x1.1e0752p+2f, 0.0f},
+    {0x1.9a0f8p-7f, -0x1.d9ac1p-9f, 0x1.4177p-11f, -0x1.3fc1eap-14f, 0x1.f9aa08p-18f, 0.0f, 0x1.4a0872p+2f, 0.0f},
+    {0x1.51611cp-7f, -0x1.781d8p-9f, 0x1.f2a82ep-12f, -0x1.e806a2p-15f, 0x1.7d6c0cp-18f, 0.0f, 0x1.760994p+2f, 0.0f},
+    {0x1.1783eep-7f, -0x1.2c4c6ep-9f, 0x1.846394p-12f, -0x1.759f86p-15f, 0x1.206d6ap-18f, 0.0f, 0x1.a20ab4p+2f, 0.0f},
+    {0x1.d26158p-8f, -0x1.e23f76p-10f, 0x1.2fd524p-12f, -0x1.1f0176p-15f, 0x1.b55c1ep-19f, 0.0f, 0x1.ce0bd4p+2f, 0.0f},
+    {0x1.87c7b2p-8f, -0x1.8576acp-10f, 0x1.dd8968p-13f, -0x1.ba7f4ep-16f, 0x1.4c837p-19f, 0.0f, 0x1.fa0cf4p+2f, 0.0f},
+    {0x1.4b5cbep-8f, -0x1.3c6274p-10f, 0x1.790b48p-13f, -0x1.565fbep-16f, 0x1.fb10e8p-20f, 0.0f, 0x1.13070ap+3f, 0.0f},
+    {0x1.1a23f4p-8f, -0x1.028ceap-10f, 0x1.2b2546p-13f, -0x1.09ec68p-16f, 0x1.83cbaap-20f, 0.0f, 0x1.29079ap+3f, 0.0f},
+    {0x1.e3998p-9f, -0x1.a91c6ep-11f, 0x1.dd0cc4p-14f, -0x1.9ebc6ep-17f, 0x1.298474p-20f, 0.0f, 0x1.3f082ap+3f, 0.0f},
+    {0x1.a11428p-9f, -0x1.5f955ep-11f, 0x1.7e5112p-14f, -0x1.44bf98p-17f, 0x1.ca043cp-21f, 0.0f, 0x1.5508bap+3f, 0.0f},
+    {0x1.69e77ap-9f, -0x1.2482aap-11f, 0x1.33fc3cp-14f, -0x1.febd4ep-18f, 0x1.61c222p-21f, 0.0f, 0x1.6b094ap+3f, 0.0f},
+    {0x1.3bde48p-9f, -0x1.e99abep-12f, 0x1.f2d1c4p-15f, -0x1.9362fcp-18f, 0x1.123466p-21f, 0.0f, 0x1.8109dcp+3f, 0.0f},
+    {0x1.153b5p-9f, -0x1.9c236p-12f, 0x1.96174p-15f, -0x1.4007b8p-18f, 0x1.aaa61p-22f, 0.0f, 0x1.970a6cp+3f, 0.0f},
+    {0x1.e93c4ep-10f, -0x1.5ce924p-12f, 0x1.4c5c2p-15f, -0x1.fe20aap-19f, 0x1.4d2f48p-22f, 0.0f, 0x1.ad0afcp+3f, 0.0f},
+    {0x1.b1dd48p-10f, -0x1.2907acp-12f, 0x1.117682p-15f, -0x1.98783cp-19f, 0x1.0536f6p-22f, 0.0f, 0x1.c30b8cp+3f, 0.0f},
+    {0x1.829a0ap-10f, -0x1.fc7794p-13f, 0x1.c46378p-16f, -0x1.489e34p-19f, 0x1.9b3a7ap-23f, 0.0f, 0x1.d90c1cp+3f, 0.0f},
+    {0x1.5a0b7cp-10f, -0x1.b57d3cp-13f, 0x1.7827bap-16f, -0x1.09a3c6p-19f, 0x1.45077ap-23f, 0.0f, 0x1.ef0cacp+3f, 0.0f},
+    {0x1.371018p-10f, -0x1.7a52dcp-13f, 0x1.3a6318p-16f, -0x1.af869ap-20f, 0x1.01fa28p-23f, 0.0f, 0x1.02869ep+4f, 0.0f},
+    {0x1.18bd54p-10f, -0x1.48c058p-13f, 0x1.08193ep-16f, -0x1.603078p-20f, 0x1.9b4378p-24f, 0.0f, 0x1.0d86e6p+4f, 0.0f},
+    {0x1.fca886p-11f, -0x1.1f01e8p-13f, 0x1.bdeb1ep-17f, -0x1.20d1e8p-20f, 0x1.493bcp-24f, 0.0f, 0x1.18872ep+4f, 0.0f},
+    {0x1.ce7176p-11f, -0x1.f75d7cp-14f, 0x1.7a4942p-17f, -0x1.dbf7c6p-21f, 0x1.08b764p-24f, 0.0f, 0x1.238776p+4f, 0.0f},
+    {0x1.a5d4c6p-11f, -0x1.bb4ae4p-14f, 0x1.426dbcp-17f, -0x1.8a0c18p-21f, 0x1.ab8e5ep-25f, 0.0f, 0x1.2e87bep+4f, 0.0f},
+    {0x1.81fe82p-11f, -0x1.87f9b4p-14f, 0x1.141434p-17f, -0x1.47c004p-21f, 0x1.5ace54p-25f, 0.0f, 0x1.398806p+4f, 0.0f},
+    {0x1.623eccp-11f, -0x1.5bf098p-14f, 0x1.dae166p-18f, -0x1.11dc2cp-21f, 0x1.1a8bfap-25f, 0.0f, 0x1.44884ep+4f, 0.0f},
+    {0x1.46031p-11f, -0x1.35fcep-14f, 0x1.9a2a86p-18f, -0x1.cbb984p-22f, 0x1.ce690ap-26f, 0.0f, 0x1.4f8896p+4f, 0.0f},
+    {0x1.2cd092p-11f, -0x1.152434p-14f, 0x1.63bb08p-18f, -0x1.839142p-22f, 0x1.7c093cp-26f, 0.0f, 0x1.5a88dep+4f, 0.0f},
+    {0x1.164016p-11f, -0x1.f1329ep-15f, 0x1.35bde6p-18f, -0x1.4824p-22f, 0x1.39af52p-26f, 0.0f, 0x1.658926p+4f, 0.0f},
+    {0x1.01fa6ap-11f, -0x1.bf6642p-15f, 0x1.0eb95ep-18f, -0x1.16fcfp-22f, 0x1.04057ep-26f, 0.0f, 0x1.70897p+4f, 0.0f},
+    {0x1.df6b38p-12f, -0x1.93cbd6p-15f, 0x1.daf8p-19f, -0x1.dc52b2p-23f, 0x1.b0e2c4p-27f, 0.0f, 0x1.7b89b8p+4f, 0.0f},
+    {0x1.be6548p-12f, -0x1.6d7b1ap-15f, 0x1.a21b7cp-19f, -0x1.98381p-23f, 0x1.69d1eap-27f, 0.0f, 0x1.868ap+4f, 0.0f},
+    {0x1.a0774p-12f, -0x1.4bb264p-15f, 0x1.7148bcp-19f, -0x1.5f2f1ap-23f, 0x1.2fa402p-27f, 0.0f, 0x1.918a48p+4f, 0.0f},
+    {0x1.854498p-12f, -0x1.2dcf9p-15f, 0x1.4734bp-19f, -0x1.2f3a74p-23f, 0x1.ffa41ep-28f, 0.0f, 0x1.9c8a9p+4f, 0.0f},
+    {0x1.6c7d96p-12f, -0x1.134a46p-15f, 0x1.22cf44p-19f, -0x1.06bfeap-23f, 0x1.b0b834p-28f, 0.0f, 0x1.a78ad8p+4f, 0.0f},
+    {0x1.55dd4p-12f, -0x1.f75eeap-16f, 0x1.0337f6p-19f, -0x1.c8e7b6p-24f, 0x1.6f572ap-28f, 0.0f, 0x1.b28b2p+4f, 0.0f},
+    {0x1.4127bep-12f, -0x1.cd3b44p-16f, 0x1.cf69aap-20f, -0x1.8e9254p-24f, 0x1.38f8fep-28f, 0.0f, 0x1.bd8b68p+4f, 0.0f},
+    {0x1.2e28fp-12f, -0x1.a783ccp-16f, 0x1.9f568p-20f, -0x1.5cc942p-24f, 0x1.0b977p-28f, 0.0f, 0x1.c88bbp+4f, 0.0f},
+    {0x1.1cb34ep-12f, -0x1.85ab08p-16f, 0x1.753456p-20f, -0x1.32260ep-24f, 0x1.cb250cp-29f, 0.0f, 0x1.d38bf8p+4f, 0.0f},
+    {0x1.0c9efp-12f, -0x1.6737dp-16f, 0x1.502af6p-20f, -0x1.0d828ap-24f, 0x1.8b3664p-29f, 0.0f, 0x1.de8c4p+4f, 0.0f},
+    {0x1.fb9186p-13f, -0x1.4bc208p-16f, 0x1.2f849cp-20f, -0x1.dbd8d6p-25f, 0x1.55452ep-29f, 0.0f, 0x1.e98c88p+4f, 0.0f},
+    {0x1.e023b8p-13f, -0x1.32efeep-16f, 0x1.12a7fp-20f, -0x1.a536dp-25f, 0x1.2799f4p-29f, 0.0f, 0x1.f48cdp+4f, 0.0f},
+    {0x1.c6bddp-13f, -0x1.1c73eap-16f, 0x1.f2263ap-21f, -0x1.75d2bp-25f, 0x1.00ceeep-29f, 0.0f, 0x1.ff8d18p+4f, 0.0f},
+    {0x1.af2f5ap-13f, -0x1.080aaap-16f, 0x1.c4afcp-21f, -0x1.4c97d4p-25f, 0x1.bf7fap-30f, 0.0f, 0x1.0546bp+5f, 0.0f},
+    {0x1.994d4cp-13f, -0x1.eaf33ep-17f, 0x1.9c3088p-21f, -0x1.289f2ap-25f, 0x1.86f9f8p-30f, 0.0f, 0x1.0ac6d4p+5f, 0.0f},
+    {0x1.84f14ep-13f, -0x1.c91b62p-17f, 0x1.7807fp-21f, -0x1.092704p-25f, 0x1.5682acp-30f, 0.0f, 0x1.1046f8p+5f, 0.0f},
+    {0x1.71f928p-13f, -0x1.aa345ep-17f, 0x1.57ac3cp-21f, -0x1.db191ap-26f, 0x1.2cd45ep-30f, 0.0f, 0x1.15c71cp+5f, 0.0f},
+    {0x1.604636p-13f, -0x1.8defcep-17f, 0x1.3aa6fcp-21f, -0x1.aa8ccep-26f, 0x1.08e19cp-30f, 0.0f, 0x1.1b474p+5f, 0.0f},
+    {0x1.4fbd02p-13f, -0x1.7408cp-17f, 0x1.20920cp-21f, -0x1.7fc0bp-26f, 0x1.d39522p-31f, 0.0f, 0x1.20c764p+5f, 0.0f},
+    {0x1.4044d8p-13f, -0x1.5c426cp-17f, 0x1.091516p-21f, -0x1.59ef76p-26f, 0x1.9da95cp-31f, 0.0f, 0x1.264788p+5f, 0.0f},
+    {0x1.31c78p-13f, -0x1.466724p-17f, 0x1.e7c702p-22f, -0x1.387134p-26f, 0x1.6ec7f8p-31f, 0.0f, 0x1.2bc7acp+5f, 0.0f},
+    {0x1.2430eap-13f, -0x1.324756p-17f, 0x1.c1756ep-22f, -0x1.1ab68cp-26f, 0x1.45eb62p-31f, 0.0f, 0x1.3147dp+5f, 0.0f},
+    {0x1.176ef4p-13f, -0x1.1fb8dp-17f, 0x1.9ec15ep-22f, -0x1.0044cap-26f, 0x1.223798p-31f, 0.0f, 0x1.36c7f4p+5f, 0.0f},
+    {0x1.0b713p-13f, -0x1.0e9604p-17f, 0x1.7f4564p-22f, -0x1.d16538p-27f, 0x1.02f304p-31f, 0.0f, 0x1.3c4818p+5f, 0.0f},
+    {0x1.0028bcp-13f, -0x1.fd7af4p-18f, 0x1.62a8b2p-22f, -0x1.a74ab6p-27f, 0x1.cf0158p-32f, 0.0f, 0x1.41c83cp+5f, 0.0f},
+    {0x1.eb1018p-14f, -0x1.e022a6p-18f, 0x1.489d5p-22f, -0x1.819da8p-27f, 0x1.9eb6c8p-32f, 0.0f, 0x1.47486p+5f, 0.0f},
+    {0x1.d7059ap-14f, -0x1.c4ed9ep-18f, 0x1.30deaep-22f, -0x1.5fd6c8p-27f, 0x1.7423d4p-32f, 0.0f, 0x1.4cc884p+5f, 0.0f},
+    {0x1.c41b88p-14f, -0x1.ababe8p-18f, 0x1.1b3054p-22f, -0x1.4180dp-27f, 0x1.4e875p-32f, 0.0f, 0x1.5248a8p+5f, 0.0f},
+    {0x1.b23d5cp-14f, -0x1.943278p-18f, 0x1.075cdcp-22f, -0x1.2635d2p-27f, 0x1.2d3c16p-32f, 0.0f, 0x1.57c8ccp+5f, 0.0f},
+    {0x1.a15854p-14f, -0x1.7e5a8cp-18f, 0x1.ea69fep-23f, -0x1.0d9d0cp-27f, 0x1.0fb498p-32f, 0.0f, 0x1.5d48fp+5f, 0.0f},
 };
 
 __device__ __forceinline__ void near32t(float r2, float sig2, float nu2, const float4* __restrict__ tab,
@@ -727,10 +759,9 @@ __device__ __forceinline__ void near32t(float r2, float sig2, float nu2, const f
   const float e = __builtin_amdgcn_exp2f(kMHalfLog2e * sb);
   // fminf keeps the index in the table for any sb (NaN included)
   const int k = static_cast<int>(fminf(sb * kCoreScale, static_cast<float>(kCorePieces - 1)));
-  const float4 lo = tab[2 * k], hi = tab[2 * k + 1];  // (c0, c1, c2, c3), (c4, c5, s_c, 0)
+  const float4 lo = tab[2 * k], hi = tab[2 * k + 1];  // (c0, c1, c2, c3), (c4, 0, s_c, 0)
   const float t = sb - hi.z;
-  float g = fmaf(hi.y, t, hi.x);
-  g = fmaf(g, t, lo.w);
+  float g = fmaf(hi.x, t, lo.w);
   g = fmaf(g, t, lo.z);
   g = fmaf(g, t, lo.y);
   g = fmaf(g, t, lo.x);
@@ -1517,8 +1548,11 @@ struct NearArgs {
   Rate64* partial;  // item x 256
   const LeafChunk* chunks;  // leaf x kLeafChunks run spheres
   // every particle of the same radius (uniform sigma): the core's sigma
-  // factors, once per launch (u_cut2 = 44 sigma^2 > 0 marks the case)
-  float u_cut2, u_isg2, u_ke, u_fk, u_ck;
+  // factors, once per launch (u_cut2 = 44 sigma^2 > 0 marks the case):
+  // 1/sigma^2, ke = sqrt(2/pi)/sigma^3 (the table is staged times ke),
+  // log2(ke/3) (added to the exponent: e' = (ke/3) e^{-s/2}) and the PSE
+  // weight 2 nu (2 pi)^-3/2 / sigma^5 divided by ke/3
+  float u_cut2, u_isg2, u_ke, u_lk3, u_ck3;
 };
 
 constexpr int kNearSlice = 64;                       // targets per wave
@@ -1530,23 +1564,22 @@ static_assert(kNearSlices == kLeafChunks && kNearSlice == 64, "a slice is one 64
 // algebraic kernel, exact for every pair, PSE within 44 sigma^2 as in
 // pair32_ho)
 // near32t for a pair radius shared by every pair: the sigma factors come
-// from the launch (isg2 = 1/sigma^2, ke = sqrt(2/pi)/sigma^3, fk = ke/sigma^2,
-// ck = 2 nu (2 pi)^-3/2 / sigma^5)
+// from the launch and ke is folded into the staged table (g = ke G) and
+// into the exponent (e = (ke/3) e^{-s/2}), so q' = s g + e, f' = g / sigma^2
 __device__ __forceinline__ void near32u(float r2, const NearArgs& a, const float4* __restrict__ tab, float& qp,
                                         float& fp, float& c) {
   const float sb = r2 * a.u_isg2;
-  const float e = __builtin_amdgcn_exp2f(kMHalfLog2e * sb);
+  const float e = __builtin_amdgcn_exp2f(fmaf(kMHalfLog2e, sb, a.u_lk3));
   const int k = static_cast<int>(fminf(sb * kCoreScale, static_cast<float>(kCorePieces - 1)));
   const float4 lo = tab[2 * k], hi = tab[2 * k + 1];
   const float t = sb - hi.z;
-  float g = fmaf(hi.y, t, hi.x);
-  g = fmaf(g, t, lo.w);
+  float g = fmaf(hi.x, t, lo.w);
   g = fmaf(g, t, lo.z);
   g = fmaf(g, t, lo.y);
   g = fmaf(g, t, lo.x);
-  qp = a.u_ke * fmaf(sb, g, e * 0x1.555556p-2f);
-  fp = a.u_fk * g;
-  c = a.u_ck * e;
+  qp = fmaf(sb, g, e);
+  fp = a.u_isg2 * g;
+  c = a.u_ck3 * e;
 }
 
 template <bool HighOrder, bool Uniform = false>
@@ -1698,11 +1731,14 @@ __device__ __forceinline__ void near_slice(const NearArgs& a, const NearItem& it
 
 template <bool HighOrder, bool Uniform = false>
 __device__ __forceinline__ void near_entry(const NearArgs& a, SrcRel* tile, float4* tab) {
-  if (!HighOrder) {  // the core table into LDS (one 64-lane block: a float4 per lane)
-    static_assert(2 * kCorePieces == kNearSlice, "one table row half per lane");
-    const int l = static_cast<int>(threadIdx.x);
-    const float* row = &kCoreTable[l >> 1][4 * (l & 1)];
-    tab[l] = make_float4(row[0], row[1], row[2], row[3]);
+  if (!HighOrder) {  // the core table into LDS (row halves; times ke for one radius, s_c kept)
+    static_assert((2 * kCorePieces) % kNearSlice == 0, "whole passes over the table");
+    for (int l = static_cast<int>(threadIdx.x); l < 2 * kCorePieces; l += kNearSlice) {
+      const float* row = &kCoreTable[l >> 1][4 * (l & 1)];
+      const float sc = Uniform ? a.u_ke : 1.0f;
+      tab[l] = (l & 1) ? make_float4(row[0] * sc, row[1] * sc, row[2], row[3])
+                       : make_float4(row[0] * sc, row[1] * sc, row[2] * sc, row[3] * sc);
+    }
     __syncthreads();
   }
   const int slices = kNearSlices * *a.total;
@@ -2017,10 +2053,11 @@ hipError_t launch_near_fp32c(const Rec64* sorted, const NodeRec* nodes, const Tr
     // factors rounded from FP64, as near32t rounds them per pair from FP32
     const double s2 = sigma_max2, is2 = 1.0 / s2, is3 = is2 / std::sqrt(s2);
     a.u_cut2 = kFar32 * sigma_max2;
+    const double ke = 0.79788456080286535588 * is3;  // sqrt(2/pi) / sigma^3
     a.u_isg2 = static_cast<float>(is2);
-    a.u_ke = static_cast<float>(0.79788456080286535588 * is3);  // sqrt(2/pi)
-    a.u_fk = static_cast<float>(0.79788456080286535588 * is3 * is2);
-    a.u_ck = static_cast<float>(2.0 * nu * 0.063493635934240969785 * is3 * is2);  // (2 pi)^-3/2
+    a.u_ke = static_cast<float>(ke);
+    a.u_lk3 = static_cast<float>(std::log2(ke / 3.0));
+    a.u_ck3 = static_cast<float>(2.0 * nu * 0.063493635934240969785 * is3 * is2 / (ke / 3.0));  // (2 pi)^-3/2
     hipLaunchKernelGGL(k_near_fp32c_u, dim3(grid), dim3(kNearSlice), 0, st, a);
   } else if (highorder) {
     hipLaunchKernelGGL(k_near_fp32c_ho, dim3(grid), dim3(kNearSlice), 0, st, a);

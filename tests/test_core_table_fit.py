@@ -1,5 +1,5 @@
 """The branch-free Gaussian core of the tree path's near field: G(s) =
-exp(-s/2) B(s) from a table of 32 degree-5 pieces on 0 <= s <= 44.  The
+exp(-s/2) B(s) from a table of 64 degree-4 pieces on 0 <= s <= 44.  The
 table compiled into pw_pair.hip is read from the source and checked against
 a fresh fit, against G by an independent high-precision sum of its series,
 and, through the kernel's formulas q' = sqrt(2/pi)(e/3 + s G)/sigma^3 and
@@ -23,11 +23,11 @@ def _source():
 
 
 def _compiled_table():
-    body = re.search(r"kCoreTable\[32\]\[8\]\s*=\s*\{(.*?)\};", _source(), re.S).group(1)
+    body = re.search(r"kCoreTable\[64\]\[8\]\s*=\s*\{(.*?)\};", _source(), re.S).group(1)
     rows = re.findall(r"\{([^}]*)\}", body)
     vals = [[float.fromhex(v.strip().rstrip("f")) if "x" in v else float(v.strip().rstrip("f"))
              for v in r.split(",")] for r in rows]
-    assert len(vals) == 32 and all(len(r) == 8 for r in vals)
+    assert len(vals) == 64 and all(len(r) == 8 for r in vals)
     return np.array(vals, dtype=np.float32)
 
 
