@@ -332,6 +332,8 @@ def pair_kernel_name(precision: str, cfg: dict | None) -> str:
     if precision == "fp64":
         return "k_pair_fp64s" if (cfg or {}).get("scalar_feed") else "k_pair_fp64"
     cfg = cfg or {}
+    if cfg.get("two_per_lane"):
+        return "k_pair_fp32s_t2"
     return ("k_pair_fp32s" if cfg.get("scalar_feed") else "k_pair_fp32c") + ("_w8" if cfg.get("waves8") else "")
 
 
@@ -376,10 +378,12 @@ def measured_traffic(workload: str, n: int, precision: str, cfg: dict | None):
         return None
     with open(path) as f:
         t = json.load(f)
-    want = {k: bool(cfg.get(k)) if k != "parts" else int(cfg.get(k, 1))
-            for k in ("parts", "persistent", "waves8", "scalar_feed")}
+    keys = ("parts", "persistent", "waves8", "scalar_feed", "two_per_lane")
+    want = {k: bool(cfg.get(k)) if k != "parts" else int(cfg.get(k, 1)) for k in keys}
     for e in t["entries"]:
-        if e["precision"] == precision and e["launch_config"] == want:
+        have = {k: (int(e["launch_config"].get(k, 1)) if k == "parts" else bool(e["launch_config"].get(k)))
+                for k in keys}
+        if e["precision"] == precision and have == want:
             fetch = e["fetch_kib"] * t["fetch_factor"] * 1024.0
             write = e["write_kib"] * t["write_factor"] * 1024.0
             out = {"bytes": fetch + write, "fetch_bytes": fetch, "write_bytes": write, "kernel": e["kernel"],
@@ -575,8 +579,8 @@ def profile_rhs(args) -> dict:
                 g.ff_set_err_coef(float(w["coef_md"]), float(w["coef_mlp"]))
             g.ff_set_params(args.theta, args.tol, use_mlp=bool(args.weights))
         elif args.config:
-            v = [int(x) for x in args.config.split(",")] + [0]
-            g.set_pair_config(args.precision, v[0], bool(v[1]), bool(v[2]), bool(v[3]))
+            v = [int(x) for x in args.config.split(",")] + [0, 0]
+            g.set_pair_config(args.precision, v[0], bool(v[1]), bool(v[2]), bool(v[3]), bool(v[4]))
         g.set_profiling(True)
         for k in range(args.reps):
             t = time.perf_counter()
@@ -652,7 +656,7 @@ def main() -> int:
                     help="profile-rhs, tree path: absolute far-field tolerance (0: opening angle only)")
     ap.add_argument("--weights", default="", help="profile-rhs, tree path: network (npz from --save-weights)")
     ap.add_argument("--config", default="",
-                    help="profile-rhs: pair-kernel launch shape 'parts,persistent,waves8[,scalar_feed]' "
+                    help="profile-rhs: pair-kernel launch shape 'parts,persistent,waves8[,scalar_feed[,two_per_lane]]' "
                          "(e.g. 8,1,1 or 4,1,1,1)")
     ap.add_argument("--kernel", default="gaussian", choices=["gaussian", "highorder"])
     args = ap.parse_args()

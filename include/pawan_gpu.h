@@ -311,13 +311,17 @@ int pw_get_pair_timing(pw_ctx *ctx, double *total_ms, int64_t *launches);
  *                           each wave of 64 targets streams the sources
  *                           through scalar loads, chunk by chunk, instead
  *                           of the block's four waves sharing LDS tiles;
+ *   PW_VARIANT_TWO_PER_LANE (FP32C with SCALAR_FEED) two targets per lane:
+ *                           a wave owns 128 targets and every source record
+ *                           it reads feeds two pairs (WAVES8 is ignored);
  * bits that do not apply to the precision are ignored.
  * No counterpart in the reference (its OpenMP loops have no launch shape);
  * defaults are parts 1, grid launch, variant 0.  A variant outside those
  * bits is PW_ERR_ARG. */
 enum {
   PW_VARIANT_WAVES8 = 1,
-  PW_VARIANT_SCALAR_FEED = 2
+  PW_VARIANT_SCALAR_FEED = 2,
+  PW_VARIANT_TWO_PER_LANE = 4
 };
 int pw_set_pair_config(pw_ctx *ctx, int precision, int parts, int persistent,
                        int variant);

@@ -20,7 +20,7 @@ PW_OK, PW_ERR_ARG, PW_ERR_HIP, PW_ERR_STATE, PW_ERR_NONFINITE, PW_ERR_UNSUPPORTE
 PW_FP64, PW_FP32C, PW_LEARNED, PW_FP64_REF = 0, 1, 2, 3
 PW_EULER, PW_RK3, PW_RK4 = 1, 3, 4
 PW_KERNEL_GAUSSIAN, PW_KERNEL_HIGHORDER = 0, 1
-PW_VARIANT_WAVES8, PW_VARIANT_SCALAR_FEED = 1, 2
+PW_VARIANT_WAVES8, PW_VARIANT_SCALAR_FEED, PW_VARIANT_TWO_PER_LANE = 1, 2, 4
 
 PRECISIONS = {"fp64": PW_FP64, "fp64_ref": PW_FP64_REF, "fp32c": PW_FP32C, "learned": PW_LEARNED}
 SCHEMES = {"euler": PW_EULER, "rk3": PW_RK3, "rk4": PW_RK4}
@@ -388,8 +388,9 @@ class Solver:
         _check(self._lib.pw_set_profiling(self._h, int(bool(on))))
 
     def set_pair_config(self, precision: str, parts: int = 1, persistent: bool = False,
-                        waves8: bool = False, scalar_feed: bool = False) -> None:
-        variant = (PW_VARIANT_WAVES8 if waves8 else 0) | (PW_VARIANT_SCALAR_FEED if scalar_feed else 0)
+                        waves8: bool = False, scalar_feed: bool = False, two_per_lane: bool = False) -> None:
+        variant = ((PW_VARIANT_WAVES8 if waves8 else 0) | (PW_VARIANT_SCALAR_FEED if scalar_feed else 0) |
+                   (PW_VARIANT_TWO_PER_LANE if two_per_lane else 0))
         _check(self._lib.pw_set_pair_config(self._h, PRECISIONS[precision], int(parts),
                                             int(bool(persistent)), variant))
 
@@ -398,7 +399,8 @@ class Solver:
         _check(self._lib.pw_get_pair_config(self._h, PRECISIONS[precision], C.byref(p), C.byref(q),
                                             C.byref(w)))
         return {"parts": p.value, "persistent": bool(q.value), "waves8": bool(w.value & PW_VARIANT_WAVES8),
-                "scalar_feed": bool(w.value & PW_VARIANT_SCALAR_FEED)}
+                "scalar_feed": bool(w.value & PW_VARIANT_SCALAR_FEED),
+                "two_per_lane": bool(w.value & PW_VARIANT_TWO_PER_LANE)}
 
     def autotune(self, precision: str) -> float:
         ms = C.c_double()

@@ -1264,12 +1264,13 @@ int pw_set_pair_config(pw_ctx* c, int precision, int parts, int persistent, int 
   if (kind == kKindLearned) return fail(PW_ERR_UNSUPPORTED, "launch configuration applies to the direct-sum kinds");
   if (parts < 1 || parts > 256) return fail(PW_ERR_ARG, "parts must be in [1, 256]");
   auto& k = c->ks[kind];
-  if (variant < 0 || variant > (PW_VARIANT_WAVES8 | PW_VARIANT_SCALAR_FEED))
-    return fail(PW_ERR_ARG, "variant must be a combination of PW_VARIANT_WAVES8 and PW_VARIANT_SCALAR_FEED");
+  if (variant < 0 || variant > (PW_VARIANT_WAVES8 | PW_VARIANT_SCALAR_FEED | PW_VARIANT_TWO_PER_LANE))
+    return fail(PW_ERR_ARG, "variant must be a combination of the PW_VARIANT_* bits");
   k.cfg.parts = parts;
   k.cfg.persistent = persistent != 0;
-  k.cfg.waves8 = (variant & PW_VARIANT_WAVES8) != 0 && kind == pw::kPairFP32C;
   k.cfg.scalar_feed = (variant & PW_VARIANT_SCALAR_FEED) != 0 && (kind == pw::kPairFP32C || kind == pw::kPairFP64);
+  k.cfg.two_per_lane = (variant & PW_VARIANT_TWO_PER_LANE) != 0 && k.cfg.scalar_feed && kind == pw::kPairFP32C;
+  k.cfg.waves8 = (variant & PW_VARIANT_WAVES8) != 0 && kind == pw::kPairFP32C && !k.cfg.two_per_lane;
   k.tuned = true;
   drop_graphs(c);
   return PW_OK;
@@ -1283,7 +1284,9 @@ int pw_get_pair_config(pw_ctx* c, int precision, int* parts, int* persistent, in
   const auto& k = c->ks[kind];
   if (parts) *parts = k.cfg.parts;
   if (persistent) *persistent = k.cfg.persistent ? 1 : 0;
-  if (variant) *variant = (k.cfg.waves8 ? PW_VARIANT_WAVES8 : 0) | (k.cfg.scalar_feed ? PW_VARIANT_SCALAR_FEED : 0);
+  if (variant)
+    *variant = (k.cfg.waves8 ? PW_VARIANT_WAVES8 : 0) | (k.cfg.scalar_feed ? PW_VARIANT_SCALAR_FEED : 0) |
+               (k.cfg.two_per_lane ? PW_VARIANT_TWO_PER_LANE : 0);
   return PW_OK;
 }
 
@@ -1330,10 +1333,11 @@ void load_tune_file() {
     pw::PairConfig cfg;
     cfg.parts = parts;
     cfg.persistent = pers != 0;
-    if (w8 < 0 || w8 > (PW_VARIANT_WAVES8 | PW_VARIANT_SCALAR_FEED)) continue;
+    if (w8 < 0 || w8 > (PW_VARIANT_WAVES8 | PW_VARIANT_SCALAR_FEED | PW_VARIANT_TWO_PER_LANE)) continue;
     const bool f32 = kind == pw::kPairFP32C || kind == pw::kPairFP32CHO;
-    cfg.waves8 = (w8 & PW_VARIANT_WAVES8) != 0 && f32;
     cfg.scalar_feed = (w8 & PW_VARIANT_SCALAR_FEED) != 0 && (f32 || kind == pw::kPairFP64);
+    cfg.two_per_lane = (w8 & PW_VARIANT_TWO_PER_LANE) != 0 && cfg.scalar_feed && f32;
+    cfg.waves8 = (w8 & PW_VARIANT_WAVES8) != 0 && f32 && !cfg.two_per_lane;
     tune_cache()[TuneKey(kind, n, nsys, largest)] = cfg;
   }
   std::fclose(f);
@@ -1347,7 +1351,8 @@ void save_tune_file(const TuneKey& key, const pw::PairConfig& cfg) {
   std::fprintf(f, "%d %lld %d %lld %d %d %d\n", std::get<0>(key), static_cast<long long>(std::get<1>(key)),
                std::get<2>(key), static_cast<long long>(std::get<3>(key)), cfg.parts,
                cfg.persistent ? 1 : 0,
-               (cfg.waves8 ? PW_VARIANT_WAVES8 : 0) | (cfg.scalar_feed ? PW_VARIANT_SCALAR_FEED : 0));
+               (cfg.waves8 ? PW_VARIANT_WAVES8 : 0) | (cfg.scalar_feed ? PW_VARIANT_SCALAR_FEED : 0) |
+                   (cfg.two_per_lane ? PW_VARIANT_TWO_PER_LANE : 0));
   std::fclose(f);
 }
 
@@ -1431,13 +1436,14 @@ int pw_autotune(pw_ctx* c, int precision, double* best_ms) {
     max_tiles = std::max(max_tiles, nt);
   }
   std::vector<pw::PairConfig> cands;
-  auto add = [&](int parts, bool pers, bool w8, bool sf = false) {
+  auto add = [&](int parts, bool pers, bool w8, bool sf = false, bool t2 = false) {
     if (parts > max_tiles) return;
     pw::PairConfig p;
     p.parts = parts;
     p.persistent = pers;
-    p.waves8 = w8;
+    p.waves8 = w8 && !t2;
     p.scalar_feed = sf;
+    p.two_per_lane = t2;
     cands.push_back(p);
   };
   const bool f32 = kind == pw::kPairFP32C;
@@ -1466,6 +1472,13 @@ int pw_autotune(pw_ctx* c, int precision, double* best_ms) {
   for (int parts = 16; parts <= 32; parts *= 2) {
     if (blocks * parts > 8 * resident) add(parts, true, f32);
     if (f32) add(parts, true, true, true);
+  }
+  // two targets per lane (particles per thread): each scalar source record
+  // feeds two pairs, at half the waves
+  if (f32) {
+    add(8, true, false, true, true);
+    add(16, true, false, true, true);
+    add(32, true, false, true, true);
   }
   // the scalar-fed FP64 kernel (Gaussian FP64 only)
   if (kind == pw::kPairFP64 && c->smoothing == PW_KERNEL_GAUSSIAN) {

@@ -85,6 +85,7 @@ struct PairConfig {
   bool scalar_feed = false;// FP32C: each wave streams the sources through
                            // scalar loads (Chunk32/Src32) instead of LDS
                            // tiles shared by the block's four waves
+  bool two_per_lane = false;// FP32C scalar-fed: two targets per lane
 };
 
 // Pair-kernel variants.

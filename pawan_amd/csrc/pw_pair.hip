@@ -1258,8 +1258,10 @@ __global__ __launch_bounds__(64) void k_src_chunks(const Rec64* __restrict__ sta
   }
 }
 
-// Targets [tb, tb + 64) of item it (a 64-lane wave of its own).
-template <bool HighOrder>
+// Targets [tb, tb + 64 TPL) of item it (a 64-lane wave of its own; with
+// TPL = 2 each lane owns targets tb + lane and tb + 64 + lane, and every
+// source record read from scalar registers feeds two pairs).
+template <bool HighOrder, int TPL>
 __device__ __forceinline__ void wave_fp32s(const WorkItem& it, int tb, double nu, const Rec64* state_g,
                                            const Chunk32* chunks_g, int nchunks, const Src32* src_g,
                                            Rate64* __restrict__ out) {
@@ -1267,23 +1269,35 @@ __device__ __forceinline__ void wave_fp32s(const WorkItem& it, int tb, double nu
   const ConstPtr<Chunk32> chunks = as_const(chunks_g);
   const ConstPtr<Src32> src = as_const(src_g);
   const int lane = static_cast<int>(threadIdx.x) & 63;
-  const int t = tb + lane;
-  const bool active = t < it.t1;
-  const Rec64 m = state_g[active ? t : tb];
-  T32 me;
-  me.x = m.x; me.y = m.y; me.z = m.z;
-  me.hs2 = static_cast<float>(0.5 * m.s2);
-  me.ax = static_cast<float>(m.ax);
-  me.ay = static_cast<float>(m.ay);
-  me.az = static_cast<float>(m.az);
-  me.vol = static_cast<float>(m.vol);
-  me.nu2 = 2.0f * static_cast<float>(nu);
-  // the wave's targets' sphere (inactive lanes repeat the first target)
+  int t[TPL];
+  bool active[TPL];
+  T32 me[TPL];
+  double lo_x = 0, lo_y = 0, lo_z = 0, hi_x = 0, hi_y = 0, hi_z = 0;
+  float hs2max = 0.0f;
+#pragma unroll
+  for (int j = 0; j < TPL; ++j) {
+    t[j] = tb + 64 * j + lane;
+    active[j] = t[j] < it.t1;
+    // inactive lanes repeat the first target (inside the sphere already)
+    const Rec64 m = state_g[active[j] ? t[j] : tb];
+    me[j].x = m.x; me[j].y = m.y; me[j].z = m.z;
+    me[j].hs2 = static_cast<float>(0.5 * m.s2);
+    me[j].ax = static_cast<float>(m.ax);
+    me[j].ay = static_cast<float>(m.ay);
+    me[j].az = static_cast<float>(m.az);
+    me[j].vol = static_cast<float>(m.vol);
+    me[j].nu2 = 2.0f * static_cast<float>(nu);
+    lo_x = j ? fmin(lo_x, m.x) : m.x; hi_x = j ? fmax(hi_x, m.x) : m.x;
+    lo_y = j ? fmin(lo_y, m.y) : m.y; hi_y = j ? fmax(hi_y, m.y) : m.y;
+    lo_z = j ? fmin(lo_z, m.z) : m.z; hi_z = j ? fmax(hi_z, m.z) : m.z;
+    hs2max = fmaxf(hs2max, me[j].hs2);
+  }
+  // the wave's targets' sphere
   double cW[3];
   float rW2 = 0.0f;
   {
-    const double lo[3] = {wave_mind(m.x), wave_mind(m.y), wave_mind(m.z)};
-    const double hi[3] = {wave_maxd(m.x), wave_maxd(m.y), wave_maxd(m.z)};
+    const double lo[3] = {wave_mind(lo_x), wave_mind(lo_y), wave_mind(lo_z)};
+    const double hi[3] = {wave_maxd(hi_x), wave_maxd(hi_y), wave_maxd(hi_z)};
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       cW[k] = uniform(0.5 * (lo[k] + hi[k]));
@@ -1292,11 +1306,13 @@ __device__ __forceinline__ void wave_fp32s(const WorkItem& it, int tb, double nu
     }
   }
   const float rW = uniform(sqrtf(rW2) * 1.00001f);
-  const float h2W = uniform(wave_max(me.hs2));
-  float hi[kAcc], lo[kAcc];
+  const float h2W = uniform(wave_max(hs2max));
+  float hi[TPL][kAcc], lo[TPL][kAcc];
 #pragma unroll
-  for (int k = 0; k < kAcc; ++k) hi[k] = lo[k] = 0.0f;
-  PSE32 pse;
+  for (int j = 0; j < TPL; ++j)
+#pragma unroll
+    for (int k = 0; k < kAcc; ++k) hi[j][k] = lo[j][k] = 0.0f;
+  PSE32 pse[TPL];
   for (int ci = it.chunk0; ci < nchunks; ++ci) {
     const Chunk32 h = load(chunks, ci);
     if (h.first >= it.s1) break;
@@ -1306,50 +1322,71 @@ __device__ __forceinline__ void wave_fp32s(const WorkItem& it, int tb, double nu
     const float gap = dist * 0.99999f - rW - h.r;
     const bool sep = gap > 0.0f && gap * gap > kFar32 * (h2W + h.h2) * 1.0001f && h.r <= gap;
     if (sep) {
-      const float tx = static_cast<float>(me.x - h.cx), ty = static_cast<float>(me.y - h.cy),
-                  tz = static_cast<float>(me.z - h.cz);
-      const float cx = ty * me.az - tz * me.ay, cy = tz * me.ax - tx * me.az, cz = tx * me.ay - ty * me.ax;
-      P32S p;
-#pragma unroll 4
-      for (int l = l0; l < l1; ++l)
-        pair32s<HighOrder>(tx, ty, tz, me.hs2, me.ax, me.ay, me.az, cx, cy, cz, load(src, l), p);
-      fold_s(p, tx, ty, tz, me.ax, me.ay, me.az, hi, lo);
+      float tx[TPL], ty[TPL], tz[TPL], cx[TPL], cy[TPL], cz[TPL];
+#pragma unroll
+      for (int j = 0; j < TPL; ++j) {
+        tx[j] = static_cast<float>(me[j].x - h.cx);
+        ty[j] = static_cast<float>(me[j].y - h.cy);
+        tz[j] = static_cast<float>(me[j].z - h.cz);
+        cx[j] = ty[j] * me[j].az - tz[j] * me[j].ay;
+        cy[j] = tz[j] * me[j].ax - tx[j] * me[j].az;
+        cz[j] = tx[j] * me[j].ay - ty[j] * me[j].ax;
+      }
+      P32S p[TPL];
+      constexpr int kSepUnroll = 4 / TPL;
+#pragma unroll kSepUnroll
+      for (int l = l0; l < l1; ++l) {
+        const Src32 sr = load(src, l);
+#pragma unroll
+        for (int j = 0; j < TPL; ++j)
+          pair32s<HighOrder>(tx[j], ty[j], tz[j], me[j].hs2, me[j].ax, me[j].ay, me[j].az, cx[j], cy[j], cz[j], sr,
+                             p[j]);
+      }
+#pragma unroll
+      for (int j = 0; j < TPL; ++j) fold_s(p[j], tx[j], ty[j], tz[j], me[j].ax, me[j].ay, me[j].az, hi[j], lo[j]);
     } else {
-      P32 p;
+      P32 p[TPL];
 #pragma unroll 1
       for (int l = l0; l < l1; ++l) {
         const Src32 sr = load(src, l);
         SrcCore sc;
         sc.x = state[l].x; sc.y = state[l].y; sc.z = state[l].z;
         sc.hs2 = sr.hs2; sc.ax = sr.ax; sc.ay = sr.ay; sc.az = sr.az; sc.vol = sr.vol;
-        if (l != t) {  // the self pair is never evaluated
-          if (HighOrder) pair32_ho(me, sc, p, pse);
-          else pair32(me, sc, p, pse);
+#pragma unroll
+        for (int j = 0; j < TPL; ++j) {
+          if (l != t[j]) {  // the self pair is never evaluated
+            if (HighOrder) pair32_ho(me[j], sc, p[j], pse[j]);
+            else pair32(me[j], sc, p[j], pse[j]);
+          }
         }
       }
-      fold(p, me.ax, me.ay, me.az, hi, lo);
+#pragma unroll
+      for (int j = 0; j < TPL; ++j) fold(p[j], me[j].ax, me[j].ay, me[j].az, hi[j], lo[j]);
     }
   }
-  if (active) {
+#pragma unroll
+  for (int j = 0; j < TPL; ++j) {
+    if (!active[j]) continue;
     double acc[kAcc];
 #pragma unroll
-    for (int k = 0; k < kAcc; ++k) acc[k] = static_cast<double>(hi[k]) + static_cast<double>(lo[k]);
+    for (int k = 0; k < kAcc; ++k) acc[k] = static_cast<double>(hi[j][k]) + static_cast<double>(lo[j][k]);
     Rate64 r;
     r.ux = kQ * acc[0];
     r.uy = kQ * acc[1];
     r.uz = kQ * acc[2];
-    r.wx = kQ * acc[3] + static_cast<double>(pse.x);
-    r.wy = kQ * acc[4] + static_cast<double>(pse.y);
-    r.wz = kQ * acc[5] + static_cast<double>(pse.z);
-    out[t] = r;
+    r.wx = kQ * acc[3] + static_cast<double>(pse[j].x);
+    r.wy = kQ * acc[4] + static_cast<double>(pse[j].y);
+    r.wz = kQ * acc[5] + static_cast<double>(pse[j].z);
+    out[t[j]] = r;
   }
 }
 
-// Entry: one-wave blocks; wave index w = 4 item + quarter (grid form: w =
-// blockIdx.x; persistent form: pulled from the queue).
-template <bool HighOrder, bool Persistent>
+// Entry: one-wave blocks; wave index w = (4 / TPL) item + part of the item
+// (grid form: w = blockIdx.x; persistent form: pulled from the queue).
+template <bool HighOrder, bool Persistent, int TPL = 1>
 __device__ __forceinline__ void fp32s_entry(const PairArgs& a) {
-  const int waves = 4 * a.nitems;
+  constexpr int kWavesPerItem = 4 / TPL;
+  const int waves = kWavesPerItem * a.nitems;
   for (int w = Persistent ? -1 : static_cast<int>(blockIdx.x);;) {
     if (Persistent) {
       int idx = 0;
@@ -1357,11 +1394,11 @@ __device__ __forceinline__ void fp32s_entry(const PairArgs& a) {
       w = __builtin_amdgcn_readfirstlane(idx);  // lane 0's: every lane is active here
     }
     if (w >= waves) break;
-    const WorkItem it = a.items[w >> 2];
-    const int tb = it.t0 + 64 * (w & 3);
+    const WorkItem it = a.items[w / kWavesPerItem];
+    const int tb = it.t0 + 64 * TPL * (w % kWavesPerItem);
     if (tb < it.t1) {
       Rate64* out = it.part < 0 ? a.rates : a.partial + static_cast<size_t>(it.part) * a.n;
-      wave_fp32s<HighOrder>(it, tb, a.nu, a.state, a.chunks, a.nchunks, a.src, out);
+      wave_fp32s<HighOrder, TPL>(it, tb, a.nu, a.state, a.chunks, a.nchunks, a.src, out);
     }
     if (!Persistent) break;
   }
@@ -1888,6 +1925,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
 }
 template <bool P>
 __global__ __launch_bounds__(64) void k_pair_fp32s_ho(PairArgs a) { fp32s_entry<true, P>(a); }
+// two targets per lane (every scalar source record feeds two pairs)
+template <bool P>
+__global__ __launch_bounds__(64) void k_pair_fp32s_t2(PairArgs a) { fp32s_entry<false, P, 2>(a); }
+template <bool P>
+__global__ __launch_bounds__(64) void k_pair_fp32s_ho_t2(PairArgs a) { fp32s_entry<true, P, 2>(a); }
 template <bool P>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_pair_fp32s_ho_w8(PairArgs a) {
   fp32s_entry<true, P>(a);
@@ -1920,7 +1962,8 @@ hipError_t launch_fp32s(int kind, const PairConfig& cfg, const PairArgs& args, i
   hipLaunchKernelGGL(k_src_chunks, dim3(args.nchunks), dim3(64), 0, st, args.state, args.chunks, args.src);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  const int waves = 4 * args.nitems;
+  const bool t2 = cfg.two_per_lane;
+  const int waves = (t2 ? 2 : 4) * args.nitems;
   const bool P = cfg.persistent;
   if (P) {
     e = launch_fill_int(args.queue, 1, 0, st);
@@ -1929,7 +1972,13 @@ hipError_t launch_fp32s(int kind, const PairConfig& cfg, const PairArgs& args, i
   const dim3 g(P ? static_cast<unsigned>(std::min(grid, waves)) : static_cast<unsigned>(waves));
   const dim3 block(64);
   const bool ho = kind == kPairFP32CHO;
-  if (ho && cfg.waves8) {
+  if (t2 && ho) {
+    if (P) hipLaunchKernelGGL(k_pair_fp32s_ho_t2<true>, g, block, 0, st, args);
+    else hipLaunchKernelGGL(k_pair_fp32s_ho_t2<false>, g, block, 0, st, args);
+  } else if (t2) {
+    if (P) hipLaunchKernelGGL(k_pair_fp32s_t2<true>, g, block, 0, st, args);
+    else hipLaunchKernelGGL(k_pair_fp32s_t2<false>, g, block, 0, st, args);
+  } else if (ho && cfg.waves8) {
     if (P) hipLaunchKernelGGL(k_pair_fp32s_ho_w8<true>, g, block, 0, st, args);
     else hipLaunchKernelGGL(k_pair_fp32s_ho_w8<false>, g, block, 0, st, args);
   } else if (ho) {
@@ -2093,7 +2142,10 @@ int pair_kernel_waves(int kind, const PairConfig& cfg) {
   if (cfg.scalar_feed && kind == kPairFP64) return blocks_per_cu(reinterpret_cast<const void*>(k_pair_fp64s<true>), 64);
   if (cfg.scalar_feed && (kind == kPairFP32C || kind == kPairFP32CHO)) {
     // one-wave blocks: resident blocks per CU = resident waves per CU
-    if (kind == kPairFP32C)
+    if (cfg.two_per_lane)
+      k = kind == kPairFP32C ? reinterpret_cast<const void*>(k_pair_fp32s_t2<true>)
+                             : reinterpret_cast<const void*>(k_pair_fp32s_ho_t2<true>);
+    else if (kind == kPairFP32C)
       k = cfg.waves8 ? reinterpret_cast<const void*>(k_pair_fp32s_w8<true>)
                      : reinterpret_cast<const void*>(k_pair_fp32s<true>);
     else

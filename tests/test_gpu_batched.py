@@ -9,11 +9,13 @@ from pawan_amd import capi, workloads as W
 
 pytestmark = pytest.mark.gpu
 
-# (parts, persistent, waves8, scalar_feed); waves8 applies to FP32C only,
+# (parts, persistent, waves8, scalar_feed, two_per_lane); waves8 and
+# two_per_lane apply to FP32C only (two_per_lane with scalar_feed),
 # scalar_feed to FP32C and FP64
-CONFIGS = [(1, False, False, False), (3, False, False, False), (1, True, False, False), (8, True, False, False),
-           (4, True, True, False), (2, False, True, False), (1, False, False, True), (3, True, True, True),
-           (8, True, False, True)]
+CONFIGS = [(1, False, False, False, False), (3, False, False, False, False), (1, True, False, False, False),
+           (8, True, False, False, False), (4, True, True, False, False), (2, False, True, False, False),
+           (1, False, False, True, False), (3, True, True, True, False), (8, True, False, True, False),
+           (1, False, False, True, True), (3, True, False, True, True), (8, True, False, True, True)]
 
 
 def _say(msg):
@@ -36,28 +38,30 @@ def _check_wakes(oracle, s, u, w, tol_u, tol_w, sample=64):
             assert normwise(w[a:b][idx], rw) < tol_w, (k, normwise(w[a:b][idx], rw))
 
 
-@pytest.mark.parametrize("parts,persistent,waves8,feed", CONFIGS)
-def test_launch_shapes_coupled(oracle, parts, persistent, waves8, feed):
+@pytest.mark.parametrize("parts,persistent,waves8,feed,t2", CONFIGS)
+def test_launch_shapes_coupled(oracle, parts, persistent, waves8, feed, t2):
     s = W.leapfrog_64k(nst=1024)  # 16,384 particles, two coupled wakes
     idx = np.unique(np.linspace(0, s.n - 1, 300).astype(np.int64))
     ru, rw = oracle.rates_targets(s, idx)
-    _say(f"leapfrog N={s.n} parts={parts} persistent={persistent} waves8={waves8} scalar_feed={feed}")
+    _say(f"leapfrog N={s.n} parts={parts} persistent={persistent} waves8={waves8} scalar_feed={feed} "
+         f"two_per_lane={t2}")
     with capi.Solver(s) as g:
         for prec, tu, tw in (("fp64", 1e-12, 1e-12), ("fp32c", 1e-6, 1e-5)):
-            g.set_pair_config(prec, parts, persistent, waves8, feed)
+            g.set_pair_config(prec, parts, persistent, waves8, feed, t2)
             u, w = g.solve(prec)
             assert normwise(u[idx], ru) < tu
             assert normwise(w[idx], rw) < tw
 
 
-@pytest.mark.parametrize("parts,persistent,waves8,feed", CONFIGS)
-def test_launch_shapes_independent_wakes(oracle, batched, parts, persistent, waves8, feed):
-    _say(f"batched {batched.nw} wakes N={batched.n} parts={parts} persistent={persistent} scalar_feed={feed}")
+@pytest.mark.parametrize("parts,persistent,waves8,feed,t2", CONFIGS)
+def test_launch_shapes_independent_wakes(oracle, batched, parts, persistent, waves8, feed, t2):
+    _say(f"batched {batched.nw} wakes N={batched.n} parts={parts} persistent={persistent} scalar_feed={feed} "
+         f"two_per_lane={t2}")
     with capi.Solver(batched) as g:
-        g.set_pair_config("fp64", parts, persistent, waves8, feed)
+        g.set_pair_config("fp64", parts, persistent, waves8, feed, t2)
         u, _ = g.solve("fp64")
         _check_wakes(oracle, batched, u, None, 1e-12, None)
-        g.set_pair_config("fp32c", parts, persistent, waves8, feed)
+        g.set_pair_config("fp32c", parts, persistent, waves8, feed, t2)
         u, _ = g.solve("fp32c")
         _check_wakes(oracle, batched, u, None, 1e-6, None)
 
@@ -165,7 +169,8 @@ def test_config3_batched_256_wakes(oracle):
                 assert max_w < tw and rms_w < 1e-4
             # graph replay against eager steps of the same launch shape
             with capi.Solver(s) as e:
-                e.set_pair_config(prec, cfg["parts"], cfg["persistent"], cfg["waves8"], cfg["scalar_feed"])
+                e.set_pair_config(prec, cfg["parts"], cfg["persistent"], cfg["waves8"], cfg["scalar_feed"],
+                                  cfg["two_per_lane"])
                 g.set_graph(True)
                 g.step("rk4", dt, prec, 2)
                 e.step("rk4", dt, prec, 2)

@@ -119,16 +119,17 @@ def test_point_velocity(ho):
     assert normwise(u, ref) < FP64_TOL
 
 
-@pytest.mark.parametrize("feed", [False, True])
-def test_fp32c_rhs(ho, feed):
-    """The compensated FP32 form of the kernel (LDS-tiled and scalar-fed), at
+@pytest.mark.parametrize("feed,t2", [(False, False), (True, False), (True, True)])
+def test_fp32c_rhs(ho, feed, t2):
+    """The compensated FP32 form of the kernel (LDS-tiled, scalar-fed, and
+    scalar-fed with two targets per lane), at
     the Gaussian FP32C bars: velocity 1e-6 and vorticity rate 1e-5 normwise
     against the FP64 oracle."""
     for s in (W.ring_1k(), W.leapfrog_64k()):
         idx = np.arange(0, s.n, max(1, s.n // 700))
         ref_u, ref_w = ho.rates_targets(s, idx)
         with _solver(s) as g:
-            g.set_pair_config("fp32c", 1, False, False, feed)
+            g.set_pair_config("fp32c", 1, False, False, feed, t2)
             u, w = g.solve("fp32c")
             u64, w64 = g.solve("fp64")
         eu, ew = normwise(u[idx], ref_u), normwise(w[idx], ref_w)
