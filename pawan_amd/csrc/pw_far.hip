@@ -143,25 +143,27 @@ __device__ __forceinline__ void quad_accumulate(const float r[3], const float Q[
     S[p] = fmaf(E[p][2], r[2], fmaf(E[p][1], r[1], E[p][0] * r[0]));
   }
   const float V[3] = {E[1][2] - E[2][1], E[2][0] - E[0][2], E[0][1] - E[1][0]};
-  const float Pr[3] = {P[1] * r[2] - P[2] * r[1], P[2] * r[0] - P[0] * r[2], P[0] * r[1] - P[1] * r[0]};
-  const float Sr[3] = {S[1] * r[2] - S[2] * r[1], S[2] * r[0] - S[0] * r[2], S[0] * r[1] - S[1] * r[0]};
   const float c5 = -3.0f * ir5, h5 = -1.5f * ir5, c7 = 7.5f * ir7;
+  // Z = -1.5 P / r^5 + 7.5 S / r^7, so that u'_Q = -3 V / r^5 + Z x r and
+  // the two cross products with a in the stretching are one, a x Z
+  const float Z[3] = {fmaf(c7, S[0], h5 * P[0]), fmaf(c7, S[1], h5 * P[1]), fmaf(c7, S[2], h5 * P[2])};
   if (with_u) {
-#pragma unroll
-    for (int k = 0; k < 3; ++k) u[k] = fmaf(c5, V[k], fmaf(h5, Pr[k], fmaf(c7, Sr[k], u[k])));
+    u[0] = fmaf(c5, V[0], fmaf(Z[1], r[2], fmaf(-Z[2], r[1], u[0])));
+    u[1] = fmaf(c5, V[1], fmaf(Z[2], r[0], fmaf(-Z[0], r[2], u[1])));
+    u[2] = fmaf(c5, V[2], fmaf(Z[0], r[1], fmaf(-Z[1], r[0], u[2])));
   }
-  const float f1 = fmaf(a[2], V[2], fmaf(a[1], V[1], a[0] * V[0]));
-  const float f2 = fmaf(a[2], Pr[2], fmaf(a[1], Pr[1], a[0] * Pr[0]));
-  const float f3 = fmaf(a[2], Sr[2], fmaf(a[1], Sr[1], a[0] * Sr[0]));
   const float cr[3] = {r[1] * a[2] - r[2] * a[1], r[2] * a[0] - r[0] * a[2], r[0] * a[1] - r[1] * a[0]};  // r x a
-  const float aP[3] = {a[1] * P[2] - a[2] * P[1], a[2] * P[0] - a[0] * P[2], a[0] * P[1] - a[1] * P[0]};
-  const float aS[3] = {a[1] * S[2] - a[2] * S[1], a[2] * S[0] - a[0] * S[2], a[0] * S[1] - a[1] * S[0]};
+  const float f1 = fmaf(a[2], V[2], fmaf(a[1], V[1], a[0] * V[0]));      // a . V
+  const float f2 = fmaf(P[2], cr[2], fmaf(P[1], cr[1], P[0] * cr[0]));  // a . (P x r)
+  const float f3 = fmaf(S[2], cr[2], fmaf(S[1], cr[1], S[0] * cr[0]));  // a . (S x r)
+  const float aZ[3] = {a[1] * Z[2] - a[2] * Z[1], a[2] * Z[0] - a[0] * Z[2], a[0] * Z[1] - a[1] * Z[0]};
   const float rad = fmaf(15.0f * ir7, f1, fmaf(7.5f * ir7, f2, -52.5f * ir9 * f3));
+  const float c72 = 2.0f * c7;
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
     const float g = fmaf(a[2], Wq[3 * m + 2], fmaf(a[1], Wq[3 * m + 1], a[0] * Wq[3 * m]));
     const float cE = fmaf(cr[2], E[2][m], fmaf(cr[1], E[1][m], cr[0] * E[0][m]));
-    w[m] += fmaf(rad, r[m], fmaf(c5, g, fmaf(h5, aP[m], c7 * fmaf(2.0f, cE, aS[m]))));
+    w[m] += fmaf(rad, r[m], fmaf(c5, g, fmaf(c72, cE, aZ[m])));
   }
 }
 
@@ -218,20 +220,20 @@ __device__ __forceinline__ void md_accumulate(const float r[3], const FarNodeMD&
   const float c7 = 5.0f * b5 * ir2;
   const float* M = n.M;
   const float* D = n.D;
-  const float mr[3] = {M[1] * r[2] - M[2] * r[1], M[2] * r[0] - M[0] * r[2], M[0] * r[1] - M[1] * r[0]};
   const float p[3] = {fmaf(D[2], r[2], fmaf(D[1], r[1], D[0] * r[0])),
                       fmaf(D[5], r[2], fmaf(D[4], r[1], D[3] * r[0])),
                       fmaf(D[8], r[2], fmaf(D[7], r[1], D[6] * r[0]))};
-  const float pr[3] = {p[1] * r[2] - p[2] * r[1], p[2] * r[0] - p[0] * r[2], p[0] * r[1] - p[1] * r[0]};
-  float v[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    u[k] = fmaf(mr[k] - n.W[k], a3, fmaf(pr[k], b5, u[k]));
-    v[k] = fmaf(b5, n.W[k] - mr[k], -c7 * pr[k]);
-  }
+  // c = M/r^3 + 3 p/r^5: u' = c x r - W/r^3 (the monopole's and the dipole's
+  // cross products with r in one)
   const float c[3] = {fmaf(M[0], a3, p[0] * b5), fmaf(M[1], a3, p[1] * b5), fmaf(M[2], a3, p[2] * b5)};
+  u[0] = fmaf(-a3, n.W[0], fmaf(c[1], r[2], fmaf(-c[2], r[1], u[0])));
+  u[1] = fmaf(-a3, n.W[1], fmaf(c[2], r[0], fmaf(-c[0], r[2], u[1])));
+  u[2] = fmaf(-a3, n.W[2], fmaf(c[0], r[1], fmaf(-c[1], r[0], u[2])));
   const float sx = r[1] * a[2] - r[2] * a[1], sy = r[2] * a[0] - r[0] * a[2], sz = r[0] * a[1] - r[1] * a[0];
-  const float av = fmaf(a[2], v[2], fmaf(a[1], v[1], a[0] * v[0]));
+  // a.v with v = (3/r^5)(W - M x r) - (15/r^7)(p x r): a.(X x r) = X.(r x a)
+  const float Y[3] = {fmaf(M[0], b5, p[0] * c7), fmaf(M[1], b5, p[1] * c7), fmaf(M[2], b5, p[2] * c7)};
+  const float aW = fmaf(a[2], n.W[2], fmaf(a[1], n.W[1], a[0] * n.W[0]));
+  const float av = fmaf(b5, aW, -fmaf(Y[2], sz, fmaf(Y[1], sy, Y[0] * sx)));
   w[0] += fmaf(av, r[0], fmaf(a[1], c[2], fmaf(-a[2], c[1], b5 * fmaf(D[6], sz, fmaf(D[3], sy, D[0] * sx)))));
   w[1] += fmaf(av, r[1], fmaf(a[2], c[0], fmaf(-a[0], c[2], b5 * fmaf(D[7], sz, fmaf(D[4], sy, D[1] * sx)))));
   w[2] += fmaf(av, r[2], fmaf(a[0], c[1], fmaf(-a[1], c[0], b5 * fmaf(D[8], sz, fmaf(D[5], sy, D[2] * sx)))));
