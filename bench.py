@@ -247,7 +247,8 @@ def _quantities(d: np.ndarray) -> dict:
 def drift_leg(steps: int, every: int, learned_cfg: dict | None, target_err: float, nus=(1e-4, 0.0)):
     """The leapfrogging rings (two coupled wakes, N = 65,536, dt = 0.002),
     advanced `steps` RK4 steps by FP64, FP32C and (when configured) the tree
-    path with the network, side by side, at each viscosity in `nus`.  Every
+    path with the network ("learned") and with the expansion to quadrupole
+    order ("quadrupole"), side by side, at each viscosity in `nus`.  Every
     `every` steps, for each path:
       * dev_Q: |Q_path(t) - Q_FP64(t)| / S_Q, the distance of its conserved
         quantities from FP64's at the same time (S_Q = |Q(0)| for the linear
@@ -258,38 +259,43 @@ def drift_leg(steps: int, every: int, learned_cfg: dict | None, target_err: floa
       * drift_Q: |Q_path(t) - Q_path(0)| / S_Q;
       * the velocity error of its right-hand side against an FP64 right-hand
         side on its own state (max-norm and RMS).
-    The tree path's tolerance is calibrated once on this system to the same
+    Each tree path's tolerance is calibrated once on this system to the same
     velocity error target as at 1M."""
     from pawan_amd import capi, workloads
 
     spec = workloads.WORKLOADS["leapfrog_64k"]
     dt = spec["dt"]
     out = {"workload": "leapfrog_64k", "steps": steps, "dt": dt, "every": every, "runs": []}
-    tol_abs = None
+    tol_abs = {}
+    # path name -> the precision it steps with
+    prec = {"fp64": "fp64", "fp32c": "fp32c"}
+    if learned_cfg is not None:
+        prec.update({"learned": "learned", "quadrupole": "learned"})
     for nu in nus:
         system = spec["make"]()
         system.nu = nu
         run = {"nu": nu, "particles": system.n, "t": [], "paths": {}}
         t0 = time.perf_counter()
-        paths = ["fp64", "fp32c"] + (["learned"] if learned_cfg is not None else [])
-        ctx = {p: capi.Solver(system) for p in paths}
+        ctx = {p: capi.Solver(system) for p in prec}
         try:
             u0, _ = ctx["fp64"].solve("fp64")
             scale = _diag_terms(system, u0)
             for p, g in ctx.items():
-                if p == "learned":
-                    g.ff_set_weights(learned_cfg["weights"])
+                if prec[p] == "learned":
+                    use_mlp = p == "learned"
+                    if use_mlp:
+                        g.ff_set_weights(learned_cfg["weights"])
                     g.ff_set_err_coef(learned_cfg["coef_md"], learned_cfg["coef_mlp"])
-                    if tol_abs is None:
+                    if p not in tol_abs:
                         ref_u, ref_w = g.solve("fp64")
                         urms = float(np.sqrt((ref_u * ref_u).sum(axis=1).mean()))
-                        best = calibrate_tol(g, target_err, ref_u, ref_w, True, learned_cfg["theta"], urms)
+                        best = calibrate_tol(g, target_err, ref_u, ref_w, use_mlp, learned_cfg["theta"], urms)
                         if best is None:
                             raise RuntimeError("no tolerance meets the target on the rings")
-                        tol_abs = best[0]
-                        out["learned_tol_abs"] = tol_abs
-                        out["learned_velocity_error_t0"] = best[1]
-                    g.ff_set_params(learned_cfg["theta"], tol_abs, use_mlp=True)
+                        tol_abs[p] = best[0]
+                        out[f"{p}_tol_abs"] = tol_abs[p]
+                        out[f"{p}_velocity_error_t0"] = best[1]
+                    g.ff_set_params(learned_cfg["theta"], tol_abs[p], use_mlp=use_mlp)
                 else:
                     g.set_graph(True)
                 run["paths"][p] = {"dev": {}, "drift": {}, "e_u_maxnorm": [], "e_u_rms": []}
@@ -297,7 +303,7 @@ def drift_leg(steps: int, every: int, learned_cfg: dict | None, target_err: floa
             for k in range(0, steps + 1, every):
                 if k > 0:
                     for p, g in ctx.items():
-                        g.step("rk4", dt, p, every)
+                        g.step("rk4", dt, prec[p], every)
                 run["t"].append(k * dt)
                 q64 = _quantities(ctx["fp64"].diagnose())
                 for p, g in ctx.items():
@@ -308,7 +314,7 @@ def drift_leg(steps: int, every: int, learned_cfg: dict | None, target_err: floa
                         rec["dev"].setdefault(name, []).append(float(np.linalg.norm(q[name] - q64[name])) / sc)
                         rec["drift"].setdefault(name, []).append(float(np.linalg.norm(q[name] - q0[p][name])) / sc)
                     if p != "fp64":
-                        up, _ = g.solve(p)
+                        up, _ = g.solve(prec[p])
                         ur, _ = g.solve("fp64")
                         rec["e_u_maxnorm"].append(maxnorm(up, ur))
                         rec["e_u_rms"].append(rmsnorm(up, ur))
