@@ -767,7 +767,7 @@ __device__ __forceinline__ void near32t(float r2, float sig2, float nu2, const f
   g = fmaf(g, t, lo.x);
   const float ke = kFourPiK * isg3;
   qp = ke * fmaf(sb, g, e * 0x1.555556p-2f);
-  fp = ke * g * isg2;
+  fp = -3.0f * (ke * g * isg2);  // -3 f', as accumulate32_pair_n3 takes it
   c = nu2 * kKf * e * isg3 * isg2;  // 2 nu Z / sigma^2
 }
 
@@ -799,6 +799,25 @@ __device__ __forceinline__ void accumulate32(float dx, float dy, float dz, float
 // there q' is large, A would gather large, nearly parallel strengths (thin
 // rings) whose cross product with a_t cancels, and forming the small
 // a_t x a_s pair by pair keeps that cancellation out of the sums.
+// accumulate32_pair with B kept as fold() would return it: the caller passes
+// f3 = -3 f', so q' (a_t x a_s) enters B unscaled and fold_n3() adds the sums
+// as they are (B_n3 = sum q' (a_t x a_s) - 3 f' (a_t.v) d = -3 B; one
+// multiply less per pair)
+template <typename S>
+__device__ __forceinline__ void accumulate32_pair_n3(float dx, float dy, float dz, float q, float f3, float tax,
+                                                     float tay, float taz, const S& s, P32& p) {
+  const float vx = s.ay * dz - s.az * dy;
+  const float vy = s.az * dx - s.ax * dz;
+  const float vz = s.ax * dy - s.ay * dx;
+  p.ux = fmaf(q, vx, p.ux);
+  p.uy = fmaf(q, vy, p.uy);
+  p.uz = fmaf(q, vz, p.uz);
+  const float g = f3 * fmaf(taz, vz, fmaf(tay, vy, tax * vx));
+  p.bx = fmaf(q, tay * s.az - taz * s.ay, fmaf(g, dx, p.bx));
+  p.by = fmaf(q, taz * s.ax - tax * s.az, fmaf(g, dy, p.by));
+  p.bz = fmaf(q, tax * s.ay - tay * s.ax, fmaf(g, dz, p.bz));
+}
+
 template <typename S>
 __device__ __forceinline__ void accumulate32_pair(float dx, float dy, float dz, float q, float f, float tax,
                                                   float tay, float taz, const S& s, P32& p) {
@@ -910,6 +929,16 @@ __device__ __forceinline__ void two_sum(float& hi, float& lo, float p) {
 }
 
 // A chunk's partial sums into the compensated totals (u, then w).
+// fold() of sums formed by accumulate32_pair_n3 alone (A = 0, B already -3 B)
+__device__ __forceinline__ void fold_n3(const P32& p, float* hi, float* lo) {
+  two_sum(hi[0], lo[0], p.ux);
+  two_sum(hi[1], lo[1], p.uy);
+  two_sum(hi[2], lo[2], p.uz);
+  two_sum(hi[3], lo[3], p.bx);
+  two_sum(hi[4], lo[4], p.by);
+  two_sum(hi[5], lo[5], p.bz);
+}
+
 __device__ __forceinline__ void fold(const P32& p, float tax, float tay, float taz, float* hi, float* lo) {
   two_sum(hi[0], lo[0], p.ux);
   two_sum(hi[1], lo[1], p.uy);
@@ -1587,9 +1616,10 @@ struct NearArgs {
   // every particle of the same radius (uniform sigma): the core's sigma
   // factors, once per launch (u_cut2 = 44 sigma^2 > 0 marks the case):
   // 1/sigma^2, ke = sqrt(2/pi)/sigma^3 (the table is staged times ke),
-  // log2(ke/3) (added to the exponent: e' = (ke/3) e^{-s/2}) and the PSE
-  // weight 2 nu (2 pi)^-3/2 / sigma^5 divided by ke/3
-  float u_cut2, u_isg2, u_ke, u_lk3, u_ck3;
+  // log2(ke/3) (added to the exponent: e' = (ke/3) e^{-s/2}; staged into
+  // the table's rows), -3/sigma^2 (f' = g/sigma^2 enters the sums as -3 f')
+  // and the PSE weight 2 nu (2 pi)^-3/2 / sigma^5 divided by ke/3
+  float u_cut2, u_isg2, u_ke, u_lk3, u_ck3, u_n3isg2;
 };
 
 constexpr int kNearSlice = 64;                       // targets per wave
@@ -1602,20 +1632,23 @@ static_assert(kNearSlices == kLeafChunks && kNearSlice == 64, "a slice is one 64
 // pair32_ho)
 // near32t for a pair radius shared by every pair: the sigma factors come
 // from the launch and ke is folded into the staged table (g = ke G) and
-// into the exponent (e = (ke/3) e^{-s/2}), so q' = s g + e, f' = g / sigma^2
+// into the exponent (e = (ke/3) e^{-s/2}, log2(ke/3) staged as each row's
+// hi.y), so q' = s g + e and -3 f' = -3 g / sigma^2.  Only for r2 <= u_cut2:
+// then sb <= fl(u_cut2 / sigma^2) and the index below stays under
+// kCorePieces, which the launch checks in FP32 (rounding is monotone).
 __device__ __forceinline__ void near32u(float r2, const NearArgs& a, const float4* __restrict__ tab, float& qp,
-                                        float& fp, float& c) {
+                                        float& fp3, float& c) {
   const float sb = r2 * a.u_isg2;
-  const float e = __builtin_amdgcn_exp2f(fmaf(kMHalfLog2e, sb, a.u_lk3));
-  const int k = static_cast<int>(fminf(sb * kCoreScale, static_cast<float>(kCorePieces - 1)));
-  const float4 lo = tab[2 * k], hi = tab[2 * k + 1];
+  const int k = static_cast<int>(sb * kCoreScale);
+  const float4 lo = tab[2 * k], hi = tab[2 * k + 1];  // (c0, c1, c2, c3), (c4, log2(ke/3), s_c, 0)
+  const float e = __builtin_amdgcn_exp2f(fmaf(kMHalfLog2e, sb, hi.y));
   const float t = sb - hi.z;
   float g = fmaf(hi.x, t, lo.w);
   g = fmaf(g, t, lo.z);
   g = fmaf(g, t, lo.y);
   g = fmaf(g, t, lo.x);
   qp = fmaf(sb, g, e);
-  fp = a.u_isg2 * g;
+  fp3 = a.u_n3isg2 * g;
   c = a.u_ck3 * e;
 }
 
@@ -1637,11 +1670,11 @@ __device__ __forceinline__ void pair32_rel_core(float tx, float ty, float tz, co
     ib7 = ib5 * ib2;
     q = fmaf(2.5f, sig2, r2) * ib5;
     f = fmaf(3.5f, sig2, r2) * ib7;
-  } else {
+  } else {  // f holds -3 f' (accumulate32_pair_n3)
     const float ir = __builtin_amdgcn_rsqf(r2);
     const float ir2 = ir * ir;
     q = ir * ir2;
-    f = q * ir2;
+    f = -3.0f * (q * ir2);
   }
   const bool core = Uniform ? r2 <= a.u_cut2 : r2 <= kFar32 * sig2;
   if (core) {
@@ -1656,7 +1689,8 @@ __device__ __forceinline__ void pair32_rel_core(float tx, float ty, float tz, co
   }
   // a run within the core range of the slice: stretching per pair (a_t x a_s
   // formed per pair keeps its accuracy where a_s is nearly parallel to a_t)
-  accumulate32_pair(dx, dy, dz, q, f, t.ax, t.ay, t.az, s, p);
+  if (HighOrder) accumulate32_pair(dx, dy, dz, q, f, t.ax, t.ay, t.az, s, p);
+  else accumulate32_pair_n3(dx, dy, dz, q, f, t.ax, t.ay, t.az, s, p);
 }
 
 template <bool HighOrder, bool Uniform>
@@ -1745,7 +1779,8 @@ __device__ __forceinline__ void near_slice(const NearArgs& a, const NearItem& it
 #pragma unroll 1
         for (int l = 0; l < nc; ++l) pair32_rel_core<HighOrder, Uniform>(rx, ry, rz, me, tile[l], tab, a, p, pse);
       }
-      fold(p, me.ax, me.ay, me.az, hi, lo);
+      if (HighOrder) fold(p, me.ax, me.ay, me.az, hi, lo);
+      else fold_n3(p, hi, lo);
     }
   }
   if (active) {
@@ -1773,7 +1808,7 @@ __device__ __forceinline__ void near_entry(const NearArgs& a, SrcRel* tile, floa
     for (int l = static_cast<int>(threadIdx.x); l < 2 * kCorePieces; l += kNearSlice) {
       const float* row = &kCoreTable[l >> 1][4 * (l & 1)];
       const float sc = Uniform ? a.u_ke : 1.0f;
-      tab[l] = (l & 1) ? make_float4(row[0] * sc, row[1] * sc, row[2], row[3])
+      tab[l] = (l & 1) ? make_float4(row[0] * sc, Uniform ? a.u_lk3 : row[1], row[2], row[3])
                        : make_float4(row[0] * sc, row[1] * sc, row[2] * sc, row[3] * sc);
     }
     __syncthreads();
@@ -1791,16 +1826,20 @@ __device__ __forceinline__ void near_entry(const NearArgs& a, SrcRel* tile, floa
   }
 }
 
+// the core table at LDS offset 0 (a row's second half is then in reach of
+// one read2 without an address add), the source tile after it
+struct NearLds {
+  float4 tab[2 * kCorePieces];
+  SrcRel tile[kNearSlice];
+};
 __global__ __launch_bounds__(kNearSlice) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_near_fp32c(NearArgs a) {
-  __shared__ SrcRel tile[kNearSlice];
-  __shared__ float4 tab[2 * kCorePieces];
-  near_entry<false>(a, tile, tab);
+  __shared__ NearLds lds;
+  near_entry<false>(a, lds.tile, lds.tab);
 }
 // every particle of one radius
 __global__ __launch_bounds__(kNearSlice) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_near_fp32c_u(NearArgs a) {
-  __shared__ SrcRel tile[kNearSlice];
-  __shared__ float4 tab[2 * kCorePieces];
-  near_entry<false, true>(a, tile, tab);
+  __shared__ NearLds lds;
+  near_entry<false, true>(a, lds.tile, lds.tab);
 }
 __global__ __launch_bounds__(kNearSlice) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_near_fp32c_ho(NearArgs a) {
   __shared__ SrcRel tile[kNearSlice];
@@ -2096,8 +2135,11 @@ hipError_t launch_near_fp32c(const Rec64* sorted, const NodeRec* nodes, const Tr
   if (e != hipSuccess) return e;
   NearArgs a{sorted, nodes, tb.near_list, tb.cap_near, tb.near_items, tb.near_total, tb.near_queue,
              static_cast<float>(nu), kFar32 * sigma_max2, rates_s, tb.near_partial, tb.leaf_chunks,
-             0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-  if (uniform_sigma && !highorder) {
+             0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  // near32u indexes the table without a clamp: every core sb must map below
+  // kCorePieces (the largest core sb, in the device's FP32 roundings)
+  const float sb_max = static_cast<float>(kFar32 * sigma_max2) * static_cast<float>(1.0 / sigma_max2);
+  if (uniform_sigma && !highorder && sb_max * kCoreScale < static_cast<float>(kCorePieces)) {
     // the pair radius is sigma itself: sigma^2 = (s_s^2 + s_t^2) / 2; the
     // factors rounded from FP64, as near32t rounds them per pair from FP32
     const double s2 = sigma_max2, is2 = 1.0 / s2, is3 = is2 / std::sqrt(s2);
@@ -2107,6 +2149,7 @@ hipError_t launch_near_fp32c(const Rec64* sorted, const NodeRec* nodes, const Tr
     a.u_ke = static_cast<float>(ke);
     a.u_lk3 = static_cast<float>(std::log2(ke / 3.0));
     a.u_ck3 = static_cast<float>(2.0 * nu * 0.063493635934240969785 * is3 * is2 / (ke / 3.0));  // (2 pi)^-3/2
+    a.u_n3isg2 = static_cast<float>(-3.0 * is2);
     hipLaunchKernelGGL(k_near_fp32c_u, dim3(grid), dim3(kNearSlice), 0, st, a);
   } else if (highorder) {
     hipLaunchKernelGGL(k_near_fp32c_ho, dim3(grid), dim3(kNearSlice), 0, st, a);
