@@ -256,7 +256,12 @@ __global__ __launch_bounds__(256) void k_far_md(FarArgs a) {
   const int T = blockIdx.x;
   const int t0 = a.nodes[T].first, t1 = a.nodes[T].last;
   const double tcx = a.nodes[T].cx, tcy = a.nodes[T].cy, tcz = a.nodes[T].cz;
-  const int t = t0 + static_cast<int>(threadIdx.x);
+  // a wave with at most 32 targets gives lanes 32-63 the targets of lanes
+  // 0-31 and each half every other node; the halves' sums are added at the end
+  const int w0 = t0 + static_cast<int>(threadIdx.x & ~63);  // this wave's first target
+  const bool halves = t1 - w0 > 0 && t1 - w0 <= 32;          // wave-uniform
+  const int h = halves ? static_cast<int>(threadIdx.x >> 5) & 1 : 0;
+  const int t = w0 + static_cast<int>(halves ? threadIdx.x & 31 : threadIdx.x & 63);
   const bool active = t < t1;
   const Rec64 me = a.sorted[active ? t : t0];
   const float rt[3] = {static_cast<float>(me.x - tcx), static_cast<float>(me.y - tcy),
@@ -274,12 +279,26 @@ __global__ __launch_bounds__(256) void k_far_md(FarArgs a) {
     }
     __syncthreads();
     if (!active) continue;
-    for (int k = 0; k < c; ++k) {
-      const float r[3] = {rt[0] - sn[k].rc[0], rt[1] - sn[k].rc[1], rt[2] - sn[k].rc[2]};
-      md_accumulate<Quad>(r, sn[k], at, u, w);
+    if (halves) {
+      for (int k = h; k < c; k += 2) {
+        const float r[3] = {rt[0] - sn[k].rc[0], rt[1] - sn[k].rc[1], rt[2] - sn[k].rc[2]};
+        md_accumulate<Quad>(r, sn[k], at, u, w);
+      }
+    } else {
+      for (int k = 0; k < c; ++k) {
+        const float r[3] = {rt[0] - sn[k].rc[0], rt[1] - sn[k].rc[1], rt[2] - sn[k].rc[2]};
+        md_accumulate<Quad>(r, sn[k], at, u, w);
+      }
     }
   }
-  if (!active) return;
+  if (halves) {  // every lane of the wave runs the shuffles
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      u[k] += __shfl_down(u[k], 32);
+      w[k] += __shfl_down(w[k], 32);
+    }
+  }
+  if (!active || h != 0) return;
   Rate64 r = a.rates[t];
   r.ux += kQd * u[0];
   r.uy += kQd * u[1];

@@ -1693,16 +1693,22 @@ __device__ __forceinline__ void pair32_rel_core(float tx, float ty, float tz, co
   else accumulate32_pair_n3(dx, dy, dz, q, f, t.ax, t.ay, t.az, s, p);
 }
 
-template <bool HighOrder, bool Uniform>
+// Halves: the slice holds at most 32 targets; lanes 32-63 take the same
+// targets as lanes 0-31 and each half every other source of a tile, and
+// the two halves' sums are added at the end (a leaf's last slice is often
+// this short: 13% of the 1M helix's slices)
+template <bool HighOrder, bool Uniform, bool Halves>
 __device__ __forceinline__ void near_slice(const NearArgs& a, const NearItem& it, int slice, SrcRel* tile,
                                            const float4* __restrict__ tab) {
   const int T = it.T;
   const int lane = static_cast<int>(threadIdx.x);
+  const int h = Halves ? lane >> 5 : 0;  // this lane's half: sources h, h + 2, ...
+  constexpr int kStep = Halves ? 2 : 1;
   const int t0 = a.nodes[T].first, t1 = a.nodes[T].last;
   const double tcx = a.nodes[T].cx, tcy = a.nodes[T].cy, tcz = a.nodes[T].cz;
   const float tr = a.nodes[T].r;
   const LeafChunk ts = a.chunks[kLeafChunks * T + slice];  // this slice's targets
-  const int self = kNearSlice * slice + lane;  // index of this lane's target in T
+  const int self = kNearSlice * slice + (Halves ? lane & 31 : lane);  // index of this lane's target in T
   const int t = t0 + self;
   const bool active = t < t1;
   const Rec64 m = a.sorted[active ? t : t0];
@@ -1765,7 +1771,7 @@ __device__ __forceinline__ void near_slice(const NearArgs& a, const NearItem& it
       if (sep) {
         P32S q;
 #pragma unroll 2
-        for (int l = 0; l < nc; ++l)
+        for (int l = h; l < nc; l += kStep)
           pair32s<HighOrder>(rx, ry, rz, me.hs2, me.ax, me.ay, me.az, ctx, cty, ctz, tile[l], q);
         fold_s(q, rx, ry, rz, me.ax, me.ay, me.az, hi, lo);
         continue;
@@ -1773,17 +1779,29 @@ __device__ __forceinline__ void near_slice(const NearArgs& a, const NearItem& it
       P32 p;
       if (diag) {
         const int own = self - c0;  // this target's slot in the tile, if there
-        for (int l = 0; l < nc; ++l)
+        for (int l = h; l < nc; l += kStep)
           if (l != own) pair32_rel_core<HighOrder, Uniform>(rx, ry, rz, me, tile[l], tab, a, p, pse);
       } else {
 #pragma unroll 1
-        for (int l = 0; l < nc; ++l) pair32_rel_core<HighOrder, Uniform>(rx, ry, rz, me, tile[l], tab, a, p, pse);
+        for (int l = h; l < nc; l += kStep) pair32_rel_core<HighOrder, Uniform>(rx, ry, rz, me, tile[l], tab, a, p, pse);
       }
       if (HighOrder) fold(p, me.ax, me.ay, me.az, hi, lo);
       else fold_n3(p, hi, lo);
     }
   }
-  if (active) {
+  if (Halves) {  // lane l (< 32) takes lane l + 32's sums; every lane runs the shuffles
+#pragma unroll
+    for (int k = 0; k < kAcc; ++k) {
+      const float oh = __shfl_down(hi[k], 32), ol = __shfl_down(lo[k], 32);
+      two_sum(hi[k], lo[k], oh);
+      lo[k] += ol;
+    }
+    pse.x += __shfl_down(pse.x, 32);
+    pse.y += __shfl_down(pse.y, 32);
+    pse.z += __shfl_down(pse.z, 32);
+    pse.w += __shfl_down(pse.w, 32);
+  }
+  if (active && h == 0) {
     double acc[kAcc];
 #pragma unroll
     for (int k = 0; k < kAcc; ++k) acc[k] = static_cast<double>(hi[k]) + static_cast<double>(lo[k]);
@@ -1821,8 +1839,10 @@ __device__ __forceinline__ void near_entry(const NearArgs& a, SrcRel* tile, floa
     if (idx >= slices) break;
     const NearItem it = a.items[idx / kNearSlices];
     const int slice = idx % kNearSlices;
-    if (a.nodes[it.T].first + kNearSlice * slice >= a.nodes[it.T].last) continue;
-    near_slice<HighOrder, Uniform>(a, it, slice, tile, tab);
+    const int left = a.nodes[it.T].last - a.nodes[it.T].first - kNearSlice * slice;  // targets from here on
+    if (left <= 0) continue;
+    if (left <= kNearSlice / 2) near_slice<HighOrder, Uniform, true>(a, it, slice, tile, tab);
+    else near_slice<HighOrder, Uniform, false>(a, it, slice, tile, tab);
   }
 }
 
